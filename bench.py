@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: elements assembled/s + CG time to 1e-8 rel. error vs FP64,
+mixed vs plain, on 128^3-cell jittered P1-tet linear elasticity.
+
+Workload (one MI355X): box_mesh 128x128x128 cells x 6 tets (12,582,912 tets,
+2,146,689 nodes, 6,440,067 DOF); interior nodes jittered by a seeded
+uniform +-0.2h per axis; E per element log-uniform in [0.1, 10], nu = 0.3; all
+DOFs clamped on x = 0; traction (0, 0, -1e-3) on x = 1.
+
+A step is one numeric assembly pass of the reduced stiffness over the whole
+mesh (the hot path), with mesh, coefficients and the CSR pattern resident in
+HBM.  value = elements assembled per second by the FP64 (plain) pass.  After
+the timed steps the reduced system is solved three ways -- FP64 PCG to
+1e-12 (the FP64 reference solution), FP64 PCG and FP32 PCG + FP64
+refinement to the same FP64 residual tolerance -- and the line reports both
+times, both errors against the reference solution and their ratio.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tpt-fem_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def asm_bytes(n_nodes, n_elem, nnz, n_free, sv, per_elem_coef=True):
+    """Algorithmic HBM bytes of one numeric assembly pass: FP64 coordinates,
+    int32 connectivity, the per-element modulus, values written once plus
+    col_ind read once, and the row pointers."""
+    return 24 * n_nodes + 16 * n_elem + (8 * n_elem if per_elem_coef else 0) + (sv + 4) * nnz + 4 * (n_free + 1)
+
+
+def cpu_baseline(n, seed=7):
+    """Single-core oracle (restated reference CPU path): assemble triplets with
+    the exact-zero skip, Coo::to_csr and reduce_system, on an n^3 sample of
+    the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], (n, n, n))
+    coords = O.jitter_interior(coords, (n, n, n), 0.2, seed)
+    ye = 10.0 ** np.random.default_rng(seed).uniform(-1.0, 1.0, len(conn))
+    bcs = [(int(i) * 3 + k, 0.0) for i in O.nodes_on_plane(coords, 0, 0.0, 1e-9) for k in range(3)]
+    rhs = np.zeros(len(coords) * 3)
+    op = O.make_op(O.TET, 3, 1, coef_e=ye, nu=0.3)
+    t0 = time.perf_counter()
+    coo = O.assemble(coords, conn, op, 3)
+    O.reduce_system(coo, rhs, bcs)
+    dt = time.perf_counter() - t0
+    return len(conn) / dt, len(conn), dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128, help="cells per axis")
+    ap.add_argument("--rtol", type=float, default=1e-10, help="FP64 residual tolerance of the timed solves")
+    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--cpu-n", type=int, default=32, help="cells per axis of the CPU-baseline sample")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-solve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import tptfem as T
+
+    n = args.n
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    ctx = T.Context(0)
+    t_setup = time.perf_counter()
+    mesh = T.Mesh.box(ctx, [0.0, 0.0, 0.0], [1.0, 1.0, 1.0], (n, n, n))
+    mesh.jitter((n, n, n), 0.2, 2024)
+    ye = 10.0 ** np.random.default_rng(2024).uniform(-1.0, 1.0, mesh.n_elem)
+    clamp = mesh.nodes_on_plane(0, 0.0, 1e-9)
+    bcs = [(int(i) * 3 + k, 0.0) for i in clamp for k in range(3)]
+    rhs = T.plane_traction(mesh, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
+    op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=ye)
+    sysm = T.System.create(mesh, op, 3, bcs, rhs, flags=T.SYS_FP64 | T.SYS_FP32)
+    ctx.synchronize()
+    log(f"setup {time.perf_counter() - t_setup:.2f}s: {mesh.n_elem} tets, {mesh.n_nodes} nodes, "
+        f"n_free {sysm.n_free}, nnz {sysm.nnz}")
+
+    # ---- timed assembly --------------------------------------------------
+    for _ in range(args.warmup):
+        sysm.assemble(None, T.SYS_FP64)
+        sysm.assemble(None, T.SYS_FP32)
+    ctx.synchronize()
+    t64, t32 = [], []
+    wall0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.timer_begin()
+        sysm.assemble(None, T.SYS_FP64)
+        t64.append(ctx.timer_end())
+    ctx.synchronize()
+    wall = time.perf_counter() - wall0
+    for _ in range(args.steps):
+        ctx.timer_begin()
+        sysm.assemble(None, T.SYS_FP32)
+        t32.append(ctx.timer_end())
+    ms64 = float(np.mean(t64))
+    ms32 = float(np.mean(t32))
+    b64 = asm_bytes(mesh.n_nodes, mesh.n_elem, sysm.nnz, sysm.n_free, 8)
+    b32 = asm_bytes(mesh.n_nodes, mesh.n_elem, sysm.nnz, sysm.n_free, 4)
+    gbs64 = b64 / (ms64 * 1e-3) / 1e9
+    gbs32 = b32 / (ms32 * 1e-3) / 1e9
+    log(f"assembly fp64 {ms64:.3f} ms ({gbs64:.0f} GB/s), fp32 {ms32:.3f} ms ({gbs32:.0f} GB/s)")
+    sysm.assemble(None, 0)
+
+    # ---- solves ------------------------------------------------------------
+    solve = {}
+    if not args.skip_solve:
+        u_ref = sysm.solve(T.SOLVE_PCG64, rtol=1e-12, max_iter=200000)
+        st = sysm.stats
+        solve["ref_fp64_rtol1e-12"] = {"s": st.seconds, "iters": st.iterations, "rel_res": st.rel_residual}
+        log(f"fp64 ref: {st.iterations} it, {st.seconds:.3f}s, res {st.rel_residual:.2e}")
+        nref = np.linalg.norm(u_ref)
+        u_p = sysm.solve(T.SOLVE_PCG64, rtol=args.rtol, max_iter=200000)
+        sp = sysm.stats
+        err_p = float(np.linalg.norm(u_p - u_ref) / nref)
+        log(f"plain: {sp.iterations} it, {sp.seconds:.3f}s, err {err_p:.2e}")
+        u_m = sysm.solve(T.SOLVE_MIXED, rtol=args.rtol, max_iter=200000, inner_rtol=args.inner_rtol,
+                         max_outer=200)
+        sm = sysm.stats
+        err_m = float(np.linalg.norm(u_m - u_ref) / nref)
+        log(f"mixed: {sm.iterations} it / {sm.outer} cycles, {sm.seconds:.3f}s, err {err_m:.2e}")
+        solve.update({
+            "rtol": args.rtol,
+            "plain_fp64": {"s": sp.seconds, "iters": sp.iterations, "rel_err_vs_fp64": err_p},
+            "mixed_fp32_fp64refine": {"s": sm.seconds, "iters": sm.iterations, "cycles": sm.outer,
+                                      "rel_err_vs_fp64": err_m},
+            "mixed_speedup_vs_plain": sp.seconds / sm.seconds if sm.seconds > 0 else None,
+        })
+
+    cpu = None
+    if not args.skip_cpu:
+        eps, ne, dt = cpu_baseline(args.cpu_n)
+        cpu = {"value": eps, "unit": "elements/s", "cores": 1, "kind": "port",
+               "sample": f"oracle assemble+to_csr+reduce_system, {args.cpu_n}^3 jittered elasticity "
+                         f"({ne} tets, {dt:.1f}s)"}
+        log(f"cpu baseline {eps:.0f} elements/s")
+
+    line = {
+        "metric": "elements assembled/s + CG time to 1e-8 rel err vs FP64, mixed vs plain",
+        "value": mesh.n_elem / (ms64 * 1e-3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jitter and moduli)",
+        "config": {"workload": f"P1-tet elasticity box_mesh {n}^3 x6, jitter 0.2h, E log-uniform [0.1,10], "
+                               "nu 0.3, x=0 clamped, traction on x=1",
+                   "n_elem": mesh.n_elem, "n_nodes": mesh.n_nodes, "n_free": sysm.n_free, "nnz_reduced": sysm.nnz},
+        "assembly": {"fp64_ms": ms64, "fp32_ms": ms32, "fp32_elements_per_s": mesh.n_elem / (ms32 * 1e-3),
+                     "fp64_GBps": gbs64, "fp32_GBps": gbs32},
+        "solve": solve,
+        "roofline": {"bound": "hbm", "achieved": gbs64, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": gbs64 / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "k_asm_tile<TET,3,F64>", "bytes_per_launch": b64},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    sysm.close()
+    mesh.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

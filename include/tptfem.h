@@ -1,0 +1,189 @@
+/*
+ * tptfem.h -- C ABI of the MI355X (gfx950) element-assembly and reduced-system
+ * solve library.  Plain C99: opaque handles, plain pointers and sizes, int
+ * status codes, no exceptions across the boundary.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the tpt-solutions/tpt-fem repository).
+ *
+ * Threading: one tfm_ctx per host thread; calls on one context are not
+ * concurrent.  Contexts are independent (the reference has no global state).
+ * Host buffers are caller-owned; device memory is owned by the handles.
+ */
+#ifndef TPTFEM_H
+#define TPTFEM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TFM_OK 0
+#define TFM_E_INVALID (-1)       /* contract violation; the reference panics  */
+#define TFM_E_NUMERIC (-2)       /* SparseError::Numeric (sparse/src/lib.rs:66-84) */
+#define TFM_E_NOT_CONVERGED (-3) /* PCG hit max_iter (no reference counterpart) */
+#define TFM_E_BREAKDOWN (-4)     /* PCG p'Ap <= 0: operator not SPD           */
+#define TFM_E_HIP (-5)           /* HIP runtime error                          */
+#define TFM_E_UNSUPPORTED (-6)   /* cell/operator combination not built        */
+#define TFM_E_MODEL (-7)         /* ElasticityError::ModelDimMismatch (elasticity/src/lib.rs:28-35) */
+
+/* CellType order: crates/tpt-fem-mesh/src/lib.rs:39-52 (P1 cells). */
+enum { TFM_LINE = 0, TFM_TRI = 1, TFM_QUAD = 2, TFM_TET = 3, TFM_HEX = 4 };
+/* ElasticModel order: crates/tpt-fem-elasticity/src/lib.rs:58-67. */
+enum { TFM_BAR = 0, TFM_PLANE_STRESS = 1, TFM_PLANE_STRAIN = 2, TFM_CONTINUUM3D = 3 };
+/* Element operators the library evaluates on the GPU. */
+enum {
+  TFM_OP_POISSON = 0,    /* poisson_element_matrix, thermal/src/lib.rs:132-160      */
+  TFM_OP_ELASTICITY = 1, /* elasticity_element_matrix, elasticity/src/lib.rs:262-309 */
+  TFM_OP_SIMP = 2        /* topopt assemble, topopt/src/lib.rs:166-238 (Q4 only)      */
+};
+enum { TFM_FP64 = 0, TFM_FP32 = 1 };
+
+typedef struct tfm_ctx tfm_ctx;
+typedef struct tfm_mesh tfm_mesh;
+typedef struct tfm_matrix tfm_matrix;
+typedef struct tfm_system tfm_system;
+
+/* Element operator: which closure the reference's assemble() would be given.
+ * coef is kappa (Poisson) or Young's modulus (elasticity) when coef_e is NULL;
+ * coef_e (host, n_elem) gives a per-element value instead, as an
+ * `|eid, m| ...element_matrix(m, eid, coef[eid], ...)` closure would.
+ * SIMP: E(e) = emin + rho[e]^penal * (e0 - emin), Ke = E(e) * Ke_unit(nu, h). */
+typedef struct {
+  int kind;
+  int model;
+  int quad_order;
+  double coef;
+  const double *coef_e;
+  double nu;
+  const double *rho;
+  double penal, e0, emin, h;
+} tfm_operator;
+
+/* ---- context ------------------------------------------------------------ */
+int tfm_ctx_create(int device, tfm_ctx **out);
+void tfm_ctx_destroy(tfm_ctx *ctx);
+/* Text of the last error on this context; mirrors SparseError's Display
+ * strings (crates/tpt-fem-sparse/src/lib.rs:76-84) where one applies. */
+const char *tfm_last_error(const tfm_ctx *ctx);
+int tfm_ctx_synchronize(tfm_ctx *ctx);
+/* Device-time bracket on the context's stream (HIP events): begin records the
+ * start, end records the stop, synchronises and returns the elapsed ms.  No
+ * reference counterpart (measurement utility). */
+int tfm_ctx_timer_begin(tfm_ctx *ctx);
+int tfm_ctx_timer_end(tfm_ctx *ctx, double *ms);
+
+/* ---- meshes (crates/tpt-fem-mesh/src/lib.rs:39-148) ---------------------- */
+/* Mesh with one cell type; coords node-major [n_nodes x pdim], conn
+ * element-major [n_elem x nodes_per_cell].  Node and element order are kept,
+ * so DOF numbering equals Mesh::number_dofs (mesh/src/lib.rs:242-250). */
+int tfm_mesh_create(tfm_ctx *ctx, int cell, int64_t n_nodes, int pdim, const double *coords,
+                    int64_t n_elem, const int64_t *conn, tfm_mesh **out);
+/* box_mesh (crates/tpt-fem-mesh-gen/src/lib.rs:422-495), generated on the GPU. */
+int tfm_mesh_box(tfm_ctx *ctx, const double mn[3], const double mx[3], int64_t nx,
+                 int64_t ny, int64_t nz, tfm_mesh **out);
+/* topopt Grid::new (crates/tpt-fem-topopt/src/lib.rs:89-116), generated on the GPU. */
+int tfm_mesh_grid(tfm_ctx *ctx, int64_t nx, int64_t ny, double h, tfm_mesh **out);
+/* Move every interior node of an (nx,ny,nz) box_mesh by (2u-1)*frac*h per axis,
+ * u = counter hash(seed, 3*node+axis).  No reference counterpart (input generator). */
+int tfm_mesh_jitter(tfm_mesh *mesh, int64_t nx, int64_t ny, int64_t nz, double frac,
+                    uint64_t seed);
+int tfm_mesh_info(const tfm_mesh *mesh, int *cell, int64_t *n_nodes, int *pdim,
+                  int64_t *n_elem);
+int tfm_mesh_download(const tfm_mesh *mesh, double *coords, int64_t *conn);
+/* Mesh::nodes_on_plane (mesh/src/lib.rs:425-431); out holds n_nodes entries. */
+int tfm_mesh_nodes_on_plane(const tfm_mesh *mesh, int axis, double coord, double tol,
+                            int64_t *out, int64_t *count);
+void tfm_mesh_destroy(tfm_mesh *mesh);
+
+/* ---- assembly (crates/tpt-fem-assembly/src/lib.rs:428-480) -------------- */
+/* K = assemble(mesh, dpn, op).to_csr() (sparse/src/lib.rs:122-175): exact-zero
+ * element entries skipped, columns ascending, nrows = max row + 1. */
+int tfm_assemble(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op,
+                 int dofs_per_node, tfm_matrix **out);
+/* A CSR matrix from host arrays (a Csr built by the caller). */
+int tfm_matrix_create(tfm_ctx *ctx, int64_t nrows, int64_t ncols, const int64_t *row_ptrs,
+                      const int64_t *col_ind, const double *values, tfm_matrix **out);
+int tfm_matrix_info(const tfm_matrix *m, int64_t *nrows, int64_t *ncols, int64_t *nnz);
+int tfm_matrix_download(const tfm_matrix *m, int64_t *row_ptrs, int64_t *col_ind,
+                        double *values);
+/* Csr::matvec (sparse/src/lib.rs:195-205): y has max(nrows, nx) entries. */
+int tfm_matrix_matvec(const tfm_matrix *m, const double *x, int64_t nx, double *y);
+void tfm_matrix_destroy(tfm_matrix *m);
+
+/* ---- loads -------------------------------------------------------------- */
+/* rhs[node] += poisson_source_vector with s(x) = f (thermal/src/lib.rs:163-196,
+ * scattered as in solve_poisson :223-230).  rhs: host, n_nodes. */
+int tfm_load_source(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, double f,
+                    double *rhs);
+/* rhs[node*dim+a] += elasticity_body_vector with constant b (elasticity/src/lib.rs:312-348). */
+int tfm_load_body(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, const double *b,
+                  double *rhs);
+/* apply_neumann (assembly/src/lib.rs:591-667) with constant flux g tiled over dpn. */
+int tfm_load_neumann(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, double g, double *rhs);
+/* Traction t[dpn] on the plane x[axis] == coord (|.| <= tol): one apply_neumann
+ * pass per nonzero component into a scalar buffer, interleaved into rhs. */
+int tfm_load_plane_traction(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, int axis,
+                            double coord, double tol, const double *t, double *rhs);
+
+/* ---- reduced systems (assembly/src/lib.rs:489-559) ----------------------- */
+enum { TFM_SYS_FP64 = 1, TFM_SYS_FP32 = 2 };
+/* reduce_system(assemble(mesh, dpn, op), rhs, bcs) without forming the full K:
+ * the reduced CSR pattern is built on the GPU (symbolic phase) and filled
+ * (numeric phase) in the values named by flags.  rhs: host, n_nodes*dpn. */
+int tfm_system_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
+                      int64_t n_bc, const int64_t *bc_dof, const double *bc_val,
+                      const double *rhs, int flags, tfm_system **out);
+/* reduce_system(K, rhs, bcs) of an assembled matrix (general lifting). */
+int tfm_system_from_matrix(tfm_ctx *ctx, const tfm_matrix *k, int64_t n, const double *rhs,
+                           int64_t n_bc, const int64_t *bc_dof, const double *bc_val,
+                           tfm_system **out);
+/* Numeric re-assembly on the existing pattern (the hot path).  op may carry
+ * new coefficients (coef_e, rho), or be NULL to reuse the device-resident
+ * ones; the pattern must stay valid for it.  which: TFM_SYS_FP64 and/or
+ * TFM_SYS_FP32 value arrays to refill (0 = every array the system holds). */
+int tfm_system_assemble(tfm_system *sys, const tfm_operator *op, int which);
+int tfm_system_info(const tfm_system *sys, int64_t *n_free, int64_t *nnz, int64_t *n_dof);
+/* ReducedSystem {kred (as its to_csr()), fred, free}. */
+int tfm_system_download(const tfm_system *sys, int64_t *row_ptrs, int64_t *col_ind,
+                        double *values, double *fred, int64_t *free_dofs);
+
+enum { TFM_SOLVE_PCG64 = 0, TFM_SOLVE_MIXED = 1 };
+typedef struct {
+  int mode;           /* TFM_SOLVE_*                                          */
+  double rtol;        /* stop when ||f - K u|| <= rtol * ||f|| (FP64 residual) */
+  int max_iter;       /* total inner iterations                               */
+  double inner_rtol;  /* mixed: FP32 inner relative reduction per cycle       */
+  int max_outer;      /* mixed: refinement cycles                             */
+} tfm_solve_opts;
+typedef struct {
+  int iterations;     /* inner PCG iterations in total          */
+  int outer;          /* refinement cycles (1 for PCG64)        */
+  double rel_residual;/* final ||f - K u|| / ||f|| in FP64      */
+  double seconds;     /* device time of the solve               */
+} tfm_solve_stats;
+/* solve_with_dirichlet (assembly/src/lib.rs:543-559): solve the reduced system
+ * by Jacobi-PCG (FP64) or FP32 PCG with FP64 refinement, scatter into u
+ * (host, n_dof) with u[bc] = value in bcs order (last duplicate wins). */
+int tfm_system_solve(tfm_system *sys, const tfm_solve_opts *opts, double *u,
+                     tfm_solve_stats *stats);
+void tfm_system_destroy(tfm_system *sys);
+
+/* ---- drivers ------------------------------------------------------------ */
+/* solve_poisson (thermal/src/lib.rs:206-240) with constant source f, no
+ * Neumann/Robin terms. */
+int tfm_solve_poisson(tfm_ctx *ctx, const tfm_mesh *mesh, double kappa, int quad_order,
+                      double f, int64_t n_bc, const int64_t *bc_dof, const double *bc_val,
+                      const tfm_solve_opts *opts, double *u);
+/* solve_elasticity (elasticity/src/lib.rs:355-381) with constant body force. */
+int tfm_solve_elasticity(tfm_ctx *ctx, const tfm_mesh *mesh, int model, double young,
+                         double nu, int quad_order, const double *body, int64_t n_bc,
+                         const int64_t *bc_dof, const double *bc_val,
+                         const tfm_solve_opts *opts, double *u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

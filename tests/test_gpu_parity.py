@@ -1,0 +1,382 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: sparsity patterns, DOF numbering and free maps identical; FP64 values
+bit-identical (up to the sign of an exact zero) wherever the reference's
+arithmetic is deterministic IEEE (every element operator except SIMP, whose
+rho^p uses the device pow: 1e-14 relative); solutions within 1e-10 relative
+of the dense-LU direct solve.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+def tp():
+    import tptfem
+
+    return tptfem
+
+
+def csr_equal(g, r, exact=True, rtol=0.0):
+    assert (g.nrows, g.ncols) == (r.nrows, r.ncols)
+    np.testing.assert_array_equal(g.row_ptrs, r.row_ptrs)
+    np.testing.assert_array_equal(g.col_ind, r.col_ind)
+    if exact:
+        bad = np.flatnonzero(g.values != r.values)
+        assert bad.size == 0, f"{bad.size} values differ, first {bad[:5]}: {g.values[bad[:5]]} vs {r.values[bad[:5]]}"
+    else:
+        scale = np.abs(r.values).max() if r.values.size else 1.0
+        assert np.abs(g.values - r.values).max() <= rtol * scale
+
+
+def hex_box(n):
+    nx, ny, nz = n
+    idx = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k
+    coords = np.array([[i / nx, j / ny, k / nz] for i in range(nx + 1) for j in range(ny + 1)
+                       for k in range(nz + 1)])
+    conn = []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                conn.append([idx(i, j, k), idx(i + 1, j, k), idx(i + 1, j + 1, k), idx(i, j + 1, k),
+                             idx(i, j, k + 1), idx(i + 1, j, k + 1), idx(i + 1, j + 1, k + 1),
+                             idx(i, j + 1, k + 1)])
+    return coords, np.array(conn, dtype=np.int64)
+
+
+def tri_grid(nx, ny):
+    coords, quads = O.grid(nx, ny, 1.0 / (nx - 1))
+    tris = []
+    for q in quads:
+        tris.append([q[0], q[1], q[2]])
+        tris.append([q[0], q[2], q[3]])
+    return coords, np.array(tris, dtype=np.int64)
+
+
+def perturb(coords, scale, seed):
+    rng = np.random.default_rng(seed)
+    return coords + rng.uniform(-scale, scale, coords.shape)
+
+
+def face_bcs(coords, dpn, faces):
+    bcs = []
+    for axis, val in faces:
+        for n in O.nodes_on_plane(coords, axis, val, 1e-9):
+            for c in range(dpn):
+                bcs.append((int(n) * dpn + c, 0.0))
+    return bcs
+
+
+# ---- generators -----------------------------------------------------------
+def test_box_mesh_matches_oracle(ctx):
+    T = tp()
+    mn, mx, n = [-1.0, 0.5, 2.0], [0.3, 1.7, 2.9], (3, 4, 5)
+    m = T.Mesh.box(ctx, mn, mx, n)
+    c, e = m.download()
+    rc, re = O.box_mesh(mn, mx, n)
+    assert np.array_equal(c, rc) and np.array_equal(e, re)
+
+
+def test_grid_and_jitter_match_oracle(ctx):
+    T = tp()
+    m = T.Mesh.grid(ctx, 9, 5, 0.25)
+    c, e = m.download()
+    rc, re = O.grid(9, 5, 0.25)
+    assert np.array_equal(c, rc) and np.array_equal(e, re)
+    b = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], (6, 5, 4))
+    b.jitter((6, 5, 4), 0.2, 1234)
+    c, _ = b.download()
+    rc, _ = O.box_mesh([0, 0, 0], [1, 1, 1], (6, 5, 4))
+    assert np.array_equal(c, O.jitter_interior(rc, (6, 5, 4), 0.2, 1234))
+    assert not np.array_equal(c, rc)
+
+
+def test_nodes_on_plane(ctx):
+    T = tp()
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], (4, 4, 4))
+    c, _ = m.download()
+    for axis in range(3):
+        np.testing.assert_array_equal(m.nodes_on_plane(axis, 1.0, 1e-9), O.nodes_on_plane(c, axis, 1.0, 1e-9))
+
+
+# ---- assembly parity --------------------------------------------------------
+def _assemble_pair(ctx, cell, coords, conn, op_gpu, op_orc, dpn):
+    T = tp()
+    m = T.Mesh.from_arrays(ctx, cell, coords, conn)
+    g = T.assemble(m, dpn, op_gpu).to_csr()
+    r = O.assemble(coords, conn, op_orc, dpn).to_csr()
+    return g, r
+
+
+def test_c1_poisson_assembly_bitexact(ctx):
+    T = tp()
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], (8, 8, 8))
+    g, r = _assemble_pair(ctx, O.TET, coords, conn, T.Operator.poisson(1.0, 2), O.make_op(O.TET, 3, 0, coef=1.0), 1)
+    assert r.nnz() == 4617
+    csr_equal(g, r)
+
+
+def test_c1_poisson_solution_matches_direct_and_closed_form(ctx):
+    T = tp()
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], (8, 8, 8))
+    coords, conn = m.download()
+    bcs = face_bcs(coords, 1, [(a, v) for a in range(3) for v in (0.0, 1.0)])
+    u = T.solve_poisson(m, 1.0, 2, 1.0, bcs)
+    ur = O.solve_poisson(coords, O.TET, conn, 1.0, 2, 1.0, bcs)
+    assert np.abs(u - ur).max() <= 1e-10 * np.abs(ur).max()
+    # 7-point FD solution of -lap u = 1 (DST-I closed form) at the centre node
+    assert abs(u.max() - 0.05491766911624052) <= 1e-12
+
+
+@pytest.mark.parametrize("n,jit", [((4, 4, 4), 0.0), ((5, 4, 3), 0.2)])
+def test_elasticity_tet_assembly_bitexact(ctx, n, jit):
+    T = tp()
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    if jit:
+        coords = O.jitter_interior(coords, n, jit, 99)
+    rng = np.random.default_rng(5)
+    ye = 10.0 ** rng.uniform(-1, 1, len(conn))
+    g, r = _assemble_pair(ctx, O.TET, coords, conn,
+                          T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=ye),
+                          O.make_op(O.TET, 3, 1, coef_e=ye, nu=0.3), 3)
+    if not jit:
+        nn = n[0]
+        assert r.nnz() == 111 * nn ** 3 + 171 * nn ** 2 + 45 * nn - 3
+    csr_equal(g, r)
+
+
+def test_elasticity_reduced_system_bitexact(ctx):
+    T = tp()
+    n = (4, 3, 3)
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    bcs = face_bcs(coords, 3, [(0, 0.0)])
+    rhs = O.plane_traction(coords, O.TET, conn, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2)
+    s = T.System.create(m, op, 3, bcs, rhs)
+    kg, fg, frg = s.download()
+    coo = O.assemble(coords, conn, O.make_op(O.TET, 3, 1, coef=1.0, nu=0.3), 3)
+    kr, fr, frr = O.reduce_system(coo, rhs, bcs)
+    np.testing.assert_array_equal(frg, frr)
+    assert np.array_equal(fg, fr)
+    csr_equal(kg, kr.to_csr())
+
+
+def test_elasticity_solution_matches_direct(ctx):
+    T = tp()
+    n = (4, 3, 3)
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    bcs = face_bcs(coords, 3, [(0, 0.0)])
+    rhs = O.plane_traction(coords, O.TET, conn, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    s = T.System.create(m, T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2), 3, bcs, rhs)
+    u = s.solve(T.SOLVE_PCG64, rtol=1e-14)
+    coo = O.assemble(coords, conn, O.make_op(O.TET, 3, 1, coef=1.0, nu=0.3), 3)
+    ur = O.solve_with_dirichlet(coo, rhs, bcs)
+    assert np.abs(u - ur).max() <= 1e-10 * np.abs(ur).max()
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_quad_poisson_and_plane_models_bitexact(ctx, order):
+    T = tp()
+    coords, conn = O.grid(7, 5, 0.3)
+    coords = perturb(coords, 0.05, 3)
+    g, r = _assemble_pair(ctx, O.QUAD, coords, conn, T.Operator.poisson(2.5, order),
+                          O.make_op(O.QUAD, 2, 0, coef=2.5, order=order), 1)
+    csr_equal(g, r)
+    for model in (T.PLANE_STRESS, T.PLANE_STRAIN):
+        g, r = _assemble_pair(ctx, O.QUAD, coords, conn, T.Operator.elasticity(model, 3.0, 0.25, order),
+                              O.make_op(O.QUAD, 2, 1, coef=3.0, nu=0.25, model=model, order=order), 2)
+        csr_equal(g, r)
+
+
+def test_hex_elasticity_and_poisson_bitexact(ctx):
+    T = tp()
+    coords, conn = hex_box((3, 2, 2))
+    coords = perturb(coords, 0.03, 11)
+    g, r = _assemble_pair(ctx, O.HEX, coords, conn, T.Operator.elasticity(T.CONTINUUM3D, 2.0, 0.3, 2),
+                          O.make_op(O.HEX, 3, 1, coef=2.0, nu=0.3, order=2), 3)
+    csr_equal(g, r)
+    g, r = _assemble_pair(ctx, O.HEX, coords, conn, T.Operator.poisson(1.0, 3),
+                          O.make_op(O.HEX, 3, 0, coef=1.0, order=3), 1)
+    csr_equal(g, r)
+
+
+def test_tri_poisson_and_plane_stress_bitexact(ctx):
+    T = tp()
+    coords, conn = tri_grid(6, 5)
+    coords = perturb(coords, 0.02, 7)
+    g, r = _assemble_pair(ctx, O.TRI, coords, conn, T.Operator.poisson(1.0, 2), O.make_op(O.TRI, 2, 0), 1)
+    csr_equal(g, r)
+    g, r = _assemble_pair(ctx, O.TRI, coords, conn, T.Operator.elasticity(T.PLANE_STRESS, 1.0, 0.3, 2),
+                          O.make_op(O.TRI, 2, 1, model=O.PSTRESS, nu=0.3), 2)
+    csr_equal(g, r)
+
+
+def test_simp_q4_pattern_exact_values_close(ctx):
+    T = tp()
+    coords, conn = O.grid(9, 5, 1.0)
+    rho = np.random.default_rng(1).uniform(0.01, 1.0, len(conn))
+    ke0 = O.simp_ke_unit(0.3, 1.0)
+    g, r = _assemble_pair(ctx, O.QUAD, coords, conn, T.Operator.simp(rho, 0.3, 3.0, 1.0, 1e-9, 1.0),
+                          O.make_op(O.QUAD, 2, 2, rho=rho, penal=3.0, e0=1.0, emin=1e-9, ke0=ke0), 2)
+    csr_equal(g, r, exact=False, rtol=1e-14)
+
+
+def test_reassembly_with_new_coefficients(ctx):
+    T = tp()
+    n = (3, 3, 3)
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    coords = O.jitter_interior(coords, n, 0.2, 3)
+    bcs = face_bcs(coords, 3, [(0, 0.0)])
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    y1 = np.full(len(conn), 1.0)
+    y2 = np.random.default_rng(2).uniform(0.5, 2.0, len(conn))
+    s = T.System.create(m, T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=y1), 3, bcs)
+    s.assemble(T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=y2))
+    kg, _, _ = s.download()
+    coo = O.assemble(coords, conn, O.make_op(O.TET, 3, 1, coef_e=y2, nu=0.3), 3)
+    kr, _, _ = O.reduce_system(coo, np.zeros(len(coords) * 3), bcs)
+    csr_equal(kg, kr.to_csr())
+
+
+# ---- loads ------------------------------------------------------------------
+def test_source_and_body_vectors_bitexact(ctx):
+    T = tp()
+    n = (4, 3, 5)
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    coords = O.jitter_interior(coords, n, 0.2, 5)
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    assert np.array_equal(T.source_vector(m, 1.7), O.poisson_rhs(coords, O.TET, conn, 1.7))
+    qc, qe = O.grid(6, 4, 0.5)
+    mq = T.Mesh.from_arrays(ctx, T.QUAD, qc, qe)
+    assert np.array_equal(T.source_vector(mq, 1.0, 3), O.poisson_rhs(qc, O.QUAD, qe, 1.0, 3))
+    b = T.body_vector(m, [0.5, -1.0, 2.0])
+    assert abs(b[2::3].sum() - 2.0) < 1e-12 and abs(b[0::3].sum() - 0.5) < 1e-12
+
+
+def test_plane_traction_and_neumann_match_oracle(ctx):
+    T = tp()
+    n = (4, 3, 3)
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    t = [0.0, 0.3, -1e-3]
+    g = T.plane_traction(m, 3, 0, 1.0, 1e-9, t)
+    r = O.plane_traction(coords, O.TET, conn, 3, 0, 1.0, 1e-9, t)
+    assert np.array_equal(g, r)
+    assert abs(r[2::3].sum() + 1e-3) < 1e-15
+    # constant flux over the whole boundary tiled over components
+    g = T.apply_neumann(m, 1, 1.0, np.zeros(len(coords)))
+    r = O.apply_neumann(coords, O.TET, conn, 1, 1.0, np.zeros(len(coords)))
+    assert np.array_equal(g, r)
+    assert abs(r.sum() - 6.0) < 1e-12
+    # 2-D: Tri3 boundary (reference neumann_conserves_total_load)
+    tc = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    mt = T.Mesh.from_arrays(ctx, T.TRI, tc, np.array([[0, 1, 2]]))
+    gt = T.apply_neumann(mt, 1, 1.0, np.zeros(3))
+    assert abs(gt.sum() - (2.0 + np.sqrt(2.0))) < 1e-9
+
+
+# ---- sparse API -------------------------------------------------------------
+def test_coo_to_csr_known_answer_and_matvec(ctx):
+    T = tp()
+    # crates/tpt-fem-sparse/src/lib.rs:378-391 and :500-524
+    coo = O.Coo([1, 0, 0, 0, 1], [1, 1, 0, 0, 0], [1.0, 1.0, 1.0, 1.0, 1.0])
+    csr = O.to_csr(coo)
+    np.testing.assert_array_equal(csr.row_ptrs, [0, 2, 4])
+    np.testing.assert_array_equal(csr.col_ind, [0, 1, 0, 1])
+    np.testing.assert_array_equal(csr.values, [2.0, 1.0, 1.0, 1.0])
+    M = T.Matrix.from_csr(ctx, T.Csr(csr.nrows, csr.ncols, csr.row_ptrs, csr.col_ind, csr.values))
+    back = M.to_csr()
+    csr_equal(back, csr)
+    a = O.to_csr(O.Coo([0, 0, 1], [0, 1, 1], [2.0, 1.0, 3.0]))
+    Ma = T.Matrix.from_csr(ctx, T.Csr(a.nrows, a.ncols, a.row_ptrs, a.col_ind, a.values))
+    y = Ma.matvec([1.0, -2.0])
+    np.testing.assert_array_equal(y, [0.0, -6.0])
+    e = O.to_csr(O.Coo())
+    Me = T.Matrix.from_csr(ctx, T.Csr(e.nrows, e.ncols, e.row_ptrs, e.col_ind, e.values))
+    np.testing.assert_array_equal(Me.matvec([1.0, 2.0, 3.0]), [0.0, 0.0, 0.0])
+
+
+def test_matvec_bitexact_on_assembled_matrix(ctx):
+    T = tp()
+    n = (5, 4, 4)
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    coords = O.jitter_interior(coords, n, 0.2, 8)
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    K = T.assemble(m, 3, T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2))
+    x = np.random.default_rng(4).standard_normal(K.ncols)
+    r = O.assemble(coords, conn, O.make_op(O.TET, 3, 1, coef=1.0, nu=0.3), 3).to_csr()
+    assert np.array_equal(K.matvec(x), r.matvec(x))
+
+
+def test_dirichlet_line_known_answer(ctx):
+    T = tp()
+    # crates/tpt-fem-assembly/src/lib.rs:795-811: 2 Line2 elements, k=1,
+    # element matrix 2*[[1,-1],[-1,1]], u(0)=0, u(1)=1 -> u(0.5) = 0.5
+    m = T.Mesh.from_arrays(ctx, T.LINE, np.array([[0.0], [0.5], [1.0]]), np.array([[0, 1], [1, 2]]))
+    K = T.assemble(m, 1, T.Operator.poisson(1.0, 2))
+    np.testing.assert_allclose(K.to_csr().values, [2.0, -2.0, -2.0, 4.0, -2.0, -2.0, 2.0], rtol=0, atol=1e-15)
+    u = T.solve_with_dirichlet(K, [0.0, 0.0, 0.0], [(0, 0.0), (2, 1.0)])
+    np.testing.assert_allclose(u, [0.0, 0.5, 1.0], atol=1e-12)
+
+
+def test_nonhomogeneous_dirichlet_lifting_matches_oracle(ctx):
+    T = tp()
+    n = (4, 4, 3)
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    bcs = [(int(i), 1.0 + 0.1 * k) for k, i in enumerate(O.nodes_on_plane(coords, 0, 0.0, 1e-9))]
+    bcs += [(int(i), -0.5) for i in O.nodes_on_plane(coords, 0, 1.0, 1e-9)]
+    bcs.append((bcs[0][0], 9.0))  # duplicate: lifting uses the first, the scatter the last
+    rhs = O.poisson_rhs(coords, O.TET, conn, 1.0)
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    s = T.System.create(m, T.Operator.poisson(1.0, 2), 1, bcs, rhs)
+    kg, fg, frg = s.download()
+    coo = O.assemble(coords, conn, O.make_op(O.TET, 3, 0), 1)
+    kr, fr, frr = O.reduce_system(coo, rhs, bcs)
+    np.testing.assert_array_equal(frg, frr)
+    assert np.array_equal(fg, fr)
+    csr_equal(kg, kr.to_csr())
+    u = s.solve(T.SOLVE_PCG64, rtol=1e-14)
+    ur = O.solve_with_dirichlet(coo, rhs, bcs)
+    assert np.abs(u - ur).max() <= 1e-10 * np.abs(ur).max()
+    assert u[bcs[0][0]] == 9.0
+
+
+# ---- precision form ---------------------------------------------------------
+def test_mixed_precision_refinement_reaches_1e8(ctx):
+    T = tp()
+    n = (10, 10, 10)
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], n)
+    m.jitter(n, 0.2, 17)
+    coords, conn = m.download()
+    ye = 10.0 ** np.random.default_rng(3).uniform(-1, 1, m.n_elem)
+    bcs = face_bcs(coords, 3, [(0, 0.0)])
+    rhs = T.plane_traction(m, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
+    op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=ye)
+    s = T.System.create(m, op, 3, bcs, rhs, flags=T.SYS_FP64 | T.SYS_FP32)
+    u64 = s.solve(T.SOLVE_PCG64, rtol=1e-13)
+    umx = s.solve(T.SOLVE_MIXED, rtol=1e-11, inner_rtol=1e-4)
+    err = np.linalg.norm(umx - u64) / np.linalg.norm(u64)
+    assert err <= 1e-8, err
+    assert s.stats.outer >= 2
+
+
+# ---- error paths --------------------------------------------------------------
+def test_model_dimension_mismatch_raises(ctx):
+    T = tp()
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], (1, 1, 1))
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    with pytest.raises(T.ElasticityError):
+        T.assemble(m, 3, T.Operator.elasticity(T.PLANE_STRESS, 1.0, 0.3, 2))
+
+
+def test_singular_system_reports_numeric_error(ctx):
+    T = tp()
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], (2, 2, 2))
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    s = T.System.create(m, T.Operator.poisson(1.0, 2), 1, [], np.ones(len(coords)))
+    with pytest.raises((T.SparseError, T.TfmError)):
+        s.solve(T.SOLVE_PCG64, rtol=1e-14, max_iter=500)

@@ -1,0 +1,313 @@
+"""CPU pins of the oracle (no GPU): every oracle function is checked against
+the reference's own known answers (tests/golden/kats.json), closed forms or
+textbook routines, chosen so that a plausible slip (dropped term, transposed
+operand, wrong sign or index) fails at least one of them."""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy import linalg
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KATS = json.load(open(os.path.join(GOLD, "kats.json")))
+CELL = {"line": O.LINE, "tri": O.TRI, "tet": O.TET}
+
+
+# ---- quadrature --------------------------------------------------------------
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_gauss_legendre_exact_to_degree(order):
+    p, w = O.gauss_legendre(order)
+    for deg in range(2 * order):
+        exact = 0.0 if deg % 2 else 2.0 / (deg + 1)
+        assert abs(np.dot(w, p ** deg) - exact) < 1e-9
+
+
+def test_tet_and_triangle_rules():
+    p, w = O.cell_quad(O.TET, 7)  # order is ignored for tets
+    assert len(w) == 4 and abs(w.sum() - 1 / 6) < 1e-15
+    x, y, z = p.T
+    assert abs(np.dot(w, x * x) - 1 / 60) < 1e-12 and abs(np.dot(w, x * y) - 1 / 120) < 1e-12
+    p, w = O.cell_quad(O.TRI, 5)
+    assert abs(w.sum() - 0.5) < 1e-15 and abs(np.dot(w, p[:, 0] ** 2) - 1 / 12) < 1e-12
+    p, w = O.cell_quad(O.HEX, 2)
+    assert len(w) == 8 and abs(w.sum() - 8.0) < 1e-14 and abs(np.dot(w, p[:, 0] ** 2 * p[:, 2] ** 2) - 8 / 9) < 1e-12
+
+
+# ---- shape functions ---------------------------------------------------------
+@pytest.mark.parametrize("cell", [O.LINE, O.TRI, O.QUAD, O.TET, O.HEX])
+def test_shape_partition_kronecker_and_gradient(cell):
+    ref = {O.LINE: [[-1], [1]], O.TRI: [[0, 0], [1, 0], [0, 1]], O.QUAD: [[-1, -1], [1, -1], [1, 1], [-1, 1]],
+           O.TET: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]],
+           O.HEX: [[x, y, z] for z in (-1, 1) for (x, y) in ((-1, -1), (1, -1), (1, 1), (-1, 1))]}[cell]
+    for i, xi in enumerate(ref):
+        np.testing.assert_allclose(O.shape(cell, xi), np.eye(len(ref))[i], atol=1e-15)
+    xi = np.array([0.13, 0.21, 0.17])[: O.DIMS[cell]]
+    assert abs(O.shape(cell, xi).sum() - 1.0) < 1e-14
+    g = O.grad(cell, xi)
+    for d in range(O.DIMS[cell]):
+        e = np.zeros(len(xi))
+        e[d] = 1e-6
+        fd = (O.shape(cell, xi + e) - O.shape(cell, xi - e)) / 2e-6
+        np.testing.assert_allclose(g[:, d], fd, atol=1e-9)
+
+
+# ---- Map ---------------------------------------------------------------------
+def test_jacobian_known_answers():
+    k = KATS["jacobian_dets"]
+    for c in k["cases"]:
+        cell = CELL[c["cell"]]
+        J, det, inv = O.map_jacobian(np.array(c["nodes"], float), O.grad(cell, c["xi"]))
+        assert abs(det - c["det"]) < k["tol"]
+        if "jac" in c:
+            np.testing.assert_allclose(J.ravel(), c["jac"], atol=k["tol"])
+        if "pgrad" in c:
+            np.testing.assert_allclose(np.array(c["pgrad_in"]) @ inv, c["pgrad"], atol=k["tol"])
+
+
+def test_inverse_is_the_inverse_not_its_transpose():
+    m = np.array([[2.0, 0.7, -0.3], [0.1, 3.0, 0.5], [-0.4, 0.2, 1.5]])
+    assert abs(O.mat_det(m) - np.linalg.det(m)) < 1e-13
+    np.testing.assert_allclose(O.mat_inv(m), np.linalg.inv(m), rtol=1e-14)
+    np.testing.assert_allclose(O.mat_inv(m[:2, :2]), np.linalg.inv(m[:2, :2]), rtol=1e-14)
+
+
+# ---- element operators -------------------------------------------------------
+def test_tri3_conductivity_known_answer():
+    k = KATS["tri3_conductivity"]
+    K = O.poisson_element_matrix(O.TRI, np.array(k["nodes"], float))
+    np.testing.assert_allclose(K, k["k"], atol=k["tol"])
+
+
+def test_tet_poisson_matrix_closed_form():
+    rng = np.random.default_rng(0)
+    X = rng.uniform(0, 1, (4, 3))
+    K = O.poisson_element_matrix(O.TET, X, kappa=2.5)
+    A = np.hstack([np.ones((4, 1)), X])
+    G = np.linalg.inv(A)[1:].T  # rows: gradients of the barycentric functions
+    vol = abs(np.linalg.det(A)) / 6
+    np.testing.assert_allclose(K, 2.5 * vol * G @ G.T, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(O.poisson_source_vector(O.TET, X, 3.0), np.full(4, 3.0 * vol / 4), rtol=1e-14)
+
+
+def test_elasticity_tet_rigid_modes_symmetry_and_closed_form():
+    rng = np.random.default_rng(1)
+    X = rng.uniform(0, 1, (4, 3))
+    E, nu = 2.0, 0.3
+    K = O.elasticity_element_matrix(O.TET, X, O.CONT3D, E, nu)
+    np.testing.assert_allclose(K, K.T, rtol=1e-12, atol=1e-14)
+    modes = []
+    for t in np.eye(3):
+        modes.append(np.tile(t, 4))
+    for w in np.eye(3):
+        modes.append(np.cross(w, X).ravel())
+    for mvec in modes:
+        assert np.abs(K @ mvec).max() < 1e-12 * np.abs(K).max()
+    # textbook B^T D B V with independent B (Voigt xx,yy,zz,xy,yz,zx)
+    A = np.hstack([np.ones((4, 1)), X])
+    G = np.linalg.inv(A)[1:].T
+    vol = abs(np.linalg.det(A)) / 6
+    B = np.zeros((6, 12))
+    for a in range(4):
+        gx, gy, gz = G[a]
+        B[:, 3 * a:3 * a + 3] = [[gx, 0, 0], [0, gy, 0], [0, 0, gz], [gy, gx, 0], [0, gz, gy], [gz, 0, gx]]
+    c = E / ((1 + nu) * (1 - 2 * nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = c * nu
+    np.fill_diagonal(D[:3, :3], c * (1 - nu))
+    D[3:, 3:] = np.eye(3) * c * (1 - 2 * nu) / 2
+    np.testing.assert_allclose(K, vol * B.T @ D @ B, rtol=1e-11, atol=1e-13)
+
+
+def test_q4_simp_unit_stiffness_matches_textbook():
+    nu = 0.3
+    k = np.array([1 / 2 - nu / 6, 1 / 8 + nu / 8, -1 / 4 - nu / 12, -1 / 8 + 3 * nu / 8, -1 / 4 + nu / 12,
+                  -1 / 8 - nu / 8, nu / 6, 1 / 8 - 3 * nu / 8])
+    idx = [[0, 1, 2, 3, 4, 5, 6, 7], [1, 0, 7, 6, 5, 4, 3, 2], [2, 7, 0, 5, 6, 3, 4, 1], [3, 6, 5, 0, 7, 2, 1, 4],
+           [4, 5, 6, 7, 0, 1, 2, 3], [5, 4, 3, 2, 1, 0, 7, 6], [6, 3, 4, 1, 2, 7, 0, 5], [7, 2, 1, 4, 3, 6, 5, 0]]
+    KE = k[np.array(idx)] / (1 - nu * nu)
+    np.testing.assert_allclose(O.simp_ke_unit(nu, 1.0), KE, atol=1e-15)
+    np.testing.assert_allclose(O.simp_ke_unit(nu, 0.37), KE, atol=1e-14)  # 2-D: independent of h
+    # and equals the generic plane-stress Q4 operator on a unit square
+    sq = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+    np.testing.assert_allclose(O.elasticity_element_matrix(O.QUAD, sq, O.PSTRESS, 1.0, nu), KE, atol=1e-15)
+
+
+def test_hex_elasticity_rigid_modes():
+    X = np.array([[x, y, z] for z in (0, 1) for (x, y) in ((0, 0), (1, 0), (1, 1), (0, 1))], float)
+    X += np.random.default_rng(2).uniform(-0.05, 0.05, X.shape)
+    K = O.elasticity_element_matrix(O.HEX, X, O.CONT3D, 1.0, 0.3)
+    for w in np.eye(3):
+        assert np.abs(K @ np.cross(w, X).ravel()).max() < 1e-12
+    assert np.abs(K @ np.tile([1.0, 0, 0], 8)).max() < 1e-13
+
+
+# ---- sparse ------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["coo_to_csr", "coo_doctest"])
+def test_to_csr_known_answers(name):
+    k = KATS[name]
+    csr = O.to_csr(O.Coo(k["rows"], k["cols"], k["vals"]))
+    np.testing.assert_array_equal(csr.row_ptrs, k["row_ptrs"])
+    np.testing.assert_array_equal(csr.values, k["values"])
+    if "col_ind" in k:
+        np.testing.assert_array_equal(csr.col_ind, k["col_ind"])
+
+
+def test_to_csr_stable_insertion_order_sum():
+    # (1e16 + 1) - 1e16 depends on the order: insertion order must be kept
+    csr = O.to_csr(O.Coo([0, 0, 0], [0, 0, 0], [1e16, 1.0, -1e16]))
+    assert csr.values[0] == (1e16 + 1.0) - 1e16
+    csr = O.to_csr(O.Coo([0, 0, 0], [0, 0, 0], [1.0, 1e16, -1e16]))
+    assert csr.values[0] == 0.0
+    e = O.to_csr(O.Coo())
+    assert (e.nrows, e.ncols, list(e.row_ptrs)) == (1, 1, [0, 0])
+
+
+@pytest.mark.parametrize("name", ["solve_2x2", "solve_3x3"])
+def test_dense_solve_known_answers(name):
+    k = KATS[name]
+    x = O.solve(O.Coo(k["rows"], k["cols"], k["vals"]), k["rhs"])
+    np.testing.assert_allclose(x, k["x"], atol=k["tol"])
+
+
+def test_dense_solve_matches_lapack():
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((40, 40)) + 40 * np.eye(40)
+    b = rng.standard_normal(40)
+    r, c = np.nonzero(A)
+    x = O.solve(O.Coo(r, c, A[r, c]), b)
+    np.testing.assert_allclose(x, linalg.solve(A, b), rtol=1e-12)
+    with pytest.raises(O.SolveError):
+        O.solve(O.Coo([0, 1], [0, 1], [1.0, 0.0]), [1.0, 1.0])
+
+
+def test_matvec_known_answer():
+    k = KATS["matvec"]
+    y = O.to_csr(O.Coo(k["rows"], k["cols"], k["vals"])).matvec(k["x"])
+    np.testing.assert_allclose(y, k["y"], atol=k["tol"])
+    np.testing.assert_array_equal(O.to_csr(O.Coo()).matvec([1.0, 2.0, 3.0]), [0.0, 0.0, 0.0])
+
+
+# ---- assembly / reduction ------------------------------------------------------
+def test_assemble_skips_exact_zeros_and_dirichlet_known_answer():
+    k = KATS["dirichlet_line"]
+    coords, conn = np.array(k["coords"]), np.array(k["conn"])
+    coo = O.assemble(coords, conn, O.make_op(O.LINE, 1, 0), 1)
+    u = O.solve_with_dirichlet(coo, np.zeros(3), [tuple(b) for b in k["bcs"]])
+    np.testing.assert_allclose(u, k["u"], atol=k["tol"])
+    # Kuhn tets: 6 of 16 Poisson entries per tet are exact zeros
+    c, e = O.box_mesh([0, 0, 0], [1, 1, 1], (2, 2, 2))
+    assert len(O.assemble(c, e, O.make_op(O.TET, 3, 0), 1).rows) == 10 * len(e)
+
+
+def test_reduce_system_lifting_first_match_and_scatter_last():
+    coo = O.Coo([0, 0, 1, 1, 1, 2, 2], [0, 1, 0, 1, 2, 1, 2], [2, -1, -1, 2, -1, -1, 2])
+    bcs = [(0, 1.0), (2, 3.0), (0, 5.0)]
+    kred, fred, free = O.reduce_system(coo, [0.0, 0.5, 0.0], bcs)
+    np.testing.assert_array_equal(free, [1])
+    assert fred[0] == 0.5 + 1.0 + 3.0  # first value of DOF 0 lifts
+    u = O.solve_with_dirichlet(coo, [0.0, 0.5, 0.0], bcs)
+    assert u[0] == 5.0 and abs(u[1] - 2.25) < 1e-15
+
+
+def test_neumann_total_load_and_boundary_faces():
+    k = KATS["neumann_total"]
+    c, e = np.array(k["coords"], float), np.array(k["conn"])
+    rhs = O.apply_neumann(c, O.TRI, e, 1, 1.0, np.zeros(3))
+    assert abs(rhs.sum() - k["total"]) < k["tol"]
+    bc, be = O.box_mesh([0, 0, 0], [1, 1, 1], (3, 2, 2))
+    assert len(O.boundary_faces(O.TET, be)) == 2 * 2 * (3 * 2 + 3 * 2 + 2 * 2)
+    t = O.plane_traction(bc, O.TET, be, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
+    assert abs(t[2::3].sum() + 1e-3) < 1e-15 and not t[0::3].any()
+
+
+def test_box_mesh_counts_orientation_and_numbering():
+    k = KATS["box_mesh_2"]
+    c, e = O.box_mesh([0, 0, 0], [1, 1, 1], (2, 2, 2))
+    assert len(c) == k["nodes"] and len(e) == k["tets"]
+    vols = [np.linalg.det(np.array([c[t[1]] - c[t[0]], c[t[2]] - c[t[0]], c[t[3]] - c[t[0]]])) for t in e]
+    assert min(vols) > 0
+    c, e = O.box_mesh([0, 0, 0], [2, 3, 4], (2, 3, 4))
+    i, j, kk = 1, 2, 3
+    np.testing.assert_array_equal(c[(i * 4 + j) * 5 + kk], [1.0, 2.0, 3.0])
+
+
+# ---- drivers against closed forms / reference KATs -------------------------------
+def test_c1_poisson_equals_dst_closed_form():
+    g = json.load(open(os.path.join(GOLD, "c1_fd_solution.json")))
+    c, e = O.box_mesh([0, 0, 0], [1, 1, 1], (8, 8, 8))
+    bcs = [(int(n), 0.0) for a in range(3) for v in (0.0, 1.0) for n in O.nodes_on_plane(c, a, v, 1e-9)]
+    u = O.solve_poisson(c, O.TET, e, 1.0, 2, 1.0, bcs)
+    interior = u.reshape(9, 9, 9)[1:-1, 1:-1, 1:-1].ravel()
+    np.testing.assert_allclose(interior, g["interior_u"], rtol=0, atol=1e-15)
+    # the reduced matrix is h*(6I - neighbours): diag 0.75, off-diagonal
+    # -0.125 in exact arithmetic; the reference's IEEE order (w = 1/24 is not
+    # representable) lands within a few ulp of those values
+    coo = O.assemble(c, e, O.make_op(O.TET, 3, 0), 1)
+    kr, fr, free = O.reduce_system(coo, O.poisson_rhs(c, O.TET, e), bcs)
+    k = kr.to_csr()
+    assert k.nnz() == 2107
+    diag = k.values[k.col_ind == np.repeat(np.arange(343), np.diff(k.row_ptrs))]
+    off = k.values[k.col_ind != np.repeat(np.arange(343), np.diff(k.row_ptrs))]
+    assert len(diag) == 343 and np.abs(diag - 0.75).max() <= 4e-16
+    assert np.abs(off + 0.125).max() <= 1e-16
+    np.testing.assert_allclose(fr, np.full(343, 1 / 512), rtol=1e-15)
+
+
+def test_stale_reference_vtk_is_not_reproduced():
+    s = json.load(open(os.path.join(GOLD, "stale_vtk.json")))
+    c, e = O.box_mesh([0, 0, 0], [1, 1, 1], (6, 6, 6))
+    bcs = [(int(n), 0.0) for a in range(3) for v in (0.0, 1.0) for n in O.nodes_on_plane(c, a, v, 1e-9)]
+    u = O.solve_poisson(c, O.TET, e, 1.0, 2, 1.0, bcs)
+    assert len(c) == s["points"] and len(e) == s["cells"]
+    assert abs(u.max() - s["temperature_max"]) > 1e-3
+
+
+def test_kuhn_elasticity_pattern_counts():
+    g = json.load(open(os.path.join(GOLD, "kuhn_counts.json")))
+    for n in (2, 4):
+        c, e = O.box_mesh([0, 0, 0], [1, 1, 1], (n, n, n))
+        coo = O.assemble(c, e, O.make_op(O.TET, 3, 1, nu=0.3), 3)
+        assert coo.to_csr().nnz() == g[str(n)]["full"]
+        bcs = [(int(i) * 3 + k, 0.0) for i in O.nodes_on_plane(c, 0, 0.0, 1e-9) for k in range(3)]
+        kr, _, _ = O.reduce_system(coo, np.zeros(len(c) * 3), bcs)
+        assert kr.to_csr().nnz() == g[str(n)]["reduced_x0_clamped"]
+
+
+def test_reference_solver_known_answers():
+    k = KATS["poisson_2d_linear"]
+    u = O.solve_poisson(np.array(k["coords"], float), O.TRI, np.array(k["conn"]), 1.0, 2, 0.0,
+                        [tuple(b) for b in k["bcs"]])
+    assert abs(u[4] - k["u_mid"]) < k["tol"]
+    k = KATS["bar_axial"]
+    c, e = np.array(k["coords"]), np.array(k["conn"])
+    coo = O.assemble(c, e, O.make_op(O.LINE, 1, 1, model=O.BAR, coef=1.0, nu=0.0, order=1), 1)
+    u = O.solve_with_dirichlet(coo, np.zeros(3), [tuple(b) for b in k["bcs"]])
+    np.testing.assert_allclose(u, k["u"], atol=k["tol"])
+    k = KATS["plane_stress_patch"]
+    c, e = np.array(k["coords"], float), np.array(k["conn"])
+    coo = O.assemble(c, e, O.make_op(O.TRI, 2, 1, model=O.PSTRESS, nu=0.3), 2)
+    u = O.solve_with_dirichlet(coo, np.zeros(10), [tuple(b) for b in k["bcs"]])
+    np.testing.assert_allclose(u[8:10], k["u_mid"], atol=k["tol"])
+    k = KATS["poisson_1d"]
+    n = k["nx"]
+    c = np.linspace(0, 1, n + 1)[:, None]
+    e = np.array([[i, i + 1] for i in range(n)])
+    u = O.solve_poisson(c, O.LINE, e, 1.0, 2, 1.0, [(0, 0.0), (n, 0.0)])
+    assert abs(u[n // 2] - k["u_mid"]) < k["tol"]
+
+
+def test_planted_transposed_inverse_is_caught():
+    """A transposed J^-1 (the reference's old bug) must fail the closed-form
+    element check; restated here in numpy so the oracle itself is untouched."""
+    rng = np.random.default_rng(9)
+    X = rng.uniform(0, 1, (4, 3))
+    J = (X[1:] - X[0]).T
+    lg = O.grad(O.TET, [0.2, 0.2, 0.2])
+    good = lg @ np.linalg.inv(J)
+    bad = lg @ np.linalg.inv(J).T
+    A = np.hstack([np.ones((4, 1)), X])
+    G = np.linalg.inv(A)[1:].T
+    assert np.allclose(good, G) and not np.allclose(bad, G)

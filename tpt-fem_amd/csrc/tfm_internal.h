@@ -1,0 +1,160 @@
+// tfm_internal.h -- internal types of the MI355X assembly/solve library.
+//
+// Device data layout (HBM):
+//   coords   double [n_nodes * pdim]   node-major, components interleaved
+//   conn     int32  [n_elem * nn]      element-major
+//   n2e_ptr  int32  [n_nodes + 1]      node -> element adjacency (CSR)
+//   n2e      int32  [n_elem * nn]      (eid << 3) | local index, eid ascending
+//   CSR      int32 row_ptr [nrows+1], int32 col_ind [nnz], double/float values
+// Reduced systems keep global DOF numbering on the device: fixed rows are
+// empty and fixed columns absent, which is entry-for-entry the reference's
+// kred (assembly/src/lib.rs:504-535) with the free map applied; the reduced
+// numbering is produced only when a caller downloads it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/tptfem.h"
+
+#define TFM_CHECK_HIP(ctx, call)                                               \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      return tfm::fail((ctx), TFM_E_HIP, std::string(#call) + ": " +            \
+                                             hipGetErrorString(e_));           \
+    }                                                                          \
+  } while (0)
+
+struct tfm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // small device scratch for reductions / scalars (grow-only)
+  void *d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct tfm_mesh {
+  tfm_ctx *ctx = nullptr;
+  int cell = 0, nn = 0, pdim = 0, dim = 0;
+  int64_t n_nodes = 0, n_elem = 0;
+  double *d_coords = nullptr;
+  int *d_conn = nullptr;
+  // node -> element adjacency, built on first use
+  int *d_n2e_ptr = nullptr;
+  int *d_n2e = nullptr;
+  int max_adj = 0;
+  std::vector<int> h_n2e_ptr;  // kept for tile scheduling
+};
+
+struct tfm_matrix {
+  tfm_ctx *ctx = nullptr;
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  int *d_rp = nullptr;
+  int *d_ci = nullptr;
+  double *d_v = nullptr;
+};
+
+namespace tfm {
+
+int fail(tfm_ctx *ctx, int code, const std::string &msg);
+int set_device(tfm_ctx *ctx);
+int ensure_adjacency(tfm_mesh *m);
+int scratch(tfm_ctx *ctx, size_t bytes, void **out);
+
+template <typename T>
+int dalloc(tfm_ctx *ctx, T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(ctx, TFM_E_HIP, std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+                                    " bytes): " + hipGetErrorString(e));
+  }
+  return TFM_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+inline int cell_nodes(int cell) {
+  static const int nn[5] = {2, 3, 4, 4, 8};
+  return (cell >= 0 && cell < 5) ? nn[cell] : 0;
+}
+inline int cell_dim(int cell) {
+  static const int dd[5] = {1, 2, 2, 3, 3};
+  return (cell >= 0 && cell < 5) ? dd[cell] : 0;
+}
+
+// Exclusive scan of n int32 counts into out[0..n] (out[n] = total) on the
+// context stream; returns the total through *total (synchronises).
+int exclusive_scan_i32(tfm_ctx *ctx, const int *d_in, int *d_out, int64_t n, int64_t *total);
+
+// Assembly engine (tfm_assemble.hip).
+struct Pattern;  // opaque
+struct AsmPlan {
+  const tfm_mesh *mesh = nullptr;
+  tfm_operator op{};
+  int dpn = 1;
+  int64_t ndof = 0;
+  // device copies of per-element coefficient arrays (owned)
+  double *d_coef_e = nullptr;
+  double *d_rho = nullptr;
+  double ke0[64] = {0};
+  // quadrature rule table then SIMP unit stiffness, one device buffer (owned)
+  double *d_tab = nullptr;
+  int nq = 0;
+  // fixed-DOF flags (owned), nullptr when no DOF is fixed
+  unsigned char *d_fixed = nullptr;
+  // tile schedule (owned)
+  int *d_tiles = nullptr;
+  int ntiles = 0;
+  int tile_pairs = 0;
+  // CSR pattern in global DOF numbering (owned)
+  int *d_rp = nullptr;
+  int *d_ci = nullptr;
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+};
+int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
+                const unsigned char *h_fixed, AsmPlan *plan);
+int plan_set_coefficients(tfm_ctx *ctx, AsmPlan *plan, const tfm_operator *op);
+int plan_numeric(tfm_ctx *ctx, const AsmPlan *plan, double *d_v64, float *d_v32);
+void plan_destroy(AsmPlan *plan);
+int validate_operator(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn);
+
+// Loads (tfm_loads.hip): device rhs of n_nodes*dpn doubles, accumulated.
+int load_source_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double f, double *d_rhs);
+int load_body_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *b, double *d_rhs);
+int load_neumann_dev(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int comp, double g, int axis,
+                     double coord, double tol, double *d_rhs);
+
+}  // namespace tfm
+
+struct tfm_system {
+  tfm_ctx *ctx = nullptr;
+  int dpn = 1;
+  int64_t ndof = 0, n_free = 0;
+  bool has_plan = false;
+  tfm::AsmPlan plan;          // when built from a mesh
+  // working matrix (global DOF numbering, fixed rows empty / cols absent)
+  int *d_rp = nullptr;        // aliases plan.d_rp when has_plan
+  int *d_ci = nullptr;
+  int64_t nrows = 0, nnz = 0;
+  double *d_v64 = nullptr;
+  float *d_v32 = nullptr;
+  double *d_f = nullptr;      // reduced rhs (fred) scattered to global DOFs, 0 at fixed
+  unsigned char *d_fixed = nullptr;  // aliases plan.d_fixed when has_plan
+  std::vector<int64_t> bc_dof;
+  std::vector<double> bc_val;
+  std::vector<int64_t> free_dofs;
+  bool owns_pattern = false;
+};

@@ -1,0 +1,327 @@
+// tfm_loads.hip -- right-hand-side loads on the GPU.
+//
+// Each kernel is a per-node gather over the node's element list (element-id
+// order), which is the order in which the reference's drivers scatter into
+// rhs, so FP64 sums match bit for bit:
+//   * source vector  -- poisson_source_vector + solve_poisson's scatter
+//     (crates/tpt-fem-thermal/src/lib.rs:163-196, 223-230);
+//   * body vector    -- elasticity_body_vector + solve_elasticity's scatter
+//     (crates/tpt-fem-elasticity/src/lib.rs:312-348, 370-379);
+//   * Neumann flux   -- boundary_faces + apply_neumann_order
+//     (crates/tpt-fem-assembly/src/lib.rs:563-667): faces seen by exactly one
+//     element, visited in (element, face, quadrature point) order.
+#include <cmath>
+
+#include "tfm_elem.h"
+#include "tfm_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace tfm {
+
+int build_rule(int cell, int order, std::vector<double> &rule);
+
+__global__ void k_load_source(const double *__restrict__ coords, int pdim, const int *__restrict__ conn,
+                              int nn, int cell, const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
+                              int64_t n_nodes, const double *__restrict__ rule, int nq, double s_const,
+                              const double *__restrict__ bvec, int dim, double *__restrict__ rhs) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  const bool body = bvec != nullptr;
+  const int dpn = body ? dim : 1;
+  double acc[3];
+  for (int a = 0; a < dpn; a++) acc[a] = rhs[n * dpn + a];
+  for (int p = n2e_ptr[n]; p < n2e_ptr[n + 1]; p++) {
+    int v = n2e[p], e = v >> 3, ln = v & 7;
+    double X[8][3];
+    for (int a = 0; a < nn; a++)
+      for (int d = 0; d < 3; d++) X[a][d] = d < dim ? coords[(int64_t)conn[(int64_t)e * nn + a] * pdim + d] : 0.0;
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < nq; q++) {
+      const double *R = rule + q * kRuleStride;
+      double w = R[0];
+      const double *ns = R + 4;
+      const double *lg = R + 12;
+      double g[8][3];
+      double det;
+      switch (cell) {
+      case TFM_LINE: { double Xl[2][3] = {{X[0][0], X[0][1], X[0][2]}, {X[1][0], X[1][1], X[1][2]}};
+        double gl[2][3]; det = map_point<2, 1>(Xl, lg, gl); break; }
+      case TFM_TRI: { double Xt[3][3]; for (int a = 0; a < 3; a++) for (int d = 0; d < 3; d++) Xt[a][d] = X[a][d];
+        double gt[3][3]; det = map_point<3, 2>(Xt, lg, gt); break; }
+      case TFM_QUAD: { double Xq[4][3]; for (int a = 0; a < 4; a++) for (int d = 0; d < 3; d++) Xq[a][d] = X[a][d];
+        double gq[4][3]; det = map_point<4, 2>(Xq, lg, gq); break; }
+      case TFM_TET: { double Xt[4][3]; for (int a = 0; a < 4; a++) for (int d = 0; d < 3; d++) Xt[a][d] = X[a][d];
+        double gt[4][3]; det = map_point<4, 3>(Xt, lg, gt); break; }
+      default: { det = map_point<8, 3>(X, lg, g); break; }
+      }
+      det = fabs(det);
+      if (body) {
+        for (int a = 0; a < dpn; a++) f[a] += w * ns[ln] * bvec[a] * det;
+      } else {
+        f[0] += w * s_const * ns[ln] * det;
+      }
+    }
+    for (int a = 0; a < dpn; a++) acc[a] += f[a];
+  }
+  for (int a = 0; a < dpn; a++) rhs[n * dpn + a] = acc[a];
+}
+
+// Boundary-face tables (crates/tpt-fem-assembly/src/lib.rs:73-117, P1 cells).
+struct FaceTab {
+  int nf, fn_;          // faces per cell, nodes per face
+  int fcell;            // reference cell of the face
+  int idx[6][4];
+};
+__constant__ FaceTab c_faces[5] = {
+    {2, 1, TFM_LINE, {{0}, {1}}},
+    {3, 2, TFM_LINE, {{1, 2}, {2, 0}, {0, 1}}},
+    {4, 2, TFM_LINE, {{0, 1}, {1, 2}, {2, 3}, {3, 0}}},
+    {4, 3, TFM_TRI, {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}}},
+    {6, 4, TFM_QUAD, {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {3, 2, 6, 7}, {0, 4, 7, 3}, {1, 5, 6, 2}}},
+};
+
+// One scalar Neumann pass: for every boundary face containing node n, in
+// (element, face) order, rhs[n*dpn + c] += w*measure*g*N_k at each face
+// quadrature point, with g = gconst, or -- when axis >= 0 -- gconst only at
+// points with |x[axis] - coord| <= tol and 0 elsewhere.  comp < 0 tiles the
+// load over all dpn components (apply_neumann's behaviour).
+__global__ void k_load_neumann(const double *__restrict__ coords, int pdim, const int *__restrict__ conn, int nn,
+                               int cell, const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
+                               int64_t n_nodes, const double *__restrict__ frule, int fnq, int dpn, int comp,
+                               double gconst, int axis, double coord, double tol, double *__restrict__ rhs) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  const FaceTab &FT = c_faces[cell];
+  const int pb = n2e_ptr[n], pe = n2e_ptr[n + 1];
+  double acc[3];
+  for (int c = 0; c < dpn; c++) acc[c] = rhs[n * dpn + c];
+  bool touched = false;
+  for (int p = pb; p < pe; p++) {
+    int v = n2e[p], e = v >> 3, ln = v & 7;
+    int en[8];
+    for (int a = 0; a < nn; a++) en[a] = conn[(int64_t)e * nn + a];
+    for (int fi = 0; fi < FT.nf; fi++) {
+      int kself = -1;
+      for (int k = 0; k < FT.fn_; k++)
+        if (FT.idx[fi][k] == ln) kself = k;
+      if (kself < 0) continue;
+      // boundary iff no other element holds every node of this face
+      bool shared = false;
+      for (int p2 = pb; p2 < pe && !shared; p2++) {
+        int e2 = n2e[p2] >> 3;
+        if (e2 == e) continue;
+        bool all = true;
+        for (int k = 0; k < FT.fn_ && all; k++) {
+          int node = en[FT.idx[fi][k]];
+          bool found = false;
+          for (int a = 0; a < nn; a++) found |= conn[(int64_t)e2 * nn + a] == node;
+          all = found;
+        }
+        shared = all;
+      }
+      if (shared) continue;
+      if (FT.fn_ == 1) continue;  // point faces of Line2 cells: no surface measure
+      double P[4][3];
+      for (int k = 0; k < FT.fn_; k++)
+        for (int d = 0; d < 3; d++) P[k][d] = d < pdim ? coords[(int64_t)en[FT.idx[fi][k]] * pdim + d] : 0.0;
+      for (int q = 0; q < fnq; q++) {
+        const double *R = frule + q * kRuleStride;
+        double w = R[0];
+        const double *ns = R + 4;
+        const double *ng = R + 12;
+        int fdim = FT.fcell == TFM_LINE ? 1 : 2;
+        double xp[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < FT.fn_; k++)
+          for (int d = 0; d < pdim; d++) xp[d] += ns[k] * P[k][d];
+        double t[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        for (int m = 0; m < fdim; m++)
+          for (int k = 0; k < FT.fn_; k++)
+            for (int d = 0; d < pdim; d++) t[m][d] += ng[k * fdim + m] * P[k][d];
+        double measure;
+        if (fdim == 1) {
+          double s = t[0][0] * t[0][0];
+          for (int d = 1; d < pdim; d++) s = s + t[0][d] * t[0][d];
+          measure = sqrt(s);
+        } else {
+          double c0 = t[0][1] * t[1][2] - t[0][2] * t[1][1];
+          double c1 = t[0][2] * t[1][0] - t[0][0] * t[1][2];
+          double c2 = t[0][0] * t[1][1] - t[0][1] * t[1][0];
+          measure = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        }
+        double g = gconst;
+        if (axis >= 0 && !(fabs(xp[axis] - coord) <= tol)) g = 0.0;
+        double val = w * measure * g * ns[kself];
+        touched = true;
+        if (comp < 0) {
+          for (int c = 0; c < dpn; c++) acc[c] += val;
+        } else {
+          acc[comp] += val;
+        }
+      }
+    }
+  }
+  if (touched)
+    for (int c = 0; c < dpn; c++) rhs[n * dpn + c] = acc[c];
+}
+
+static int upload_rule(tfm_ctx *ctx, int cell, int order, double **d_rule, int *nq) {
+  std::vector<double> rule;
+  int q = build_rule(cell, order, rule);
+  if (q <= 0) return fail(ctx, TFM_E_INVALID, "gauss_legendre: order must be in 1..=5");
+  int rc = dalloc(ctx, d_rule, rule.size());
+  if (rc) return rc;
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(*d_rule, rule.data(), sizeof(double) * rule.size(), hipMemcpyHostToDevice,
+                                    ctx->stream));
+  *nq = q;
+  return TFM_OK;
+}
+
+static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int load_source_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double f, double *d_rhs) {
+  int rc;
+  if ((rc = ensure_adjacency(mesh))) return rc;
+  double *d_rule = nullptr;
+  int nq = 0;
+  if ((rc = upload_rule(ctx, mesh->cell, order, &d_rule, &nq))) return rc;
+  if (mesh->n_nodes > 0)
+    hipLaunchKernelGGL(k_load_source, dim3(grid1(mesh->n_nodes, 128)), dim3(128), 0, ctx->stream, mesh->d_coords,
+                       mesh->pdim, mesh->d_conn, mesh->nn, mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, mesh->n_nodes,
+                       d_rule, nq, f, (const double *)nullptr, mesh->dim, d_rhs);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  dfree(d_rule);
+  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, "source-vector kernel failed");
+  return TFM_OK;
+}
+
+int load_body_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *b, double *d_rhs) {
+  int rc;
+  if ((rc = ensure_adjacency(mesh))) return rc;
+  double *d_rule = nullptr, *d_b = nullptr;
+  int nq = 0;
+  if ((rc = upload_rule(ctx, mesh->cell, order, &d_rule, &nq))) return rc;
+  if ((rc = dalloc(ctx, &d_b, 3))) return rc;
+  double hb[3] = {0, 0, 0};
+  for (int a = 0; a < mesh->dim; a++) hb[a] = b[a];
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(d_b, hb, sizeof hb, hipMemcpyHostToDevice, ctx->stream));
+  if (mesh->n_nodes > 0)
+    hipLaunchKernelGGL(k_load_source, dim3(grid1(mesh->n_nodes, 128)), dim3(128), 0, ctx->stream, mesh->d_coords,
+                       mesh->pdim, mesh->d_conn, mesh->nn, mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, mesh->n_nodes,
+                       d_rule, nq, 0.0, d_b, mesh->dim, d_rhs);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  dfree(d_rule);
+  dfree(d_b);
+  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, "body-vector kernel failed");
+  return TFM_OK;
+}
+
+int load_neumann_dev(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int comp, double g, int axis, double coord,
+                     double tol, double *d_rhs) {
+  int rc;
+  if ((rc = ensure_adjacency(mesh))) return rc;
+  // face_quad (assembly/src/lib.rs:336-358) at FACE_QUAD_ORDER = 2: Line
+  // faces Gauss-Legendre 2, Tri faces the Degree2 rule, Quad faces 2x2.
+  int fcell = mesh->cell == TFM_TET ? TFM_TRI : (mesh->cell == TFM_HEX ? TFM_QUAD : TFM_LINE);
+  double *d_rule = nullptr;
+  int nq = 0;
+  if ((rc = upload_rule(ctx, fcell, 2, &d_rule, &nq))) return rc;
+  if (mesh->n_nodes > 0)
+    hipLaunchKernelGGL(k_load_neumann, dim3(grid1(mesh->n_nodes, 128)), dim3(128), 0, ctx->stream, mesh->d_coords,
+                       mesh->pdim, mesh->d_conn, mesh->nn, mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, mesh->n_nodes,
+                       d_rule, nq, dpn, comp, g, axis, coord, tol, d_rhs);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  dfree(d_rule);
+  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, "Neumann kernel failed");
+  return TFM_OK;
+}
+
+}  // namespace tfm
+
+using namespace tfm;
+
+static int host_rhs_roundtrip(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, double *rhs,
+                              int (*fn)(tfm_ctx *, tfm_mesh *, double *, const void *), const void *arg) {
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  int64_t n = mesh->n_nodes * dpn;
+  double *d = nullptr;
+  if ((rc = dalloc(ctx, &d, n))) return rc;
+  hipError_t e = hipMemcpy(d, rhs, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { dfree(d); return fail(ctx, TFM_E_HIP, "rhs upload failed"); }
+  rc = fn(ctx, const_cast<tfm_mesh *>(mesh), d, arg);
+  if (!rc && hipMemcpy(rhs, d, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(ctx, TFM_E_HIP, "rhs download failed");
+  dfree(d);
+  return rc;
+}
+
+struct SrcArg { int order; double f; };
+struct BodyArg { int order; const double *b; };
+struct NeuArg { int dpn, comp; double g; int axis; double coord, tol; };
+
+extern "C" int tfm_load_source(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, double f, double *rhs) {
+  if (!ctx || !mesh || !rhs) return TFM_E_INVALID;
+  SrcArg a{quad_order, f};
+  return host_rhs_roundtrip(ctx, mesh, 1, rhs,
+                            [](tfm_ctx *c, tfm_mesh *m, double *d, const void *p) {
+                              const SrcArg *s = (const SrcArg *)p;
+                              return load_source_dev(c, m, s->order, s->f, d);
+                            },
+                            &a);
+}
+
+extern "C" int tfm_load_body(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, const double *b, double *rhs) {
+  if (!ctx || !mesh || !rhs || !b) return TFM_E_INVALID;
+  BodyArg a{quad_order, b};
+  return host_rhs_roundtrip(ctx, mesh, mesh->dim, rhs,
+                            [](tfm_ctx *c, tfm_mesh *m, double *d, const void *p) {
+                              const BodyArg *s = (const BodyArg *)p;
+                              return load_body_dev(c, m, s->order, s->b, d);
+                            },
+                            &a);
+}
+
+extern "C" int tfm_load_neumann(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, double g, double *rhs) {
+  if (!ctx || !mesh || !rhs || dpn < 1 || dpn > 3) return TFM_E_INVALID;
+  NeuArg a{dpn, -1, g, -1, 0.0, 0.0};
+  return host_rhs_roundtrip(ctx, mesh, dpn, rhs,
+                            [](tfm_ctx *c, tfm_mesh *m, double *d, const void *p) {
+                              const NeuArg *s = (const NeuArg *)p;
+                              return load_neumann_dev(c, m, s->dpn, s->comp, s->g, s->axis, s->coord, s->tol, d);
+                            },
+                            &a);
+}
+
+// The reference's composition for a vector traction: one scalar pass per
+// non-zero component into a zeroed buffer, then rhs[n*dpn + c] += buf[n].
+__global__ void k_interleave_add(const double *buf, int64_t n, int dpn, int c, double *rhs) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rhs[i * dpn + c] += buf[i];
+}
+
+extern "C" int tfm_load_plane_traction(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, int axis, double coord,
+                                       double tol, const double *t, double *rhs) {
+  if (!ctx || !mesh || !rhs || !t || dpn < 1 || dpn > 3 || axis < 0) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  int64_t nn = mesh->n_nodes;
+  double *d_rhs = nullptr, *d_buf = nullptr;
+  if ((rc = dalloc(ctx, &d_rhs, nn * dpn)) || (rc = dalloc(ctx, &d_buf, nn))) return rc;
+  TFM_CHECK_HIP(ctx, hipMemcpy(d_rhs, rhs, sizeof(double) * nn * dpn, hipMemcpyHostToDevice));
+  for (int c = 0; c < dpn && !rc; c++) {
+    if (t[c] == 0.0) continue;
+    if (hipMemsetAsync(d_buf, 0, sizeof(double) * nn, ctx->stream) != hipSuccess) {
+      rc = fail(ctx, TFM_E_HIP, "memset failed");
+      break;
+    }
+    rc = load_neumann_dev(ctx, const_cast<tfm_mesh *>(mesh), 1, 0, t[c], axis, coord, tol, d_buf);
+    if (!rc && nn > 0)
+      hipLaunchKernelGGL(k_interleave_add, dim3(grid1(nn, 256)), dim3(256), 0, ctx->stream, d_buf, nn, dpn, c, d_rhs);
+  }
+  if (!rc && hipMemcpy(rhs, d_rhs, sizeof(double) * nn * dpn, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(ctx, TFM_E_HIP, "rhs download failed");
+  dfree(d_rhs);
+  dfree(d_buf);
+  return rc;
+}

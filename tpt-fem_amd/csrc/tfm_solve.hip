@@ -1,0 +1,921 @@
+// tfm_solve.hip -- CSR SpMV and Jacobi-preconditioned CG on the GPU.
+//
+// The reference solves the reduced system by densifying it and calling a
+// partial-pivot LU (crates/tpt-fem-sparse/src/lib.rs:244-278); that cannot
+// scale past a few thousand DOFs, so this library solves the same reduced
+// system with Jacobi PCG:
+//   * plain form: FP64 matrix, vectors and reductions;
+//   * precision form: FP32 matrix, vectors and SpMV accumulation, FP64 dot
+//     products, wrapped in FP64 iterative refinement (r = f - K u with the FP64
+//     matrix, u += d) until the FP64 residual meets the tolerance.
+// Iterations run in batches captured in a hipGraph; every reduction is a
+// fixed-order two-level sum (per-block partials, then the last block to
+// finish sums the partials in index order), so results are deterministic.
+#include <cmath>
+#include <cstring>
+
+#include "tfm_elem.h"
+#include "tfm_internal.h"
+
+namespace tfm {
+
+constexpr int kBlock = 256;
+constexpr int kMaxBlocks = 2048;
+
+struct Scal {
+  double rz, pq, rr, alpha, beta, tol2, rr0;
+  int done, breakdown, iters, pad;
+  unsigned int ticket[4];
+};
+
+__device__ __forceinline__ double warp_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block sum of v, valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  v = warp_sum(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += sh[i];
+  __syncthreads();
+  return t;
+}
+
+// Deterministic grid reduction: block partials to part[], last block sums.
+// Returns true in thread 0 of the last block with the totals in out[].
+template <int K>
+__device__ __forceinline__ bool grid_reduce(const double (&v)[K], double *part, unsigned int *ticket,
+                                            double (&out)[K]) {
+  __shared__ double sh[16];
+  __shared__ bool last;
+  for (int k = 0; k < K; k++) {
+    double s = block_sum(v[k], sh);
+    if (threadIdx.x == 0) part[k * kMaxBlocks + blockIdx.x] = s;
+  }
+  __threadfence();
+  if (threadIdx.x == 0) last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  for (int k = 0; k < K; k++) {
+    double t = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) t += part[k * kMaxBlocks + i];
+    // fixed-order combine of the strided partial sums
+    t = block_sum(t, sh);
+    out[k] = t;
+  }
+  if (threadIdx.x == 0) *ticket = 0u;
+  return threadIdx.x == 0;
+}
+
+// ---- SpMV + p.q : q = K p, alpha = rz / (p.q) -----------------------------
+template <typename VT, int S>
+__global__ __launch_bounds__(kBlock) void k_spmv_pq(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                     const VT *__restrict__ v, const VT *__restrict__ p,
+                                                     VT *__restrict__ q, Scal *sc, double *part) {
+  if (sc->done) return;
+  const int lane = threadIdx.x % S;
+  const int group = (blockIdx.x * blockDim.x + threadIdx.x) / S;
+  const int ngroups = gridDim.x * blockDim.x / S;
+  double dot = 0.0;
+  for (int row = group; row < n; row += ngroups) {
+    int rb = rp[row], re = rp[row + 1];
+    VT acc = 0;
+    for (int j = rb + lane; j < re; j += S) acc += v[j] * p[ci[j]];
+#pragma unroll
+    for (int o = S / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, S);
+    if (lane == 0) {
+      q[row] = acc;
+      dot += (double)p[row] * (double)acc;
+    }
+  }
+  double tot[1];
+  double vv[1] = {dot};
+  if (grid_reduce<1>(vv, part, &sc->ticket[0], tot)) {
+    sc->pq = tot[0];
+    if (!(tot[0] > 0.0)) {
+      sc->breakdown = 1;
+      sc->done = 1;
+      sc->alpha = 0.0;
+    } else {
+      sc->alpha = sc->rz / tot[0];
+    }
+  }
+}
+
+// ---- x += a p, r -= a q, z = Dinv r, partial r.z and r.r ------------------
+template <typename VT, typename XT>
+__global__ __launch_bounds__(kBlock) void k_update(int n, XT *__restrict__ x, VT *__restrict__ r,
+                                                    VT *__restrict__ z, const VT *__restrict__ p,
+                                                    const VT *__restrict__ q, const VT *__restrict__ dinv,
+                                                    Scal *sc, double *part) {
+  if (sc->done) return;
+  const double a = sc->alpha;
+  const VT af = (VT)a;
+  double rz = 0.0, rr = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] += (XT)(a * (double)p[i]);
+    VT ri = r[i] - af * q[i];
+    r[i] = ri;
+    VT zi = dinv[i] * ri;
+    z[i] = zi;
+    rz += (double)ri * (double)zi;
+    rr += (double)ri * (double)ri;
+  }
+  double tot[2];
+  double vv[2] = {rz, rr};
+  if (grid_reduce<2>(vv, part, &sc->ticket[1], tot)) {
+    sc->beta = tot[0] / sc->rz;
+    sc->rz = tot[0];
+    sc->rr = tot[1];
+    sc->iters += 1;
+    if (tot[1] <= sc->tol2) sc->done = 1;
+  }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_pdir(int n, VT *__restrict__ p, const VT *__restrict__ z, Scal *sc) {
+  if (sc->done) return;
+  const VT b = (VT)sc->beta;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + b * p[i];
+}
+
+// ---- setup: z = Dinv r, p = z, rz = r.z, rr = r.r --------------------------
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_init(int n, const VT *__restrict__ r, VT *__restrict__ z,
+                                                  VT *__restrict__ p, const VT *__restrict__ dinv, Scal *sc,
+                                                  double *part, double rtol2_rel) {
+  double rz = 0.0, rr = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    VT zi = dinv[i] * r[i];
+    z[i] = zi;
+    p[i] = zi;
+    rz += (double)r[i] * (double)zi;
+    rr += (double)r[i] * (double)r[i];
+  }
+  double tot[2];
+  double vv[2] = {rz, rr};
+  if (grid_reduce<2>(vv, part, &sc->ticket[2], tot)) {
+    sc->rz = tot[0];
+    sc->rr = tot[1];
+    sc->rr0 = tot[1];
+    sc->tol2 = rtol2_rel * tot[1];
+    sc->iters = 0;
+    sc->breakdown = 0;
+    sc->done = (tot[1] <= sc->tol2 || tot[1] == 0.0) ? 1 : 0;
+  }
+}
+
+// Jacobi: dinv[i] = 1 / K[i][i] on non-empty rows; 0 on empty (fixed) rows.
+// A free row without a positive diagonal is flagged (singular / not SPD).
+template <typename VT>
+__global__ void k_diag_inv(int n, const int *__restrict__ rp, const int *__restrict__ ci, const VT *__restrict__ v,
+                           const unsigned char *__restrict__ fixed, VT *__restrict__ dinv, int *bad) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int rb = rp[i], re = rp[i + 1];
+  VT d = 0;
+  int lo = rb, hi = re;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (ci[mid] < i) lo = mid + 1; else hi = mid;
+  }
+  if (lo < re && ci[lo] == i) d = v[lo];
+  bool isfixed = fixed && fixed[i];
+  if (isfixed || rb == re) {
+    dinv[i] = 0;
+    if (!isfixed) atomicExch(bad, 1);
+  } else if (!(d > (VT)0)) {
+    dinv[i] = 0;
+    atomicExch(bad, 1);
+  } else {
+    dinv[i] = (VT)1 / d;
+  }
+}
+
+// r = f - K u (FP64), and partial ||r||^2, ||f||^2 -- the refinement residual.
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_residual64(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                        const double *__restrict__ v, const double *__restrict__ u,
+                                                        const double *__restrict__ f, double *__restrict__ r,
+                                                        float *__restrict__ r32, Scal *sc, double *part) {
+  const int lane = threadIdx.x % S;
+  const int group = (blockIdx.x * blockDim.x + threadIdx.x) / S;
+  const int ngroups = gridDim.x * blockDim.x / S;
+  double rr = 0.0, ff = 0.0;
+  for (int row = group; row < n; row += ngroups) {
+    int rb = rp[row], re = rp[row + 1];
+    double acc = 0.0;
+    for (int j = rb + lane; j < re; j += S) acc += v[j] * u[ci[j]];
+#pragma unroll
+    for (int o = S / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, S);
+    if (lane == 0) {
+      double ri = (rb == re) ? 0.0 : f[row] - acc;
+      r[row] = ri;
+      if (r32) r32[row] = (float)ri;
+      rr += ri * ri;
+      ff += f[row] * f[row];
+    }
+  }
+  double tot[2];
+  double vv[2] = {rr, ff};
+  if (grid_reduce<2>(vv, part, &sc->ticket[3], tot)) {
+    sc->rr = tot[0];
+    sc->rr0 = tot[1];
+  }
+}
+
+__global__ void k_add_f32_to_f64(int n, double *u, const float *d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] += (double)d[i];
+}
+
+// Csr::matvec in the reference's order (sparse/src/lib.rs:195-205): one lane
+// per row, left-to-right sum from 0.0, no fused multiply-add.
+__global__ void k_matvec_exact(int64_t nrows, const int *__restrict__ rp, const int *__restrict__ ci,
+                               const double *__restrict__ v, const double *__restrict__ x, double *__restrict__ y) {
+#pragma clang fp contract(off)
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  double s = 0.0;
+  for (int j = rp[r]; j < rp[r + 1]; j++) s += v[j] * x[ci[j]];
+  y[r] = s;
+}
+
+static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+static int nblocks_for(int64_t work, int per_block) {
+  int64_t b = (work + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > kMaxBlocks) b = kMaxBlocks;
+  return (int)b;
+}
+
+// Lanes per row for the CSR SpMV from the average row length.
+static int lanes_for(double avg) {
+  if (avg <= 6) return 2;
+  if (avg <= 12) return 4;
+  if (avg <= 24) return 8;
+  if (avg <= 48) return 16;
+  return 32;
+}
+
+template <typename VT>
+struct PcgBufs {
+  VT *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr;
+};
+
+template <typename VT, int S>
+static void launch_spmv(hipStream_t st, int nb, int n, const int *rp, const int *ci, const VT *v, const VT *p, VT *q,
+                        Scal *sc, double *part) {
+  hipLaunchKernelGGL((k_spmv_pq<VT, S>), dim3(nb), dim3(kBlock), 0, st, n, rp, ci, v, p, q, sc, part);
+}
+
+template <typename VT>
+static void spmv_dispatch(int S, hipStream_t st, int nb, int n, const int *rp, const int *ci, const VT *v, const VT *p,
+                          VT *q, Scal *sc, double *part) {
+  switch (S) {
+  case 2: launch_spmv<VT, 2>(st, nb, n, rp, ci, v, p, q, sc, part); break;
+  case 4: launch_spmv<VT, 4>(st, nb, n, rp, ci, v, p, q, sc, part); break;
+  case 8: launch_spmv<VT, 8>(st, nb, n, rp, ci, v, p, q, sc, part); break;
+  case 16: launch_spmv<VT, 16>(st, nb, n, rp, ci, v, p, q, sc, part); break;
+  default: launch_spmv<VT, 32>(st, nb, n, rp, ci, v, p, q, sc, part); break;
+  }
+}
+
+static void residual_dispatch(int S, hipStream_t st, int nb, int n, const int *rp, const int *ci, const double *v,
+                              const double *u, const double *f, double *r, float *r32, Scal *sc, double *part) {
+  switch (S) {
+  case 2: hipLaunchKernelGGL((k_residual64<2>), dim3(nb), dim3(kBlock), 0, st, n, rp, ci, v, u, f, r, r32, sc, part); break;
+  case 4: hipLaunchKernelGGL((k_residual64<4>), dim3(nb), dim3(kBlock), 0, st, n, rp, ci, v, u, f, r, r32, sc, part); break;
+  case 8: hipLaunchKernelGGL((k_residual64<8>), dim3(nb), dim3(kBlock), 0, st, n, rp, ci, v, u, f, r, r32, sc, part); break;
+  case 16: hipLaunchKernelGGL((k_residual64<16>), dim3(nb), dim3(kBlock), 0, st, n, rp, ci, v, u, f, r, r32, sc, part); break;
+  default: hipLaunchKernelGGL((k_residual64<32>), dim3(nb), dim3(kBlock), 0, st, n, rp, ci, v, u, f, r, r32, sc, part); break;
+  }
+}
+
+// Runs PCG on K x = r0 (x starts at 0, r holds r0 on entry) until ||r|| <=
+// rtol * ||r0|| or max_iter.  x may be FP64 (plain form) or the FP32 inner
+// correction.  Returns iterations through *iters.
+template <typename VT, typename XT>
+static int pcg_run(tfm_ctx *ctx, int n, const int *rp, const int *ci, const VT *v, PcgBufs<VT> &B, XT *x,
+                   double rtol, int max_iter, int S, Scal *sc, double *part, int *iters, int *breakdown,
+                   double *rr_out) {
+  hipStream_t st = ctx->stream;
+  int nbv = nblocks_for(n, kBlock * 4);
+  int nbs = nblocks_for((int64_t)n * S, kBlock * 4);
+  hipLaunchKernelGGL((k_init<VT>), dim3(nbv), dim3(kBlock), 0, st, n, B.r, B.z, B.p, B.dinv, sc, part, rtol * rtol);
+  TFM_CHECK_HIP(ctx, hipGetLastError());
+  const int CH = 16;  // iterations per graph replay
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  TFM_CHECK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < CH; k++) {
+    spmv_dispatch<VT>(S, st, nbs, n, rp, ci, v, B.p, B.q, sc, part);
+    hipLaunchKernelGGL((k_update<VT, XT>), dim3(nbv), dim3(kBlock), 0, st, n, x, B.r, B.z, B.p, B.q, B.dinv, sc,
+                       part);
+    hipLaunchKernelGGL((k_pdir<VT>), dim3(nbv), dim3(kBlock), 0, st, n, B.p, B.z, sc);
+  }
+  TFM_CHECK_HIP(ctx, hipStreamEndCapture(st, &graph));
+  TFM_CHECK_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  Scal h{};
+  int done_it = 0;
+  int rc = TFM_OK;
+  for (int it = 0; it < max_iter + CH; it += CH) {
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
+    TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+    if (h.done || h.iters >= max_iter) break;
+    if (hipGraphLaunch(exec, st) != hipSuccess) {
+      rc = fail(ctx, TFM_E_HIP, "hipGraphLaunch failed");
+      break;
+    }
+  }
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  done_it = h.iters;
+  *iters = done_it;
+  *breakdown = h.breakdown;
+  *rr_out = h.rr0 > 0 ? std::sqrt(h.rr / h.rr0) : 0.0;
+  return rc;
+}
+
+}  // namespace tfm
+
+using namespace tfm;
+
+// ---------------------------------------------------------------------------
+// Matrices
+// ---------------------------------------------------------------------------
+extern "C" int tfm_matrix_create(tfm_ctx *ctx, int64_t nrows, int64_t ncols, const int64_t *row_ptrs,
+                                 const int64_t *col_ind, const double *values, tfm_matrix **out) {
+  if (!ctx || !out || nrows < 0 || ncols < 0 || !row_ptrs) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  int64_t nnz = row_ptrs[nrows];
+  if (nnz >= INT32_MAX || nrows >= INT32_MAX) return fail(ctx, TFM_E_UNSUPPORTED, "matrix too large for int32 indexing");
+  std::vector<int> rp(nrows + 1), ci(nnz);
+  for (int64_t i = 0; i <= nrows; i++) rp[i] = (int)row_ptrs[i];
+  for (int64_t i = 0; i < nnz; i++) {
+    if (col_ind[i] < 0 || col_ind[i] >= ncols) return fail(ctx, TFM_E_INVALID, "column index out of range");
+    ci[i] = (int)col_ind[i];
+  }
+  tfm_matrix *m = new tfm_matrix();
+  m->ctx = ctx;
+  m->nrows = nrows;
+  m->ncols = ncols;
+  m->nnz = nnz;
+  if ((rc = dalloc(ctx, &m->d_rp, nrows + 1)) || (rc = dalloc(ctx, &m->d_ci, nnz)) || (rc = dalloc(ctx, &m->d_v, nnz))) {
+    tfm_matrix_destroy(m);
+    return rc;
+  }
+  if (hipMemcpy(m->d_rp, rp.data(), sizeof(int) * (nrows + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      (nnz && hipMemcpy(m->d_ci, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice) != hipSuccess) ||
+      (nnz && hipMemcpy(m->d_v, values, sizeof(double) * nnz, hipMemcpyHostToDevice) != hipSuccess)) {
+    tfm_matrix_destroy(m);
+    return fail(ctx, TFM_E_HIP, "matrix upload failed");
+  }
+  *out = m;
+  return TFM_OK;
+}
+
+extern "C" int tfm_assemble(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn, tfm_matrix **out) {
+  if (!ctx || !mesh || !op || !out) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  AsmPlan pl;
+  if ((rc = plan_create(ctx, mesh, op, dpn, nullptr, &pl))) {
+    plan_destroy(&pl);
+    return rc;
+  }
+  tfm_matrix *m = new tfm_matrix();
+  m->ctx = ctx;
+  m->nrows = pl.nrows;
+  m->ncols = pl.ncols;
+  m->nnz = pl.nnz;
+  if ((rc = dalloc(ctx, &m->d_v, pl.nnz)) || (rc = plan_numeric(ctx, &pl, m->d_v, nullptr))) {
+    plan_destroy(&pl);
+    tfm_matrix_destroy(m);
+    return rc;
+  }
+  // keep the pattern; the plan's row pointers cover all ndof rows, and rows
+  // past nrows are empty, so the first nrows+1 entries are the CSR's.
+  m->d_rp = pl.d_rp;
+  m->d_ci = pl.d_ci;
+  pl.d_rp = nullptr;
+  pl.d_ci = nullptr;
+  if (pl.nnz == 0) {
+    // an empty assembly is the 1 x 1 empty CSR of Coo::to_csr
+    TFM_CHECK_HIP(ctx, hipMemsetAsync(m->d_rp, 0, sizeof(int) * 2, ctx->stream));
+  }
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  plan_destroy(&pl);
+  if (e != hipSuccess) {
+    tfm_matrix_destroy(m);
+    return fail(ctx, TFM_E_HIP, "assembly failed");
+  }
+  *out = m;
+  return TFM_OK;
+}
+
+extern "C" int tfm_matrix_info(const tfm_matrix *m, int64_t *nrows, int64_t *ncols, int64_t *nnz) {
+  if (!m) return TFM_E_INVALID;
+  if (nrows) *nrows = m->nrows;
+  if (ncols) *ncols = m->ncols;
+  if (nnz) *nnz = m->nnz;
+  return TFM_OK;
+}
+
+extern "C" int tfm_matrix_download(const tfm_matrix *m, int64_t *row_ptrs, int64_t *col_ind, double *values) {
+  if (!m) return TFM_E_INVALID;
+  tfm_ctx *ctx = m->ctx;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  std::vector<int> rp(m->nrows + 1), ci(m->nnz);
+  TFM_CHECK_HIP(ctx, hipMemcpy(rp.data(), m->d_rp, sizeof(int) * (m->nrows + 1), hipMemcpyDeviceToHost));
+  if (m->nnz) TFM_CHECK_HIP(ctx, hipMemcpy(ci.data(), m->d_ci, sizeof(int) * m->nnz, hipMemcpyDeviceToHost));
+  if (row_ptrs)
+    for (int64_t i = 0; i <= m->nrows; i++) row_ptrs[i] = rp[i];
+  if (col_ind)
+    for (int64_t i = 0; i < m->nnz; i++) col_ind[i] = ci[i];
+  if (values && m->nnz) TFM_CHECK_HIP(ctx, hipMemcpy(values, m->d_v, sizeof(double) * m->nnz, hipMemcpyDeviceToHost));
+  return TFM_OK;
+}
+
+extern "C" int tfm_matrix_matvec(const tfm_matrix *m, const double *x, int64_t nx, double *y) {
+  if (!m || (!x && nx) || !y) return TFM_E_INVALID;
+  tfm_ctx *ctx = m->ctx;
+  if (nx < m->ncols) return fail(ctx, TFM_E_INVALID, "x shorter than ncols");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  int64_t ny = std::max(m->nrows, nx);
+  double *dx = nullptr, *dy = nullptr;
+  if ((rc = dalloc(ctx, &dx, nx)) || (rc = dalloc(ctx, &dy, ny))) return rc;
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(dx, x, sizeof(double) * nx, hipMemcpyHostToDevice, ctx->stream));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(dy, 0, sizeof(double) * ny, ctx->stream));
+  if (m->nrows > 0)
+    hipLaunchKernelGGL(k_matvec_exact, dim3(grid1(m->nrows, 256)), dim3(256), 0, ctx->stream, m->nrows, m->d_rp,
+                       m->d_ci, m->d_v, dx, dy);
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(y, dy, sizeof(double) * ny, hipMemcpyDeviceToHost, ctx->stream));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dfree(dx);
+  dfree(dy);
+  return TFM_OK;
+}
+
+extern "C" void tfm_matrix_destroy(tfm_matrix *m) {
+  if (!m) return;
+  dfree(m->d_rp);
+  dfree(m->d_ci);
+  dfree(m->d_v);
+  delete m;
+}
+
+// ---------------------------------------------------------------------------
+// Reduced systems
+// ---------------------------------------------------------------------------
+static int collect_bcs(tfm_ctx *ctx, int64_t ndof, int64_t n_bc, const int64_t *bc_dof, const double *bc_val,
+                       tfm_system *s, std::vector<unsigned char> &fixed, std::vector<double> &first) {
+  fixed.assign((size_t)ndof, 0);
+  first.assign((size_t)ndof, 0.0);
+  s->bc_dof.assign(bc_dof, bc_dof + n_bc);
+  s->bc_val.assign(bc_val, bc_val + n_bc);
+  for (int64_t b = 0; b < n_bc; b++) {
+    int64_t d = bc_dof[b];
+    if (d < 0 || d >= ndof) return fail(ctx, TFM_E_INVALID, "Dirichlet DOF out of range");
+    if (!fixed[d]) first[d] = bc_val[b];  // lifting uses the first match (assembly/src/lib.rs:525)
+    fixed[d] = 1;
+  }
+  s->free_dofs.clear();
+  for (int64_t i = 0; i < ndof; i++)
+    if (!fixed[i]) s->free_dofs.push_back(i);
+  s->n_free = (int64_t)s->free_dofs.size();
+  return TFM_OK;
+}
+
+// fred = rhs - K_fc g over each free row, in ascending column order, exact.
+__global__ void k_lift(int64_t ndof, const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
+                       const unsigned char *__restrict__ fixed, const double *__restrict__ gval,
+                       const double *__restrict__ rhs, double *__restrict__ f, int nrows) {
+#pragma clang fp contract(off)
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ndof) return;
+  if (fixed[i]) { f[i] = 0.0; return; }
+  double r = rhs[i];
+  if (i < nrows)
+    for (int j = rp[i]; j < rp[i + 1]; j++) {
+      int c = ci[j];
+      if (c < ndof && fixed[c]) r -= v[j] * gval[c];
+    }
+  f[i] = r;
+}
+
+// Row counts / fill of the reduced pattern of a full matrix.
+__global__ void k_reduce_count(int64_t ndof, int nrows, const int *rp, const int *ci, const unsigned char *fixed,
+                               int64_t ncol_lim, int *cnt, int *bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ndof) return;
+  int c = 0;
+  if (!fixed[i] && i < nrows)
+    for (int j = rp[i]; j < rp[i + 1]; j++) {
+      int col = ci[j];
+      if (col >= ncol_lim) { atomicExch(bad, 1); continue; }
+      if (!fixed[col]) c++;
+    }
+  cnt[i] = c;
+}
+
+__global__ void k_reduce_fill(int64_t ndof, int nrows, const int *rp, const int *ci, const double *v,
+                              const unsigned char *fixed, const int *rp2, int *ci2, double *v2) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ndof || fixed[i] || i >= nrows) return;
+  int o = rp2[i];
+  for (int j = rp[i]; j < rp[i + 1]; j++) {
+    int col = ci[j];
+    if (col < ndof && !fixed[col]) {
+      ci2[o] = col;
+      v2[o] = v[j];
+      o++;
+    }
+  }
+}
+
+extern "C" int tfm_system_from_matrix(tfm_ctx *ctx, const tfm_matrix *k, int64_t n, const double *rhs, int64_t n_bc,
+                                      const int64_t *bc_dof, const double *bc_val, tfm_system **out) {
+  if (!ctx || !k || !out || (n && !rhs) || (n_bc && (!bc_dof || !bc_val))) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  tfm_system *s = new tfm_system();
+  s->ctx = ctx;
+  s->dpn = 1;
+  s->ndof = n;
+  std::vector<unsigned char> fixed;
+  std::vector<double> first;
+  if ((rc = collect_bcs(ctx, n, n_bc, bc_dof, bc_val, s, fixed, first))) {
+    delete s;
+    return rc;
+  }
+  // reduce_system indexes row_ptrs[fdof + 1] for every free DOF: the matrix
+  // must have a row for each (the reference panics otherwise).
+  for (int64_t f : s->free_dofs)
+    if (f >= k->nrows) {
+      delete s;
+      return fail(ctx, TFM_E_INVALID, "reduce_system: free DOF has no row in the assembled matrix");
+    }
+  double *d_g = nullptr, *d_rhs = nullptr;
+  int *cnt = nullptr, *bad = nullptr;
+  s->owns_pattern = true;
+  if ((rc = dalloc(ctx, &s->d_fixed, n)) || (rc = dalloc(ctx, &d_g, n)) || (rc = dalloc(ctx, &d_rhs, n)) ||
+      (rc = dalloc(ctx, &s->d_f, n)) || (rc = dalloc(ctx, &cnt, n + 1)) || (rc = dalloc(ctx, &bad, 1)) ||
+      (rc = dalloc(ctx, &s->d_rp, n + 1))) {
+    tfm_system_destroy(s);
+    return rc;
+  }
+  hipStream_t st = ctx->stream;
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(s->d_fixed, fixed.data(), n, hipMemcpyHostToDevice, st));
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(d_g, first.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(d_rhs, rhs, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(bad, 0, sizeof(int), st));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_reduce_count, dim3(grid1(n, 256)), dim3(256), 0, st, n, (int)k->nrows, k->d_rp, k->d_ci,
+                       s->d_fixed, n, cnt, bad);
+    hipLaunchKernelGGL(k_lift, dim3(grid1(n, 256)), dim3(256), 0, st, n, k->d_rp, k->d_ci, k->d_v, s->d_fixed, d_g,
+                       d_rhs, s->d_f, (int)k->nrows);
+  }
+  int64_t nnz = 0;
+  if ((rc = exclusive_scan_i32(ctx, cnt, s->d_rp, n, &nnz))) {
+    tfm_system_destroy(s);
+    return rc;
+  }
+  int hbad = 0;
+  TFM_CHECK_HIP(ctx, hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost));
+  if (hbad) {
+    tfm_system_destroy(s);
+    return fail(ctx, TFM_E_INVALID, "reduce_system: column beyond the rhs length");
+  }
+  s->nnz = nnz;
+  s->nrows = n;
+  if ((rc = dalloc(ctx, &s->d_ci, nnz)) || (rc = dalloc(ctx, &s->d_v64, nnz))) {
+    tfm_system_destroy(s);
+    return rc;
+  }
+  if (n > 0)
+    hipLaunchKernelGGL(k_reduce_fill, dim3(grid1(n, 256)), dim3(256), 0, st, n, (int)k->nrows, k->d_rp, k->d_ci,
+                       k->d_v, s->d_fixed, s->d_rp, s->d_ci, s->d_v64);
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  dfree(d_g);
+  dfree(d_rhs);
+  dfree(cnt);
+  dfree(bad);
+  *out = s;
+  return TFM_OK;
+}
+
+extern "C" int tfm_system_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn, int64_t n_bc,
+                                 const int64_t *bc_dof, const double *bc_val, const double *rhs, int flags,
+                                 tfm_system **out) {
+  if (!ctx || !mesh || !op || !out || (n_bc && (!bc_dof || !bc_val))) return TFM_E_INVALID;
+  if (!(flags & (TFM_SYS_FP64 | TFM_SYS_FP32))) flags = TFM_SYS_FP64;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  tfm_system *s = new tfm_system();
+  s->ctx = ctx;
+  s->dpn = dpn;
+  s->ndof = mesh->n_nodes * dpn;
+  std::vector<unsigned char> fixed;
+  std::vector<double> first;
+  if ((rc = collect_bcs(ctx, s->ndof, n_bc, bc_dof, bc_val, s, fixed, first))) {
+    delete s;
+    return rc;
+  }
+  bool homogeneous = true;
+  for (int64_t b = 0; b < n_bc; b++) homogeneous &= (bc_val[b] == 0.0);
+  if (!homogeneous) {
+    // General lifting needs K[f, c] of fixed columns: assemble the full
+    // matrix and reduce it (reduce_system's own order of operations).
+    tfm_matrix *K = nullptr;
+    delete s;
+    if ((rc = tfm_assemble(ctx, mesh, op, dpn, &K))) return rc;
+    std::vector<double> zero;
+    if (!rhs) zero.assign((size_t)(mesh->n_nodes * dpn), 0.0);
+    rc = tfm_system_from_matrix(ctx, K, mesh->n_nodes * dpn, rhs ? rhs : zero.data(), n_bc, bc_dof, bc_val, out);
+    tfm_matrix_destroy(K);
+    return rc;
+  }
+  s->has_plan = true;
+  if ((rc = plan_create(ctx, mesh, op, dpn, n_bc ? fixed.data() : nullptr, &s->plan))) {
+    tfm_system_destroy(s);
+    return rc;
+  }
+  if (!s->plan.d_fixed) {
+    if ((rc = dalloc(ctx, &s->plan.d_fixed, s->ndof))) { tfm_system_destroy(s); return rc; }
+    TFM_CHECK_HIP(ctx, hipMemsetAsync(s->plan.d_fixed, 0, s->ndof, ctx->stream));
+  }
+  s->d_rp = s->plan.d_rp;
+  s->d_ci = s->plan.d_ci;
+  s->d_fixed = s->plan.d_fixed;
+  s->nnz = s->plan.nnz;
+  s->nrows = s->ndof;
+  if ((flags & TFM_SYS_FP64) && (rc = dalloc(ctx, &s->d_v64, s->nnz))) { tfm_system_destroy(s); return rc; }
+  if ((flags & TFM_SYS_FP32) && (rc = dalloc(ctx, &s->d_v32, s->nnz))) { tfm_system_destroy(s); return rc; }
+  if ((rc = plan_numeric(ctx, &s->plan, s->d_v64, s->d_v32))) { tfm_system_destroy(s); return rc; }
+  if ((rc = dalloc(ctx, &s->d_f, s->ndof))) { tfm_system_destroy(s); return rc; }
+  // Homogeneous Dirichlet values: rhs[f] - K[f,c] * 0 leaves every free entry
+  // unchanged, so fred is rhs restricted to the free DOFs.
+  std::vector<double> f((size_t)s->ndof, 0.0);
+  if (rhs)
+    for (int64_t i = 0; i < s->ndof; i++) f[i] = fixed[i] ? 0.0 : rhs[i];
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(s->d_f, f.data(), sizeof(double) * s->ndof, hipMemcpyHostToDevice, ctx->stream));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *out = s;
+  return TFM_OK;
+}
+
+extern "C" int tfm_system_assemble(tfm_system *s, const tfm_operator *op, int which) {
+  if (!s) return TFM_E_INVALID;
+  tfm_ctx *ctx = s->ctx;
+  if (!s->has_plan) return fail(ctx, TFM_E_INVALID, "system was not built from a mesh");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (op) {
+    if (op->kind != s->plan.op.kind || op->model != s->plan.op.model || op->quad_order != s->plan.op.quad_order)
+      return fail(ctx, TFM_E_INVALID, "re-assembly must keep the operator kind, model and rule");
+    if ((rc = plan_set_coefficients(ctx, &s->plan, op))) return rc;
+    if (op->kind == TFM_OP_SIMP)
+      TFM_CHECK_HIP(ctx, hipMemcpyAsync(s->plan.d_tab + (size_t)s->plan.nq * kRuleStride, s->plan.ke0,
+                                        sizeof(double) * 64, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (which == 0) which = TFM_SYS_FP64 | TFM_SYS_FP32;
+  return plan_numeric(ctx, &s->plan, (which & TFM_SYS_FP64) ? s->d_v64 : nullptr,
+                      (which & TFM_SYS_FP32) ? s->d_v32 : nullptr);
+}
+
+extern "C" int tfm_system_info(const tfm_system *s, int64_t *n_free, int64_t *nnz, int64_t *n_dof) {
+  if (!s) return TFM_E_INVALID;
+  if (n_free) *n_free = s->n_free;
+  if (nnz) *nnz = s->nnz;
+  if (n_dof) *n_dof = s->ndof;
+  return TFM_OK;
+}
+
+extern "C" int tfm_system_download(const tfm_system *s, int64_t *row_ptrs, int64_t *col_ind, double *values,
+                                   double *fred, int64_t *free_dofs) {
+  if (!s) return TFM_E_INVALID;
+  tfm_ctx *ctx = s->ctx;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  int64_t n = s->ndof;
+  std::vector<int> rp(n + 1), ci(s->nnz);
+  std::vector<double> v(s->d_v64 ? s->nnz : 0), f(n);
+  std::vector<float> v32(!s->d_v64 && s->d_v32 ? s->nnz : 0);
+  TFM_CHECK_HIP(ctx, hipMemcpy(rp.data(), s->d_rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  if (s->nnz) TFM_CHECK_HIP(ctx, hipMemcpy(ci.data(), s->d_ci, sizeof(int) * s->nnz, hipMemcpyDeviceToHost));
+  if (s->nnz && s->d_v64) TFM_CHECK_HIP(ctx, hipMemcpy(v.data(), s->d_v64, sizeof(double) * s->nnz, hipMemcpyDeviceToHost));
+  if (s->nnz && v32.size()) TFM_CHECK_HIP(ctx, hipMemcpy(v32.data(), s->d_v32, sizeof(float) * s->nnz, hipMemcpyDeviceToHost));
+  TFM_CHECK_HIP(ctx, hipMemcpy(f.data(), s->d_f, sizeof(double) * n, hipMemcpyDeviceToHost));
+  std::vector<int64_t> red((size_t)n, -1);
+  for (int64_t i = 0; i < s->n_free; i++) red[s->free_dofs[i]] = i;
+  int64_t o = 0;
+  if (row_ptrs) row_ptrs[0] = 0;
+  for (int64_t q = 0; q < s->n_free; q++) {
+    int64_t g = s->free_dofs[q];
+    for (int j = rp[g]; j < rp[g + 1]; j++) {
+      if (col_ind) col_ind[o] = red[ci[j]];
+      if (values) values[o] = s->d_v64 ? v[j] : (double)v32[j];
+      o++;
+    }
+    if (row_ptrs) row_ptrs[q + 1] = o;
+    if (fred) fred[q] = f[g];
+    if (free_dofs) free_dofs[q] = g;
+  }
+  return TFM_OK;
+}
+
+extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, double *u, tfm_solve_stats *stats) {
+  if (!s || !u) return TFM_E_INVALID;
+  tfm_ctx *ctx = s->ctx;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 1e-5, 40};
+  if (opts) o = *opts;
+  if (o.max_iter <= 0) o.max_iter = 20000;
+  const int n = (int)s->ndof;
+  const bool mixed = o.mode == TFM_SOLVE_MIXED;
+  if (mixed && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values (TFM_SYS_FP32)");
+  if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values (TFM_SYS_FP64)");
+  hipStream_t st = ctx->stream;
+  int S = lanes_for(n ? (double)s->nnz / std::max<int64_t>(1, s->n_free) : 1.0);
+  // device work space
+  Scal *sc = nullptr;
+  double *part = nullptr, *x = nullptr;
+  int *bad = nullptr;
+  PcgBufs<double> B64;
+  PcgBufs<float> B32;
+  float *x32 = nullptr;
+  auto cleanup = [&]() {
+    dfree(sc); dfree(part); dfree(x); dfree(bad); dfree(x32);
+    dfree(B64.r); dfree(B64.z); dfree(B64.p); dfree(B64.q); dfree(B64.dinv);
+    dfree(B32.r); dfree(B32.z); dfree(B32.p); dfree(B32.q); dfree(B32.dinv);
+  };
+  if ((rc = dalloc(ctx, &sc, 1)) || (rc = dalloc(ctx, &part, 4 * kMaxBlocks)) || (rc = dalloc(ctx, &x, n)) ||
+      (rc = dalloc(ctx, &bad, 1)) || (rc = dalloc(ctx, &B64.r, n)) || (rc = dalloc(ctx, &B64.dinv, n))) {
+    cleanup();
+    return rc;
+  }
+  if (!mixed && ((rc = dalloc(ctx, &B64.z, n)) || (rc = dalloc(ctx, &B64.p, n)) || (rc = dalloc(ctx, &B64.q, n)))) {
+    cleanup();
+    return rc;
+  }
+  if (mixed && ((rc = dalloc(ctx, &x32, n)) || (rc = dalloc(ctx, &B32.r, n)) || (rc = dalloc(ctx, &B32.z, n)) ||
+                (rc = dalloc(ctx, &B32.p, n)) || (rc = dalloc(ctx, &B32.q, n)) || (rc = dalloc(ctx, &B32.dinv, n)))) {
+    cleanup();
+    return rc;
+  }
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(sc, 0, sizeof(Scal), st));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(x, 0, sizeof(double) * n, st));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(bad, 0, sizeof(int), st));
+  int nb1 = (int)grid1(n, 256);
+  hipLaunchKernelGGL(k_diag_inv<double>, dim3(nb1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v64, s->d_fixed,
+                     B64.dinv, bad);
+  if (mixed)
+    hipLaunchKernelGGL(k_diag_inv<float>, dim3(nb1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v32, s->d_fixed,
+                       B32.dinv, bad);
+  int hbad = 0;
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  if (hbad) {
+    cleanup();
+    return fail(ctx, TFM_E_NUMERIC,
+                "sparse numeric factorization/solve failed: singular matrix (a free DOF has no positive diagonal)");
+  }
+  TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev0, st));
+  int total_it = 0, outer = 0, brk = 0;
+  double relres = 0.0;
+  int nbs = nblocks_for((int64_t)n * S, kBlock * 4);
+  if (!mixed) {
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(B64.r, s->d_f, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    double rr = 0;
+    rc = pcg_run<double, double>(ctx, n, s->d_rp, s->d_ci, s->d_v64, B64, x, o.rtol, o.max_iter, S, sc, part,
+                                 &total_it, &brk, &rr);
+    outer = 1;
+  } else {
+    // FP64 iterative refinement around FP32 PCG.
+    for (outer = 0; outer < o.max_outer && !rc; outer++) {
+      residual_dispatch(S, st, nbs, n, s->d_rp, s->d_ci, s->d_v64, x, s->d_f, B64.r, B32.r, sc, part);
+      Scal h{};
+      TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
+      TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+      relres = h.rr0 > 0 ? std::sqrt(h.rr / h.rr0) : 0.0;
+      if (relres <= o.rtol || total_it >= o.max_iter) break;
+      TFM_CHECK_HIP(ctx, hipMemsetAsync(x32, 0, sizeof(float) * n, st));
+      int it = 0;
+      double rr = 0;
+      rc = pcg_run<float, float>(ctx, n, s->d_rp, s->d_ci, s->d_v32, B32, x32, o.inner_rtol,
+                                 o.max_iter - total_it, S, sc, part, &it, &brk, &rr);
+      total_it += it;
+      hipLaunchKernelGGL(k_add_f32_to_f64, dim3(nb1), dim3(256), 0, st, n, x, x32);
+      if (brk) break;
+    }
+  }
+  TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, st));
+  // final FP64 residual certificate
+  residual_dispatch(S, st, nbs, n, s->d_rp, s->d_ci, s->d_v64, x, s->d_f, B64.r, nullptr, sc, part);
+  Scal h{};
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
+  std::vector<double> hx((size_t)n);
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(hx.data(), x, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  relres = h.rr0 > 0 ? std::sqrt(h.rr / h.rr0) : 0.0;
+  cleanup();
+  if (rc) return rc;
+  // solve_with_dirichlet scatter: u[free] = x, then u[bc] = value in order.
+  for (int64_t i = 0; i < n; i++) u[i] = 0.0;
+  for (int64_t f : s->free_dofs) u[f] = hx[f];
+  for (size_t b = 0; b < s->bc_dof.size(); b++) u[s->bc_dof[b]] = s->bc_val[b];
+  if (stats) {
+    stats->iterations = total_it;
+    stats->outer = outer;
+    stats->rel_residual = relres;
+    stats->seconds = ms * 1e-3;
+  }
+  if (brk) return fail(ctx, TFM_E_BREAKDOWN, "PCG breakdown: p'Kp <= 0 (matrix not SPD)");
+  if (!(relres <= o.rtol))
+    return fail(ctx, TFM_E_NOT_CONVERGED, "PCG did not reach the tolerance: rel. residual " + std::to_string(relres));
+  return TFM_OK;
+}
+
+extern "C" void tfm_system_destroy(tfm_system *s) {
+  if (!s) return;
+  if (s->has_plan) {
+    plan_destroy(&s->plan);
+    s->d_rp = nullptr;
+    s->d_ci = nullptr;
+    s->d_fixed = nullptr;
+  } else {
+    dfree(s->d_rp);
+    dfree(s->d_ci);
+    dfree(s->d_fixed);
+  }
+  dfree(s->d_v64);
+  dfree(s->d_v32);
+  dfree(s->d_f);
+  delete s;
+}
+
+// ---------------------------------------------------------------------------
+// Drivers
+// ---------------------------------------------------------------------------
+extern "C" int tfm_solve_poisson(tfm_ctx *ctx, const tfm_mesh *mesh, double kappa, int quad_order, double f,
+                                 int64_t n_bc, const int64_t *bc_dof, const double *bc_val, const tfm_solve_opts *opts,
+                                 double *u) {
+  if (!ctx || !mesh || !u) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  std::vector<double> rhs((size_t)mesh->n_nodes, 0.0);
+  if ((rc = tfm_load_source(ctx, mesh, quad_order, f, rhs.data()))) return rc;
+  tfm_operator op{};
+  op.kind = TFM_OP_POISSON;
+  op.quad_order = quad_order;
+  op.coef = kappa;
+  tfm_system *s = nullptr;
+  if ((rc = tfm_system_create(ctx, mesh, &op, 1, n_bc, bc_dof, bc_val, rhs.data(), TFM_SYS_FP64, &s))) return rc;
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 40};
+  if (opts) o = *opts;
+  rc = tfm_system_solve(s, &o, u, nullptr);
+  tfm_system_destroy(s);
+  return rc;
+}
+
+extern "C" int tfm_solve_elasticity(tfm_ctx *ctx, const tfm_mesh *mesh, int model, double young, double nu,
+                                    int quad_order, const double *body, int64_t n_bc, const int64_t *bc_dof,
+                                    const double *bc_val, const tfm_solve_opts *opts, double *u) {
+  if (!ctx || !mesh || !u) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  int dim = mesh->dim;
+  std::vector<double> rhs((size_t)(mesh->n_nodes * dim), 0.0);
+  double zero[3] = {0, 0, 0};
+  if ((rc = tfm_load_body(ctx, mesh, quad_order, body ? body : zero, rhs.data()))) return rc;
+  tfm_operator op{};
+  op.kind = TFM_OP_ELASTICITY;
+  op.model = model;
+  op.quad_order = quad_order;
+  op.coef = young;
+  op.nu = nu;
+  tfm_system *s = nullptr;
+  if ((rc = tfm_system_create(ctx, mesh, &op, dim, n_bc, bc_dof, bc_val, rhs.data(), TFM_SYS_FP64, &s))) return rc;
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 40};
+  if (opts) o = *opts;
+  rc = tfm_system_solve(s, &o, u, nullptr);
+  tfm_system_destroy(s);
+  return rc;
+}

@@ -1,0 +1,447 @@
+"""Python binding of libtptfem.so (the MI355X assembly/solve library).
+
+Mirrors the reference crates' API on this path -- tpt_fem_sparse (Coo, Csr,
+solve), tpt_fem_assembly (assemble, reduce_system, solve_with_dirichlet,
+apply_neumann), tpt_fem_thermal (solve_poisson), tpt_fem_elasticity
+(solve_elasticity), tpt_fem_mesh_gen (box_mesh) -- over the C ABI in
+include/tptfem.h.  There is no CPU fallback: if the HIP library is not built,
+importing this module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libtptfem.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C tpt-fem_amd)")
+
+_lib = C.CDLL(LIB_PATH)
+
+OK, E_INVALID, E_NUMERIC, E_NOT_CONVERGED, E_BREAKDOWN, E_HIP, E_UNSUPPORTED, E_MODEL = (
+    0, -1, -2, -3, -4, -5, -6, -7)
+LINE, TRI, QUAD, TET, HEX = 0, 1, 2, 3, 4
+BAR, PLANE_STRESS, PLANE_STRAIN, CONTINUUM3D = 0, 1, 2, 3
+OP_POISSON, OP_ELASTICITY, OP_SIMP = 0, 1, 2
+SYS_FP64, SYS_FP32 = 1, 2
+SOLVE_PCG64, SOLVE_MIXED = 0, 1
+NODES = {LINE: 2, TRI: 3, QUAD: 4, TET: 4, HEX: 8}
+DIMS = {LINE: 1, TRI: 2, QUAD: 2, TET: 3, HEX: 3}
+
+P, I64, D, INT = C.c_void_p, C.c_int64, C.c_double, C.c_int
+PP = C.POINTER(C.c_void_p)
+
+
+class Operator(C.Structure):
+    """tfm_operator: the element operator an assemble() closure would apply."""
+
+    _fields_ = [("kind", INT), ("model", INT), ("quad_order", INT), ("coef", D), ("coef_e", P),
+                ("nu", D), ("rho", P), ("penal", D), ("e0", D), ("emin", D), ("h", D)]
+
+    @staticmethod
+    def poisson(kappa=1.0, quad_order=2, kappa_e=None):
+        op = Operator(kind=OP_POISSON, quad_order=quad_order, coef=kappa)
+        op._attach("coef_e", kappa_e)
+        return op
+
+    @staticmethod
+    def elasticity(model, young, nu, quad_order=2, young_e=None):
+        op = Operator(kind=OP_ELASTICITY, model=model, quad_order=quad_order, coef=young, nu=nu)
+        op._attach("coef_e", young_e)
+        return op
+
+    @staticmethod
+    def simp(rho, nu=0.3, penal=3.0, e0=1.0, emin=1e-9, h=1.0):
+        op = Operator(kind=OP_SIMP, quad_order=2, nu=nu, penal=penal, e0=e0, emin=emin, h=h)
+        op._attach("rho", rho)
+        return op
+
+    def _attach(self, name, arr):
+        if not hasattr(self, "_keep"):
+            self._keep = {}
+        if arr is None:
+            setattr(self, name, None)
+            return
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._keep[name] = a
+        setattr(self, name, a.ctypes.data)
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("mode", INT), ("rtol", D), ("max_iter", INT), ("inner_rtol", D), ("max_outer", INT)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", INT), ("outer", INT), ("rel_residual", D), ("seconds", D)]
+
+
+_SIGS = {
+    "tfm_ctx_create": (INT, [INT, PP]),
+    "tfm_ctx_destroy": (None, [P]),
+    "tfm_last_error": (C.c_char_p, [P]),
+    "tfm_ctx_synchronize": (INT, [P]),
+    "tfm_ctx_timer_begin": (INT, [P]),
+    "tfm_ctx_timer_end": (INT, [P, C.POINTER(D)]),
+    "tfm_mesh_create": (INT, [P, INT, I64, INT, P, I64, P, PP]),
+    "tfm_mesh_box": (INT, [P, P, P, I64, I64, I64, PP]),
+    "tfm_mesh_grid": (INT, [P, I64, I64, D, PP]),
+    "tfm_mesh_jitter": (INT, [P, I64, I64, I64, D, C.c_uint64]),
+    "tfm_mesh_info": (INT, [P, P, P, P, P]),
+    "tfm_mesh_download": (INT, [P, P, P]),
+    "tfm_mesh_nodes_on_plane": (INT, [P, INT, D, D, P, P]),
+    "tfm_mesh_destroy": (None, [P]),
+    "tfm_assemble": (INT, [P, P, C.POINTER(Operator), INT, PP]),
+    "tfm_matrix_create": (INT, [P, I64, I64, P, P, P, PP]),
+    "tfm_matrix_info": (INT, [P, P, P, P]),
+    "tfm_matrix_download": (INT, [P, P, P, P]),
+    "tfm_matrix_matvec": (INT, [P, P, I64, P]),
+    "tfm_matrix_destroy": (None, [P]),
+    "tfm_load_source": (INT, [P, P, INT, D, P]),
+    "tfm_load_body": (INT, [P, P, INT, P, P]),
+    "tfm_load_neumann": (INT, [P, P, INT, D, P]),
+    "tfm_load_plane_traction": (INT, [P, P, INT, INT, D, D, P, P]),
+    "tfm_system_create": (INT, [P, P, C.POINTER(Operator), INT, I64, P, P, P, INT, PP]),
+    "tfm_system_from_matrix": (INT, [P, P, I64, P, I64, P, P, PP]),
+    "tfm_system_assemble": (INT, [P, C.POINTER(Operator), INT]),
+    "tfm_system_info": (INT, [P, P, P, P]),
+    "tfm_system_download": (INT, [P, P, P, P, P, P]),
+    "tfm_system_solve": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
+    "tfm_system_destroy": (None, [P]),
+    "tfm_solve_poisson": (INT, [P, P, D, INT, D, I64, P, P, C.POINTER(SolveOpts), P]),
+    "tfm_solve_elasticity": (INT, [P, P, INT, D, D, INT, P, I64, P, P, C.POINTER(SolveOpts), P]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+EXPORTED = tuple(_SIGS)
+
+
+class SparseError(RuntimeError):
+    """Mirrors tpt_fem_sparse::SparseError (crates/tpt-fem-sparse/src/lib.rs:66-84)."""
+
+
+class ElasticityError(RuntimeError):
+    """Mirrors tpt_fem_elasticity::ElasticityError::ModelDimMismatch."""
+
+
+class TfmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+class Context:
+    """One tfm_ctx (device + stream).  Not shared across host threads."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = _lib.tfm_ctx_create(device, C.byref(h))
+        if rc != OK:
+            raise TfmError(rc, "tfm_ctx_create failed (no usable GPU?)")
+        self.h = h
+
+    def check(self, rc):
+        if rc == OK:
+            return
+        msg = (_lib.tfm_last_error(self.h) or b"").decode()
+        if rc == E_NUMERIC:
+            raise SparseError(msg)
+        if rc == E_MODEL:
+            raise ElasticityError(msg)
+        raise TfmError(rc, msg)
+
+    def close(self):
+        if self.h:
+            _lib.tfm_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def synchronize(self):
+        self.check(_lib.tfm_ctx_synchronize(self.h))
+
+    def timer_begin(self):
+        self.check(_lib.tfm_ctx_timer_begin(self.h))
+
+    def timer_end(self):
+        """Device milliseconds since timer_begin on this context's stream."""
+        ms = D()
+        self.check(_lib.tfm_ctx_timer_end(self.h, C.byref(ms)))
+        return ms.value
+
+
+class Mesh:
+    """Device-resident mesh (crates/tpt-fem-mesh/src/lib.rs:39-148)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+        cell, nn, pdim, ne = INT(), I64(), INT(), I64()
+        _lib.tfm_mesh_info(handle, C.byref(cell), C.byref(nn), C.byref(pdim), C.byref(ne))
+        self.cell, self.n_nodes, self.pdim, self.n_elem = cell.value, nn.value, pdim.value, ne.value
+        self.dim = DIMS[self.cell]
+
+    @staticmethod
+    def from_arrays(ctx, cell, coords, conn):
+        coords, conn = _f64(coords), _i64(conn)
+        h = C.c_void_p()
+        ctx.check(_lib.tfm_mesh_create(ctx.h, cell, coords.shape[0], coords.shape[1], _ptr(coords),
+                                       conn.shape[0], _ptr(conn), C.byref(h)))
+        return Mesh(ctx, h)
+
+    @staticmethod
+    def box(ctx, mn, mx, n):
+        """box_mesh(min, max, [nx, ny, nz]) (crates/tpt-fem-mesh-gen/src/lib.rs:422-495)."""
+        h = C.c_void_p()
+        ctx.check(_lib.tfm_mesh_box(ctx.h, _ptr(_f64(mn)), _ptr(_f64(mx)), n[0], n[1], n[2], C.byref(h)))
+        return Mesh(ctx, h)
+
+    @staticmethod
+    def grid(ctx, nx, ny, h=1.0):
+        """topopt Grid::new (crates/tpt-fem-topopt/src/lib.rs:89-116) as a Q4 mesh."""
+        hh = C.c_void_p()
+        ctx.check(_lib.tfm_mesh_grid(ctx.h, nx, ny, h, C.byref(hh)))
+        return Mesh(ctx, hh)
+
+    def jitter(self, n, frac, seed):
+        self.ctx.check(_lib.tfm_mesh_jitter(self.h, n[0], n[1], n[2], frac, seed))
+
+    def download(self):
+        coords = np.zeros((self.n_nodes, self.pdim))
+        conn = np.zeros((self.n_elem, NODES[self.cell]), dtype=np.int64)
+        self.ctx.check(_lib.tfm_mesh_download(self.h, _ptr(coords), _ptr(conn)))
+        return coords, conn
+
+    def nodes_on_plane(self, axis, coord, tol):
+        out = np.zeros(max(1, self.n_nodes), dtype=np.int64)
+        cnt = I64()
+        self.ctx.check(_lib.tfm_mesh_nodes_on_plane(self.h, axis, coord, tol, _ptr(out), C.byref(cnt)))
+        return out[: cnt.value].copy()
+
+    def close(self):
+        if self.h:
+            _lib.tfm_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Csr:
+    """Host copy of a tpt_fem_sparse::Csr (crates/tpt-fem-sparse/src/lib.rs:50-63)."""
+
+    def __init__(self, nrows, ncols, row_ptrs, col_ind, values):
+        self.nrows, self.ncols = nrows, ncols
+        self.row_ptrs, self.col_ind, self.values = row_ptrs, col_ind, values
+
+    def nnz(self):
+        return len(self.values)
+
+
+class Matrix:
+    """Device CSR (tfm_matrix)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+        nr, nc, nz = I64(), I64(), I64()
+        _lib.tfm_matrix_info(handle, C.byref(nr), C.byref(nc), C.byref(nz))
+        self.nrows, self.ncols, self.nnz = nr.value, nc.value, nz.value
+
+    @staticmethod
+    def from_csr(ctx, csr):
+        h = C.c_void_p()
+        rp, ci, v = _i64(csr.row_ptrs), _i64(csr.col_ind), _f64(csr.values)
+        ctx.check(_lib.tfm_matrix_create(ctx.h, csr.nrows, csr.ncols, _ptr(rp), _ptr(ci), _ptr(v), C.byref(h)))
+        return Matrix(ctx, h)
+
+    def to_csr(self):
+        rp = np.zeros(self.nrows + 1, dtype=np.int64)
+        ci = np.zeros(max(1, self.nnz), dtype=np.int64)
+        v = np.zeros(max(1, self.nnz))
+        self.ctx.check(_lib.tfm_matrix_download(self.h, _ptr(rp), _ptr(ci), _ptr(v)))
+        return Csr(self.nrows, self.ncols, rp, ci[: self.nnz].copy(), v[: self.nnz].copy())
+
+    def matvec(self, x):
+        """Csr::matvec (crates/tpt-fem-sparse/src/lib.rs:195-205)."""
+        x = _f64(x)
+        y = np.zeros(max(self.nrows, len(x)))
+        self.ctx.check(_lib.tfm_matrix_matvec(self.h, _ptr(x), len(x), _ptr(y)))
+        return y
+
+    def close(self):
+        if self.h:
+            _lib.tfm_matrix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def assemble(mesh, dofs_per_node, op):
+    """assemble(mesh, dpn, |eid, m| element_matrix) -> device CSR equal to Coo::to_csr()."""
+    h = C.c_void_p()
+    mesh.ctx.check(_lib.tfm_assemble(mesh.ctx.h, mesh.h, C.byref(op), dofs_per_node, C.byref(h)))
+    return Matrix(mesh.ctx, h)
+
+
+# ---- loads ------------------------------------------------------------------
+def source_vector(mesh, f=1.0, quad_order=2, rhs=None):
+    rhs = np.zeros(mesh.n_nodes) if rhs is None else _f64(rhs).copy()
+    mesh.ctx.check(_lib.tfm_load_source(mesh.ctx.h, mesh.h, quad_order, f, _ptr(rhs)))
+    return rhs
+
+
+def body_vector(mesh, b, quad_order=2, rhs=None):
+    rhs = np.zeros(mesh.n_nodes * mesh.dim) if rhs is None else _f64(rhs).copy()
+    bb = _f64(list(b) + [0.0] * (3 - len(b)))
+    mesh.ctx.check(_lib.tfm_load_body(mesh.ctx.h, mesh.h, quad_order, _ptr(bb), _ptr(rhs)))
+    return rhs
+
+
+def apply_neumann(mesh, dofs_per_node, g, rhs):
+    rhs = _f64(rhs).copy()
+    mesh.ctx.check(_lib.tfm_load_neumann(mesh.ctx.h, mesh.h, dofs_per_node, g, _ptr(rhs)))
+    return rhs
+
+
+def plane_traction(mesh, dofs_per_node, axis, coord, tol, t, rhs=None):
+    rhs = np.zeros(mesh.n_nodes * dofs_per_node) if rhs is None else _f64(rhs).copy()
+    tt = _f64(list(t) + [0.0] * (3 - len(t)))
+    mesh.ctx.check(_lib.tfm_load_plane_traction(mesh.ctx.h, mesh.h, dofs_per_node, axis, coord, tol,
+                                                _ptr(tt), _ptr(rhs)))
+    return rhs
+
+
+# ---- reduced systems ----------------------------------------------------------
+def _bc_arrays(bcs):
+    if not bcs:
+        return 0, np.zeros(1, np.int64), np.zeros(1)
+    return len(bcs), _i64([b[0] for b in bcs]), _f64([b[1] for b in bcs])
+
+
+class System:
+    """Reduced system on the device (ReducedSystem + its solver)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+        nf, nz, nd = I64(), I64(), I64()
+        _lib.tfm_system_info(handle, C.byref(nf), C.byref(nz), C.byref(nd))
+        self.n_free, self.nnz, self.n_dof = nf.value, nz.value, nd.value
+
+    @staticmethod
+    def create(mesh, op, dofs_per_node, bcs, rhs=None, flags=SYS_FP64):
+        nb, bd, bv = _bc_arrays(bcs)
+        r = None if rhs is None else _f64(rhs)
+        h = C.c_void_p()
+        mesh.ctx.check(_lib.tfm_system_create(mesh.ctx.h, mesh.h, C.byref(op), dofs_per_node, nb, _ptr(bd),
+                                              _ptr(bv), _ptr(r), flags, C.byref(h)))
+        return System(mesh.ctx, h)
+
+    @staticmethod
+    def from_matrix(matrix, rhs, bcs):
+        nb, bd, bv = _bc_arrays(bcs)
+        rhs = _f64(rhs)
+        h = C.c_void_p()
+        matrix.ctx.check(_lib.tfm_system_from_matrix(matrix.ctx.h, matrix.h, len(rhs), _ptr(rhs), nb, _ptr(bd),
+                                                     _ptr(bv), C.byref(h)))
+        return System(matrix.ctx, h)
+
+    def assemble(self, op=None, which=0):
+        """Numeric re-assembly (values only) on the existing pattern."""
+        self.ctx.check(_lib.tfm_system_assemble(self.h, C.byref(op) if op is not None else None, which))
+
+    def download(self):
+        """(kred Csr, fred, free) like reduce_system's ReducedSystem."""
+        rp = np.zeros(self.n_free + 1, np.int64)
+        ci = np.zeros(max(1, self.nnz), np.int64)
+        v = np.zeros(max(1, self.nnz))
+        fred = np.zeros(max(1, self.n_free))
+        free = np.zeros(max(1, self.n_free), np.int64)
+        self.ctx.check(_lib.tfm_system_download(self.h, _ptr(rp), _ptr(ci), _ptr(v), _ptr(fred), _ptr(free)))
+        ncols = int(ci[: self.nnz].max()) + 1 if self.nnz else 1
+        nrows = self.n_free
+        return (Csr(nrows, ncols, rp, ci[: self.nnz].copy(), v[: self.nnz].copy()),
+                fred[: self.n_free].copy(), free[: self.n_free].copy())
+
+    def solve(self, mode=SOLVE_PCG64, rtol=1e-12, max_iter=100000, inner_rtol=1e-5, max_outer=60):
+        u = np.zeros(max(1, self.n_dof))
+        o = SolveOpts(mode=mode, rtol=rtol, max_iter=max_iter, inner_rtol=inner_rtol, max_outer=max_outer)
+        st = SolveStats()
+        self.ctx.check(_lib.tfm_system_solve(self.h, C.byref(o), _ptr(u), C.byref(st)))
+        self.stats = st
+        return u[: self.n_dof]
+
+    def close(self):
+        if self.h:
+            _lib.tfm_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def reduce_system(matrix, rhs, bcs):
+    """reduce_system (crates/tpt-fem-assembly/src/lib.rs:504-535) -> (kred, fred, free)."""
+    s = System.from_matrix(matrix, rhs, bcs)
+    try:
+        return s.download()
+    finally:
+        s.close()
+
+
+def solve_with_dirichlet(matrix, rhs, bcs, rtol=1e-13):
+    """solve_with_dirichlet (crates/tpt-fem-assembly/src/lib.rs:543-559), solved by FP64 PCG."""
+    s = System.from_matrix(matrix, rhs, bcs)
+    try:
+        return s.solve(SOLVE_PCG64, rtol=rtol)
+    finally:
+        s.close()
+
+
+def solve_poisson(mesh, kappa, quad_order, f, bcs, rtol=1e-13):
+    """solve_poisson (crates/tpt-fem-thermal/src/lib.rs:206-240), constant source, no Neumann/Robin."""
+    nb, bd, bv = _bc_arrays(bcs)
+    u = np.zeros(mesh.n_nodes)
+    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
+    mesh.ctx.check(_lib.tfm_solve_poisson(mesh.ctx.h, mesh.h, kappa, quad_order, f, nb, _ptr(bd), _ptr(bv),
+                                          C.byref(o), _ptr(u)))
+    return u
+
+
+def solve_elasticity(mesh, model, young, nu, quad_order, body, bcs, rtol=1e-13):
+    """solve_elasticity (crates/tpt-fem-elasticity/src/lib.rs:355-381), constant body force."""
+    nb, bd, bv = _bc_arrays(bcs)
+    u = np.zeros(mesh.n_nodes * mesh.dim)
+    b = _f64(list(body) + [0.0] * (3 - len(body)))
+    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
+    mesh.ctx.check(_lib.tfm_solve_elasticity(mesh.ctx.h, mesh.h, model, young, nu, quad_order, _ptr(b), nb,
+                                             _ptr(bd), _ptr(bv), C.byref(o), _ptr(u)))
+    return u
