@@ -171,6 +171,24 @@ int tfm_system_solve(tfm_system *sys, const tfm_solve_opts *opts, double *u,
                      tfm_solve_stats *stats);
 void tfm_system_destroy(tfm_system *sys);
 
+/* ---- batching form (no reference counterpart: the reference solves one
+ * problem per call; these run many through one assembly and one PCG) ------ */
+/* P unit-cube box_mesh problems with per-problem cells n3[3p..3p+2],
+ * concatenated (node_off / elem_off: P+1 entries, may be NULL). */
+int tfm_mesh_box_batch(tfm_ctx *ctx, int64_t P, const int64_t *n3, tfm_mesh **out,
+                       int64_t *node_off, int64_t *elem_off);
+/* P copies of the topopt Grid (one per SIMP design), concatenated. */
+int tfm_mesh_grid_batch(tfm_ctx *ctx, int64_t P, int64_t nx, int64_t ny, double h,
+                        tfm_mesh **out);
+/* Solve a block-diagonal system (P problems, DOF ranges dof_off[p]..dof_off[p+1])
+ * for nrhs (1 or 8) right-hand sides each, every (problem, rhs) segment with
+ * its own PCG scalars and convergence.  rhs: host [nrhs][n_dof] or NULL (the
+ * system's own rhs, nrhs = 1); u: host [nrhs][n_dof]; stats: [P*nrhs]
+ * (segment p*nrhs + k).  Homogeneous Dirichlet values only. */
+int tfm_system_solve_batch(tfm_system *sys, int64_t P, const int64_t *dof_off, int nrhs,
+                           const double *rhs, const tfm_solve_opts *opts, double *u,
+                           tfm_solve_stats *stats);
+
 /* ---- drivers ------------------------------------------------------------ */
 /* solve_poisson (thermal/src/lib.rs:206-240) with constant source f, no
  * Neumann/Robin terms. */

@@ -194,7 +194,8 @@ def box_mesh(mn, mx, n):
     nx, ny, nz = n
     coords = np.zeros(((nx + 1) * (ny + 1) * (nz + 1), 3))
     conn = np.zeros((nx * ny * nz * 6, 4), dtype=np.int64)
-    lib().orc_box_mesh(_p(f64(mn)), _p(f64(mx)), nx, ny, nz, _p(coords), _p(conn))
+    lo, hi = f64(mn), f64(mx)  # kept alive across the call
+    lib().orc_box_mesh(_p(lo), _p(hi), nx, ny, nz, _p(coords), _p(conn))
     return coords, conn
 
 
@@ -365,8 +366,9 @@ def apply_neumann(coords, cell, conn, dpn, g, rhs, order=2):
 def plane_traction(coords, cell, conn, dpn, axis, coord, tol, t, rhs=None, order=2):
     coords, conn = f64(coords), i64(conn)
     rhs = np.zeros(coords.shape[0] * dpn) if rhs is None else f64(rhs).copy()
+    tt = f64(t)
     lib().orc_plane_traction(coords.shape[0], coords.shape[1], _p(coords), conn.shape[0], cell,
-                             _p(conn), dpn, axis, coord, tol, _p(f64(t)), order, _p(rhs))
+                             _p(conn), dpn, axis, coord, tol, _p(tt), order, _p(rhs))
     return rhs
 
 

