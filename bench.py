@@ -63,7 +63,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128, help="cells per axis")
     ap.add_argument("--rtol", type=float, default=1e-10, help="FP64 residual tolerance of the timed solves")
-    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--inner-rtol", type=float, default=0.1,
+                    help="FP32 residual reduction between FP64 residual replacements")
     ap.add_argument("--cpu-n", type=int, default=32, help="cells per axis of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve", action="store_true")

@@ -123,7 +123,18 @@ struct AsmPlan {
   int *d_rp = nullptr;
   int *d_ci = nullptr;
   int64_t nrows = 0, ncols = 0, nnz = 0;
+  // P1-tet fast path (tfm_asm_tet.hip): its own tile schedule (owned) and an
+  // overflow flag checked after the first launch (then the generic kernel).
+  bool use_tet = false;
+  bool tet_checked = false;
+  int *d_tet_tiles = nullptr;
+  int tet_ntiles = 0;
+  int *d_err = nullptr;
 };
+int tet_tiles(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, int **d_tiles, int *ntiles);
+struct OpParams;
+int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_tiles, int ntiles, const OpParams &P,
+                const int *d_rp, const int *d_ci, double *v64, float *v32, int *d_err);
 int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
                 const unsigned char *h_fixed, AsmPlan *plan);
 int plan_set_coefficients(tfm_ctx *ctx, AsmPlan *plan, const tfm_operator *op);
