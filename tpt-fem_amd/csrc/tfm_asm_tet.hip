@@ -98,7 +98,7 @@ __device__ __forceinline__ int lower_bound_lds(const int *s, int lo, int hi, int
 template <int DPN, typename VT>
 __global__ __launch_bounds__(kTetTP) void k_asm_tet(const double *__restrict__ coords, const int *__restrict__ conn,
                                                      const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
-                                                     const int *__restrict__ tiles, OpParams P,
+                                                     const int *__restrict__ tiles, int ntiles, OpParams P,
                                                      const int *__restrict__ rp, const int *__restrict__ ci,
                                                      VT *__restrict__ vals, int *__restrict__ err) {
   __shared__ TetRec<VT> s_rec[kTetTP];
@@ -115,7 +115,9 @@ __global__ __launch_bounds__(kTetTP) void k_asm_tet(const double *__restrict__ c
   __shared__ int s_ci[kTetSlots];
 
   const int tid = threadIdx.x;
-  const int n0 = tiles[blockIdx.x], n1 = tiles[blockIdx.x + 1];
+  const int tile = xcd_item(blockIdx.x, ntiles);  // neighbouring tiles share an XCD's L2
+  if (tile >= ntiles) return;
+  const int n0 = tiles[tile], n1 = tiles[tile + 1];
   const int nt = n1 - n0;
   const int pa = n2e_ptr[n0];
   const int np = n2e_ptr[n1] - pa;
@@ -308,17 +310,17 @@ int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_tiles,
                 const int *d_rp, const int *d_ci, double *v64, float *v32, int *d_err) {
   if (ntiles == 0) return TFM_OK;
   hipStream_t st = ctx->stream;
-  dim3 g(ntiles), t(kTetTP);
+  dim3 g(xcd_grid(ntiles)), t(kTetTP);
   if (dpn == 1) {
     if (v64) hipLaunchKernelGGL((k_asm_tet<1, double>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e_ptr,
-                                mesh->d_n2e, d_tiles, P, d_rp, d_ci, v64, d_err);
+                                mesh->d_n2e, d_tiles, ntiles, P, d_rp, d_ci, v64, d_err);
     if (v32) hipLaunchKernelGGL((k_asm_tet<1, float>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e_ptr,
-                                mesh->d_n2e, d_tiles, P, d_rp, d_ci, v32, d_err);
+                                mesh->d_n2e, d_tiles, ntiles, P, d_rp, d_ci, v32, d_err);
   } else {
     if (v64) hipLaunchKernelGGL((k_asm_tet<3, double>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e_ptr,
-                                mesh->d_n2e, d_tiles, P, d_rp, d_ci, v64, d_err);
+                                mesh->d_n2e, d_tiles, ntiles, P, d_rp, d_ci, v64, d_err);
     if (v32) hipLaunchKernelGGL((k_asm_tet<3, float>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e_ptr,
-                                mesh->d_n2e, d_tiles, P, d_rp, d_ci, v32, d_err);
+                                mesh->d_n2e, d_tiles, ntiles, P, d_rp, d_ci, v32, d_err);
   }
   TFM_CHECK_HIP(ctx, hipGetLastError());
   return TFM_OK;

@@ -40,6 +40,18 @@ struct OpParams {
   const double *ke0;     // device [64] (SIMP unit stiffness)
 };
 
+// MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs,
+// each with a private L2.  Work items that share data (consecutive node tiles,
+// consecutive row chunks) should land on the same XCD: launch a grid of
+// 8 * ceil(nwork / 8) blocks and map block b to work item
+// (b % 8) * ceil(nwork / 8) + b / 8 -- a bijection; items >= nwork exit.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item(int b, int nwork) {
+  const int q = (nwork + kXcds - 1) / kXcds;
+  return (b % kXcds) * q + b / kXcds;
+}
+inline int xcd_grid(int nwork) { return kXcds * ((nwork + kXcds - 1) / kXcds); }
+
 template <int CELL> struct Cell;
 template <> struct Cell<TFM_LINE> { static constexpr int NN = 2, DIM = 1; };
 template <> struct Cell<TFM_TRI> { static constexpr int NN = 3, DIM = 2; };
