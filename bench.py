@@ -56,6 +56,20 @@ def cpu_baseline(n, seed=7):
     return len(conn) / dt, len(conn), dt
 
 
+def copy_probe():
+    """Five copies of 1 GiB (2^28 float32) between two device buffers: a
+    kernel whose HBM traffic is known (1 GiB read + 1 GiB written per launch),
+    profiled beside the assembly to scale the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_launch": 1 << 30, "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,11 +82,16 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=32, help="cells per axis of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run five 1 GiB device-to-device copies (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the sharded design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import tptfem as T
 

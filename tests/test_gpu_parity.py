@@ -400,3 +400,27 @@ def test_tet_fast_path_equals_generic_kernel(ctx, monkeypatch):
     coo = O.assemble(coords, conn, O.make_op(O.TET, 3, 1, coef_e=ye, nu=0.3), 3)
     kr, _, _ = O.reduce_system(coo, np.zeros(len(coords) * 3), bcs)
     csr_equal(kf, kr.to_csr())
+
+
+@pytest.mark.parametrize("kind", ["poisson", "elasticity"])
+def test_tet_fp32_values_close_to_fp64(ctx, kind):
+    """The precision form's FP32 matrix: same pattern, values within FP32
+    rounding of the exact FP64 ones (1e-5 of the largest entry)."""
+    T = tp()
+    n = (5, 4, 4)
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], n)
+    m.jitter(n, 0.2, 8)
+    coords, conn = m.download()
+    cf = 10.0 ** np.random.default_rng(2).uniform(-1, 1, m.n_elem)
+    dpn = 1 if kind == "poisson" else 3
+    if kind == "poisson":
+        op = T.Operator.poisson(1.0, 2, kappa_e=cf)
+        ref = O.make_op(O.TET, 3, 0, coef_e=cf)
+    else:
+        op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=cf)
+        ref = O.make_op(O.TET, 3, 1, coef_e=cf, nu=0.3)
+    bcs = face_bcs(coords, dpn, [(0, 0.0)])
+    s = T.System.create(m, op, dpn, bcs, flags=T.SYS_FP32)
+    k32, _, _ = s.download()
+    kr, _, _ = O.reduce_system(O.assemble(coords, conn, ref, dpn), np.zeros(len(coords) * dpn), bcs)
+    csr_equal(k32, kr.to_csr(), exact=False, rtol=1e-5)

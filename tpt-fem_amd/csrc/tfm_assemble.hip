@@ -569,13 +569,10 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
   const bool tet_op = mesh->cell == TFM_TET && ((op->kind == TFM_OP_POISSON && dpn == 1) ||
                                                 (op->kind == TFM_OP_ELASTICITY && dpn == 3));
   if (tet_op && mesh->max_adj <= 32 && !getenv("TFM_GENERIC_ASSEMBLY")) {
-    if (tet_tiles(ctx, mesh, dpn, pl->d_rp, &pl->d_tet_tiles, &pl->tet_ntiles) == TFM_OK &&
-        dalloc(ctx, &pl->d_err, 1) == TFM_OK &&
-        hipMemsetAsync(pl->d_err, 0, sizeof(int), ctx->stream) == hipSuccess) {
+    if (tet_plan_create(ctx, mesh, dpn, pl->d_rp, pl->d_ci, &pl->tet) == TFM_OK) {
       pl->use_tet = true;
     } else {
-      dfree(pl->d_tet_tiles);
-      dfree(pl->d_err);
+      tet_plan_destroy(&pl->tet);
       ctx->err.clear();
     }
   }
@@ -587,21 +584,7 @@ int plan_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *d_v64, float *d_v32) {
   OpParams P = make_params(pl, pl->d_tab, pl->nq, pl->d_tab + (size_t)pl->nq * kRuleStride);
   int rc;
   if (pl->ntiles == 0) return TFM_OK;
-  if (pl->use_tet) {
-    if ((rc = tet_numeric(ctx, pl->mesh, pl->dpn, pl->d_tet_tiles, pl->tet_ntiles, P, pl->d_rp, pl->d_ci, d_v64, d_v32,
-                          pl->d_err)))
-      return rc;
-    if (pl->tet_checked) return TFM_OK;
-    // first launch: a node beyond the fast path's hash / bitmask limits
-    // sends this plan to the generic kernel for good
-    int herr = 0;
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(&herr, pl->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    TFM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    AsmPlan *mp = const_cast<AsmPlan *>(pl);
-    mp->tet_checked = true;
-    if (!herr) return TFM_OK;
-    mp->use_tet = false;
-  }
+  if (pl->use_tet) return tet_numeric(ctx, pl->mesh, pl->dpn, pl->tet, P, d_v64, d_v32);
   if (d_v64 && (rc = dispatch<M_F64>(ctx, pl, P, nullptr, nullptr, nullptr, d_v64, nullptr, nullptr))) return rc;
   if (d_v32 && (rc = dispatch<M_F32>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, d_v32, nullptr))) return rc;
   TFM_CHECK_HIP(ctx, hipGetLastError());
@@ -609,8 +592,7 @@ int plan_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *d_v64, float *d_v32) {
 }
 
 void plan_destroy(AsmPlan *pl) {
-  dfree(pl->d_tet_tiles);
-  dfree(pl->d_err);
+  tet_plan_destroy(&pl->tet);
   dfree(pl->d_tab);
   dfree(pl->d_coef_e);
   dfree(pl->d_rho);

@@ -99,8 +99,26 @@ inline int cell_dim(int cell) {
 // context stream; returns the total through *total (synchronises).
 int exclusive_scan_i32(tfm_ctx *ctx, const int *d_in, int *d_out, int64_t n, int64_t *total);
 
+// P1-tet assembly plan (tfm_asm_tet.hip), built once per pattern:
+//   tiles  int32 [ntiles + 1]          first node of each tile
+//   bptr   int32 [n_nodes + 1]         first row block of each node
+//   bmask  u32   [nblocks]             contributing (node, element) pairs of a block
+//   blmb   u64   [nblocks]             2-bit local index of the column node per pair
+//   ipos   int32 [nblocks * dpn]       CSR position of a (row, block)'s first entry,
+//                                      items ordered (node, row a, block)
+//   ipm    u8    [nblocks * dpn]       presence mask of the block's dpn columns
+struct TetPlan {
+  int *d_tiles = nullptr;
+  int ntiles = 0;
+  int *d_bptr = nullptr;
+  unsigned *d_bmask = nullptr;
+  unsigned long long *d_blmb = nullptr;
+  int *d_ipos = nullptr;
+  unsigned char *d_ipm = nullptr;
+  int64_t nblocks = 0;
+};
+
 // Assembly engine (tfm_assemble.hip).
-struct Pattern;  // opaque
 struct AsmPlan {
   const tfm_mesh *mesh = nullptr;
   tfm_operator op{};
@@ -123,24 +141,43 @@ struct AsmPlan {
   int *d_rp = nullptr;
   int *d_ci = nullptr;
   int64_t nrows = 0, ncols = 0, nnz = 0;
-  // P1-tet fast path (tfm_asm_tet.hip): its own tile schedule (owned) and an
-  // overflow flag checked after the first launch (then the generic kernel).
+  // P1-tet fast path (tfm_asm_tet.hip), when its plan could be built
   bool use_tet = false;
-  bool tet_checked = false;
-  int *d_tet_tiles = nullptr;
-  int tet_ntiles = 0;
-  int *d_err = nullptr;
+  TetPlan tet;
 };
-int tet_tiles(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, int **d_tiles, int *ntiles);
 struct OpParams;
-int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_tiles, int ntiles, const OpParams &P,
-                const int *d_rp, const int *d_ci, double *v64, float *v32, int *d_err);
+int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci, TetPlan *tp);
+void tet_plan_destroy(TetPlan *tp);
+int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const TetPlan &tp, const OpParams &P, double *v64,
+                float *v32);
 int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
                 const unsigned char *h_fixed, AsmPlan *plan);
 int plan_set_coefficients(tfm_ctx *ctx, AsmPlan *plan, const tfm_operator *op);
 int plan_numeric(tfm_ctx *ctx, const AsmPlan *plan, double *d_v64, float *d_v32);
 void plan_destroy(AsmPlan *plan);
 int validate_operator(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn);
+
+// Solver copy of a system matrix in node-blocked sliced ELL (tfm_solve.hip):
+// block row b holds the bs DOF rows of node b; 64 consecutive block rows form
+// a slice stored lane-interleaved, so the SpMV runs one lane per block row
+// with coalesced loads and one column index per bs x bs block.
+//   soff  int32 [nslices + 1]   first block of each slice (multiple of 64)
+//   bcol  int32 [nblk]          block column of (slice, k, lane) at soff + 64k + lane;
+//                               padding blocks point at column 0 with zero values
+//   map   int32 [nnz]           CSR entry -> value index
+//   value index of entry (i, j) of block (slice, k, lane):
+//                               (soff + 64k) * bs^2 + (i * bs + j) * 64 + lane
+struct BSell {
+  int state = 0;  // 0 not built, 1 built, -1 not usable (index range)
+  int bs = 0, nbr = 0, nslices = 0;
+  int64_t nblk = 0, nnz = 0;
+  int *d_soff = nullptr;
+  int *d_bcol = nullptr;
+  int *d_map = nullptr;
+  double *d_v64 = nullptr;
+  float *d_v32 = nullptr;
+};
+void bsell_destroy(BSell *b);
 
 // Loads (tfm_loads.hip): device rhs of n_nodes*dpn doubles, accumulated.
 int load_source_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double f, double *d_rhs);
@@ -168,4 +205,5 @@ struct tfm_system {
   std::vector<double> bc_val;
   std::vector<int64_t> free_dofs;
   bool owns_pattern = false;
+  tfm::BSell bsell;           // solver copy, built on the first solve
 };

@@ -12,6 +12,7 @@
 // fences or atomics: each block writes its partial sum, and every block of
 // the next kernel re-sums the partials in the same fixed order, so all blocks
 // agree on alpha, beta and convergence and results are deterministic.
+#include <climits>
 #include <cmath>
 #include <cstring>
 
@@ -82,6 +83,36 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// Start of iteration i (parity e) in every block: converged / iteration
+// limit (sc->done) decided identically by all blocks from the partials of the
+// previous update; beta_i = rz_i / rz_{i-1}.  Returns false (uniformly per
+// block) when the block has nothing to do.
+__device__ __forceinline__ bool dir_prologue(Scal *sc, const double *part, int e, double *beta) {
+  __shared__ int s_done, s_it;
+  if (threadIdx.x == 0) {
+    s_done = sc->done;
+    s_it = sc->it;
+  }
+  __syncthreads();
+  if (s_done) return false;
+  const int nb = gridDim.x;
+  const int it = s_it;
+  if (it > 0) {
+    double rr = sum_part(part, e ? P_RR0 : P_RR1, nb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->rr = rr;  // latest recurrence residual, for the host
+    if (rr <= sc->tol2 || it >= sc->max_it) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->done = 1;
+        sc->rr = rr;
+      }
+      return false;
+    }
+  }
+  *beta = 0.0;
+  if (it > 0) *beta = sum_part(part, e ? P_RZ0 : P_RZ1, nb) / sum_part(part, e ? P_RZ1 : P_RZ0, nb);
+  return true;
+}
+
 // Iteration i (parity e): p_i = z_i + beta_i p_{i-1} on the fly, q = K p_i,
 // partial p_i.q.  beta_i = rz_i / rz_{i-1}; rz_i, rr_i come from the previous
 // update kernel (slot parity 1-e), rz_{i-1} from the one before (parity e).
@@ -94,28 +125,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_spmv(int n, const int *__res
   extern __shared__ unsigned char dyn_lds[];
   VT *prod = (VT *)dyn_lds;
   constexpr int T = Stream<R>::T;
-  __shared__ int s_done, s_it;
-  if (threadIdx.x == 0) {
-    s_done = sc->done;
-    s_it = sc->it;
-  }
-  __syncthreads();
-  if (s_done) return;  // uniform per block (barriers follow)
-  const int nb = gridDim.x;
-  const int it = s_it;
-  if (it > 0) {
-    double rr = sum_part(part, e ? P_RR0 : P_RR1, nb);
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->rr = rr;  // latest recurrence residual, for the host
-    if (rr <= sc->tol2 || it >= sc->max_it) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        sc->done = 1;
-        sc->rr = rr;
-      }
-      return;
-    }
-  }
-  double beta = 0.0;
-  if (it > 0) beta = sum_part(part, e ? P_RZ0 : P_RZ1, nb) / sum_part(part, e ? P_RZ1 : P_RZ0, nb);
+  double beta;
+  if (!dir_prologue(sc, part, e, &beta)) return;  // uniform per block
   const VT b = (VT)beta;
   const int lane = threadIdx.x & 63, lr = lane / T, sub = lane % T;
   VT *wprod = prod + (threadIdx.x >> 6) * kWaveCap;
@@ -145,6 +156,132 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_spmv(int n, const int *__res
     wave_lds_sync();
   }
   put_part(part, P_PQ, dot);
+}
+
+// The same iteration step on the node-blocked sliced ELL copy (BSell): one
+// lane per block row (node), 64 block rows per slice, each wave a contiguous
+// run of slices and each XCD a contiguous run of waves (neighbouring nodes'
+// vector entries stay in one L2).  Per block: one column index, BS*BS
+// coalesced values, BS gathered entries of z and p_{i-1}.  Padding blocks
+// carry zeros against column 0, so the loop has no branch.
+template <typename VT, int BS>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir_bsell(int n, int nslices, const int *__restrict__ soff,
+                                                           const int *__restrict__ bcol, const VT *__restrict__ bv,
+                                                           const VT *__restrict__ z, const VT *__restrict__ pold,
+                                                           VT *__restrict__ pnew, VT *__restrict__ q, Scal *sc,
+                                                           double *part, int e) {
+  double beta;
+  if (!dir_prologue(sc, part, e, &beta)) return;  // uniform per block
+  const VT b = (VT)beta;
+  constexpr int BB = BS * BS;
+  const int lane = threadIdx.x & 63;
+  const int nwaves = gridDim.x * (kBlock / 64);
+  const int gw = xcd_item(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  const int per = (nslices + nwaves - 1) / nwaves;
+  const int s1 = min(nslices, (gw + 1) * per);
+  double dot = 0.0;
+  for (int s = gw * per; s < s1; s++) {
+    const int o = soff[s], w = (soff[s + 1] - o) >> 6;
+    const int *cp = bcol + o + lane;
+    const VT *vp = bv + (int64_t)o * BB + lane;
+    VT acc[BS];
+#pragma unroll
+    for (int i = 0; i < BS; i++) acc[i] = 0;
+#pragma unroll 2
+    for (int k = 0; k < w; k++) {
+      const int c = cp[k * 64] * BS;
+      VT xv[BS];
+#pragma unroll
+      for (int j = 0; j < BS; j++) xv[j] = z[c + j] + b * pold[c + j];
+      const VT *vk = vp + (int64_t)k * (64 * BB);
+#pragma unroll
+      for (int i = 0; i < BS; i++)
+#pragma unroll
+        for (int j = 0; j < BS; j++) acc[i] += vk[(i * BS + j) * 64] * xv[j];
+    }
+    const int r0 = (s * 64 + lane) * BS;
+    if (r0 < n) {
+#pragma unroll
+      for (int i = 0; i < BS; i++) {
+        const VT pn = z[r0 + i] + b * pold[r0 + i];
+        pnew[r0 + i] = pn;
+        q[r0 + i] = acc[i];
+        dot += (double)pn * (double)acc[i];
+      }
+    }
+  }
+  put_part(part, P_PQ, dot);
+}
+
+// ---- building the sliced copy from the CSR ---------------------------------
+// Distinct column blocks of block row b: a merge of its BS sorted rows.
+template <int BS>
+__device__ __forceinline__ int bs_merge_step(const int *__restrict__ ci, const int *j, const int *e) {
+  int m = INT_MAX;
+#pragma unroll
+  for (int i = 0; i < BS; i++)
+    if (j[i] < e[i]) m = min(m, ci[j[i]] / BS);
+  return m;
+}
+
+template <int BS>
+__global__ void k_bs_count(int nbr, const int *__restrict__ rp, const int *__restrict__ ci, int *__restrict__ cnt) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbr) return;
+  int j[BS], e[BS];
+#pragma unroll
+  for (int i = 0; i < BS; i++) {
+    j[i] = rp[b * BS + i];
+    e[i] = rp[b * BS + i + 1];
+  }
+  int k = 0;
+  for (int m; (m = bs_merge_step<BS>(ci, j, e)) != INT_MAX; k++)
+#pragma unroll
+    for (int i = 0; i < BS; i++)
+      while (j[i] < e[i] && ci[j[i]] / BS == m) j[i]++;
+  cnt[b] = k;
+}
+
+// 64 * (widest block row) per slice.
+__global__ void k_bs_width(int nbr, int nslices, const int *__restrict__ cnt, int *__restrict__ w) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslices) return;
+  int m = 0;
+  for (int b = s * 64; b < min(nbr, s * 64 + 64); b++) m = max(m, cnt[b]);
+  w[s] = 64 * m;
+}
+
+template <int BS>
+__global__ void k_bs_fill(int nbr, int nslices, const int *__restrict__ rp, const int *__restrict__ ci,
+                          const int *__restrict__ soff, int *__restrict__ bcol, int *__restrict__ map) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nslices * 64) return;
+  constexpr int BB = BS * BS;
+  const int s = b >> 6, lane = b & 63;
+  const int o = soff[s], w = (soff[s + 1] - o) >> 6;
+  int k = 0;
+  if (b < nbr) {
+    int j[BS], e[BS];
+#pragma unroll
+    for (int i = 0; i < BS; i++) {
+      j[i] = rp[b * BS + i];
+      e[i] = rp[b * BS + i + 1];
+    }
+    for (int m; (m = bs_merge_step<BS>(ci, j, e)) != INT_MAX; k++) {
+      bcol[o + k * 64 + lane] = m;
+#pragma unroll
+      for (int i = 0; i < BS; i++)
+        for (; j[i] < e[i] && ci[j[i]] / BS == m; j[i]++)
+          map[j[i]] = (o + k * 64) * BB + (i * BS + ci[j[i]] % BS) * 64 + lane;
+    }
+  }
+  for (; k < w; k++) bcol[o + k * 64 + lane] = 0;
+}
+
+template <typename VT, typename ST>
+__global__ void k_bs_values(int64_t nnz, const int *__restrict__ map, const ST *__restrict__ src, VT *__restrict__ dst) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nnz) dst[map[j]] = (VT)src[j];
 }
 
 // alpha = rz_i / p.q; x += alpha p, r -= alpha q, z = Dinv r; partial rz, rr.
@@ -381,6 +518,67 @@ static void residual_dispatch(int R, hipStream_t st, int nb, int n, const int *r
   hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(kBlock), 0, st, sc, part, nb);
 }
 
+// The matrix operand of the PCG iteration: the CSR stream form, or the
+// node-blocked sliced copy when one was built (bs > 0).  nb is the grid of
+// every kernel of the iteration (their partial sums are indexed by block).
+template <typename VT>
+struct SpOp {
+  int n = 0, nb = 1;
+  const int *rp = nullptr, *ci = nullptr;
+  const VT *v = nullptr;
+  int R = 1;
+  int bs = 0, nslices = 0;
+  const int *soff = nullptr, *bcol = nullptr;
+  const VT *bv = nullptr;
+};
+
+template <typename VT>
+static SpOp<VT> csr_op(int n, const int *rp, const int *ci, const VT *v, int R) {
+  SpOp<VT> o;
+  o.n = n;
+  o.rp = rp;
+  o.ci = ci;
+  o.v = v;
+  o.R = R;
+  o.nb = stream_blocks(n, R);
+  return o;
+}
+
+template <typename VT>
+static SpOp<VT> bsell_op(int n, const BSell &B, const VT *bv) {
+  SpOp<VT> o;
+  o.n = n;
+  o.bs = B.bs;
+  o.nslices = B.nslices;
+  o.soff = B.d_soff;
+  o.bcol = B.d_bcol;
+  o.bv = bv;
+  // four waves per block; a multiple of 8 blocks so that the XCD mapping is
+  // a permutation of the grid
+  int nb = (B.nslices + 3) / 4;
+  nb = std::min(kMaxBlocks, std::max(8, (nb + 7) / 8 * 8));
+  o.nb = nb;
+  return o;
+}
+
+template <typename VT>
+static void dir_launch(const SpOp<VT> &A, hipStream_t st, const VT *z, const VT *pold, VT *pnew, VT *q, Scal *sc,
+                       double *part, int e) {
+  if (A.bs == 0) {
+    dir_dispatch<VT>(A.R, st, A.nb, A.n, A.rp, A.ci, A.v, z, pold, pnew, q, sc, part, e);
+    return;
+  }
+#define TFM_BS(BSZ)                                                                                          \
+  hipLaunchKernelGGL((k_pcg_dir_bsell<VT, BSZ>), dim3(A.nb), dim3(kBlock), 0, st, A.n, A.nslices, A.soff, A.bcol, \
+                     A.bv, z, pold, pnew, q, sc, part, e)
+  switch (A.bs) {
+  case 3: TFM_BS(3); break;
+  case 2: TFM_BS(2); break;
+  default: TFM_BS(1); break;
+  }
+#undef TFM_BS
+}
+
 __global__ void k_max_row_nnz(int n, const int *rp, int *out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) atomicMax(out, rp[i + 1] - rp[i]);
@@ -390,11 +588,10 @@ __global__ void k_max_row_nnz(int n, const int *rp, int *out) {
 // ||r|| <= rtol ||r0|| or max_iter.  Two kernels per iteration (direction +
 // SpMV + p.q, then the update + r.z, r.r); 16 iterations per graph replay.
 template <typename VT, typename XT>
-static int pcg_run(tfm_ctx *ctx, int n, const int *rp, const int *ci, const VT *v, PcgBufs<VT> &B, XT *x,
-                   double rtol, int max_iter, int S, Scal *sc, double *part, int *iters, int *breakdown,
-                   double *rr_out) {
+static int pcg_run(tfm_ctx *ctx, const SpOp<VT> &A, PcgBufs<VT> &B, XT *x, double rtol, int max_iter, Scal *sc,
+                   double *part, int *iters, int *breakdown, double *rr_out) {
   hipStream_t st = ctx->stream;
-  const int nb = stream_blocks(n, S);  // S: rows per wave chunk
+  const int n = A.n, nb = A.nb;
   hipLaunchKernelGGL((k_pcg_init<VT>), dim3(nb), dim3(kBlock), 0, st, n, B.r, B.z, B.p, B.p2, B.dinv, part);
   hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, st, sc, part, nb, rtol * rtol, max_iter);
   TFM_CHECK_HIP(ctx, hipGetLastError());
@@ -403,7 +600,7 @@ static int pcg_run(tfm_ctx *ctx, int n, const int *rp, const int *ci, const VT *
   auto chunk = [&]() {
     for (int k = 0; k < CH; k++) {
       const int e = k & 1;
-      dir_dispatch<VT>(S, st, nb, n, rp, ci, v, B.z, pb[1 - e], pb[e], B.q, sc, part, e);
+      dir_launch<VT>(A, st, B.z, pb[1 - e], pb[e], B.q, sc, part, e);
       hipLaunchKernelGGL((k_pcg_update<VT, XT>), dim3(nb), dim3(kBlock), 0, st, n, x, B.r, B.z, pb[e], B.q, B.dinv,
                          sc, part, e);
     }
@@ -473,18 +670,20 @@ __global__ void k_fold(int n, double *__restrict__ x, float *__restrict__ d) {
 // (FP64 matrix), keeping the search direction.  Stops when the FP64 residual
 // meets rtol.  Keeping the Krylov direction (instead of restarting per
 // refinement cycle) keeps the iteration count near plain FP64 PCG's.
-static int mixed_run(tfm_ctx *ctx, int n, int S, const int *rp, const int *ci, const double *v64, const float *v32,
-                     const double *f, PcgBufs<float> &B, double *r64, float *x32, double *x, double rtol,
-                     double delta, int max_iter, Scal *sc, double *part, int *iters, int *repl, int *breakdown,
-                     double *relres) {
+static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A, const double *f,
+                     PcgBufs<float> &B, double *r64, float *x32, double *x, double rtol, double delta, int max_iter,
+                     Scal *sc, double *part, int *iters, int *repl, int *breakdown, double *relres) {
   hipStream_t st = ctx->stream;
-  const int nb = stream_blocks(n, S);
+  const int n = A.n, nb = A.nb;
+  const int S = K64.R, nbr = K64.nb;
+  const int *rp = K64.rp, *ci = K64.ci;
+  const double *v64 = K64.v;
   const unsigned g1 = (unsigned)((n + 255) / 256);
   Scal h{};
   *iters = 0;
   *repl = 0;
   *breakdown = 0;
-  residual_dispatch(S, st, nb, n, rp, ci, v64, x, f, r64, B.r, sc, part);
+  residual_dispatch(S, st, nbr, n, rp, ci, v64, x, f, r64, B.r, sc, part);
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
   *relres = h.rr0 > 0 ? std::sqrt(h.rr / h.rr0) : 0.0;
@@ -498,7 +697,7 @@ static int mixed_run(tfm_ctx *ctx, int n, int S, const int *rp, const int *ci, c
   auto chunk = [&]() {
     for (int k = 0; k < CH; k++) {
       const int e = k & 1;
-      dir_dispatch<float>(S, st, nb, n, rp, ci, v32, B.z, pb[1 - e], pb[e], B.q, sc, part, e);
+      dir_launch<float>(A, st, B.z, pb[1 - e], pb[e], B.q, sc, part, e);
       hipLaunchKernelGGL((k_pcg_update<float, float>), dim3(nb), dim3(kBlock), 0, st, n, x32, B.r, B.z, pb[e], B.q,
                          B.dinv, sc, part, e);
     }
@@ -529,7 +728,7 @@ static int mixed_run(tfm_ctx *ctx, int n, int S, const int *rp, const int *ci, c
     }
     if (h.rr <= delta * delta * rr_ref || h.done) {
       hipLaunchKernelGGL(k_fold, dim3(g1), dim3(256), 0, st, n, x, x32);
-      residual_dispatch(S, st, nb, n, rp, ci, v64, x, f, r64, B.r, sc, part);
+      residual_dispatch(S, st, nbr, n, rp, ci, v64, x, f, r64, B.r, sc, part);
       Scal h2{};
       TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h2, sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
       TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
@@ -543,6 +742,106 @@ static int mixed_run(tfm_ctx *ctx, int n, int S, const int *rp, const int *ci, c
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   return rc;
+}
+
+// Builds the node-blocked sliced copy's structure from the system's CSR
+// (once per system; the pattern never changes).  Leaves state -1 when the
+// value index would not fit int32.
+static int bsell_build(tfm_system *s) {
+  BSell &B = s->bsell;
+  if (B.state != 0) return TFM_OK;
+  tfm_ctx *ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int bs = (s->dpn >= 1 && s->dpn <= 3 && s->ndof % s->dpn == 0) ? s->dpn : 1;
+  const int64_t nbr = s->ndof / bs;
+  const int64_t nsl = (nbr + 63) / 64;
+  if (nbr == 0 || s->nnz == 0 || nsl * 64 >= INT32_MAX / 16) {
+    B.state = -1;
+    return TFM_OK;
+  }
+  int *cnt = nullptr, *w = nullptr;
+  int rc;
+  if ((rc = dalloc(ctx, &cnt, nbr)) || (rc = dalloc(ctx, &w, nsl)) || (rc = dalloc(ctx, &B.d_soff, nsl + 1))) {
+    dfree(cnt);
+    dfree(w);
+    bsell_destroy(&B);
+    return rc;
+  }
+  const unsigned g = (unsigned)((nbr + 255) / 256);
+  switch (bs) {
+  case 3: hipLaunchKernelGGL(k_bs_count<3>, dim3(g), dim3(256), 0, st, (int)nbr, s->d_rp, s->d_ci, cnt); break;
+  case 2: hipLaunchKernelGGL(k_bs_count<2>, dim3(g), dim3(256), 0, st, (int)nbr, s->d_rp, s->d_ci, cnt); break;
+  default: hipLaunchKernelGGL(k_bs_count<1>, dim3(g), dim3(256), 0, st, (int)nbr, s->d_rp, s->d_ci, cnt); break;
+  }
+  hipLaunchKernelGGL(k_bs_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (int)nbr, (int)nsl, cnt, w);
+  int64_t nblk = 0;
+  rc = exclusive_scan_i32(ctx, w, B.d_soff, nsl, &nblk);
+  dfree(cnt);
+  dfree(w);
+  if (rc) {
+    bsell_destroy(&B);
+    return rc;
+  }
+  if (nblk * bs * bs >= INT32_MAX) {
+    bsell_destroy(&B);
+    B.state = -1;
+    return TFM_OK;
+  }
+  if ((rc = dalloc(ctx, &B.d_bcol, nblk)) || (rc = dalloc(ctx, &B.d_map, s->nnz))) {
+    bsell_destroy(&B);
+    return rc;
+  }
+  const unsigned g2 = (unsigned)((nsl * 64 + 255) / 256);
+  switch (bs) {
+  case 3:
+    hipLaunchKernelGGL(k_bs_fill<3>, dim3(g2), dim3(256), 0, st, (int)nbr, (int)nsl, s->d_rp, s->d_ci, B.d_soff,
+                       B.d_bcol, B.d_map);
+    break;
+  case 2:
+    hipLaunchKernelGGL(k_bs_fill<2>, dim3(g2), dim3(256), 0, st, (int)nbr, (int)nsl, s->d_rp, s->d_ci, B.d_soff,
+                       B.d_bcol, B.d_map);
+    break;
+  default:
+    hipLaunchKernelGGL(k_bs_fill<1>, dim3(g2), dim3(256), 0, st, (int)nbr, (int)nsl, s->d_rp, s->d_ci, B.d_soff,
+                       B.d_bcol, B.d_map);
+    break;
+  }
+  TFM_CHECK_HIP(ctx, hipGetLastError());
+  B.bs = bs;
+  B.nbr = (int)nbr;
+  B.nslices = (int)nsl;
+  B.nblk = nblk;
+  B.nnz = s->nnz;
+  B.state = 1;
+  return TFM_OK;
+}
+
+// Copies the system's current FP64 or FP32 values into the sliced copy
+// (entries absent from the CSR stay zero).
+template <typename VT>
+static int bsell_values(tfm_system *s, const VT *src, VT **dst) {
+  BSell &B = s->bsell;
+  tfm_ctx *ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t nv = B.nblk * B.bs * B.bs;
+  if (!*dst) {
+    int rc = dalloc(ctx, dst, nv);
+    if (rc) return rc;
+    TFM_CHECK_HIP(ctx, hipMemsetAsync(*dst, 0, sizeof(VT) * nv, st));
+  }
+  hipLaunchKernelGGL((k_bs_values<VT, VT>), dim3((unsigned)((B.nnz + 255) / 256)), dim3(256), 0, st, B.nnz, B.d_map,
+                     src, *dst);
+  TFM_CHECK_HIP(ctx, hipGetLastError());
+  return TFM_OK;
+}
+
+void bsell_destroy(BSell *b) {
+  dfree(b->d_soff);
+  dfree(b->d_bcol);
+  dfree(b->d_map);
+  dfree(b->d_v64);
+  dfree(b->d_v32);
+  *b = BSell{};
 }
 
 }  // namespace tfm
@@ -1007,11 +1306,30 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
     return fail(ctx, TFM_E_NUMERIC,
                 "sparse numeric factorization/solve failed: singular matrix (a free DOF has no positive diagonal)");
   }
+  // node-blocked sliced copy of the matrix for the iteration (structure once
+  // per system, values on every solve)
+  if ((rc = bsell_build(s))) {
+    cleanup();
+    return rc;
+  }
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev0, st));
+  const SpOp<double> K64 = csr_op<double>(n, s->d_rp, s->d_ci, s->d_v64, S);
+  SpOp<double> A64 = K64;
+  SpOp<float> A32 = csr_op<float>(n, s->d_rp, s->d_ci, s->d_v32, S);
+  if (s->bsell.state > 0) {
+    if (!mixed && !(rc = bsell_values<double>(s, s->d_v64, &s->bsell.d_v64)))
+      A64 = bsell_op<double>(n, s->bsell, s->bsell.d_v64);
+    if (mixed && !(rc = bsell_values<float>(s, s->d_v32, &s->bsell.d_v32)))
+      A32 = bsell_op<float>(n, s->bsell, s->bsell.d_v32);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
+  }
   int total_it = 0, outer = 0, brk = 0;
   bool pcg_conv = false;
   double relres = 0.0;
-  const int nbs = stream_blocks(n, S);
+  const int nbs = K64.nb;
   if (!mixed) {
     TFM_CHECK_HIP(ctx, hipMemcpyAsync(B64.r, s->d_f, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     // Restarted PCG: when the recurrence residual says converged but the true
@@ -1033,8 +1351,7 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
       } else {
         TFM_CHECK_HIP(ctx, hipMemcpyAsync(B64.r, s->d_f, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
       }
-      rc = pcg_run<double, double>(ctx, n, s->d_rp, s->d_ci, s->d_v64, B64, x, target, o.max_iter - total_it, S,
-                                   sc, part, &it, &brk, &rr);
+      rc = pcg_run<double, double>(ctx, A64, B64, x, target, o.max_iter - total_it, sc, part, &it, &brk, &rr);
       total_it += it;
       pcg_conv = rr <= target;
       if (brk || !pcg_conv || total_it >= o.max_iter) break;
@@ -1044,7 +1361,7 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
     // FP32 PCG with FP64 reliable-update refinement; inner_rtol is the
     // residual reduction between FP64 replacements (0 picks 0.1).
     const double delta = (o.inner_rtol > 0.0 && o.inner_rtol < 1.0) ? o.inner_rtol : 0.1;
-    rc = mixed_run(ctx, n, S, s->d_rp, s->d_ci, s->d_v64, s->d_v32, s->d_f, B32, B64.r, x32, x, o.rtol, delta,
+    rc = mixed_run(ctx, K64, A32, s->d_f, B32, B64.r, x32, x, o.rtol, delta,
                    o.max_iter, sc, part, &total_it, &outer, &brk, &relres);
   }
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, st));
@@ -1098,6 +1415,7 @@ extern "C" void tfm_system_destroy(tfm_system *s) {
   }
   dfree(s->d_v64);
   dfree(s->d_v32);
+  bsell_destroy(&s->bsell);
   dfree(s->d_f);
   delete s;
 }
