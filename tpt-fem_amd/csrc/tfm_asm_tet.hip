@@ -110,11 +110,72 @@ __device__ __forceinline__ void tet_rowblock(const double (&X)[4][3], int ln, do
   }
 }
 
+// One phase-2 item: row a of block blk of tile node t.
+struct TetItem {
+  unsigned mask, pm;
+  unsigned long long lmb;
+  int pos, a, kb;
+};
+
+template <int DPN>
+__device__ __forceinline__ TetItem tet_item(int it, int nt, int b0, const int *s_bptr, const int *s_nptr,
+                                            const unsigned *__restrict__ bmask,
+                                            const unsigned long long *__restrict__ blmb,
+                                            const int *__restrict__ ipos, const unsigned char *__restrict__ ipm) {
+  TetItem I;
+  int t = 0;
+  while (t + 1 < nt && DPN * (s_bptr[t + 1] - b0) <= it) t++;
+  const int nb = s_bptr[t + 1] - s_bptr[t];
+  const int loc = it - DPN * (s_bptr[t] - b0);
+  I.a = loc / nb;
+  const int blk = s_bptr[t] + (loc - I.a * nb);
+  const int64_t gi = (int64_t)DPN * b0 + it;
+  I.mask = bmask[blk];
+  I.lmb = blmb[blk];
+  I.pos = ipos[gi];
+  I.pm = ipm[gi];
+  I.kb = s_nptr[t];
+  return I;
+}
+
+// Sums the block's contributions for row a in ascending pair (= element)
+// order.  LDS reads are issued four contributions at a time; missing ones
+// read as +0.0, which leaves the sum unchanged (it is never -0.0: it starts
+// at +0.0 and exact cancellation rounds to +0.0).
+template <int DPN, typename VT>
+__device__ __forceinline__ void tet_sum(const VT *s_c, const TetItem &I, VT (&acc)[DPN]) {
+#pragma unroll
+  for (int b = 0; b < DPN; b++) acc[b] = 0;
+  unsigned mask = I.mask;
+  while (mask) {
+    VT c[4][DPN];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (mask) {
+        const int j = __ffs((int)mask) - 1;
+        mask &= mask - 1;
+        const int lm = (int)((I.lmb >> (2 * j)) & 3ull);
+        const VT *src = s_c + ((lm * DPN + I.a) * DPN) * kTetTP + I.kb + j;
+#pragma unroll
+        for (int b = 0; b < DPN; b++) c[u][b] = src[b * kTetTP];
+      } else {
+#pragma unroll
+        for (int b = 0; b < DPN; b++) c[u][b] = 0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int b = 0; b < DPN; b++) acc[b] += c[u][b];
+  }
+}
+
 template <int DPN, typename VT>
 __global__ __launch_bounds__(kTetTP) void k_asm_tet(const double *__restrict__ coords, const int *__restrict__ conn,
                                                      const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
-                                                     const int *__restrict__ tiles, int ntiles, OpParams P,
-                                                     const int *__restrict__ bptr, const unsigned *__restrict__ bmask,
+                                                     const int *__restrict__ tiles, const int *__restrict__ tpa,
+                                                     int ntiles, OpParams P, const int *__restrict__ bptr,
+                                                     const unsigned *__restrict__ bmask,
                                                      const unsigned long long *__restrict__ blmb,
                                                      const int *__restrict__ ipos,
                                                      const unsigned char *__restrict__ ipm, VT *__restrict__ vals) {
@@ -127,61 +188,50 @@ __global__ __launch_bounds__(kTetTP) void k_asm_tet(const double *__restrict__ c
   const int tile = xcd_item(blockIdx.x, ntiles);  // neighbouring tiles share an XCD's L2
   if (tile >= ntiles) return;
   const int n0 = tiles[tile], n1 = tiles[tile + 1];
+  const int pa = tpa[tile], np = tpa[tile + 1] - pa;
   const int nt = n1 - n0;
-  const int pa = n2e_ptr[n0];
-  const int np = n2e_ptr[n1] - pa;
   if (tid <= nt) {
     s_nptr[tid] = n2e_ptr[n0 + tid] - pa;
     s_bptr[tid] = bptr[n0 + tid];
   }
 
-  // ---- phase 1: one (node, element) pair per lane -------------------------
-  if (tid < np) {
-    const int v = n2e[pa + tid];
-    const int e = v >> 3, ln = v & 7;
-    double X[4][3];
+  // ---- phase 1 loads: one (node, element) pair per lane --------------------
+  // (unconditional: inactive lanes read pair 0 of the tile, or of the mesh
+  // when the tile's nodes have no elements)
+  const bool pair = tid < np;
+  const int v = n2e[np > 0 ? pa + (pair ? tid : 0) : 0];
+  const int e = v >> 3, ln = v & 7;
+  double X[4][3];
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const int nd = conn[(int64_t)e * 4 + a];
+  for (int a = 0; a < 4; a++) {
+    const int nd = conn[(int64_t)e * 4 + a];
 #pragma unroll
-      for (int p = 0; p < 3; p++) X[a][p] = coords[(int64_t)nd * 3 + p];
-    }
-    const double coef = P.coef_e ? P.coef_e[e] : P.coef;
-    tet_rowblock<DPN, VT>(X, ln, coef, P, s_c + tid, kTetTP);
+    for (int p = 0; p < 3; p++) X[a][p] = coords[(int64_t)nd * 3 + p];
   }
+  const double coef = P.coef_e ? P.coef_e[e] : P.coef;
+  __syncthreads();  // s_nptr / s_bptr (waits on LDS only: the loads stay in flight)
+
+  // first phase-2 item's plan entries, fetched while the element math runs
+  const int b0 = s_bptr[0];
+  const int nitems = DPN * (s_bptr[nt] - b0);
+  TetItem cur{};
+  if (tid < nitems) cur = tet_item<DPN>(tid, nt, b0, s_bptr, s_nptr, bmask, blmb, ipos, ipm);
+
+  // ---- phase 1 math: the pair's row block into LDS -------------------------
+  if (pair) tet_rowblock<DPN, VT>(X, ln, coef, P, s_c + tid, kTetTP);
   __syncthreads();
 
   // ---- phase 2: one (row, block) item per lane ----------------------------
-  const int b0 = s_bptr[0];
-  const int nitems = DPN * (s_bptr[nt] - b0);
   for (int it = tid; it < nitems; it += kTetTP) {
-    int t = 0;
-    while (t + 1 < nt && DPN * (s_bptr[t + 1] - b0) <= it) t++;
-    const int nb = s_bptr[t + 1] - s_bptr[t];
-    const int loc = it - DPN * (s_bptr[t] - b0);
-    const int a = loc / nb, k = loc - a * nb;
-    const int blk = s_bptr[t] + k;
-    const int64_t gi = (int64_t)DPN * b0 + it;
-    unsigned mask = bmask[blk];
-    const unsigned long long lmb = blmb[blk];
-    const int pos = ipos[gi];
-    const unsigned pm = ipm[gi];
-    const int kb = s_nptr[t];
+    TetItem nxt{};
+    if (it + kTetTP < nitems) nxt = tet_item<DPN>(it + kTetTP, nt, b0, s_bptr, s_nptr, bmask, blmb, ipos, ipm);
     VT acc[DPN];
-#pragma unroll
-    for (int b = 0; b < DPN; b++) acc[b] = 0;
-    while (mask) {
-      const int j = __ffs((int)mask) - 1;
-      mask &= mask - 1;
-      const int lm = (int)((lmb >> (2 * j)) & 3ull);
-      const VT *c = s_c + ((lm * DPN + a) * DPN) * kTetTP + kb + j;
-#pragma unroll
-      for (int b = 0; b < DPN; b++) acc[b] += c[b * kTetTP];
-    }
-    int q = pos;
+    tet_sum<DPN, VT>(s_c, cur, acc);
+    int q = cur.pos;
 #pragma unroll
     for (int b = 0; b < DPN; b++)
-      if ((pm >> b) & 1u) vals[q++] = acc[b];
+      if ((cur.pm >> b) & 1u) vals[q++] = acc[b];
+    cur = nxt;
   }
 }
 
@@ -278,9 +328,12 @@ static int tet_tiles(tfm_ctx *ctx, const tfm_mesh *mesh, TetPlan *tp) {
   }
   if (N > 0) t.push_back((int)N);
   tp->ntiles = (int)t.size() - 1;
-  int rc = dalloc(ctx, &tp->d_tiles, t.size());
-  if (rc) return rc;
+  std::vector<int> pa(t.size());
+  for (size_t i = 0; i < t.size(); i++) pa[i] = p[t[i]];
+  int rc;
+  if ((rc = dalloc(ctx, &tp->d_tiles, t.size())) || (rc = dalloc(ctx, &tp->d_tpa, t.size()))) return rc;
   TFM_CHECK_HIP(ctx, hipMemcpy(tp->d_tiles, t.data(), sizeof(int) * t.size(), hipMemcpyHostToDevice));
+  TFM_CHECK_HIP(ctx, hipMemcpy(tp->d_tpa, pa.data(), sizeof(int) * t.size(), hipMemcpyHostToDevice));
   return TFM_OK;
 }
 
@@ -334,6 +387,7 @@ int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp
 
 void tet_plan_destroy(TetPlan *tp) {
   dfree(tp->d_tiles);
+  dfree(tp->d_tpa);
   dfree(tp->d_bptr);
   dfree(tp->d_bmask);
   dfree(tp->d_blmb);
@@ -349,7 +403,7 @@ int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const TetPlan &tp, 
   dim3 g(xcd_grid(tp.ntiles)), t(kTetTP);
 #define TFM_TET(D, VT, V)                                                                                           \
   hipLaunchKernelGGL((k_asm_tet<D, VT>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e_ptr, mesh->d_n2e, \
-                     tp.d_tiles, tp.ntiles, P, tp.d_bptr, tp.d_bmask, tp.d_blmb, tp.d_ipos, tp.d_ipm, V)
+                     tp.d_tiles, tp.d_tpa, tp.ntiles, P, tp.d_bptr, tp.d_bmask, tp.d_blmb, tp.d_ipos, tp.d_ipm, V)
   if (dpn == 1) {
     if (v64) TFM_TET(1, double, v64);
     if (v32) TFM_TET(1, float, v32);

@@ -566,7 +566,7 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
   pl->ncols = nnz ? hd[1] : 1;
   // P1-tet Poisson / 3-D elasticity: the balanced fast path, when every node
   // fits its tile limits (otherwise the generic kernel keeps the job).
-  const bool tet_op = mesh->cell == TFM_TET && ((op->kind == TFM_OP_POISSON && dpn == 1) ||
+  const bool tet_op = mesh->cell == TFM_TET && mesh->n_elem > 0 && ((op->kind == TFM_OP_POISSON && dpn == 1) ||
                                                 (op->kind == TFM_OP_ELASTICITY && dpn == 3));
   if (tet_op && mesh->max_adj <= 32 && !getenv("TFM_GENERIC_ASSEMBLY")) {
     if (tet_plan_create(ctx, mesh, dpn, pl->d_rp, pl->d_ci, &pl->tet) == TFM_OK) {

@@ -101,6 +101,7 @@ int exclusive_scan_i32(tfm_ctx *ctx, const int *d_in, int *d_out, int64_t n, int
 
 // P1-tet assembly plan (tfm_asm_tet.hip), built once per pattern:
 //   tiles  int32 [ntiles + 1]          first node of each tile
+//   tpa    int32 [ntiles + 1]          first (node, element) pair of each tile
 //   bptr   int32 [n_nodes + 1]         first row block of each node
 //   bmask  u32   [nblocks]             contributing (node, element) pairs of a block
 //   blmb   u64   [nblocks]             2-bit local index of the column node per pair
@@ -109,6 +110,7 @@ int exclusive_scan_i32(tfm_ctx *ctx, const int *d_in, int *d_out, int64_t n, int
 //   ipm    u8    [nblocks * dpn]       presence mask of the block's dpn columns
 struct TetPlan {
   int *d_tiles = nullptr;
+  int *d_tpa = nullptr;
   int ntiles = 0;
   int *d_bptr = nullptr;
   unsigned *d_bmask = nullptr;
