@@ -53,100 +53,136 @@ __device__ __forceinline__ float bdb3f(const float *gi, const float *gj, int a, 
   }
 }
 
-// Row block of element-local node ln (4 x DPN x DPN entries) into
-// out[((lm * DPN + a) * DPN + b) * stride].
+// Element quantities of one (node, element) pair, kept in registers while
+// its row block is emitted one DOF row at a time.
 template <int DPN, typename VT>
-__device__ __forceinline__ void tet_rowblock(const double (&X)[4][3], int ln, double coef, const OpParams &P,
-                                             VT *out, int stride) {
-  double g[4][3];
-  const double det = fabs(map_tet(X, g));
+struct TetPair {
+  double g[4][3];  // physical gradients (FP64 path)
+  double gl[3];    // the pair's own node's gradient, g[ln]
+  float gf[4][3];  // the same in FP32 (precision form)
+  float gfl[3];
+  double k0, k1;   // FP64 Poisson: w * coef, |det|; FP64 elasticity: w * |det|
+  DConst D;        // FP64 elasticity
+  float f0, m, cnu, dd;  // FP32: vol (* coef for Poisson) and the D entries
+};
+
+template <int DPN, typename VT>
+__device__ __forceinline__ void tet_setup(const double (&X)[4][3], int ln, double coef, const OpParams &P,
+                                          TetPair<DPN, VT> &pr) {
+  const double det = fabs(map_tet(X, pr.g));
+  // g[ln] by selects (a dynamic index would put the array in scratch memory)
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    pr.gl[p] = pr.g[0][p];
+#pragma unroll
+    for (int i = 1; i < 4; i++)
+      if (ln == i) pr.gl[p] = pr.g[i][p];
+  }
   const double w = 1.0 / 24.0;
   if constexpr (sizeof(VT) == 8) {
     if constexpr (DPN == 1) {
-      const double wk = w * coef;
-#pragma unroll
-      for (int lm = 0; lm < 4; lm++) {
-        const double dot = g[ln][0] * g[lm][0] + g[ln][1] * g[lm][1] + g[ln][2] * g[lm][2];
-        const double c = wk * dot * det;
-        out[lm * stride] = c + c + c + c;
-      }
+      pr.k0 = w * coef;
+      pr.k1 = det;
     } else {
-      const DConst D = make_d(TFM_CONTINUUM3D, coef, P.nu);
-      const double wd = w * det;
-#pragma unroll
-      for (int lm = 0; lm < 4; lm++)
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-          for (int b = 0; b < 3; b++) {
-            const double c = wd * bdb3(g[ln], g[lm], a, b, D);
-            out[((lm * 3 + a) * 3 + b) * stride] = c + c + c + c;
-          }
+      pr.D = make_d(TFM_CONTINUUM3D, coef, P.nu);
+      pr.k0 = w * det;
     }
   } else {
-    float gf[4][3];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-      for (int p = 0; p < 3; p++) gf[i][p] = (float)g[i][p];
+      for (int p = 0; p < 3; p++) pr.gf[i][p] = (float)pr.g[i][p];
+#pragma unroll
+    for (int p = 0; p < 3; p++) pr.gfl[p] = (float)pr.gl[p];
     const double vol = det * (1.0 / 6.0);  // the four equal-weight points
     if constexpr (DPN == 1) {
-      const float vk = (float)(vol * coef);
-#pragma unroll
-      for (int lm = 0; lm < 4; lm++)
-        out[lm * stride] = vk * (gf[ln][0] * gf[lm][0] + gf[ln][1] * gf[lm][1] + gf[ln][2] * gf[lm][2]);
+      pr.f0 = (float)(vol * coef);
     } else {
       const double c = coef / ((1.0 + P.nu) * (1.0 - 2.0 * P.nu));
-      const float vf = (float)vol, m = (float)((1.0 - P.nu) * c), cnu = (float)(c * P.nu),
-                  dd = (float)((1.0 - 2.0 * P.nu) / 2.0 * c);
-#pragma unroll
-      for (int lm = 0; lm < 4; lm++)
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-          for (int b = 0; b < 3; b++)
-            out[((lm * 3 + a) * 3 + b) * stride] = vf * bdb3f(gf[ln], gf[lm], a, b, m, cnu, dd);
+      pr.f0 = (float)vol;
+      pr.m = (float)((1.0 - P.nu) * c);
+      pr.cnu = (float)(c * P.nu);
+      pr.dd = (float)((1.0 - 2.0 * P.nu) / 2.0 * c);
     }
   }
 }
 
-// One phase-2 item: row a of block blk of tile node t.
-struct TetItem {
-  unsigned mask, pm;
+// Row a of element-local node ln's row block: K_e[ln*DPN + a][lm*DPN + b]
+// into out[(lm * DPN + b) * stride].
+template <int DPN, typename VT>
+__device__ __forceinline__ void tet_row(const TetPair<DPN, VT> &pr, int a, VT *out, int stride) {
+  if constexpr (sizeof(VT) == 8) {
+    if constexpr (DPN == 1) {
+#pragma unroll
+      for (int lm = 0; lm < 4; lm++) {
+        const double dot = pr.gl[0] * pr.g[lm][0] + pr.gl[1] * pr.g[lm][1] + pr.gl[2] * pr.g[lm][2];
+        const double c = pr.k0 * dot * pr.k1;
+        out[lm * stride] = c + c + c + c;
+      }
+    } else {
+#pragma unroll
+      for (int lm = 0; lm < 4; lm++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          const double c = pr.k0 * bdb3(pr.gl, pr.g[lm], a, b, pr.D);
+          out[(lm * 3 + b) * stride] = c + c + c + c;
+        }
+    }
+  } else {
+    if constexpr (DPN == 1) {
+#pragma unroll
+      for (int lm = 0; lm < 4; lm++)
+        out[lm * stride] =
+            pr.f0 * (pr.gfl[0] * pr.gf[lm][0] + pr.gfl[1] * pr.gf[lm][1] + pr.gfl[2] * pr.gf[lm][2]);
+    } else {
+#pragma unroll
+      for (int lm = 0; lm < 4; lm++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+          out[(lm * 3 + b) * stride] = pr.f0 * bdb3f(pr.gfl, pr.gf[lm], a, b, pr.m, pr.cnu, pr.dd);
+    }
+  }
+}
+
+// Plan entries of one row block of a tile, for all DPN rows.
+struct TetBlk {
+  unsigned mask;
   unsigned long long lmb;
-  int pos, a, kb;
+  int kb;             // first pair of the block's node within the tile
+  int pos[3];         // per row: CSR position of the first present entry
+  unsigned pm[3];     // per row: presence mask of the DPN columns
 };
 
 template <int DPN>
-__device__ __forceinline__ TetItem tet_item(int it, int nt, int b0, const int *s_bptr, const int *s_nptr,
-                                            const unsigned *__restrict__ bmask,
-                                            const unsigned long long *__restrict__ blmb,
-                                            const int *__restrict__ ipos, const unsigned char *__restrict__ ipm) {
-  TetItem I;
+__device__ __forceinline__ TetBlk tet_blk(int blk, int nt, const int *s_bptr, const int *s_nptr,
+                                          const unsigned *__restrict__ bmask,
+                                          const unsigned long long *__restrict__ blmb,
+                                          const int *__restrict__ ipos, const unsigned char *__restrict__ ipm) {
+  TetBlk B;
   int t = 0;
-  while (t + 1 < nt && DPN * (s_bptr[t + 1] - b0) <= it) t++;
-  const int nb = s_bptr[t + 1] - s_bptr[t];
-  const int loc = it - DPN * (s_bptr[t] - b0);
-  I.a = loc / nb;
-  const int blk = s_bptr[t] + (loc - I.a * nb);
-  const int64_t gi = (int64_t)DPN * b0 + it;
-  I.mask = bmask[blk];
-  I.lmb = blmb[blk];
-  I.pos = ipos[gi];
-  I.pm = ipm[gi];
-  I.kb = s_nptr[t];
-  return I;
+  while (t + 1 < nt && s_bptr[t + 1] <= blk) t++;
+  const int nb = s_bptr[t + 1] - s_bptr[t], k = blk - s_bptr[t];
+  B.mask = bmask[blk];
+  B.lmb = blmb[blk];
+  B.kb = s_nptr[t];
+#pragma unroll
+  for (int a = 0; a < DPN; a++) {
+    const int64_t gi = (int64_t)DPN * s_bptr[t] + a * nb + k;
+    B.pos[a] = ipos[gi];
+    B.pm[a] = ipm[gi];
+  }
+  return B;
 }
 
-// Sums the block's contributions for row a in ascending pair (= element)
-// order.  LDS reads are issued four contributions at a time; missing ones
-// read as +0.0, which leaves the sum unchanged (it is never -0.0: it starts
-// at +0.0 and exact cancellation rounds to +0.0).
+// Sums one row of a block from the pairs' rows in LDS, in ascending pair (=
+// element) order.  LDS reads are issued four contributions at a time; missing
+// ones read as +0.0, which leaves the sum unchanged (it is never -0.0: it
+// starts at +0.0 and exact cancellation rounds to +0.0).
 template <int DPN, typename VT>
-__device__ __forceinline__ void tet_sum(const VT *s_c, const TetItem &I, VT (&acc)[DPN]) {
+__device__ __forceinline__ void tet_sum(const VT *s_c, const TetBlk &B, VT (&acc)[DPN]) {
 #pragma unroll
   for (int b = 0; b < DPN; b++) acc[b] = 0;
-  unsigned mask = I.mask;
+  unsigned mask = B.mask;
   while (mask) {
     VT c[4][DPN];
 #pragma unroll
@@ -154,8 +190,8 @@ __device__ __forceinline__ void tet_sum(const VT *s_c, const TetItem &I, VT (&ac
       if (mask) {
         const int j = __ffs((int)mask) - 1;
         mask &= mask - 1;
-        const int lm = (int)((I.lmb >> (2 * j)) & 3ull);
-        const VT *src = s_c + ((lm * DPN + I.a) * DPN) * kTetTP + I.kb + j;
+        const int lm = (int)((B.lmb >> (2 * j)) & 3ull);
+        const VT *src = s_c + lm * DPN * kTetTP + B.kb + j;
 #pragma unroll
         for (int b = 0; b < DPN; b++) c[u][b] = src[b * kTetTP];
       } else {
@@ -171,6 +207,21 @@ __device__ __forceinline__ void tet_sum(const VT *s_c, const TetItem &I, VT (&ac
 }
 
 template <int DPN, typename VT>
+__device__ __forceinline__ void tet_write(const TetBlk &B, int a, const VT (&acc)[DPN], VT *__restrict__ vals) {
+  int q = B.pos[0];
+  unsigned pm = B.pm[0];
+#pragma unroll
+  for (int i = 1; i < DPN; i++)  // selects: a dynamic index would use scratch memory
+    if (a == i) {
+      q = B.pos[i];
+      pm = B.pm[i];
+    }
+#pragma unroll
+  for (int b = 0; b < DPN; b++)
+    if ((pm >> b) & 1u) vals[q++] = acc[b];
+}
+
+template <int DPN, typename VT>
 __global__ __launch_bounds__(kTetTP) void k_asm_tet(const double *__restrict__ coords, const int *__restrict__ conn,
                                                      const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
                                                      const int *__restrict__ tiles, const int *__restrict__ tpa,
@@ -179,8 +230,7 @@ __global__ __launch_bounds__(kTetTP) void k_asm_tet(const double *__restrict__ c
                                                      const unsigned long long *__restrict__ blmb,
                                                      const int *__restrict__ ipos,
                                                      const unsigned char *__restrict__ ipm, VT *__restrict__ vals) {
-  constexpr int NS = 4 * DPN * DPN;  // entries of one pair's row block
-  __shared__ VT s_c[NS * kTetTP];    // slot-major: s_c[slot * kTetTP + pair]
+  __shared__ VT s_c[4 * DPN * kTetTP];  // one DOF row of every pair: s_c[(lm * DPN + b) * kTetTP + pair]
   __shared__ int s_nptr[kTetNodes + 1];
   __shared__ int s_bptr[kTetNodes + 1];
 
@@ -195,9 +245,9 @@ __global__ __launch_bounds__(kTetTP) void k_asm_tet(const double *__restrict__ c
     s_bptr[tid] = bptr[n0 + tid];
   }
 
-  // ---- phase 1 loads: one (node, element) pair per lane --------------------
-  // (unconditional: inactive lanes read pair 0 of the tile, or of the mesh
-  // when the tile's nodes have no elements)
+  // element loads, one (node, element) pair per lane (unconditional: inactive
+  // lanes read pair 0 of the tile, or of the mesh when the tile's nodes have
+  // no elements)
   const bool pair = tid < np;
   const int v = n2e[np > 0 ? pa + (pair ? tid : 0) : 0];
   const int e = v >> 3, ln = v & 7;
@@ -211,27 +261,30 @@ __global__ __launch_bounds__(kTetTP) void k_asm_tet(const double *__restrict__ c
   const double coef = P.coef_e ? P.coef_e[e] : P.coef;
   __syncthreads();  // s_nptr / s_bptr (waits on LDS only: the loads stay in flight)
 
-  // first phase-2 item's plan entries, fetched while the element math runs
+  // the first round's plan entries, fetched while the element math runs
   const int b0 = s_bptr[0];
-  const int nitems = DPN * (s_bptr[nt] - b0);
-  TetItem cur{};
-  if (tid < nitems) cur = tet_item<DPN>(tid, nt, b0, s_bptr, s_nptr, bmask, blmb, ipos, ipm);
+  const int nblk = s_bptr[nt] - b0;
+  TetBlk B0{};
+  if (tid < nblk) B0 = tet_blk<DPN>(b0 + tid, nt, s_bptr, s_nptr, bmask, blmb, ipos, ipm);
 
-  // ---- phase 1 math: the pair's row block into LDS -------------------------
-  if (pair) tet_rowblock<DPN, VT>(X, ln, coef, P, s_c + tid, kTetTP);
-  __syncthreads();
+  TetPair<DPN, VT> pr;
+  tet_setup<DPN, VT>(X, ln, coef, P, pr);
 
-  // ---- phase 2: one (row, block) item per lane ----------------------------
-  for (int it = tid; it < nitems; it += kTetTP) {
-    TetItem nxt{};
-    if (it + kTetTP < nitems) nxt = tet_item<DPN>(it + kTetTP, nt, b0, s_bptr, s_nptr, bmask, blmb, ipos, ipm);
-    VT acc[DPN];
-    tet_sum<DPN, VT>(s_c, cur, acc);
-    int q = cur.pos;
 #pragma unroll
-    for (int b = 0; b < DPN; b++)
-      if ((cur.pm >> b) & 1u) vals[q++] = acc[b];
-    cur = nxt;
+  for (int a = 0; a < DPN; a++) {
+    if (a > 0) __syncthreads();  // the previous row's sums are done with s_c
+    if (pair) tet_row<DPN, VT>(pr, a, s_c + tid, kTetTP);
+    __syncthreads();
+    VT acc[DPN];
+    if (tid < nblk) {
+      tet_sum<DPN, VT>(s_c, B0, acc);
+      tet_write<DPN, VT>(B0, a, acc, vals);
+    }
+    for (int it = tid + kTetTP; it < nblk; it += kTetTP) {
+      const TetBlk B = tet_blk<DPN>(b0 + it, nt, s_bptr, s_nptr, bmask, blmb, ipos, ipm);
+      tet_sum<DPN, VT>(s_c, B, acc);
+      tet_write<DPN, VT>(B, a, acc, vals);
+    }
   }
 }
 
