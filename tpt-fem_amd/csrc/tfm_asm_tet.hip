@@ -175,7 +175,7 @@ __device__ __forceinline__ TetBlk tet_blk(int blk, int nt, const int *s_bptr, co
 }
 
 // Sums one row of a block from the pairs' rows in LDS, in ascending pair (=
-// element) order.  LDS reads are issued four contributions at a time; missing
+// element) order.  LDS reads are issued two contributions at a time; missing
 // ones read as +0.0, which leaves the sum unchanged (it is never -0.0: it
 // starts at +0.0 and exact cancellation rounds to +0.0).
 template <int DPN, typename VT>
@@ -184,9 +184,9 @@ __device__ __forceinline__ void tet_sum(const VT *s_c, const TetBlk &B, VT (&acc
   for (int b = 0; b < DPN; b++) acc[b] = 0;
   unsigned mask = B.mask;
   while (mask) {
-    VT c[4][DPN];
+    VT c[2][DPN];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < 2; u++) {
       if (mask) {
         const int j = __ffs((int)mask) - 1;
         mask &= mask - 1;
@@ -200,7 +200,7 @@ __device__ __forceinline__ void tet_sum(const VT *s_c, const TetBlk &B, VT (&acc
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < 2; u++)
 #pragma unroll
       for (int b = 0; b < DPN; b++) acc[b] += c[u][b];
   }
@@ -222,7 +222,7 @@ __device__ __forceinline__ void tet_write(const TetBlk &B, int a, const VT (&acc
 }
 
 template <int DPN, typename VT>
-__global__ __launch_bounds__(kTetTP) void k_asm_tet(const double *__restrict__ coords, const int *__restrict__ conn,
+__global__ __launch_bounds__(kTetTP, 5) void k_asm_tet(const double *__restrict__ coords, const int *__restrict__ conn,
                                                      const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
                                                      const int *__restrict__ tiles, const int *__restrict__ tpa,
                                                      int ntiles, OpParams P, const int *__restrict__ bptr,
