@@ -88,6 +88,84 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_per_launch": 1 << 30, "launches": 5}), flush=True)
 
 
+def box_face_nodes(n):
+    """Node ids of box_mesh(n) on any of its six faces (node (i, j, k) is
+    (i*(ny+1) + j)*(nz+1) + k)."""
+    nx, ny, nz = n
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    on = (i == 0) | (i == nx) | (j == 0) | (j == ny) | (k == 0) | (k == nz)
+    return ((i * (ny + 1) + j) * (nz + 1) + k)[on]
+
+
+def bench_batch(args, T, log):
+    """The batching forms: 256 uneven Poisson boxes (kappa_e = exp(N(0,1)),
+    f = 1, u = 0 on the faces), or 64 SIMP Q4 cantilever designs x 8 point
+    loads.  One assembly pass fills the block-diagonal system; one segmented
+    PCG solves every problem (and right-hand side) together."""
+    ctx = T.Context(0)
+    rng = np.random.default_rng(2024)
+    t0 = time.perf_counter()
+    if args.workload == "c4":
+        dims = [tuple(int(v) for v in d) for d in rng.integers(8, 49, size=(256, 3))]
+        mesh, noff, eoff = T.Mesh.box_batch(ctx, dims)
+        kappa = np.exp(rng.standard_normal(mesh.n_elem))
+        ids = np.concatenate([noff[p] + box_face_nodes(d) for p, d in enumerate(dims)])
+        bcs = np.stack([ids.astype(np.float64), np.zeros(len(ids))], axis=1)
+        rhs = T.source_vector(mesh, 1.0)
+        op = T.Operator.poisson(1.0, 2, kappa_e=kappa)
+        dpn, nrhs, off, f = 1, 1, noff, None
+        desc = (f"256 Poisson boxes, n per axis U{{8..48}}, {mesh.n_elem} tets, kappa_e = exp(N(0,1)), "
+                "f = 1, u = 0 on faces")
+    else:
+        P, nx, ny = 64, 257, 129
+        mesh = T.Mesh.grid_batch(ctx, P, nx, ny, 1.0)
+        rho = rng.uniform(0.01, 1.0, mesh.n_elem)
+        nd1 = 2 * nx * ny
+        left = (np.arange(P)[:, None] * nx * ny + np.arange(ny)[None, :] * nx).ravel()
+        ids = np.concatenate([2 * left, 2 * left + 1])
+        bcs = np.stack([ids.astype(np.float64), np.zeros(len(ids))], axis=1)
+        f = np.zeros((8, P * nd1))
+        rows = rng.integers(0, ny, size=(P, 8))
+        for p in range(P):
+            for k in range(8):
+                f[k, 2 * (p * nx * ny + rows[p, k] * nx + nx - 1) + 1] = -1.0
+        op = T.Operator.simp(rho, nu=0.3, penal=3.0, e0=1.0, emin=1e-9, h=1.0)
+        dpn, nrhs, off, rhs = 2, 8, np.arange(P + 1) * nd1, None
+        desc = f"64 SIMP Q4 designs 257x129 nodes x 8 point loads, rho U[0.01,1], p 3, {mesh.n_elem} quads"
+    s = T.System.create(mesh, op, dpn, bcs, rhs, flags=T.SYS_FP64 | T.SYS_FP32)
+    ctx.synchronize()
+    log(f"{args.workload} setup {time.perf_counter() - t0:.2f}s: {desc}; n_free {s.n_free}, nnz {s.nnz}")
+    times = {}
+    for which, name in ((T.SYS_FP64, "fp64"), (T.SYS_FP32, "fp32")):
+        for _ in range(args.warmup):
+            s.assemble(None, which)
+        ts = []
+        for _ in range(args.steps):
+            ctx.timer_begin()
+            s.assemble(None, which)
+            ts.append(ctx.timer_end())
+        times[name] = float(np.mean(ts))
+    log(f"assembly fp64 {times['fp64']:.3f} ms, fp32 {times['fp32']:.3f} ms")
+    s.assemble(None, 0)
+    solves = {}
+    for mode, name, rtol in ((T.SOLVE_PCG64, "plain_fp64", 1e-12), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", 1e-10)):
+        u = s.solve_batch(off, rhs=f, nrhs=nrhs, mode=mode, rtol=rtol, max_iter=200000, max_outer=400)
+        st = s.batch_stats
+        rec = {"rtol": rtol, "s": st[0][3], "max_iters": max(x[0] for x in st),
+               "max_rel_res": max(x[2] for x in st)}
+        if args.workload == "c5":
+            rec["compliance_sum"] = float(sum(f[k] @ u[k] for k in range(nrhs)))
+        solves[name] = rec
+        log(f"{name}: {rec}")
+    line = {"metric": "batched assembly + solve", "workload": args.workload, "config": {"workload": desc},
+            "n_elem": mesh.n_elem, "n_free": s.n_free, "nnz": s.nnz,
+            "assembly_ms": times, "elements_per_s_fp64": mesh.n_elem / (times["fp64"] * 1e-3), "solve": solves}
+    print(json.dumps(line), flush=True)
+    s.close()
+    mesh.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +181,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=32, help="cells per axis of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve", action="store_true")
+    ap.add_argument("--workload", choices=["c3", "c4", "c5"], default="c3",
+                    help="c3: the headline single problem; c4 / c5: the batching forms (own JSON line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run five 1 GiB device-to-device copies (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -118,6 +198,9 @@ def main():
 
     n = args.n
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.workload != "c3":
+        bench_batch(args, T, log)
+        return
     ctx = T.Context(0)
     t_setup = time.perf_counter()
     mesh = T.Mesh.box(ctx, [0.0, 0.0, 0.0], [1.0, 1.0, 1.0], (n, n, n))

@@ -360,6 +360,12 @@ def plane_traction(mesh, dofs_per_node, axis, coord, tol, t, rhs=None):
 
 # ---- reduced systems ----------------------------------------------------------
 def _bc_arrays(bcs):
+    """Dirichlet list as (dof, value) pairs, or a (k, 2) array of them."""
+    if isinstance(bcs, np.ndarray):
+        if bcs.size == 0:
+            return 0, np.zeros(1, np.int64), np.zeros(1)
+        b = bcs.reshape(-1, 2)
+        return len(b), _i64(b[:, 0]), _f64(b[:, 1])
     if not bcs:
         return 0, np.zeros(1, np.int64), np.zeros(1)
     return len(bcs), _i64([b[0] for b in bcs]), _f64([b[1] for b in bcs])
