@@ -174,9 +174,9 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="cells per axis")
     ap.add_argument("--err-target", type=float, default=1e-8,
                     help="relative error vs the FP64 solution (rtol 1e-12) that the timed solves must reach")
-    ap.add_argument("--rtol-sweep", type=float, nargs="+", default=[1e-7, 3e-8, 1e-8, 3e-9, 1e-9, 3e-10, 1e-10],
+    ap.add_argument("--rtol-sweep", type=float, nargs="+", default=[4e-7, 2e-7, 1e-7, 7e-8, 5e-8, 3e-8, 2e-8, 1e-8, 5e-9, 2e-9, 1e-9],
                     help="residual tolerances tried, loosest first, until the error target is met")
-    ap.add_argument("--inner-rtol", type=float, default=0.1,
+    ap.add_argument("--inner-rtol", type=float, default=0.3,
                     help="FP32 residual reduction between FP64 residual replacements")
     ap.add_argument("--cpu-n", type=int, default=32, help="cells per axis of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")

@@ -712,6 +712,9 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
     TFM_CHECK_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
   int rc = TFM_OK;
+  constexpr int kStall = 64;  // chunks of 16 iterations
+  double best = rr_ref;
+  int since = 0;
   for (;;) {
     if (!use_graph) {
       chunk();
@@ -726,7 +729,17 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
       *breakdown = 1;
       break;
     }
-    if (h.rr <= delta * delta * rr_ref || h.done) {
+    // FP32 stagnation guard: when the cycle's smallest recurrence residual
+    // has not improved by 10% for kStall chunks, replace anyway (an FP32
+    // iteration that stalls above delta would otherwise run to max_iter).
+    if (h.rr < 0.81 * best) {
+      best = h.rr;
+      since = 0;
+    } else {
+      since++;
+    }
+    const bool stalled = since >= kStall;
+    if (h.rr <= delta * delta * rr_ref || h.done || stalled) {
       hipLaunchKernelGGL(k_fold, dim3(g1), dim3(256), 0, st, n, x, x32);
       residual_dispatch(S, st, nbr, n, rp, ci, v64, x, f, r64, B.r, sc, part);
       Scal h2{};
@@ -737,6 +750,8 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
       if (*relres <= rtol || h.done || h.it >= max_iter) break;
       hipLaunchKernelGGL((k_pcg_replace<float>), dim3(nb), dim3(kBlock), 0, st, n, B.r, B.z, B.dinv, part);
       rr_ref = h2.rr;
+      best = rr_ref;
+      since = 0;
     }
   }
   if (exec) (void)hipGraphExecDestroy(exec);
@@ -1359,8 +1374,8 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
     outer = std::max(outer, 1);
   } else {
     // FP32 PCG with FP64 reliable-update refinement; inner_rtol is the
-    // residual reduction between FP64 replacements (0 picks 0.1).
-    const double delta = (o.inner_rtol > 0.0 && o.inner_rtol < 1.0) ? o.inner_rtol : 0.1;
+    // residual reduction between FP64 replacements (0 picks 0.3).
+    const double delta = (o.inner_rtol > 0.0 && o.inner_rtol < 1.0) ? o.inner_rtol : 0.3;
     rc = mixed_run(ctx, K64, A32, s->d_f, B32, B64.r, x32, x, o.rtol, delta,
                    o.max_iter, sc, part, &total_it, &outer, &brk, &relres);
   }

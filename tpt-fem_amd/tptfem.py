@@ -415,7 +415,7 @@ class System:
         return (Csr(nrows, ncols, rp, ci[: self.nnz].copy(), v[: self.nnz].copy()),
                 fred[: self.n_free].copy(), free[: self.n_free].copy())
 
-    def solve(self, mode=SOLVE_PCG64, rtol=1e-12, max_iter=100000, inner_rtol=0.1, max_outer=60):
+    def solve(self, mode=SOLVE_PCG64, rtol=1e-12, max_iter=100000, inner_rtol=0.3, max_outer=60):
         """Single-system solve.  Mixed mode: inner_rtol is the FP32 residual
         reduction between FP64 residual replacements."""
         u = np.zeros(max(1, self.n_dof))
