@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "tfm_bsell.h"
 #include "tfm_elem.h"
 #include "tfm_internal.h"
 
@@ -211,77 +212,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_bsell(int n, int nslices, co
     }
   }
   put_part(part, P_PQ, dot);
-}
-
-// ---- building the sliced copy from the CSR ---------------------------------
-// Distinct column blocks of block row b: a merge of its BS sorted rows.
-template <int BS>
-__device__ __forceinline__ int bs_merge_step(const int *__restrict__ ci, const int *j, const int *e) {
-  int m = INT_MAX;
-#pragma unroll
-  for (int i = 0; i < BS; i++)
-    if (j[i] < e[i]) m = min(m, ci[j[i]] / BS);
-  return m;
-}
-
-template <int BS>
-__global__ void k_bs_count(int nbr, const int *__restrict__ rp, const int *__restrict__ ci, int *__restrict__ cnt) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbr) return;
-  int j[BS], e[BS];
-#pragma unroll
-  for (int i = 0; i < BS; i++) {
-    j[i] = rp[b * BS + i];
-    e[i] = rp[b * BS + i + 1];
-  }
-  int k = 0;
-  for (int m; (m = bs_merge_step<BS>(ci, j, e)) != INT_MAX; k++)
-#pragma unroll
-    for (int i = 0; i < BS; i++)
-      while (j[i] < e[i] && ci[j[i]] / BS == m) j[i]++;
-  cnt[b] = k;
-}
-
-// 64 * (widest block row) per slice.
-__global__ void k_bs_width(int nbr, int nslices, const int *__restrict__ cnt, int *__restrict__ w) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslices) return;
-  int m = 0;
-  for (int b = s * 64; b < min(nbr, s * 64 + 64); b++) m = max(m, cnt[b]);
-  w[s] = 64 * m;
-}
-
-template <int BS>
-__global__ void k_bs_fill(int nbr, int nslices, const int *__restrict__ rp, const int *__restrict__ ci,
-                          const int *__restrict__ soff, int *__restrict__ bcol, int *__restrict__ map) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nslices * 64) return;
-  constexpr int BB = BS * BS;
-  const int s = b >> 6, lane = b & 63;
-  const int o = soff[s], w = (soff[s + 1] - o) >> 6;
-  int k = 0;
-  if (b < nbr) {
-    int j[BS], e[BS];
-#pragma unroll
-    for (int i = 0; i < BS; i++) {
-      j[i] = rp[b * BS + i];
-      e[i] = rp[b * BS + i + 1];
-    }
-    for (int m; (m = bs_merge_step<BS>(ci, j, e)) != INT_MAX; k++) {
-      bcol[o + k * 64 + lane] = m;
-#pragma unroll
-      for (int i = 0; i < BS; i++)
-        for (; j[i] < e[i] && ci[j[i]] / BS == m; j[i]++)
-          map[j[i]] = (o + k * 64) * BB + (i * BS + ci[j[i]] % BS) * 64 + lane;
-    }
-  }
-  for (; k < w; k++) bcol[o + k * 64 + lane] = 0;
-}
-
-template <typename VT, typename ST>
-__global__ void k_bs_values(int64_t nnz, const int *__restrict__ map, const ST *__restrict__ src, VT *__restrict__ dst) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < nnz) dst[map[j]] = (VT)src[j];
 }
 
 // alpha = rz_i / p.q; x += alpha p, r -= alpha q, z = Dinv r; partial rz, rr.
@@ -788,7 +718,8 @@ static int bsell_build(tfm_system *s) {
   case 2: hipLaunchKernelGGL(k_bs_count<2>, dim3(g), dim3(256), 0, st, (int)nbr, s->d_rp, s->d_ci, cnt); break;
   default: hipLaunchKernelGGL(k_bs_count<1>, dim3(g), dim3(256), 0, st, (int)nbr, s->d_rp, s->d_ci, cnt); break;
   }
-  hipLaunchKernelGGL(k_bs_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (int)nbr, (int)nsl, cnt, w);
+  hipLaunchKernelGGL(k_bs_width, dim3((unsigned)((nsl + 255) / 256)), dim3(256), 0, st, (int)nbr, (int)nsl,
+                     (const int *)nullptr, cnt, w);
   int64_t nblk = 0;
   rc = exclusive_scan_i32(ctx, w, B.d_soff, nsl, &nblk);
   dfree(cnt);
@@ -809,16 +740,16 @@ static int bsell_build(tfm_system *s) {
   const unsigned g2 = (unsigned)((nsl * 64 + 255) / 256);
   switch (bs) {
   case 3:
-    hipLaunchKernelGGL(k_bs_fill<3>, dim3(g2), dim3(256), 0, st, (int)nbr, (int)nsl, s->d_rp, s->d_ci, B.d_soff,
-                       B.d_bcol, B.d_map);
+    hipLaunchKernelGGL(k_bs_fill<3>, dim3(g2), dim3(256), 0, st, (int)nbr, (int)nsl, (const int *)nullptr,
+                       s->d_rp, s->d_ci, B.d_soff, B.d_bcol, B.d_map);
     break;
   case 2:
-    hipLaunchKernelGGL(k_bs_fill<2>, dim3(g2), dim3(256), 0, st, (int)nbr, (int)nsl, s->d_rp, s->d_ci, B.d_soff,
-                       B.d_bcol, B.d_map);
+    hipLaunchKernelGGL(k_bs_fill<2>, dim3(g2), dim3(256), 0, st, (int)nbr, (int)nsl, (const int *)nullptr,
+                       s->d_rp, s->d_ci, B.d_soff, B.d_bcol, B.d_map);
     break;
   default:
-    hipLaunchKernelGGL(k_bs_fill<1>, dim3(g2), dim3(256), 0, st, (int)nbr, (int)nsl, s->d_rp, s->d_ci, B.d_soff,
-                       B.d_bcol, B.d_map);
+    hipLaunchKernelGGL(k_bs_fill<1>, dim3(g2), dim3(256), 0, st, (int)nbr, (int)nsl, (const int *)nullptr,
+                       s->d_rp, s->d_ci, B.d_soff, B.d_bcol, B.d_map);
     break;
   }
   TFM_CHECK_HIP(ctx, hipGetLastError());
