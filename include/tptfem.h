@@ -155,10 +155,10 @@ typedef struct {
   int mode;           /* TFM_SOLVE_*                                          */
   double rtol;        /* stop when ||f - K u|| <= rtol * ||f|| (FP64 residual) */
   int max_iter;       /* total inner iterations                               */
-  double inner_rtol;  /* mixed, one system: FP32 residual reduction between FP64
-                         residual replacements (outside (0,1): 0.3); mixed,
-                         batched: FP32 relative tolerance per refinement cycle */
-  int max_outer;      /* mixed: refinement cycles                             */
+  double inner_rtol;  /* mixed: FP32 residual reduction between FP64 residual
+                         replacements (outside (0,1): 0.3), per segment when
+                         batched                                              */
+  int max_outer;      /* unused (replacements are driven by inner_rtol)       */
 } tfm_solve_opts;
 typedef struct {
   int iterations;     /* inner PCG iterations in total          */

@@ -24,6 +24,7 @@
 namespace tfm {
 
 constexpr int kBW = 4;         // waves (= slices) per block
+constexpr double kFloor2 = 1e-12;  // precision form: FP32 residual floor (squared, relative to the last replacement)
 constexpr int kBT = 64 * kBW;  // threads per block
 
 // Partial-sum slots, each [nblk][NR]: p.q; r.z and r.r by iteration parity;
@@ -165,12 +166,15 @@ __global__ __launch_bounds__(kBT) void kbx_dir(BView V, const VT *__restrict__ b
   bool any = false;
 #pragma unroll
   for (int k = 0; k < NR; k++) {
-    const double rr0 = it == 0 ? rr[k] : S[k].rr0;
+    // rr0: the reference of the stop test -- the first residual, or (rr0 < 0
+    // marks a residual replacement) the replaced one
+    const bool fresh = it == 0 || S[k].rr0 < 0.0;
+    const double rr0 = fresh ? rr[k] : S[k].rr0;
     const bool d = s_done[k] || rr0 == 0.0 || rr[k] <= V.rtol2 * rr0 || it >= V.max_it;
     any |= !d;
-    beta[k] = (VT)((it > 0 && !d) ? rzn[k] / rzo[k] : 0.0);
+    beta[k] = (VT)((it > 0 && !d && rzo[k] > 0.0) ? rzn[k] / rzo[k] : 0.0);
     if (first && !s_done[k]) {
-      if (it == 0) S[k].rr0 = rr[k];
+      if (fresh) S[k].rr0 = rr[k];
       S[k].rr = rr[k];
       if (d) {
         S[k].done = 1;
@@ -292,17 +296,20 @@ __global__ __launch_bounds__(kBT) void kbx_update(BView V, XT *__restrict__ x, V
 
 // FP64 residual r = f - K u (CSR, fixed order) over the block's rows and
 // partial ||r||^2, ||f||^2, f.u; segments flagged in `keep` get r = 0 (their
-// refinement is over).
+// refinement is over); the FP32 copy is refreshed for the segments in `sel`.
 template <int NR>
 __global__ __launch_bounds__(kBT) void kbx_resid64(BView V, const int *__restrict__ rp, const int *__restrict__ ci,
                                                    const double *__restrict__ v, const double *__restrict__ u,
                                                    const double *__restrict__ f, double *__restrict__ r,
-                                                   float *__restrict__ r32, const int *__restrict__ keep) {
+                                                   float *__restrict__ r32, const int *__restrict__ keep,
+                                                   const int *__restrict__ sel) {
   const int b = blockIdx.x, pr = V.blk_prob[b];
   int r0, r1;
   blk_rows(V, b, r0, r1);
   const int k = threadIdx.x % NR;
   const bool kept = keep && keep[pr * NR + k];
+  // the FP32 copy is written only for the selected segments (all when sel is null)
+  const bool w32 = r32 && (!sel || sel[pr * NR + k]);
   double rr = 0.0, ff = 0.0, fu = 0.0;
   for (int i = r0 * NR + threadIdx.x; i < r1 * NR; i += kBT) {
     const int row = i / NR;
@@ -311,7 +318,7 @@ __global__ __launch_bounds__(kBT) void kbx_resid64(BView V, const int *__restric
     for (int j = rb; j < re; j++) acc += v[j] * u[(size_t)ci[j] * NR + k];
     const double ri = rb == re ? 0.0 : f[i] - acc;
     r[i] = kept ? 0.0 : ri;
-    if (r32) r32[i] = kept ? 0.0f : (float)ri;
+    if (w32) r32[i] = kept ? 0.0f : (float)ri;
     rr += ri * ri;
     ff += f[i] * f[i];
     fu += f[i] * u[i];
@@ -340,6 +347,83 @@ __global__ __launch_bounds__(kBT) void kbx_resid_finish(BView V) {
       g.ff = c[k];
       g.fu = d[k];
     }
+}
+
+// Reliable-update helpers of the precision form, for the segments in sel:
+// fold the FP32 correction into the FP64 solution ...
+template <int NR>
+__global__ __launch_bounds__(kBT) void kbx_fold(BView V, double *__restrict__ x, float *__restrict__ x32,
+                                                const int *__restrict__ sel) {
+  const int b = blockIdx.x, pr = V.blk_prob[b];
+  if (!sel[pr * NR + threadIdx.x % NR]) return;  // kBT % NR == 0: a thread keeps one rhs
+  int r0, r1;
+  blk_rows(V, b, r0, r1);
+  for (int i = r0 * NR + threadIdx.x; i < r1 * NR; i += kBT) {
+    x[i] += (double)x32[i];
+    x32[i] = 0.0f;
+  }
+}
+
+// ... and, after the FP64 residual replaced r, z = Dinv r with the r.z / r.r
+// partials of the coming iteration (parity-1 slots); the direction is kept.
+template <int NR>
+__global__ __launch_bounds__(kBT) void kbx_replace(BView V, const float *__restrict__ r, float *__restrict__ z,
+                                                   const float *__restrict__ dinv, const int *__restrict__ sel) {
+  const int b = blockIdx.x, pr = V.blk_prob[b];
+  const bool on = sel[pr * NR + threadIdx.x % NR];
+  int r0, r1;
+  blk_rows(V, b, r0, r1);
+  double rz = 0.0, rr = 0.0;
+  if (on)
+    for (int i = r0 * NR + threadIdx.x; i < r1 * NR; i += kBT) {
+      const float zi = dinv[i / NR] * r[i];
+      z[i] = zi;
+      rz += (double)r[i] * (double)zi;
+      rr += (double)r[i] * (double)r[i];
+    }
+  double a[NR], c[NR];
+  rhs_sum<NR>(rz, a);
+  rhs_sum<NR>(rr, c);
+  bsum<NR>(a);
+  bsum<NR>(c);
+  if (threadIdx.x < NR && sel[pr * NR + threadIdx.x]) {
+    double ta = a[0], tc = c[0];
+#pragma unroll
+    for (int k = 1; k < NR; k++)
+      if ((int)threadIdx.x == k) {
+        ta = a[k];
+        tc = c[k];
+      }
+    V.part[((size_t)Q_RZ1 * V.nblk + b) * NR + threadIdx.x] = ta;
+    V.part[((size_t)Q_RR1 * V.nblk + b) * NR + threadIdx.x] = tc;
+  }
+}
+
+__global__ void kb_mark_done(Seg *seg, int nseg, const int *flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nseg && flags[i]) seg[i].done = 1;
+}
+
+// Resume of the flagged segments after a residual replacement: running again
+// with the replaced residual as the new stop reference; segments flagged in
+// zp (an FP32 breakdown) also restart their direction (both buffers zero).
+template <int NR>
+__global__ __launch_bounds__(kBT) void kbx_resume(BView V, float *__restrict__ p0, float *__restrict__ p1,
+                                                  const int *__restrict__ flags, const int *__restrict__ zp) {
+  const int b = blockIdx.x, pr = V.blk_prob[b];
+  if (b == V.prob_blk0[pr] && threadIdx.x < NR && flags[pr * NR + threadIdx.x]) {
+    Seg &g = V.seg[(size_t)pr * NR + threadIdx.x];
+    g.done = 0;
+    g.breakdown = 0;
+    g.rr0 = -1.0;
+  }
+  if (!zp[pr * NR + threadIdx.x % NR]) return;
+  int r0, r1;
+  blk_rows(V, b, r0, r1);
+  for (int i = r0 * NR + threadIdx.x; i < r1 * NR; i += kBT) {
+    p0[i] = 0.0f;
+    p1[i] = 0.0f;
+  }
 }
 
 __global__ void kb_add(int64_t n, double *u, const float *d) {
@@ -567,6 +651,151 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
   return rc;
 }
 
+// Precision form, per segment: FP32 PCG on the correction with reliable
+// updates.  Whenever a segment's recurrence residual has dropped by delta
+// since its last replacement (or stalls), its FP32 correction is folded into
+// the FP64 solution and its residual replaced by the true FP64 f - K u,
+// keeping its search direction; a segment ends when that residual meets rtol.
+template <int NR>
+static int bmixed(tfm_system *s, const BBatch &B, const BView &V, const double *f, double *x, double *r64,
+                  float *r32, float *z32, float *p32, float *p32b, float *q32, const float *d32, float *x32,
+                  int *odone, const tfm_solve_opts &o, std::vector<int> &od, std::vector<int> &iters,
+                  std::vector<int> &cycles, std::vector<int> &brk) {
+  tfm_ctx *ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int P = B.P, nseg = P * NR;
+  const size_t nv = (size_t)s->ndof * NR;
+  const double delta = (o.inner_rtol > 0.0 && o.inner_rtol < 1.0) ? o.inner_rtol : 0.3;
+  dim3 g(B.nblk), t(kBT);
+  int *sel = nullptr, *rstf = nullptr;
+  int rc;
+  if ((rc = dalloc(ctx, &sel, nseg)) || (rc = dalloc(ctx, &rstf, nseg))) {
+    dfree(sel);
+    return rc;
+  }
+  std::vector<Seg> hs(nseg);
+  std::vector<double> rr_ref(nseg), best(nseg);
+  std::vector<int> since(nseg, 0), hsel(nseg, 0), hrst(nseg, 0), nbrk(nseg, 0);
+  auto residual = [&](const int *sl) -> int {
+    hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, x, f, r64, r32, odone, sl);
+    hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
+    TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+    return TFM_OK;
+  };
+  // initial residual of every segment
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(odone, od.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+  if ((rc = residual(nullptr))) {
+    dfree(sel);
+    dfree(rstf);
+    return rc;
+  }
+  for (int q = 0; q < nseg; q++) {
+    const double rel = hs[q].ff > 0 ? std::sqrt(hs[q].rt / hs[q].ff) : 0.0;
+    if (rel <= o.rtol) od[q] = 1;
+    rr_ref[q] = best[q] = hs[q].rt;
+    cycles[q] = 1;
+  }
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(odone, od.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(x32, 0, sizeof(float) * nv, st));
+  // Replacements keeping the direction happen at chunk boundaries (the
+  // buffer and partial-slot parity is then consistent).  Inside a chunk a
+  // segment freezes only at the FP32 accuracy floor -- its residual 1e-6
+  // below the last replacement -- or on a breakdown, so a small problem never
+  // iterates on rounding noise; a frozen segment restarts its direction.
+  const BView V32 = B.view(o.max_iter, kFloor2);
+  const int nsb = (nseg + 255) / 256;
+  hipLaunchKernelGGL(kb_reset, dim3(nsb), dim3(256), 0, st, B.seg, nseg, odone, B.it);
+  hipLaunchKernelGGL((kbx_init<float, NR>), g, t, 0, st, V32, r32, z32, p32, p32b, d32);
+  const int CH = 16;
+  float *pb[2] = {p32, p32b};
+  auto chunk = [&]() {
+    for (int k = 0; k < CH; k++) {
+      const int e = k & 1;
+      bb_dir<float, NR>(B, V32, st, B.v32, z32, pb[1 - e], pb[e], q32, e);
+      hipLaunchKernelGGL((kbx_update<float, float, NR>), g, t, 0, st, V32, x32, r32, z32, pb[e], q32, d32, e);
+    }
+  };
+  const bool use_graph = !getenv("TFM_NO_GRAPH");
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    TFM_CHECK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    chunk();
+    TFM_CHECK_HIP(ctx, hipStreamEndCapture(st, &graph));
+    TFM_CHECK_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  constexpr int kStall = 64;  // chunks without a 10 % residual improvement
+  for (int it = 0;; it += CH) {
+    bool all = true;
+    for (int q = 0; q < nseg; q++) all &= od[q] != 0;
+    if (all) break;
+    if (!use_graph) {
+      chunk();
+    } else if (hipGraphLaunch(exec, st) != hipSuccess) {
+      rc = fail(ctx, TFM_E_HIP, "hipGraphLaunch failed");
+      break;
+    }
+    const int done_it = it + CH;
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
+    TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+    bool any = false;
+    for (int q = 0; q < nseg; q++) {
+      hsel[q] = 0;
+      if (od[q]) continue;
+      if (hs[q].rr < 0.81 * best[q]) {
+        best[q] = hs[q].rr;
+        since[q] = 0;
+      } else {
+        since[q]++;
+      }
+      if (hs[q].rr <= delta * delta * rr_ref[q] || hs[q].done || since[q] >= kStall || done_it >= o.max_iter) {
+        hsel[q] = 1;
+        any = true;
+      }
+    }
+    if (!any) continue;
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(sel, hsel.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((kbx_fold<NR>), g, t, 0, st, V, x, x32, sel);
+    if ((rc = residual(sel))) break;
+    for (int q = 0; q < nseg; q++) {
+      hrst[q] = 0;
+      if (!hsel[q]) continue;
+      const double rel = hs[q].ff > 0 ? std::sqrt(hs[q].rt / hs[q].ff) : 0.0;
+      cycles[q] += 1;
+      rr_ref[q] = best[q] = hs[q].rt;
+      since[q] = 0;
+      if (rel <= o.rtol || done_it >= o.max_iter) {
+        od[q] = 1;
+        iters[q] = done_it;
+        hsel[q] = 0;
+      } else if (hs[q].breakdown && ++nbrk[q] > 4) {
+        // an FP32 breakdown (p.q <= 0) that keeps coming back fails the segment
+        brk[q] = 1;
+        od[q] = 1;
+        iters[q] = done_it;
+        hsel[q] = 0;
+      } else if (hs[q].done) {
+        hrst[q] = 1;  // frozen (floor or breakdown): restart the direction
+      }
+    }
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(odone, od.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(sel, hsel.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(rstf, hrst.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(kb_mark_done, dim3(nsb), dim3(256), 0, st, B.seg, nseg, odone);
+    hipLaunchKernelGGL((kbx_replace<NR>), g, t, 0, st, V32, r32, z32, d32, sel);
+    hipLaunchKernelGGL((kbx_resume<NR>), g, t, 0, st, V32, p32, p32b, sel, rstf);
+  }
+  // segments still running (none in normal operation) count their iterations
+  for (int q = 0; q < nseg; q++)
+    if (!iters[q]) iters[q] = hs[q].iters;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  dfree(sel);
+  dfree(rstf);
+  return rc;
+}
+
 template <int NR>
 static int solve_batch_nr(tfm_system *s, int P, const int64_t *dof_off, const double *rhs,
                           const tfm_solve_opts &o, double *u, tfm_solve_stats *stats) {
@@ -640,7 +869,7 @@ static int solve_batch_nr(tfm_system *s, int P, const int64_t *dof_off, const do
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev0, st));
   if (!mixed) {
     hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, x, f, r64, (float *)nullptr,
-                       (const int *)nullptr);
+                       (const int *)nullptr, (const int *)nullptr);
     rc = bpcg<double, double, NR>(ctx, B, B.v64, r64, z64, p64, p64b, q64, d64, x, o.rtol, o.max_iter, nullptr, hs);
     for (int q = 0; q < nseg; q++) {
       iters[q] = hs[q].iters;
@@ -649,38 +878,12 @@ static int solve_batch_nr(tfm_system *s, int P, const int64_t *dof_off, const do
       od[q] = hs[q].done && !hs[q].breakdown && hs[q].iters < o.max_iter;  // converged on its recurrence residual
     }
   } else {
-    // FP64 iterative refinement: FP32 PCG on the correction to inner_rtol,
-    // per segment, until the FP64 residual meets rtol.
-    for (int cyc = 0; cyc < o.max_outer && !rc; cyc++) {
-      TFM_CHECK_HIP(ctx, hipMemcpyAsync(odone, od.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, x, f, r64, r32, odone);
-      hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
-      TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
-      TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
-      bool all = true;
-      for (int q = 0; q < nseg; q++) {
-        const double rel = hs[q].ff > 0 ? std::sqrt(hs[q].rt / hs[q].ff) : 0.0;
-        if (!od[q] && (rel <= o.rtol || iters[q] >= o.max_iter || brk[q])) od[q] = 1;
-        all &= od[q] != 0;
-      }
-      if (all) break;
-      TFM_CHECK_HIP(ctx, hipMemcpyAsync(odone, od.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-      TFM_CHECK_HIP(ctx, hipMemsetAsync(x32, 0, sizeof(float) * nv, st));
-      rc = bpcg<float, float, NR>(ctx, B, B.v32, r32, z32, p32, p32b, q32, d32, x32, o.inner_rtol, o.max_iter, odone,
-                                  hs);
-      for (int q = 0; q < nseg; q++)
-        if (!od[q]) {
-          iters[q] += hs[q].iters;
-          cycles[q] += 1;
-          brk[q] |= hs[q].breakdown;
-        }
-      hipLaunchKernelGGL(kb_add, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, (int64_t)nv, x, x32);
-    }
+    rc = bmixed<NR>(s, B, V, f, x, r64, r32, z32, p32, p32b, q32, d32, x32, odone, o, od, iters, cycles, brk);
   }
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, st));
   // final FP64 residual certificate of every segment
   hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, x, f, r64, (float *)nullptr,
-                     (const int *)nullptr);
+                     (const int *)nullptr, (const int *)nullptr);
   hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
   std::vector<double> hx(nv);
@@ -850,7 +1053,7 @@ extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *d
   if (nrhs > 1 && !rhs) return fail(ctx, TFM_E_INVALID, "several right-hand sides need rhs");
   int rc = set_device(ctx);
   if (rc) return rc;
-  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 1e-4, 60};
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 0.3, 60};
   if (opts) o = *opts;
   if (o.mode == TFM_SOLVE_MIXED && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values");
   if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values");

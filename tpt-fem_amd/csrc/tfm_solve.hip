@@ -24,6 +24,7 @@ namespace tfm {
 
 constexpr int kBlock = 256;
 constexpr int kMaxBlocks = 1024;
+constexpr double kFloor2 = 1e-12;  // precision form: FP32 residual floor (squared, relative to the last replacement)
 
 // Partial-sum slots (each kMaxBlocks doubles): p.q, r.z and r.r by iteration
 // parity, and the refinement residual's ||r||^2, ||f||^2.
@@ -110,7 +111,11 @@ __device__ __forceinline__ bool dir_prologue(Scal *sc, const double *part, int e
     }
   }
   *beta = 0.0;
-  if (it > 0) *beta = sum_part(part, e ? P_RZ0 : P_RZ1, nb) / sum_part(part, e ? P_RZ1 : P_RZ0, nb);
+  if (it > 0) {
+    const double rzo = sum_part(part, e ? P_RZ1 : P_RZ0, nb);
+    const double rzn = sum_part(part, e ? P_RZ0 : P_RZ1, nb);
+    if (rzo > 0.0) *beta = rzn / rzo;  // 0 after a restart whose old slot is empty
+  }
   return true;
 }
 
@@ -584,6 +589,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_replace(int n, const VT *__restr
   put_part(part, P_RR1, rr);
 }
 
+// After a residual replacement: the FP32 iteration runs on, freezing itself
+// only when its residual falls by sqrt(d2) below the replaced one (sc->rr).
+__global__ void k_pcg_rearm(Scal *sc, double d2) {
+  sc->tol2 = d2 * sc->rr;
+  sc->breakdown = 0;
+  sc->done = 0;
+}
+
 // x += d (FP64), d = 0.
 __global__ void k_fold(int n, double *__restrict__ x, float *__restrict__ d) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -621,7 +634,7 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
   double rr_ref = h.rr;
   TFM_CHECK_HIP(ctx, hipMemsetAsync(x32, 0, sizeof(float) * n, st));
   hipLaunchKernelGGL((k_pcg_init<float>), dim3(nb), dim3(kBlock), 0, st, n, B.r, B.z, B.p, B.p2, B.dinv, part);
-  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, st, sc, part, nb, 0.0, max_iter);
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, st, sc, part, nb, kFloor2, max_iter);
   const int CH = 16;
   float *pb[2] = {B.p, B.p2};
   auto chunk = [&]() {
@@ -644,7 +657,7 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
   int rc = TFM_OK;
   constexpr int kStall = 64;  // chunks of 16 iterations
   double best = rr_ref;
-  int since = 0;
+  int since = 0, nbrk = 0;
   for (;;) {
     if (!use_graph) {
       chunk();
@@ -655,21 +668,20 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
     TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
     TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
     *iters = h.it;
-    if (h.breakdown) {
-      *breakdown = 1;
-      break;
-    }
-    // FP32 stagnation guard: when the cycle's smallest recurrence residual
-    // has not improved by 10% for kStall chunks, replace anyway (an FP32
-    // iteration that stalls above delta would otherwise run to max_iter).
+    // Replacements keeping the direction happen at chunk boundaries, when the
+    // residual has dropped by delta (the buffer and partial-slot parity is
+    // then consistent).  Inside a chunk the FP32 iteration freezes itself
+    // (done) only at its accuracy floor -- 1e-6 below the last replacement --
+    // or on a breakdown (p.q <= 0), and then restarts its direction.  The
+    // stagnation guard also replaces when the smallest recurrence residual
+    // has not improved by 10% for kStall chunks.
     if (h.rr < 0.81 * best) {
       best = h.rr;
       since = 0;
     } else {
       since++;
     }
-    const bool stalled = since >= kStall;
-    if (h.rr <= delta * delta * rr_ref || h.done || stalled) {
+    if (h.rr <= delta * delta * rr_ref || h.done || since >= kStall) {
       hipLaunchKernelGGL(k_fold, dim3(g1), dim3(256), 0, st, n, x, x32);
       residual_dispatch(S, st, nbr, n, rp, ci, v64, x, f, r64, B.r, sc, part);
       Scal h2{};
@@ -677,8 +689,17 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
       TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
       *repl += 1;
       *relres = h2.rr0 > 0 ? std::sqrt(h2.rr / h2.rr0) : 0.0;
-      if (*relres <= rtol || h.done || h.it >= max_iter) break;
+      if (*relres <= rtol || h.it >= max_iter) break;
+      if (h.breakdown && ++nbrk > 4) {  // a breakdown that keeps coming back fails the solve
+        *breakdown = 1;
+        break;
+      }
       hipLaunchKernelGGL((k_pcg_replace<float>), dim3(nb), dim3(kBlock), 0, st, n, B.r, B.z, B.dinv, part);
+      if (h.done) {  // frozen inside the chunk: restart the direction
+        TFM_CHECK_HIP(ctx, hipMemsetAsync(B.p, 0, sizeof(float) * n, st));
+        TFM_CHECK_HIP(ctx, hipMemsetAsync(B.p2, 0, sizeof(float) * n, st));
+      }
+      hipLaunchKernelGGL(k_pcg_rearm, dim3(1), dim3(1), 0, st, sc, kFloor2);
       rr_ref = h2.rr;
       best = rr_ref;
       since = 0;

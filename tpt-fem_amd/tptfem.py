@@ -426,7 +426,7 @@ class System:
         return u[: self.n_dof]
 
     def solve_batch(self, dof_off, rhs=None, nrhs=1, mode=SOLVE_PCG64, rtol=1e-10, max_iter=20000,
-                    inner_rtol=1e-4, max_outer=60):
+                    inner_rtol=0.3, max_outer=60):
         """Block-diagonal batched solve: returns u [nrhs, n_dof] and per-segment stats."""
         off = _i64(dof_off)
         P = len(off) - 1
