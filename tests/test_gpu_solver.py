@@ -121,3 +121,40 @@ def test_elasticity_resolve_after_reassembly(ctx):
     u4 = s.solve(T.SOLVE_PCG64, rtol=1e-13)
     # linear in 1/E
     assert np.abs(u4 * 4.0 - u1).max() <= 1e-9 * np.abs(u1).max()
+
+
+@pytest.mark.parametrize("n", [(2, 3, 2), (3, 3, 3)])
+def test_mixed_tiny_system_converges(ctx, n):
+    """A system with 2 (or 8) free DOFs: its FP32 iteration converges (and
+    breaks down) inside one 16-iteration chunk; the precision form must still
+    end at the FP64 tolerance."""
+    T = tp()
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    bnd = [i for i, c in enumerate(coords) if np.any(np.isclose(c, 0.0)) or np.any(np.isclose(c, 1.0))]
+    bcs = [(int(i), 0.0) for i in bnd]
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    rhs = T.source_vector(m, 1.0, 2)
+    s = T.System.create(m, T.Operator.poisson(1.0, 2), 1, bcs, rhs, flags=T.SYS_FP64 | T.SYS_FP32)
+    um = s.solve(T.SOLVE_MIXED, rtol=1e-12, max_iter=2000)
+    assert s.stats.rel_residual <= 1e-12
+    ur = O.solve_with_dirichlet(O.assemble(coords, conn, O.make_op(O.TET, 3, 0, coef=1.0), 1),
+                                O.poisson_rhs(coords, O.TET, conn, 1.0), bcs)
+    assert rel_max(um, ur) <= 1e-10
+
+
+def test_mixed_small_replacement_factor_still_converges(ctx):
+    """inner_rtol far below what FP32 can reach: the stagnation guard makes
+    the replacements, and the solve still meets rtol."""
+    T = tp()
+    n = (6, 5, 4)
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], n)
+    m.jitter(n, 0.2, 5)
+    coords, conn = m.download()
+    bcs = plane_bcs(coords, 3, 0, 0.0, (0, 1, 2))
+    rhs = O.plane_traction(coords, O.TET, conn, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
+    s = T.System.create(m, T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2), 3, bcs, rhs,
+                        flags=T.SYS_FP64 | T.SYS_FP32)
+    u64 = s.solve(T.SOLVE_PCG64, rtol=1e-12)
+    um = s.solve(T.SOLVE_MIXED, rtol=1e-11, inner_rtol=1e-4, max_iter=100000)
+    assert s.stats.rel_residual <= 1e-11
+    assert np.linalg.norm(um - u64) <= 1e-8 * np.linalg.norm(u64)
