@@ -102,7 +102,8 @@ int exclusive_scan_i32(tfm_ctx *ctx, const int *d_in, int *d_out, int64_t n, int
 // P1-tet assembly plan (tfm_asm_tet.hip), built once per pattern:
 //   tiles  int32 [ntiles + 1]          first node of each tile
 //   tpa    int32 [ntiles + 1]          first (node, element) pair of each tile
-//   bptr   int32 [n_nodes + 1]         first row block of each node
+//   bptr   int32 [n_nodes + 1]         first row block of each node (its diagonal block
+//                                      first when it has one, hasdiag u8 [n_nodes])
 //   bmask  u32   [nblocks]             contributing (node, element) pairs of a block
 //   blmb   u64   [nblocks]             2-bit local index of the column node per pair
 //   ipos   int32 [nblocks * dpn]       CSR position of a (row, block)'s first entry,
@@ -113,6 +114,7 @@ struct TetPlan {
   int *d_tpa = nullptr;
   int ntiles = 0;
   int *d_bptr = nullptr;
+  unsigned char *d_hasdiag = nullptr;  // the node has a diagonal block (stored first)
   unsigned *d_bmask = nullptr;
   unsigned long long *d_blmb = nullptr;
   int *d_ipos = nullptr;
