@@ -1212,6 +1212,11 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
   if (o.max_iter <= 0) o.max_iter = 20000;
   const int n = (int)s->ndof;
   const bool mixed = o.mode == TFM_SOLVE_MIXED;
+  // No free DOF: the reference's solve gets the 1 x 1 CSR of an empty
+  // triplet list against a length-0 rhs and fails (sparse/src/lib.rs:244-258).
+  if (s->n_free == 0)
+    return fail(ctx, TFM_E_NUMERIC,
+                "sparse numeric factorization/solve failed: rhs length 0 does not match matrix dimension 1");
   if (mixed && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values (TFM_SYS_FP32)");
   if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values (TFM_SYS_FP64)");
   hipStream_t st = ctx->stream;
