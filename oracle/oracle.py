@@ -78,6 +78,10 @@ def _declare(L):
         "orc_dense_solve": (INT, [I64, P, P]),
         "orc_solve_coo": (INT, [I64, P, P, P, I64, P]),
         "orc_solve_with_dirichlet": (INT, [I64, P, P, P, I64, P, I64, P, P, P]),
+        "orc_topopt_quad_form": (D, [P, P, P]),
+        "orc_topopt_filter": (None, [I64, I64, D, P, D, P]),
+        "orc_topopt_oc": (None, [I64, P, P, D, D, P]),
+        "orc_topopt_simp": (INT, [I64, I64, D, D, D, D, D, D, INT, D, P, I64, P, P, P, P, C.POINTER(INT)]),
         "orc_boundary_faces": (I64, [I64, INT, P, P, P]),
         "orc_apply_neumann": (None, [I64, INT, P, I64, INT, P, INT, D, INT, P]),
         "orc_plane_traction": (None, [I64, INT, P, I64, INT, P, INT, INT, D, D, P, INT, P]),
@@ -391,3 +395,53 @@ def solve_poisson(coords, cell, conn, kappa, order, f_const, bcs):
     if rc != 0:
         raise SolveError(_SOLVE_MSG[rc])
     return u
+
+
+# ---- SIMP topology optimization (crates/tpt-fem-topopt/src/lib.rs) -------------
+def topopt_quad_form(ke0, u, nodes):
+    """element_quad_form (:241-254)."""
+    ke0, u, nodes = f64(ke0).ravel(), f64(u), i64(nodes)
+    return lib().orc_topopt_quad_form(_p(ke0), _p(u), _p(nodes))
+
+
+def topopt_filter(nx, ny, h, raw, radius):
+    """filter_sensitivity (:259-279) on Grid::new(nx, ny, h)."""
+    raw = f64(raw)
+    out = np.zeros(len(raw))
+    lib().orc_topopt_filter(nx, ny, h, _p(raw), radius, _p(out))
+    return out
+
+
+def topopt_oc(rho, sens, vol_frac, move_limit):
+    """oc_update (:284-331)."""
+    rho, sens = f64(rho), f64(sens)
+    out = np.zeros(len(rho))
+    lib().orc_topopt_oc(len(rho), _p(rho), _p(sens), vol_frac, move_limit, _p(out))
+    return out
+
+
+def cantilever_load(nx, ny, load):
+    """cantilever_load (:398-416): left edge clamped, -load in y at the
+    bottom-right node."""
+    f = np.zeros(2 * nx * ny)
+    bcs = []
+    for j in range(ny):
+        n = j * nx
+        bcs += [(2 * n, 0.0), (2 * n + 1, 0.0)]
+    f[2 * (nx - 1) + 1] = -load
+    return f, bcs
+
+
+def topopt_simp(nx, ny, h, e0, nu, vol_frac, penal, radius, max_iter, move_limit, f, bcs):
+    """topopt_simp (:342-385) -> (densities, compliance history, iterations)."""
+    f = f64(f)
+    bd = i64([b[0] for b in bcs]) if bcs else np.zeros(1, np.int64)
+    bv = f64([b[1] for b in bcs]) if bcs else np.zeros(1)
+    rho = np.zeros((nx - 1) * (ny - 1))
+    comp = np.zeros(max_iter + 1)
+    it = INT(0)
+    rc = lib().orc_topopt_simp(nx, ny, h, e0, nu, vol_frac, penal, radius, max_iter, move_limit, _p(f),
+                               len(bcs), _p(bd), _p(bv), _p(rho), _p(comp), C.byref(it))
+    if rc != 0:
+        raise SolveError(_SOLVE_MSG[rc])
+    return rho, comp, it.value

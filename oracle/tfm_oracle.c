@@ -988,3 +988,140 @@ void orc_jitter_interior(int64_t nx, int64_t ny, int64_t nz, double h_frac, uint
 void orc_loguniform(int64_t n, double lo, double hi, uint64_t seed, double *out) {
   for (int64_t i = 0; i < n; i++) out[i] = pow(10.0, lo + (hi - lo) * hash_u01(seed, (uint64_t)i));
 }
+
+
+/* ======================================================================== *
+ * SIMP minimum-compliance loop: crates/tpt-fem-topopt/src/lib.rs.           *
+ * ======================================================================== */
+#define TOPOPT_E_MIN_FACTOR 1e-3 /* :53 */
+#define TOPOPT_RHO_MIN 1e-3      /* :57 */
+#define TOPOPT_RHO_MAX 1.0       /* :59 */
+#define TOPOPT_OC_DAMP 0.5       /* :61 */
+
+/* element_quad_form (:241-254): u_e^T K0 u_e, i then j, from +0.0.          */
+double orc_topopt_quad_form(const double *ke0, const double *u, const int64_t *nodes) {
+  double ue[8];
+  for (int k = 0; k < 4; k++) {
+    ue[2 * k] = u[2 * nodes[k]];
+    ue[2 * k + 1] = u[2 * nodes[k] + 1];
+  }
+  double s = 0.0;
+  for (int i = 0; i < 8; i++)
+    for (int j = 0; j < 8; j++) s += ue[i] * ke0[i * 8 + j] * ue[j];
+  return s;
+}
+
+/* filter_sensitivity (:259-279): every element f, ascending, with
+ * |c_e - c_f|^2 <= R^2 weighted by R - |c_e - c_f|; centres from Grid::new
+ * (:94-107).                                                                */
+void orc_topopt_filter(int64_t nx, int64_t ny, double h, const double *raw, double radius,
+                       double *out) {
+  const int64_t nex = nx - 1, ne = (nx - 1) * (ny - 1);
+  const double r2 = radius * radius;
+  for (int64_t e = 0; e < ne; e++) {
+    const double cex = ((double)(e % nex) + 0.5) * h, cey = ((double)(e / nex) + 0.5) * h;
+    double num = 0.0, den = 0.0;
+    for (int64_t f = 0; f < ne; f++) {
+      const double dx = cex - ((double)(f % nex) + 0.5) * h;
+      const double dy = cey - ((double)(f / nex) + 0.5) * h;
+      const double d2 = dx * dx + dy * dy;
+      if (d2 <= r2) {
+        const double w = radius - sqrt(d2);
+        num += w * raw[f];
+        den += w;
+      }
+    }
+    out[e] = den > 0.0 ? num / den : raw[e];
+  }
+}
+
+static double topopt_clamp(double x, double lo, double hi) {
+  if (x < lo) x = lo; /* f64::clamp */
+  if (x > hi) x = hi;
+  return x;
+}
+
+/* oc_update (:284-331): 60 bisection steps on lambda in [1e-9, 1e9], then
+ * the damped, move-limited update at the last midpoint.                    */
+void orc_topopt_oc(int64_t n, const double *rho, const double *sens, double vol_frac,
+                   double move_limit, double *out) {
+  const double target = vol_frac * (double)n;
+  double lo = 1e-9, hi = 1e9, lambda = 1.0;
+  for (int it = 0; it < 60; it++) {
+    lambda = 0.5 * (lo + hi);
+    double sum = 0.0;
+    for (int64_t e = 0; e < n; e++) {
+      const double b = -sens[e];
+      const double ratio = b > 0.0 ? pow(b / lambda, TOPOPT_OC_DAMP) : 0.0;
+      const double cand = rho[e] * ratio;
+      sum += topopt_clamp(cand, fmax(rho[e] - move_limit, TOPOPT_RHO_MIN),
+                          fmin(rho[e] + move_limit, TOPOPT_RHO_MAX));
+    }
+    if (sum > target) lo = lambda;
+    else hi = lambda;
+  }
+  for (int64_t e = 0; e < n; e++) {
+    const double b = -sens[e];
+    const double ratio = b > 0.0 ? pow(b / lambda, TOPOPT_OC_DAMP) : 0.0;
+    out[e] = topopt_clamp(rho[e] * ratio, fmax(rho[e] - move_limit, TOPOPT_RHO_MIN),
+                          fmin(rho[e] + move_limit, TOPOPT_RHO_MAX));
+  }
+}
+
+/* dot (:388-390): Iterator::sum folds from -0.0.                           */
+static double topopt_dot(int64_t n, const double *a, const double *b) {
+  double s = -0.0;
+  for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+/* topopt_simp (:342-385) on Grid::new(nx, ny, h): compliance_out gets
+ * max_iter + 1 values (the uniform start, then one per OC iteration, before
+ * its update), rho_out the final densities.  Returns the solve's status.  */
+int orc_topopt_simp(int64_t nx, int64_t ny, double h, double e0, double nu, double vol_frac,
+                    double penal, double radius, int max_iter, double move_limit,
+                    const double *f, int64_t nbc, const int64_t *bc_dof, const double *bc_val,
+                    double *rho_out, double *compliance_out, int *iterations) {
+  const int64_t nn = nx * ny, ne = (nx - 1) * (ny - 1), ndof = 2 * nn;
+  const double emin = TOPOPT_E_MIN_FACTOR * e0;
+  double ke0[64];
+  orc_simp_ke_unit(nu, h, ke0);
+  double *coords = (double *)malloc(sizeof(double) * 2 * nn);
+  int64_t *conn = (int64_t *)malloc(sizeof(int64_t) * 4 * ne);
+  orc_grid(nx, ny, h, coords, conn);
+  double *rho = (double *)malloc(sizeof(double) * ne);
+  double *raw = (double *)malloc(sizeof(double) * ne);
+  double *sens = (double *)malloc(sizeof(double) * ne);
+  double *u = (double *)malloc(sizeof(double) * ndof);
+  int64_t *r = (int64_t *)malloc(sizeof(int64_t) * 64 * ne);
+  int64_t *c = (int64_t *)malloc(sizeof(int64_t) * 64 * ne);
+  double *v = (double *)malloc(sizeof(double) * 64 * ne);
+  for (int64_t e = 0; e < ne; e++) rho[e] = vol_frac;
+  orc_op op;
+  memset(&op, 0, sizeof op);
+  op.cell = ORC_QUAD;
+  op.pdim = 2;
+  op.kind = 2;
+  op.rho = rho;
+  op.penal = penal;
+  op.e0 = e0;
+  op.emin = emin;
+  op.ke0 = ke0;
+  int rc = 0, it = 0;
+  for (int k = 0; k <= max_iter && !rc; k++) {
+    const int64_t t = orc_assemble(nn, coords, ne, conn, &op, 2, r, c, v);
+    rc = orc_solve_with_dirichlet(t, r, c, v, ndof, f, nbc, bc_dof, bc_val, u);
+    if (rc) break;
+    compliance_out[k] = topopt_dot(ndof, f, u);
+    if (k == 0) continue; /* the uniform start is solved once more in iteration 1 */
+    it++;
+    for (int64_t e = 0; e < ne; e++)
+      raw[e] = -penal * pow(rho[e], penal - 1.0) * (e0 - emin) * orc_topopt_quad_form(ke0, u, conn + 4 * e);
+    orc_topopt_filter(nx, ny, h, raw, radius, sens);
+    orc_topopt_oc(ne, rho, sens, vol_frac, move_limit, rho);
+  }
+  memcpy(rho_out, rho, sizeof(double) * ne);
+  *iterations = it;
+  free(coords); free(conn); free(rho); free(raw); free(sens); free(u); free(r); free(c); free(v);
+  return rc;
+}

@@ -123,6 +123,17 @@ int orc_solve_poisson(int64_t nnodes, int pdim, const double *coords,
                       double kappa, int order, double f_const, int64_t nbc,
                       const int64_t *bc_dof, const double *bc_val, double *u);
 
+/* ---- SIMP topology optimization (crates/tpt-fem-topopt/src/lib.rs) ---- */
+double orc_topopt_quad_form(const double *ke0, const double *u, const int64_t *nodes);
+void orc_topopt_filter(int64_t nx, int64_t ny, double h, const double *raw, double radius,
+                       double *out);
+void orc_topopt_oc(int64_t n, const double *rho, const double *sens, double vol_frac,
+                   double move_limit, double *out);
+int orc_topopt_simp(int64_t nx, int64_t ny, double h, double e0, double nu, double vol_frac,
+                    double penal, double radius, int max_iter, double move_limit,
+                    const double *f, int64_t nbc, const int64_t *bc_dof, const double *bc_val,
+                    double *rho_out, double *compliance_out, int *iterations);
+
 /* Deterministic input generators shared with the product's generators. */
 uint64_t orc_splitmix64(uint64_t *state);
 double orc_uniform01(uint64_t *state);

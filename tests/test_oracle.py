@@ -311,3 +311,90 @@ def test_planted_transposed_inverse_is_caught():
     A = np.hstack([np.ones((4, 1)), X])
     G = np.linalg.inv(A)[1:].T
     assert np.allclose(good, G) and not np.allclose(bad, G)
+
+
+# ---- SIMP outer loop (crates/tpt-fem-topopt/src/lib.rs:241-500) ------------------
+def _simp(nx, ny, vf, iters):
+    f, bcs = O.cantilever_load(nx, ny, 1.0)
+    return O.topopt_simp(nx, ny, 1.0, 1.0, 0.3, vf, 3.0, 2.0, iters, 0.2, f, bcs)
+
+
+def test_cantilever_load_layout():
+    """cantilever_load (:398-416): both DOFs of every left-edge node fixed to
+    zero, -load on the y DOF of node nx-1 (bottom right), nothing else."""
+    f, bcs = O.cantilever_load(5, 3, 2.5)
+    assert bcs == [(0, 0.0), (1, 0.0), (10, 0.0), (11, 0.0), (20, 0.0), (21, 0.0)]
+    assert f[9] == -2.5 and np.count_nonzero(f) == 1 and len(f) == 30
+
+
+def test_simp_reference_properties():
+    """The reference's three tests (:421-495): compliance falls from the
+    uniform start, never rises by more than 1e-9, the volume holds to 1e-6;
+    full volume stays solid; a 40 % optimized design beats the uniform one."""
+    rho, c, it = _simp(20, 10, 0.5, 40)
+    assert it == 40 and len(rho) == 19 * 9 and len(c) == 41
+    assert c[-1] < c[0] and np.all(np.diff(c) <= 1e-9)
+    assert abs(rho.sum() - 0.5 * len(rho)) < 1e-6
+    # index 0 and 1 solve the same uniform design
+    assert c[0] == c[1]
+    rho, c, _ = _simp(16, 8, 1.0, 10)
+    assert abs(rho.sum() - len(rho)) < 1e-6 and np.all(np.abs(rho - 1.0) < 1e-6)
+    _, c, _ = _simp(24, 12, 0.4, 30)
+    assert c[-1] < c[0]
+
+
+def test_simp_quad_form_matches_numpy():
+    """element_quad_form (:241-254): u_e^T K0 u_e with u_e = (u[2n], u[2n+1])
+    per node, against numpy's product (other summation order)."""
+    rng = np.random.default_rng(3)
+    ke0 = O.simp_ke_unit(0.3, 1.0)
+    u = rng.standard_normal(2 * 12)
+    nodes = np.array([1, 2, 6, 5])
+    ue = np.ravel([[u[2 * n], u[2 * n + 1]] for n in nodes])
+    assert abs(O.topopt_quad_form(ke0, u, nodes) - ue @ ke0 @ ue) <= 1e-13 * abs(ue @ ke0 @ ue)
+    # a rigid translation carries no strain energy
+    t = np.tile([0.7, -0.2], 12)
+    assert abs(O.topopt_quad_form(ke0, t, nodes)) < 1e-15
+
+
+def test_simp_filter_closed_forms():
+    """filter_sensitivity (:259-279) on a 9 x 7 node grid (8 x 6 elements):
+    radius 0.5 < h keeps only the element itself (exactly); uniform input is
+    a fixed point; an interior element at radius 2 weighs itself 2, its four
+    edge neighbours 1, its diagonals 2 - sqrt(2) and the distance-2 ones 0."""
+    rng = np.random.default_rng(5)
+    raw = rng.standard_normal(8 * 6)
+    assert np.array_equal(O.topopt_filter(9, 7, 1.0, raw, 0.5), raw)
+    np.testing.assert_allclose(O.topopt_filter(9, 7, 1.0, np.full(48, -3.25), 2.0), -3.25, rtol=1e-15)
+    out = O.topopt_filter(9, 7, 1.0, raw, 2.0)
+    R = raw.reshape(6, 8)
+    i, j = 3, 2  # element (ei, ej) = (3, 2)
+    d = 2.0 - np.sqrt(2.0)
+    num = 2 * R[j, i] + R[j, i - 1] + R[j, i + 1] + R[j - 1, i] + R[j + 1, i] + d * (
+        R[j - 1, i - 1] + R[j - 1, i + 1] + R[j + 1, i - 1] + R[j + 1, i + 1])
+    assert abs(out[j * 8 + i] - num / (6 + 4 * d)) < 1e-14
+    # corner element (0, 0): itself, two edge neighbours, one diagonal
+    num = 2 * R[0, 0] + R[0, 1] + R[1, 0] + d * R[1, 1]
+    assert abs(out[0] - num / (4 + d)) < 1e-14
+
+
+def test_simp_oc_closed_form_and_bounds():
+    """oc_update (:284-331): with no bound active the update is
+    rho_e sqrt(B_e / lambda) with lambda fixing the volume, i.e.
+    target rho_e sqrt(B_e) / sum(rho sqrt(B)); otherwise the move limit and
+    [1e-3, 1] hold and the volume meets the target to bisection accuracy."""
+    rng = np.random.default_rng(11)
+    n = 200
+    rho = np.full(n, 0.5)
+    b = rng.uniform(0.9, 1.1, n)
+    out = O.topopt_oc(rho, -b, 0.5, 0.2)
+    w = rho * np.sqrt(b)
+    np.testing.assert_allclose(out, 0.5 * n * w / w.sum(), rtol=1e-8)
+    rho = rng.uniform(0.01, 1.0, n)
+    s = -np.exp(rng.standard_normal(n) * 2)
+    s[:7] = 0.0  # no gain: ratio 0, pushed to the lower bound
+    out = O.topopt_oc(rho, s, 0.3, 0.2)
+    assert abs(out.sum() - 0.3 * n) < 1e-6
+    assert np.all(out >= np.maximum(rho - 0.2, 1e-3) - 1e-15) and np.all(out <= np.minimum(rho + 0.2, 1.0) + 1e-15)
+    np.testing.assert_array_equal(out[:7], np.maximum(rho[:7] - 0.2, 1e-3))
+
