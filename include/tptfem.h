@@ -191,6 +191,39 @@ int tfm_system_solve_batch(tfm_system *sys, int64_t P, const int64_t *dof_off, i
                            const double *rhs, const tfm_solve_opts *opts, double *u,
                            tfm_solve_stats *stats);
 
+/* ---- SIMP topology optimization (crates/tpt-fem-topopt/src/lib.rs) ------ */
+/* TopOptParams (topopt/src/lib.rs:119-137), the Grid given by Grid::new's
+ * arguments (:89-116): nx x ny nodes, (nx-1)(ny-1) Q4 elements of side h. */
+typedef struct {
+  int64_t nx, ny;
+  double h;
+  double e0, nu, vol_frac, penal, filter_radius;
+  int max_iter;
+  double move_limit;
+} tfm_topopt_params;
+/* topopt_simp(params, f, bcs) (topopt/src/lib.rs:342-385) -> TopOptResult
+ * {densities, compliance, iterations} (:140-148).  f: host, 2*nx*ny; bcs as
+ * (dof, value) pairs with value 0 (other values: TFM_E_UNSUPPORTED).  Each
+ * state equation K(rho) u = f is solved on the GPU by the PCG of opts (NULL:
+ * FP64 PCG to rtol 1e-12) instead of the dense LU, warm-started from the
+ * previous iteration.  densities: (nx-1)(ny-1); compliance: max_iter + 1
+ * (index 0 the uniform start); iterations: max_iter; stats (may be NULL):
+ * PCG iterations and device seconds summed over the solves, worst residual. */
+int tfm_topopt_simp(tfm_ctx *ctx, const tfm_topopt_params *prm, const double *f, int64_t n_bc,
+                    const int64_t *bc_dof, const double *bc_val, const tfm_solve_opts *opts,
+                    double *densities, double *compliance, int *iterations,
+                    tfm_solve_stats *stats);
+/* Batching form of topopt_simp (no reference counterpart): P designs on the
+ * same grid and supports, design p with its own load f[p*2*nx*ny ...] and
+ * volume fraction vol_frac[p] (NULL: prm->vol_frac), one assembly, one
+ * batched PCG and one filter/OC pass per iteration for all of them.
+ * densities [P][(nx-1)(ny-1)], compliance [P][max_iter+1], iterations and
+ * stats [P]. */
+int tfm_topopt_simp_batch(tfm_ctx *ctx, int64_t P, const tfm_topopt_params *prm, const double *f,
+                          const double *vol_frac, int64_t n_bc, const int64_t *bc_dof,
+                          const double *bc_val, const tfm_solve_opts *opts, double *densities,
+                          double *compliance, int *iterations, tfm_solve_stats *stats);
+
 /* ---- drivers ------------------------------------------------------------ */
 /* solve_poisson (thermal/src/lib.rs:206-240) with constant source f, no
  * Neumann/Robin terms. */

@@ -166,9 +166,10 @@ __global__ __launch_bounds__(kBT) void kbx_dir(BView V, const VT *__restrict__ b
   bool any = false;
 #pragma unroll
   for (int k = 0; k < NR; k++) {
-    // rr0: the reference of the stop test -- the first residual, or (rr0 < 0
-    // marks a residual replacement) the replaced one
-    const bool fresh = it == 0 || S[k].rr0 < 0.0;
+    // rr0: the reference of the stop test, set before the run (||f||^2) or,
+    // when negative, taken from the first residual seen (the start, or the
+    // one a residual replacement installed)
+    const bool fresh = S[k].rr0 < 0.0;
     const double rr0 = fresh ? rr[k] : S[k].rr0;
     const bool d = s_done[k] || rr0 == 0.0 || rr[k] <= V.rtol2 * rr0 || it >= V.max_it;
     any |= !d;
@@ -457,14 +458,16 @@ __global__ void kb_dinv(int n, const int *__restrict__ rp, const int *__restrict
 }
 
 // Segment state before an iteration run: done = flags (segments whose
-// refinement is over), counters cleared.
-__global__ void kb_reset(Seg *seg, int nseg, const int *flags, int *it) {
+// refinement is over), counters cleared, the stop reference ||f||^2 (ref_ff,
+// when nonzero) or the first residual.
+__global__ void kb_reset(Seg *seg, int nseg, const int *flags, int *it, int ref_ff) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *it = 0;
   if (i < nseg) {
     seg[i].done = flags ? flags[i] : 0;
     seg[i].iters = 0;
     seg[i].breakdown = 0;
+    seg[i].rr0 = (ref_ff && seg[i].ff > 0.0) ? seg[i].ff : -1.0;
   }
 }
 
@@ -579,7 +582,7 @@ static int bb_values(tfm_system *s, BBatch &B, const VT *src, VT **dst) {
   tfm_ctx *ctx = s->ctx;
   hipStream_t st = ctx->stream;
   const int64_t nv = B.nblkst * B.bs * B.bs;
-  int rc = dalloc(ctx, dst, nv);
+  int rc = *dst ? TFM_OK : dalloc(ctx, dst, nv);
   if (rc) return rc;
   TFM_CHECK_HIP(ctx, hipMemsetAsync(*dst, 0, sizeof(VT) * nv, st));
   if (s->nnz)
@@ -601,8 +604,9 @@ static void bb_dir(const BBatch &B, const BView &V, hipStream_t st, const VT *bv
 }
 
 // Batched PCG from r (x accumulates).  Segments flagged in `flags` are left
-// alone; the others stop at ||r|| <= rtol ||r0|| (recurrence) or max_iter.
-// Two kernels per iteration, 16 iterations per graph replay.
+// alone; the others stop at ||r|| <= rtol ||f|| (recurrence residual; the
+// segments' ||f||^2 must be in place) or max_iter.  Two kernels per
+// iteration, 16 iterations per graph replay.
 template <typename VT, typename XT, int NR>
 static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p0, VT *p1, VT *q, const VT *dinv,
                 XT *x, double rtol, int max_iter, const int *flags, std::vector<Seg> &hs) {
@@ -610,7 +614,7 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
   const BView V = B.view(max_iter, rtol * rtol);
   const int nseg = B.P * B.NR;
   dim3 g(B.nblk), t(kBT);
-  hipLaunchKernelGGL(kb_reset, dim3((nseg + 255) / 256), dim3(256), 0, st, B.seg, nseg, flags, B.it);
+  hipLaunchKernelGGL(kb_reset, dim3((nseg + 255) / 256), dim3(256), 0, st, B.seg, nseg, flags, B.it, 1);
   hipLaunchKernelGGL((kbx_init<VT, NR>), g, t, 0, st, V, r, z, p0, p1, dinv);
   TFM_CHECK_HIP(ctx, hipGetLastError());
   const int CH = 16;  // iterations per replay (even: buffer parity is static)
@@ -705,7 +709,7 @@ static int bmixed(tfm_system *s, const BBatch &B, const BView &V, const double *
   // iterates on rounding noise; a frozen segment restarts its direction.
   const BView V32 = B.view(o.max_iter, kFloor2);
   const int nsb = (nseg + 255) / 256;
-  hipLaunchKernelGGL(kb_reset, dim3(nsb), dim3(256), 0, st, B.seg, nseg, odone, B.it);
+  hipLaunchKernelGGL(kb_reset, dim3(nsb), dim3(256), 0, st, B.seg, nseg, odone, B.it, 0);
   hipLaunchKernelGGL((kbx_init<float, NR>), g, t, 0, st, V32, r32, z32, p32, p32b, d32);
   const int CH = 16;
   float *pb[2] = {p32, p32b};
@@ -796,81 +800,137 @@ static int bmixed(tfm_system *s, const BBatch &B, const BView &V, const double *
   return rc;
 }
 
-template <int NR>
-static int solve_batch_nr(tfm_system *s, int P, const int64_t *dof_off, const double *rhs,
-                          const tfm_solve_opts &o, double *u, tfm_solve_stats *stats) {
-  tfm_ctx *ctx = s->ctx;
-  hipStream_t st = ctx->stream;
-  const int n = (int)s->ndof;
-  const size_t nv = (size_t)n * NR;
-  const bool mixed = o.mode == TFM_SOLVE_MIXED;
+// Host side of a batched solve: the sliced matrix, work vectors and the
+// device solution x ([n_dof][NR], kept between runs for warm starts).
+struct BatchSolver {
+  tfm_system *s = nullptr;
+  int P = 0, NR = 1;
+  bool mixed = false;
   BBatch B;
-  double *f = nullptr, *x = nullptr, *r64 = nullptr, *d64 = nullptr, *z64 = nullptr, *p64 = nullptr, *p64b = nullptr,
-         *q64 = nullptr;
+  double *f = nullptr, *x = nullptr, *r64 = nullptr, *d64 = nullptr, *z64 = nullptr, *p64 = nullptr,
+         *p64b = nullptr, *q64 = nullptr;
   float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *d32 = nullptr,
         *x32 = nullptr;
   int *bad = nullptr, *odone = nullptr;
-  auto cleanup = [&]() {
-    bb_free(B);
-    dfree(f); dfree(x); dfree(r64); dfree(d64); dfree(z64); dfree(p64); dfree(p64b); dfree(q64);
-    dfree(r32); dfree(z32); dfree(p32); dfree(p32b); dfree(q32); dfree(d32); dfree(x32); dfree(bad); dfree(odone);
-  };
-  int rc = bb_setup(s, P, dof_off, NR, B);
-  if (rc || (rc = dalloc(ctx, &f, nv)) || (rc = dalloc(ctx, &x, nv)) || (rc = dalloc(ctx, &r64, nv)) ||
-      (rc = dalloc(ctx, &d64, n)) || (rc = dalloc(ctx, &bad, 1)) || (rc = dalloc(ctx, &odone, P * NR))) {
-    cleanup();
+};
+
+void bsolver_destroy(BatchSolver *b) {
+  if (!b) return;
+  bb_free(b->B);
+  dfree(b->f); dfree(b->x); dfree(b->r64); dfree(b->d64); dfree(b->z64); dfree(b->p64); dfree(b->p64b);
+  dfree(b->q64); dfree(b->r32); dfree(b->z32); dfree(b->p32); dfree(b->p32b); dfree(b->q32); dfree(b->d32);
+  dfree(b->x32); dfree(b->bad); dfree(b->odone);
+  delete b;
+}
+
+int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, bool mixed, BatchSolver **out) {
+  tfm_ctx *ctx = s->ctx;
+  *out = nullptr;
+  if (NR != 1 && NR != 8) return fail(ctx, TFM_E_UNSUPPORTED, "batched solve supports 1 or 8 right-hand sides");
+  if (mixed && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values");
+  if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values");
+  auto *b = new BatchSolver();
+  b->s = s;
+  b->P = P;
+  b->NR = NR;
+  b->mixed = mixed;
+  const int64_t n = s->ndof;
+  const size_t nv = (size_t)n * NR;
+  int rc = bb_setup(s, P, dof_off, NR, b->B);
+  if (!rc && !((rc = dalloc(ctx, &b->f, nv)) || (rc = dalloc(ctx, &b->x, nv)) || (rc = dalloc(ctx, &b->r64, nv)) ||
+               (rc = dalloc(ctx, &b->d64, n)) || (rc = dalloc(ctx, &b->bad, 1)) ||
+               (rc = dalloc(ctx, &b->odone, P * NR)))) {
+    if (!mixed)
+      (void)((rc = dalloc(ctx, &b->z64, nv)) || (rc = dalloc(ctx, &b->p64, nv)) ||
+             (rc = dalloc(ctx, &b->p64b, nv)) || (rc = dalloc(ctx, &b->q64, nv)));
+    else
+      (void)((rc = dalloc(ctx, &b->r32, nv)) || (rc = dalloc(ctx, &b->z32, nv)) ||
+             (rc = dalloc(ctx, &b->p32, nv)) || (rc = dalloc(ctx, &b->p32b, nv)) ||
+             (rc = dalloc(ctx, &b->q32, nv)) || (rc = dalloc(ctx, &b->d32, n)) || (rc = dalloc(ctx, &b->x32, nv)));
+  }
+  if (!rc && hipMemsetAsync(b->x, 0, sizeof(double) * nv, ctx->stream) != hipSuccess)
+    rc = fail(ctx, TFM_E_HIP, "hipMemsetAsync failed");
+  if (rc) {
+    bsolver_destroy(b);
     return rc;
   }
-  if (!mixed && ((rc = dalloc(ctx, &z64, nv)) || (rc = dalloc(ctx, &p64, nv)) || (rc = dalloc(ctx, &p64b, nv)) ||
-                 (rc = dalloc(ctx, &q64, nv)) || (rc = bb_values<double>(s, B, s->d_v64, &B.v64)))) {
-    cleanup();
-    return rc;
+  *out = b;
+  return TFM_OK;
+}
+
+// Right-hand sides: host [NR][n_dof], or null for the system's own (NR = 1);
+// homogeneous-Dirichlet systems take f[fixed] = 0 (fred on the free DOFs).
+int bsolver_set_rhs(BatchSolver *b, const double *rhs) {
+  tfm_system *s = b->s;
+  tfm_ctx *ctx = s->ctx;
+  const int64_t n = s->ndof;
+  const int NR = b->NR;
+  std::vector<unsigned char> fx((size_t)n, 0);
+  TFM_CHECK_HIP(ctx, hipMemcpy(fx.data(), s->d_fixed, n, hipMemcpyDeviceToHost));
+  std::vector<double> hf((size_t)n * NR, 0.0);
+  if (rhs) {
+    for (int k = 0; k < NR; k++)
+      for (int64_t i = 0; i < n; i++) hf[(size_t)i * NR + k] = fx[i] ? 0.0 : rhs[(size_t)k * n + i];
+  } else {
+    std::vector<double> fr((size_t)n);
+    TFM_CHECK_HIP(ctx, hipMemcpy(fr.data(), s->d_f, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < NR; k++)
+      for (int64_t i = 0; i < n; i++) hf[(size_t)i * NR + k] = fr[i];
   }
-  if (mixed && ((rc = dalloc(ctx, &r32, nv)) || (rc = dalloc(ctx, &z32, nv)) || (rc = dalloc(ctx, &p32, nv)) ||
-                (rc = dalloc(ctx, &p32b, nv)) || (rc = dalloc(ctx, &q32, nv)) || (rc = dalloc(ctx, &d32, n)) ||
-                (rc = dalloc(ctx, &x32, nv)) || (rc = bb_values<float>(s, B, s->d_v32, &B.v32)))) {
-    cleanup();
-    return rc;
-  }
-  // rhs: homogeneous-Dirichlet systems take f[fixed] = 0 (fred on the free DOFs)
-  {
-    std::vector<unsigned char> fx((size_t)n, 0);
-    TFM_CHECK_HIP(ctx, hipMemcpy(fx.data(), s->d_fixed, n, hipMemcpyDeviceToHost));
-    std::vector<double> hf(nv, 0.0);
-    if (rhs) {
-      for (int k = 0; k < NR; k++)
-        for (int i = 0; i < n; i++) hf[(size_t)i * NR + k] = fx[i] ? 0.0 : rhs[(size_t)k * n + i];
-    } else {
-      std::vector<double> fr((size_t)n);
-      TFM_CHECK_HIP(ctx, hipMemcpy(fr.data(), s->d_f, sizeof(double) * n, hipMemcpyDeviceToHost));
-      for (int k = 0; k < NR; k++)
-        for (int i = 0; i < n; i++) hf[(size_t)i * NR + k] = fr[i];
-    }
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(f, hf.data(), sizeof(double) * nv, hipMemcpyHostToDevice, st));
-  }
-  TFM_CHECK_HIP(ctx, hipMemsetAsync(x, 0, sizeof(double) * nv, st));
-  TFM_CHECK_HIP(ctx, hipMemsetAsync(bad, 0, sizeof(int), st));
-  TFM_CHECK_HIP(ctx, hipMemsetAsync(odone, 0, sizeof(int) * P * NR, st));
-  unsigned g1 = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(kb_dinv<double>, dim3(g1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v64, s->d_fixed, d64, bad);
-  if (mixed)
-    hipLaunchKernelGGL(kb_dinv<float>, dim3(g1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v32, s->d_fixed, d32, bad);
+  TFM_CHECK_HIP(ctx, hipMemcpy(b->f, hf.data(), sizeof(double) * hf.size(), hipMemcpyHostToDevice));
+  return TFM_OK;
+}
+
+// Sliced copy and Jacobi of the system's current values (after a
+// re-assembly); a free DOF without a positive diagonal is a numeric error.
+int bsolver_refresh(BatchSolver *b) {
+  tfm_system *s = b->s;
+  tfm_ctx *ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int n = (int)s->ndof;
+  int rc = b->mixed ? bb_values<float>(s, b->B, s->d_v32, &b->B.v32) : bb_values<double>(s, b->B, s->d_v64, &b->B.v64);
+  if (rc) return rc;
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(b->bad, 0, sizeof(int), st));
+  const unsigned g1 = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(kb_dinv<double>, dim3(g1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v64, s->d_fixed, b->d64,
+                     b->bad);
+  if (b->mixed)
+    hipLaunchKernelGGL(kb_dinv<float>, dim3(g1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v32, s->d_fixed, b->d32,
+                       b->bad);
   int hbad = 0;
-  TFM_CHECK_HIP(ctx, hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost));
-  if (hbad) {
-    cleanup();
-    return fail(ctx, TFM_E_NUMERIC, "sparse numeric factorization/solve failed: singular matrix (a free DOF has no positive diagonal)");
-  }
-  const int nseg = P * NR;
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hbad, b->bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  if (hbad)
+    return fail(ctx, TFM_E_NUMERIC,
+                "sparse numeric factorization/solve failed: singular matrix (a free DOF has no positive diagonal)");
+  return TFM_OK;
+}
+
+const double *bsolver_x(const BatchSolver *b) { return b->x; }
+
+template <int NR>
+static int bsolver_run_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_stats *stats, double *fu) {
+  tfm_system *s = b->s;
+  tfm_ctx *ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const BBatch &B = b->B;
+  const int P = b->P, nseg = P * NR;
+  const size_t nv = (size_t)s->ndof * NR;
+  int rc = TFM_OK;
+  if (!warm) TFM_CHECK_HIP(ctx, hipMemsetAsync(b->x, 0, sizeof(double) * nv, st));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(b->odone, 0, sizeof(int) * nseg, st));
   std::vector<Seg> hs(nseg);
   std::vector<int> iters(nseg, 0), cycles(nseg, 0), brk(nseg, 0), od(nseg, 0);
   const BView V = B.view(o.max_iter, 0.0);
   dim3 g(B.nblk), t(kBT);
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev0, st));
-  if (!mixed) {
-    hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, x, f, r64, (float *)nullptr,
-                       (const int *)nullptr, (const int *)nullptr);
-    rc = bpcg<double, double, NR>(ctx, B, B.v64, r64, z64, p64, p64b, q64, d64, x, o.rtol, o.max_iter, nullptr, hs);
+  if (!b->mixed) {
+    // r = f - K x (x = 0 or the warm start), and ||f||^2 as the stop reference
+    hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, b->x, b->f, b->r64,
+                       (float *)nullptr, (const int *)nullptr, (const int *)nullptr);
+    hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
+    rc = bpcg<double, double, NR>(ctx, B, B.v64, b->r64, b->z64, b->p64, b->p64b, b->q64, b->d64, b->x, o.rtol,
+                                  o.max_iter, nullptr, hs);
     for (int q = 0; q < nseg; q++) {
       iters[q] = hs[q].iters;
       brk[q] = hs[q].breakdown;
@@ -878,43 +938,41 @@ static int solve_batch_nr(tfm_system *s, int P, const int64_t *dof_off, const do
       od[q] = hs[q].done && !hs[q].breakdown && hs[q].iters < o.max_iter;  // converged on its recurrence residual
     }
   } else {
-    rc = bmixed<NR>(s, B, V, f, x, r64, r32, z32, p32, p32b, q32, d32, x32, odone, o, od, iters, cycles, brk);
+    rc = bmixed<NR>(s, B, V, b->f, b->x, b->r64, b->r32, b->z32, b->p32, b->p32b, b->q32, b->d32, b->x32, b->odone,
+                    o, od, iters, cycles, brk);
   }
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, st));
   // final FP64 residual certificate of every segment
-  hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, x, f, r64, (float *)nullptr,
-                     (const int *)nullptr, (const int *)nullptr);
+  hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, b->x, b->f, b->r64,
+                     (float *)nullptr, (const int *)nullptr, (const int *)nullptr);
   hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
-  std::vector<double> hx(nv);
-  TFM_CHECK_HIP(ctx, hipMemcpyAsync(hx.data(), x, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-  cleanup();
   if (rc) return rc;
-  // u[k][*]: free entries from x, then Dirichlet values in bcs order
   bool worst_ok = true;
-  for (int k = 0; k < NR; k++) {
-    double *uk = u + (size_t)k * n;
-    for (int i = 0; i < n; i++) uk[i] = 0.0;
-    for (int64_t fdof : s->free_dofs) uk[fdof] = hx[(size_t)fdof * NR + k];
-    for (size_t b = 0; b < s->bc_dof.size(); b++) uk[s->bc_dof[b]] = s->bc_val[b];
-  }
   for (int q = 0; q < nseg; q++) {
     const double rel = hs[q].ff > 0 ? std::sqrt(hs[q].rt / hs[q].ff) : 0.0;
     // plain: PCG converged on its recurrence residual (the true residual is
     // reported); mixed: the FP64 refinement residual itself met rtol
-    worst_ok &= (mixed ? rel <= o.rtol : od[q] != 0) && !brk[q];
+    worst_ok &= (b->mixed ? rel <= o.rtol : od[q] != 0) && !brk[q];
     if (stats) {
       stats[q].iterations = iters[q];
       stats[q].outer = cycles[q];
       stats[q].rel_residual = rel;
       stats[q].seconds = ms * 1e-3;
     }
+    if (fu) fu[q] = hs[q].fu;
   }
   if (!worst_ok) return fail(ctx, TFM_E_NOT_CONVERGED, "a batched segment missed the tolerance");
   return TFM_OK;
+}
+
+// One batched solve from x = 0 (or, warm, from the last solution): per
+// segment stats and f.u (the compliance of a load case) when asked for.
+int bsolver_run(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_stats *stats, double *fu) {
+  return b->NR == 1 ? bsolver_run_nr<1>(b, o, warm, stats, fu) : bsolver_run_nr<8>(b, o, warm, stats, fu);
 }
 
 }  // namespace tfm
@@ -1055,10 +1113,30 @@ extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *d
   if (rc) return rc;
   tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 0.3, 60};
   if (opts) o = *opts;
-  if (o.mode == TFM_SOLVE_MIXED && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values");
-  if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values");
   // A segment's rows must not couple to another problem's rows (block
   // diagonal): the kernels reduce per problem and would silently mix them.
-  if (nrhs == 1) return solve_batch_nr<1>(s, (int)P, dof_off, rhs, o, u, stats);
-  return solve_batch_nr<8>(s, (int)P, dof_off, rhs, o, u, stats);
+  BatchSolver *b = nullptr;
+  if ((rc = bsolver_create(s, (int)P, dof_off, nrhs, o.mode == TFM_SOLVE_MIXED, &b))) return rc;
+  if ((rc = bsolver_set_rhs(b, rhs)) || (rc = bsolver_refresh(b))) {
+    bsolver_destroy(b);
+    return rc;
+  }
+  rc = bsolver_run(b, o, false, stats, nullptr);
+  if (rc && rc != TFM_E_NOT_CONVERGED) {
+    bsolver_destroy(b);
+    return rc;
+  }
+  const int64_t n = s->ndof;
+  std::vector<double> hx((size_t)n * nrhs);
+  hipError_t e = hipMemcpy(hx.data(), b->x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost);
+  bsolver_destroy(b);
+  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  // u[k][*]: free entries from x, then Dirichlet values in bcs order
+  for (int k = 0; k < nrhs; k++) {
+    double *uk = u + (size_t)k * n;
+    for (int64_t i = 0; i < n; i++) uk[i] = 0.0;
+    for (int64_t fdof : s->free_dofs) uk[fdof] = hx[(size_t)fdof * nrhs + k];
+    for (size_t q = 0; q < s->bc_dof.size(); q++) uk[s->bc_dof[q]] = s->bc_val[q];
+  }
+  return rc;
 }

@@ -183,6 +183,18 @@ struct BSell {
 };
 void bsell_destroy(BSell *b);
 
+// Batched solver (tfm_batch.hip) over a block-diagonal system of P problems
+// (DOF ranges dof_off[p]..dof_off[p+1], homogeneous Dirichlet values), NR
+// (1 or 8) right-hand sides each; the device solution stays in place between
+// runs, so a run can warm-start from the previous one.
+struct BatchSolver;
+int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, bool mixed, BatchSolver **out);
+int bsolver_set_rhs(BatchSolver *b, const double *rhs);
+int bsolver_refresh(BatchSolver *b);
+int bsolver_run(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_stats *stats, double *fu);
+const double *bsolver_x(const BatchSolver *b);  // [n_dof][NR], fixed DOFs 0
+void bsolver_destroy(BatchSolver *b);
+
 // Loads (tfm_loads.hip): device rhs of n_nodes*dpn doubles, accumulated.
 int load_source_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double f, double *d_rhs);
 int load_body_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *b, double *d_rhs);

@@ -77,6 +77,12 @@ class SolveStats(C.Structure):
     _fields_ = [("iterations", INT), ("outer", INT), ("rel_residual", D), ("seconds", D)]
 
 
+class TopOptParams(C.Structure):
+    """TopOptParams (crates/tpt-fem-topopt/src/lib.rs:119-137), the grid as Grid::new(nx, ny, h)."""
+    _fields_ = [("nx", I64), ("ny", I64), ("h", D), ("e0", D), ("nu", D), ("vol_frac", D), ("penal", D),
+                ("filter_radius", D), ("max_iter", INT), ("move_limit", D)]
+
+
 _SIGS = {
     "tfm_ctx_create": (INT, [INT, PP]),
     "tfm_ctx_destroy": (None, [P]),
@@ -112,6 +118,9 @@ _SIGS = {
     "tfm_mesh_box_batch": (INT, [P, I64, P, PP, P, P]),
     "tfm_mesh_grid_batch": (INT, [P, I64, I64, I64, D, PP]),
     "tfm_system_solve_batch": (INT, [P, I64, P, INT, P, C.POINTER(SolveOpts), P, P]),
+    "tfm_topopt_simp": (INT, [P, C.POINTER(TopOptParams), P, I64, P, P, C.POINTER(SolveOpts), P, P, P, P]),
+    "tfm_topopt_simp_batch": (INT, [P, I64, C.POINTER(TopOptParams), P, P, I64, P, P, C.POINTER(SolveOpts), P, P,
+                                    P, P]),
     "tfm_solve_poisson": (INT, [P, P, D, INT, D, I64, P, P, C.POINTER(SolveOpts), P]),
     "tfm_solve_elasticity": (INT, [P, P, INT, D, D, INT, P, I64, P, P, C.POINTER(SolveOpts), P]),
 }
@@ -488,3 +497,39 @@ def solve_elasticity(mesh, model, young, nu, quad_order, body, bcs, rtol=1e-13):
     mesh.ctx.check(_lib.tfm_solve_elasticity(mesh.ctx.h, mesh.h, model, young, nu, quad_order, _ptr(b), nb,
                                              _ptr(bd), _ptr(bv), C.byref(o), _ptr(u)))
     return u
+
+
+class TopOptResult:
+    """TopOptResult (crates/tpt-fem-topopt/src/lib.rs:140-148) plus the solver statistics."""
+
+    def __init__(self, densities, compliance, iterations, stats):
+        self.densities, self.compliance, self.iterations, self.stats = densities, compliance, iterations, stats
+
+
+def topopt_simp(ctx, nx, ny, h, e0, nu, vol_frac, penal, filter_radius, max_iter, move_limit, f, bcs,
+                mode=SOLVE_PCG64, rtol=1e-12, pcg_max_iter=100000, inner_rtol=0.3):
+    """topopt_simp(params, f, bcs) (crates/tpt-fem-topopt/src/lib.rs:342-385) on the GPU.
+
+    With f of shape (P, 2*nx*ny) (and vol_frac a scalar or P values) the P
+    designs run as one batch; the result's arrays then gain a leading P axis."""
+    prm = TopOptParams(nx=nx, ny=ny, h=h, e0=e0, nu=nu, vol_frac=float(np.ravel(vol_frac)[0]), penal=penal,
+                       filter_radius=filter_radius, max_iter=max_iter, move_limit=move_limit)
+    ff = _f64(f)
+    batched = ff.ndim == 2
+    ff = ff.reshape(-1, 2 * nx * ny)
+    Pn = ff.shape[0]
+    vf = _f64(np.broadcast_to(np.ravel(vol_frac), (Pn,)))
+    nb, bd, bv = _bc_arrays(bcs)
+    ne = (nx - 1) * (ny - 1)
+    rho = np.zeros((Pn, ne))
+    comp = np.zeros((Pn, max_iter + 1))
+    its = np.zeros(Pn, dtype=np.int32)
+    st = (SolveStats * Pn)()
+    o = SolveOpts(mode=mode, rtol=rtol, max_iter=pcg_max_iter, inner_rtol=inner_rtol, max_outer=60)
+    ctx.check(_lib.tfm_topopt_simp_batch(ctx.h, Pn, C.byref(prm), _ptr(ff), _ptr(vf), nb, _ptr(bd), _ptr(bv),
+                                         C.byref(o), _ptr(rho), _ptr(comp), its.ctypes.data,
+                                         C.cast(st, C.c_void_p)))
+    stats = [(s.iterations, s.outer, s.rel_residual, s.seconds) for s in st]
+    if batched:
+        return TopOptResult(rho, comp, its.astype(np.int64), stats)
+    return TopOptResult(rho[0], comp[0], int(its[0]), stats[0])
