@@ -753,7 +753,8 @@ static int bmixed(tfm_system *s, const BBatch &B, const BView &V, const double *
       } else {
         since[q]++;
       }
-      if (hs[q].rr <= delta * delta * rr_ref[q] || hs[q].done || since[q] >= kStall || done_it >= o.max_iter) {
+      if (hs[q].rr <= delta * delta * rr_ref[q] || hs[q].rr <= o.rtol * o.rtol * hs[q].ff || hs[q].done ||
+          since[q] >= kStall || done_it >= o.max_iter) {
         hsel[q] = 1;
         any = true;
       }

@@ -632,6 +632,7 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
   *relres = h.rr0 > 0 ? std::sqrt(h.rr / h.rr0) : 0.0;
   if (*relres <= rtol) return TFM_OK;
   double rr_ref = h.rr;
+  const double target2 = rtol * rtol * h.rr0;  // ||f||^2 rtol^2
   TFM_CHECK_HIP(ctx, hipMemsetAsync(x32, 0, sizeof(float) * n, st));
   hipLaunchKernelGGL((k_pcg_init<float>), dim3(nb), dim3(kBlock), 0, st, n, B.r, B.z, B.p, B.p2, B.dinv, part);
   hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, st, sc, part, nb, kFloor2, max_iter);
@@ -674,14 +675,16 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
     // (done) only at its accuracy floor -- 1e-6 below the last replacement --
     // or on a breakdown (p.q <= 0), and then restarts its direction.  The
     // stagnation guard also replaces when the smallest recurrence residual
-    // has not improved by 10% for kStall chunks.
+    // has not improved by 10% for kStall chunks.  A recurrence residual below
+    // the final target triggers the check at once, so the last cycle stops
+    // near rtol rather than at the next multiple of delta.
     if (h.rr < 0.81 * best) {
       best = h.rr;
       since = 0;
     } else {
       since++;
     }
-    if (h.rr <= delta * delta * rr_ref || h.done || since >= kStall) {
+    if (h.rr <= delta * delta * rr_ref || h.rr <= target2 || h.done || since >= kStall) {
       hipLaunchKernelGGL(k_fold, dim3(g1), dim3(256), 0, st, n, x, x32);
       residual_dispatch(S, st, nbr, n, rp, ci, v64, x, f, r64, B.r, sc, part);
       Scal h2{};
