@@ -279,6 +279,28 @@ def test_plane_traction_and_neumann_match_oracle(ctx):
     assert abs(gt.sum() - (2.0 + np.sqrt(2.0))) < 1e-9
 
 
+@pytest.mark.parametrize("axis,coord,tol", [(0, 1.0, 1e-9), (1, 0.0, 0.3), (2, 0.62, 0.25), (0, 1.0, 0.45)])
+def test_plane_traction_wide_bands_on_jittered_mesh(ctx, axis, coord, tol):
+    """Traction bands that cut through elements of a jittered box: faces with
+    some nodes inside the band and some outside, quadrature points on either
+    side, and planes with no boundary face at all -- bit-identical to the
+    oracle."""
+    T = tp()
+    n = (5, 4, 4)
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], n)
+    coords = O.jitter_interior(coords, n, 0.2, 11)
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    t = [0.25, -1.5, 1e-3]
+    g = T.plane_traction(m, 3, axis, coord, tol, t)
+    r = O.plane_traction(coords, O.TET, conn, 3, axis, coord, tol, t)
+    assert np.array_equal(g, r)
+    qc, qe = O.grid(6, 5, 0.2)
+    mq = T.Mesh.from_arrays(ctx, T.QUAD, qc, qe)
+    g = T.plane_traction(mq, 2, axis % 2, coord * 0.8, tol, t[:2])
+    r = O.plane_traction(qc, O.QUAD, qe, 2, axis % 2, coord * 0.8, tol, t[:2])
+    assert np.array_equal(g, r)
+
+
 # ---- sparse API -------------------------------------------------------------
 def test_coo_to_csr_known_answer_and_matvec(ctx):
     T = tp()

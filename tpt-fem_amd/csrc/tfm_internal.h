@@ -198,8 +198,9 @@ void bsolver_destroy(BatchSolver *b);
 // Loads (tfm_loads.hip): device rhs of n_nodes*dpn doubles, accumulated.
 int load_source_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double f, double *d_rhs);
 int load_body_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *b, double *d_rhs);
+// zeroed: d_rhs holds +0.0 everywhere, so faces that can only add +0.0 may be skipped.
 int load_neumann_dev(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int comp, double g, int axis,
-                     double coord, double tol, double *d_rhs);
+                     double coord, double tol, double *d_rhs, bool zeroed);
 
 }  // namespace tfm
 

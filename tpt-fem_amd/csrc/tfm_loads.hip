@@ -86,10 +86,18 @@ __constant__ FaceTab c_faces[5] = {
 // quadrature point, with g = gconst, or -- when axis >= 0 -- gconst only at
 // points with |x[axis] - coord| <= tol and 0 elsewhere.  comp < 0 tiles the
 // load over all dpn components (apply_neumann's behaviour).
+//
+// skip_far (axis >= 0 and rhs holding +0.0 at every node, as in the traction
+// composition): a face whose nodes all lie outside the band |x[axis] -
+// coord| <= tol (with a rounding margin) cannot have a quadrature point in
+// it -- the points are convex combinations of the nodes -- so it would only
+// add +0.0 to sums that start at +0.0; such faces are skipped before the
+// boundary test, which is the expensive part.
 __global__ void k_load_neumann(const double *__restrict__ coords, int pdim, const int *__restrict__ conn, int nn,
                                int cell, const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
                                int64_t n_nodes, const double *__restrict__ frule, int fnq, int dpn, int comp,
-                               double gconst, int axis, double coord, double tol, double *__restrict__ rhs) {
+                               double gconst, int axis, double coord, double tol, int skip_far,
+                               double *__restrict__ rhs) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   const FaceTab &FT = c_faces[cell];
@@ -106,6 +114,18 @@ __global__ void k_load_neumann(const double *__restrict__ coords, int pdim, cons
       for (int k = 0; k < FT.fn_; k++)
         if (FT.idx[fi][k] == ln) kself = k;
       if (kself < 0) continue;
+      if (skip_far) {
+        double lo = coords[(int64_t)en[FT.idx[fi][0]] * pdim + axis], hi = lo;
+        bool finite = lo == lo;
+        for (int k = 1; k < FT.fn_; k++) {
+          const double x = coords[(int64_t)en[FT.idx[fi][k]] * pdim + axis];
+          finite &= x == x;
+          lo = x < lo ? x : lo;
+          hi = x > hi ? x : hi;
+        }
+        const double margin = 8.0 * 2.220446049250313e-16 * (fabs(lo) + fabs(hi)) + 1e-300;
+        if (finite && (hi + margin < coord - tol || lo - margin > coord + tol)) continue;
+      }
       // boundary iff no other element holds every node of this face
       bool shared = false;
       for (int p2 = pb; p2 < pe && !shared; p2++) {
@@ -217,7 +237,7 @@ int load_body_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *b, doub
 }
 
 int load_neumann_dev(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int comp, double g, int axis, double coord,
-                     double tol, double *d_rhs) {
+                     double tol, double *d_rhs, bool zeroed) {
   int rc;
   if ((rc = ensure_adjacency(mesh))) return rc;
   // face_quad (assembly/src/lib.rs:336-358) at FACE_QUAD_ORDER = 2: Line
@@ -229,7 +249,8 @@ int load_neumann_dev(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int comp, double g, 
   if (mesh->n_nodes > 0)
     hipLaunchKernelGGL(k_load_neumann, dim3(grid1(mesh->n_nodes, 128)), dim3(128), 0, ctx->stream, mesh->d_coords,
                        mesh->pdim, mesh->d_conn, mesh->nn, mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, mesh->n_nodes,
-                       d_rule, nq, dpn, comp, g, axis, coord, tol, d_rhs);
+                       d_rule, nq, dpn, comp, g, axis, coord, tol, (int)(zeroed && axis >= 0 && axis < mesh->pdim),
+                       d_rhs);
   hipError_t e = hipStreamSynchronize(ctx->stream);
   dfree(d_rule);
   if (e != hipSuccess) return fail(ctx, TFM_E_HIP, "Neumann kernel failed");
@@ -288,7 +309,8 @@ extern "C" int tfm_load_neumann(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, dou
   return host_rhs_roundtrip(ctx, mesh, dpn, rhs,
                             [](tfm_ctx *c, tfm_mesh *m, double *d, const void *p) {
                               const NeuArg *s = (const NeuArg *)p;
-                              return load_neumann_dev(c, m, s->dpn, s->comp, s->g, s->axis, s->coord, s->tol, d);
+                              return load_neumann_dev(c, m, s->dpn, s->comp, s->g, s->axis, s->coord, s->tol, d,
+                                                      false);
                             },
                             &a);
 }
@@ -315,7 +337,7 @@ extern "C" int tfm_load_plane_traction(tfm_ctx *ctx, const tfm_mesh *mesh, int d
       rc = fail(ctx, TFM_E_HIP, "memset failed");
       break;
     }
-    rc = load_neumann_dev(ctx, const_cast<tfm_mesh *>(mesh), 1, 0, t[c], axis, coord, tol, d_buf);
+    rc = load_neumann_dev(ctx, const_cast<tfm_mesh *>(mesh), 1, 0, t[c], axis, coord, tol, d_buf, true);
     if (!rc && nn > 0)
       hipLaunchKernelGGL(k_interleave_add, dim3(grid1(nn, 256)), dim3(256), 0, ctx->stream, d_buf, nn, dpn, c, d_rhs);
   }
