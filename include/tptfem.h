@@ -37,7 +37,9 @@ enum { TFM_BAR = 0, TFM_PLANE_STRESS = 1, TFM_PLANE_STRAIN = 2, TFM_CONTINUUM3D 
 enum {
   TFM_OP_POISSON = 0,    /* poisson_element_matrix, thermal/src/lib.rs:132-160      */
   TFM_OP_ELASTICITY = 1, /* elasticity_element_matrix, elasticity/src/lib.rs:262-309 */
-  TFM_OP_SIMP = 2        /* topopt assemble, topopt/src/lib.rs:166-238 (Q4 only)      */
+  TFM_OP_SIMP = 2,       /* topopt assemble, topopt/src/lib.rs:166-238 (Q4 only)      */
+  TFM_OP_MASS = 3        /* elasticity_mass_matrix, elasticity/src/lib.rs:386-428:
+                            coef = density, dofs_per_node = the cell's dimension  */
 };
 enum { TFM_FP64 = 0, TFM_FP32 = 1 };
 
@@ -103,6 +105,11 @@ void tfm_mesh_destroy(tfm_mesh *mesh);
  * element entries skipped, columns ascending, nrows = max row + 1. */
 int tfm_assemble(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op,
                  int dofs_per_node, tfm_matrix **out);
+/* elasticity_lumped_mass(mesh, model, density, quad_order).to_csr()
+ * (elasticity/src/lib.rs:432-457): the diagonal CSR whose entry on each DOF
+ * of a node is the sum of the node's consistent-mass rows over dim.  */
+int tfm_lumped_mass(tfm_ctx *ctx, const tfm_mesh *mesh, double density, int quad_order,
+                    tfm_matrix **out);
 /* A CSR matrix from host arrays (a Csr built by the caller). */
 int tfm_matrix_create(tfm_ctx *ctx, int64_t nrows, int64_t ncols, const int64_t *row_ptrs,
                       const int64_t *col_ind, const double *values, tfm_matrix **out);

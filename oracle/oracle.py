@@ -78,6 +78,8 @@ def _declare(L):
         "orc_dense_solve": (INT, [I64, P, P]),
         "orc_solve_coo": (INT, [I64, P, P, P, I64, P]),
         "orc_solve_with_dirichlet": (INT, [I64, P, P, P, I64, P, I64, P, P, P]),
+        "orc_mass_element_matrix": (None, [INT, INT, P, D, INT, P]),
+        "orc_lumped_mass": (I64, [I64, P, P, INT, P]),
         "orc_topopt_quad_form": (D, [P, P, P]),
         "orc_topopt_filter": (None, [I64, I64, D, P, D, P]),
         "orc_topopt_oc": (None, [I64, P, P, D, D, P]),
@@ -185,6 +187,29 @@ def elasticity_element_matrix(cell, phys, model, young, nu, order=2):
     if rc != 0:
         raise ValueError("model/dimension mismatch")
     return k.reshape(nd, nd)
+
+
+def mass_element_matrix(cell, phys, density=1.0, order=2):
+    """elasticity_mass_matrix's element block (crates/tpt-fem-elasticity/src/lib.rs:392-424)."""
+    phys = f64(phys)
+    nd = NODES[cell] * DIMS[cell]
+    k = np.zeros(nd * nd)
+    lib().orc_mass_element_matrix(cell, phys.shape[1], _p(phys), density, order, _p(k))
+    return k.reshape(nd, nd)
+
+
+def mass_matrix(coords, cell, conn, density=1.0, order=2):
+    """elasticity_mass_matrix (crates/tpt-fem-elasticity/src/lib.rs:386-428) -> Coo."""
+    return assemble(coords, conn, make_op(cell, f64(coords).shape[1], 3, order=order, coef=density), DIMS[cell])
+
+
+def lumped_mass(coords, cell, conn, density=1.0, order=2):
+    """elasticity_lumped_mass (crates/tpt-fem-elasticity/src/lib.rs:432-457) -> Coo (diagonal)."""
+    csr = mass_matrix(coords, cell, conn, density, order).to_csr()
+    d = np.zeros(max(csr.nrows, 1))
+    m = lib().orc_lumped_mass(csr.nrows, _p(csr.row_ptrs), _p(csr.values), DIMS[cell], _p(d))
+    idx = np.arange(m, dtype=np.int64)
+    return Coo(idx, idx, d[:m].copy())
 
 
 def simp_ke_unit(nu, h=1.0):

@@ -251,7 +251,7 @@ static double dotn(int n, const double *a, const double *b) {
 /* poisson_element_matrix: crates/tpt-fem-thermal/src/lib.rs:132-160.      */
 void orc_poisson_element_matrix(int cell, int pdim, const double *phys,
                                 double kappa, int order, double *k) {
-  double qp[27 * 3], qw[27], lg[8 * 3], jac[9], inv[9], g[8 * 3];
+  double qp[125 * 3], qw[125], lg[8 * 3], jac[9], inv[9], g[8 * 3];
   int nn = orc_cell_nodes(cell), dim = orc_cell_dim(cell);
   int nq = orc_cell_quad(cell, order, qp, qw);
   for (int i = 0; i < nn * nn; i++) k[i] = 0.0;
@@ -270,7 +270,7 @@ void orc_poisson_element_matrix(int cell, int pdim, const double *phys,
  * formed, as in the reference, but a constant source ignores it).          */
 void orc_poisson_source_vector(int cell, int pdim, const double *phys,
                                double f_const, int order, double *f) {
-  double qp[27 * 3], qw[27], ns[8], lg[8 * 3], jac[9], inv[9];
+  double qp[125 * 3], qw[125], ns[8], lg[8 * 3], jac[9], inv[9];
   int nn = orc_cell_nodes(cell), dim = orc_cell_dim(cell);
   int nq = orc_cell_quad(cell, order, qp, qw);
   for (int i = 0; i < nn; i++) f[i] = 0.0;
@@ -329,7 +329,7 @@ static void b_matrix(const double *g, int dim, double b[6][3]) {
 int orc_elasticity_element_matrix(int cell, int pdim, const double *phys,
                                   int model, double young, double nu,
                                   int order, double *k) {
-  double qp[27 * 3], qw[27], lg[8 * 3], jac[9], inv[9], g[8 * 3];
+  double qp[125 * 3], qw[125], lg[8 * 3], jac[9], inv[9], g[8 * 3];
   double d[6][6], b[8][6][3];
   int nn = orc_cell_nodes(cell), dim = orc_cell_dim(cell);
   int nstr = dim == 1 ? 1 : (dim == 2 ? 3 : 6);
@@ -475,6 +475,48 @@ int64_t orc_nodes_on_plane(int64_t nnodes, int pdim, const double *coords, int a
   return m;
 }
 
+/* elasticity_mass_matrix's element block: crates/tpt-fem-elasticity/src/lib.rs
+ * :392-414.  m_ij = sum_q w * det * rho * N_i * N_j (left to right, from 0.0),
+ * det = |det J| of Map::from_nodes_and_grad; k is the (nn*dim)^2 matrix whose
+ * only non-zeros are the component-diagonal copies m_ij (:418-424 pushes
+ * (n_i*dim + a, n_j*dim + a) for a < dim, skipping m_ij == 0).              */
+void orc_mass_element_matrix(int cell, int pdim, const double *phys, double density, int order,
+                             double *k) {
+  double qp[125 * 3], qw[125], ns[8], lg[8 * 3], jac[9], inv[9], m[64];
+  int nn = orc_cell_nodes(cell), dim = orc_cell_dim(cell), nd = nn * dim;
+  int nq = orc_cell_quad(cell, order, qp, qw);
+  for (int i = 0; i < nn * nn; i++) m[i] = 0.0;
+  for (int q = 0; q < nq; q++) {
+    orc_grad(cell, qp + 3 * q, lg);
+    double det = fabs(orc_map(nn, pdim, phys, dim, lg, jac, inv));
+    orc_shape(cell, qp + 3 * q, ns);
+    for (int i = 0; i < nn; i++)
+      for (int j = 0; j < nn; j++) m[i * nn + j] += qw[q] * det * density * ns[i] * ns[j];
+  }
+  for (int i = 0; i < nd * nd; i++) k[i] = 0.0;
+  for (int i = 0; i < nn; i++)
+    for (int j = 0; j < nn; j++)
+      for (int a = 0; a < dim; a++) k[(i * dim + a) * nd + j * dim + a] = m[i * nn + j];
+}
+
+/* elasticity_lumped_mass: crates/tpt-fem-elasticity/src/lib.rs:432-457 on the
+ * consistent mass CSR: per node the sum of its dim rows (row order, then CSR
+ * order, from 0.0) divided by dim, on each of its dim diagonal entries.
+ * Returns the number of diagonal entries, (nrows / dim) * dim.              */
+int64_t orc_lumped_mass(int64_t nrows, const int64_t *rp, const double *vals, int dim, double *diag) {
+  int64_t nodes = nrows / dim;
+  for (int64_t node = 0; node < nodes; node++) {
+    double total = 0.0;
+    for (int a = 0; a < dim; a++) {
+      int64_t row = node * dim + a;
+      for (int64_t c = rp[row]; c < rp[row + 1]; c++) total += vals[c];
+    }
+    double lump = total / (double)dim;
+    for (int a = 0; a < dim; a++) diag[node * dim + a] = lump;
+  }
+  return nodes * dim;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Element operator dispatch used by orc_assemble.                           */
 static int elem_matrix(const orc_op *op, const double *phys, int64_t e, double *k) {
@@ -497,6 +539,9 @@ static int elem_matrix(const orc_op *op, const double *phys, int64_t e, double *
     for (int i = 0; i < 64; i++) k[i] = e_rho * op->ke0[i];
     return 8;
   }
+  case 3: /* consistent mass, density in coef (or per element in coef_e) */
+    orc_mass_element_matrix(op->cell, op->pdim, phys, op->coef_e ? op->coef_e[e] : op->coef, op->order, k);
+    return orc_cell_nodes(op->cell) * orc_cell_dim(op->cell);
   }
   return -1;
 }
@@ -511,7 +556,7 @@ int64_t orc_assemble(int64_t nnodes, const double *coords, int64_t nelem,
   (void)nnodes;
   int nn = orc_cell_nodes(op->cell);
   double phys[8 * 3];
-  double *k = (double *)malloc(sizeof(double) * 24 * 24);
+  double *k = (double *)malloc(sizeof(double) * 24 * 24);  /* Hex8 x 3 DOFs */
   int64_t t = 0;
   for (int64_t e = 0; e < nelem; e++) {
     for (int a = 0; a < nn; a++)
@@ -693,7 +738,7 @@ int orc_solve_coo(int64_t ntrip, const int64_t *rows, const int64_t *cols,
   int64_t *ci = (int64_t *)malloc(sizeof(int64_t) * (ntrip ? ntrip : 1));
   double *v = (double *)malloc(sizeof(double) * (ntrip ? ntrip : 1));
   orc_to_csr(ntrip, rows, cols, vals, n, rp, ci, v);
-  double *a = (double *)calloc((size_t)(n * n ? n * n : 1), sizeof(double));
+  double *a = (double *)calloc((size_t)(n > 0 ? n * n : 1), sizeof(double));
   for (int64_t r = 0; r < n; r++)
     for (int64_t c = rp[r]; c < rp[r + 1]; c++) a[r + ci[c] * n] += v[c];
   int rc = orc_dense_solve(n, a, b);
@@ -838,7 +883,7 @@ static void neumann_pass(int64_t nnodes, int pdim, const double *coords, int64_t
     for (int a = 0; a < fd->nn; a++)
       for (int d = 0; d < dim; d++) fcen[d] += phys[fd->idx[a]][d];
     for (int d = 0; d < dim; d++) fcen[d] /= (double)fd->nn;
-    double qp[27 * 3], qw[27];
+    double qp[125 * 3], qw[125];
     int nq = 0;
     if (fd->fcell == ORC_LINE && fd->nn == 1) {
       /* point face of a Line2 element: GL rule of the given order on a 0-d

@@ -62,7 +62,7 @@ int64_t orc_nodes_on_plane(int64_t nnodes, int pdim, const double *coords,
 typedef struct {
   int cell;        /* ORC_* */
   int pdim;        /* coordinate components per node */
-  int kind;        /* 0 poisson, 1 elasticity, 2 simp-q4 */
+  int kind;        /* 0 poisson, 1 elasticity, 2 simp-q4, 3 consistent mass (density in coef) */
   int model;       /* elastic model */
   int order;       /* quadrature order */
   double coef;     /* kappa or young when coef_e == NULL */
@@ -74,6 +74,9 @@ typedef struct {
   const double *ke0;
 } orc_op;
 
+void orc_mass_element_matrix(int cell, int pdim, const double *phys, double density, int order,
+                             double *k);
+int64_t orc_lumped_mass(int64_t nrows, const int64_t *rp, const double *vals, int dim, double *diag);
 int64_t orc_assemble(int64_t nnodes, const double *coords, int64_t nelem,
                      const int64_t *conn, const orc_op *op, int dpn,
                      int64_t *rows, int64_t *cols, double *vals);

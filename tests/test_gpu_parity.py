@@ -301,6 +301,49 @@ def test_plane_traction_wide_bands_on_jittered_mesh(ctx, axis, coord, tol):
     assert np.array_equal(g, r)
 
 
+def _mass_meshes():
+    c, e = O.box_mesh([0, 0, 0], [1.0, 0.7, 1.3], (3, 2, 3))
+    yield "tet", O.TET, O.jitter_interior(c, (3, 2, 3), 0.2, 3), e
+    hc, he = hex_box((2, 3, 2))
+    yield "hex", O.HEX, perturb(hc, 0.05, 4), he
+    tc, te = tri_grid(5, 4)
+    yield "tri", O.TRI, perturb(tc, 0.03, 5), te
+    qc, qe = O.grid(5, 4, 0.3)
+    yield "quad", O.QUAD, perturb(qc, 0.03, 6), qe
+    lc = np.linspace(0.0, 2.0, 7)[:, None] ** 1.3
+    yield "line", O.LINE, lc, np.array([[i, i + 1] for i in range(6)], dtype=np.int64)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_mass_matrices_bitexact(ctx, order):
+    """Consistent mass (elasticity_mass_matrix) for every cell type and the
+    lumped diagonal built from it, bit-identical to the oracle; per-element
+    densities through the operator's coefficient array."""
+    T = tp()
+    for name, cell, c, e in _mass_meshes():
+        m = T.Mesh.from_arrays(ctx, cell, c, e)
+        g = T.mass_matrix(m, 2.5, order).to_csr()
+        r = O.mass_matrix(c, cell, e, 2.5, order).to_csr()
+        csr_equal(g, r)
+        gl = T.lumped_mass(m, 2.5, order).to_csr()
+        rl = O.lumped_mass(c, cell, e, 2.5, order).to_csr()
+        csr_equal(gl, rl)
+        rho_e = np.random.default_rng(order).uniform(0.5, 2.0, len(e))
+        dim = O.DIMS[cell]
+        g = T.assemble(m, dim, T.Operator.mass(1.0, order, density_e=rho_e)).to_csr()
+        r = O.assemble(c, e, O.make_op(cell, c.shape[1], 3, order=order, coef_e=rho_e), dim).to_csr()
+        csr_equal(g, r)
+
+
+def test_mass_contract(ctx):
+    """The mass operator has one DOF per spatial dimension."""
+    T = tp()
+    c, e = O.box_mesh([0, 0, 0], [1, 1, 1], (1, 1, 1))
+    m = T.Mesh.from_arrays(ctx, T.TET, c, e)
+    with pytest.raises(T.TfmError):
+        T.assemble(m, 1, T.Operator.mass(1.0))
+
+
 # ---- sparse API -------------------------------------------------------------
 def test_coo_to_csr_known_answer_and_matvec(ctx):
     T = tp()

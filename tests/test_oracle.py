@@ -398,3 +398,49 @@ def test_simp_oc_closed_form_and_bounds():
     assert np.all(out >= np.maximum(rho - 0.2, 1e-3) - 1e-15) and np.all(out <= np.minimum(rho + 0.2, 1.0) + 1e-15)
     np.testing.assert_array_equal(out[:7], np.maximum(rho[:7] - 0.2, 1e-3))
 
+
+
+# ---- consistent / lumped mass (crates/tpt-fem-elasticity/src/lib.rs:386-457) ------
+def test_mass_element_closed_forms():
+    """Exact-integration closed forms: P1 tet rho V/20 (1 + d_ij), Tri3
+    rho A/12 (1 + d_ij), Line2 rho L/6 (1 + d_ij), and the tensor products
+    rho V/6^d * 2^(#shared coordinates) for axis-aligned Q4/Hex8 at order 2;
+    each copied on the component diagonal only (nothing across components)."""
+    rng = np.random.default_rng(21)
+    X = np.array([[0, 0, 0], [1.1, 0.1, 0], [0.2, 0.9, 0.1], [0.3, 0.2, 1.2]]) + rng.uniform(0, 0.1, (4, 3))
+    V = abs(np.linalg.det((X[1:] - X[0]).T)) / 6
+    k = O.mass_element_matrix(O.TET, X, 2.5, 2)
+    np.testing.assert_allclose(k[0::3, 0::3], 2.5 * V / 20 * (1 + np.eye(4)), rtol=1e-14)
+    assert np.count_nonzero(k) == 48 and np.array_equal(k[0::3, 0::3], k[2::3, 2::3])
+    T3 = np.array([[0.1, 0.2], [1.3, 0.1], [0.4, 0.8]])
+    A = abs(np.linalg.det(T3[1:] - T3[0])) / 2
+    np.testing.assert_allclose(O.mass_element_matrix(O.TRI, T3, 3.0, 2)[0::2, 0::2], 3.0 * A / 12 * (1 + np.eye(3)),
+                               rtol=1e-14)
+    np.testing.assert_allclose(O.mass_element_matrix(O.LINE, np.array([[0.5], [2.0]]), 2.0, 2),
+                               2.0 * 1.5 / 6 * (1 + np.eye(2)), rtol=1e-15)
+    a, b, c = 0.7, 1.3, 0.4
+    Q = np.array([[0, 0], [a, 0], [a, b], [0, b]])
+    rq = np.array([[0, 0], [1, 0], [1, 1], [0, 1]])
+    shared = (rq[:, None, :] == rq[None, :, :]).sum(-1)
+    np.testing.assert_allclose(O.mass_element_matrix(O.QUAD, Q, 1.5, 2)[0::2, 0::2], 1.5 * a * b / 36 * 2.0 ** shared,
+                               rtol=1e-14)
+    rh = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+    H = rh * [a, b, c]
+    shared = (rh[:, None, :] == rh[None, :, :]).sum(-1)
+    np.testing.assert_allclose(O.mass_element_matrix(O.HEX, H, 1.5, 2)[0::3, 0::3],
+                               1.5 * a * b * c / 216 * 2.0 ** shared, rtol=1e-14)
+
+
+def test_mass_matrix_total_and_lumped():
+    """Sum of the consistent mass = rho |Omega| per component; the lumped
+    diagonal holds each node's row sums over dim (restated in numpy)."""
+    c, e = O.box_mesh([0, 0, 0], [1.0, 2.0, 0.5], (3, 2, 2))
+    c = O.jitter_interior(c, (3, 2, 2), 0.2, 5)
+    coo = O.mass_matrix(c, O.TET, e, 3.0)
+    assert abs(coo.vals.sum() - 3.0 * 1.0 * 3) < 1e-13
+    csr = coo.to_csr()
+    lump = O.lumped_mass(c, O.TET, e, 3.0)
+    rows = np.repeat(np.arange(csr.nrows), np.diff(csr.row_ptrs))
+    per_node = np.bincount(rows // 3, weights=csr.values)
+    np.testing.assert_allclose(lump.vals, np.repeat(per_node / 3, 3), rtol=1e-14)
+    assert np.array_equal(lump.rows, np.arange(len(c) * 3))

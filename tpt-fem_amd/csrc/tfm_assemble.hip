@@ -394,6 +394,8 @@ int validate_operator(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op
   } else if (op->kind == TFM_OP_SIMP) {
     if (mesh->cell != TFM_QUAD || dpn != 2) return fail(ctx, TFM_E_INVALID, "SIMP operator is Q4 with 2 DOFs per node");
     if (!op->rho) return fail(ctx, TFM_E_INVALID, "SIMP operator needs densities");
+  } else if (op->kind == TFM_OP_MASS) {
+    if (dpn != dim) return fail(ctx, TFM_E_INVALID, "mass matrix has one DOF per spatial dimension");
   } else {
     return fail(ctx, TFM_E_INVALID, "unknown operator kind");
   }

@@ -213,6 +213,27 @@ __device__ __forceinline__ void rowblock_f64(const double (&X)[Cell<CELL>::NN][3
   }
 #pragma unroll
   for (int i = 0; i < DPN * W; i++) out[i] = 0.0;
+  if (P.kind == TFM_OP_MASS) {
+    // elasticity_mass_matrix (crates/tpt-fem-elasticity/src/lib.rs:403-424):
+    // m_ij = sum_q w det rho N_i N_j on every component of the diagonal
+    // block, nothing across components.
+    for (int q = 0; q < P.nq; q++) {
+      const double *R = P.rule + q * kRuleStride;
+      const double w = R[0];
+      const double *ns = R + 4;
+      const double *lg = R + 4 + 8;
+      double g[NN][3];
+      const double det = fabs(map_point<NN, DIM>(X, lg, g));
+      const double wdr = w * det * coef * ns[ln];
+#pragma unroll
+      for (int j = 0; j < NN; j++) {
+        const double v = wdr * ns[j];
+#pragma unroll
+        for (int a = 0; a < DPN; a++) out[a * W + j * DPN + a] += v;
+      }
+    }
+    return;
+  }
   DConst D{};
   if (P.kind == TFM_OP_ELASTICITY) D = make_d(P.model, coef, P.nu);
 
@@ -286,7 +307,7 @@ __device__ __forceinline__ void rowblock_f32(const double (&X)[Cell<CELL>::NN][3
                                              float *out) {
   constexpr int NN = Cell<CELL>::NN, W = NN * DPN;
   if constexpr (CELL == TFM_TET) {
-    if (P.kind != TFM_OP_SIMP) {
+    if (P.kind == TFM_OP_POISSON || P.kind == TFM_OP_ELASTICITY) {
       double gd[4][3];
       double det = fabs(map_tet(X, gd));
       float g[4][3];

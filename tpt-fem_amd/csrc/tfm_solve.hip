@@ -892,6 +892,67 @@ extern "C" int tfm_assemble(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operat
   return TFM_OK;
 }
 
+// elasticity_lumped_mass (crates/tpt-fem-elasticity/src/lib.rs:436-456): one
+// thread per node sums its dim consistent-mass rows (row order, CSR order,
+// from 0.0), divides by dim and writes the node's dim diagonal entries.
+__global__ void k_lumped(int64_t nodes, int dim, const int *__restrict__ rp, const double *__restrict__ v,
+                         int *__restrict__ orp, int *__restrict__ oci, double *__restrict__ ov) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nodes) return;
+  double total = 0.0;
+  for (int a = 0; a < dim; a++) {
+    const int64_t row = node * dim + a;
+    for (int c = rp[row]; c < rp[row + 1]; c++) total += v[c];
+  }
+  const double lump = total / (double)dim;
+  for (int a = 0; a < dim; a++) {
+    const int64_t i = node * dim + a;
+    orp[i] = (int)i;
+    oci[i] = (int)i;
+    ov[i] = lump;
+  }
+  if (node == nodes - 1) orp[nodes * dim] = (int)(nodes * dim);
+}
+
+extern "C" int tfm_lumped_mass(tfm_ctx *ctx, const tfm_mesh *mesh, double density, int quad_order,
+                               tfm_matrix **out) {
+  if (!ctx || !mesh || !out) return TFM_E_INVALID;
+  tfm_operator op{};
+  op.kind = TFM_OP_MASS;
+  op.quad_order = quad_order;
+  op.coef = density;
+  tfm_matrix *M = nullptr;
+  int rc = tfm_assemble(ctx, mesh, &op, mesh->dim, &M);
+  if (rc) return rc;
+  const int dim = mesh->dim;
+  const int64_t nodes = M->nrows / dim, n = nodes * dim;
+  tfm_matrix *L = new tfm_matrix();
+  L->ctx = ctx;
+  L->nrows = n > 0 ? n : 1;  // Coo::to_csr of no triplets is 1 x 1
+  L->ncols = L->nrows;
+  L->nnz = n;
+  if ((rc = dalloc(ctx, &L->d_rp, L->nrows + 1)) || (rc = dalloc(ctx, &L->d_ci, n)) ||
+      (rc = dalloc(ctx, &L->d_v, n))) {
+    tfm_matrix_destroy(M);
+    tfm_matrix_destroy(L);
+    return rc;
+  }
+  if (n > 0) {
+    hipLaunchKernelGGL(k_lumped, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, ctx->stream, nodes, dim,
+                       (const int *)M->d_rp, (const double *)M->d_v, L->d_rp, L->d_ci, L->d_v);
+  } else {
+    TFM_CHECK_HIP(ctx, hipMemsetAsync(L->d_rp, 0, sizeof(int) * 2, ctx->stream));
+  }
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  tfm_matrix_destroy(M);
+  if (e != hipSuccess) {
+    tfm_matrix_destroy(L);
+    return fail(ctx, TFM_E_HIP, "lumped-mass kernel failed");
+  }
+  *out = L;
+  return TFM_OK;
+}
+
 extern "C" int tfm_matrix_info(const tfm_matrix *m, int64_t *nrows, int64_t *ncols, int64_t *nnz) {
   if (!m) return TFM_E_INVALID;
   if (nrows) *nrows = m->nrows;

@@ -24,7 +24,7 @@ OK, E_INVALID, E_NUMERIC, E_NOT_CONVERGED, E_BREAKDOWN, E_HIP, E_UNSUPPORTED, E_
     0, -1, -2, -3, -4, -5, -6, -7)
 LINE, TRI, QUAD, TET, HEX = 0, 1, 2, 3, 4
 BAR, PLANE_STRESS, PLANE_STRAIN, CONTINUUM3D = 0, 1, 2, 3
-OP_POISSON, OP_ELASTICITY, OP_SIMP = 0, 1, 2
+OP_POISSON, OP_ELASTICITY, OP_SIMP, OP_MASS = 0, 1, 2, 3
 SYS_FP64, SYS_FP32 = 1, 2
 SOLVE_PCG64, SOLVE_MIXED = 0, 1
 NODES = {LINE: 2, TRI: 3, QUAD: 4, TET: 4, HEX: 8}
@@ -50,6 +50,14 @@ class Operator(C.Structure):
     def elasticity(model, young, nu, quad_order=2, young_e=None):
         op = Operator(kind=OP_ELASTICITY, model=model, quad_order=quad_order, coef=young, nu=nu)
         op._attach("coef_e", young_e)
+        return op
+
+    @staticmethod
+    def mass(density, quad_order=2, density_e=None):
+        """elasticity_mass_matrix's operator (crates/tpt-fem-elasticity/src/lib.rs:386-428):
+        assemble with dofs_per_node equal to the cell's dimension."""
+        op = Operator(kind=OP_MASS, quad_order=quad_order, coef=density)
+        op._attach("coef_e", density_e)
         return op
 
     @staticmethod
@@ -99,6 +107,7 @@ _SIGS = {
     "tfm_mesh_nodes_on_plane": (INT, [P, INT, D, D, P, P]),
     "tfm_mesh_destroy": (None, [P]),
     "tfm_assemble": (INT, [P, P, C.POINTER(Operator), INT, PP]),
+    "tfm_lumped_mass": (INT, [P, P, D, INT, PP]),
     "tfm_matrix_create": (INT, [P, I64, I64, P, P, P, PP]),
     "tfm_matrix_info": (INT, [P, P, P, P]),
     "tfm_matrix_download": (INT, [P, P, P, P]),
@@ -340,6 +349,19 @@ def assemble(mesh, dofs_per_node, op):
 
 
 # ---- loads ------------------------------------------------------------------
+def mass_matrix(mesh, density, quad_order=2):
+    """elasticity_mass_matrix(mesh, model, density, quad_order).to_csr()
+    (crates/tpt-fem-elasticity/src/lib.rs:386-428) as a device Matrix."""
+    return assemble(mesh, mesh.dim, Operator.mass(density, quad_order))
+
+
+def lumped_mass(mesh, density, quad_order=2):
+    """elasticity_lumped_mass(...).to_csr() (crates/tpt-fem-elasticity/src/lib.rs:432-457)."""
+    h = C.c_void_p()
+    mesh.ctx.check(_lib.tfm_lumped_mass(mesh.ctx.h, mesh.h, density, quad_order, C.byref(h)))
+    return Matrix(mesh.ctx, h)
+
+
 def source_vector(mesh, f=1.0, quad_order=2, rhs=None):
     rhs = np.zeros(mesh.n_nodes) if rhs is None else _f64(rhs).copy()
     mesh.ctx.check(_lib.tfm_load_source(mesh.ctx.h, mesh.h, quad_order, f, _ptr(rhs)))
