@@ -231,6 +231,17 @@ int tfm_topopt_simp_batch(tfm_ctx *ctx, int64_t P, const tfm_topopt_params *prm,
                           const double *bc_val, const tfm_solve_opts *opts, double *densities,
                           double *compliance, int *iterations, tfm_solve_stats *stats);
 
+/* ---- eigenpairs (crates/tpt-fem-eigen/src/lib.rs) ------------------------ */
+enum { TFM_EIG_SMALLEST = 0, TFM_EIG_LARGEST = 1 }; /* EigWhich (eigen/src/lib.rs:125-131) */
+/* lanczos_eigs(coo, num, which, lanczos_dim) (eigen/src/lib.rs:223-268) on the
+ * square CSR A (n = nrows): a Lanczos basis of dimension lanczos_dim from e_0
+ * with full re-orthogonalisation, built with device SpMV and dot products,
+ * the tridiagonal solved by the reference's cyclic Jacobi.  evals [num],
+ * evecs [num][n] (unit Ritz vectors; may be NULL); *count = min(num, basis
+ * size), the basis stopping early at an invariant subspace (beta < 1e-12). */
+int tfm_lanczos_eigs(const tfm_matrix *A, int num, int which, int lanczos_dim, double *evals,
+                     double *evecs, int *count);
+
 /* ---- drivers ------------------------------------------------------------ */
 /* solve_poisson (thermal/src/lib.rs:206-240) with constant source f, no
  * Neumann/Robin terms. */

@@ -80,6 +80,8 @@ def _declare(L):
         "orc_solve_with_dirichlet": (INT, [I64, P, P, P, I64, P, I64, P, P, P]),
         "orc_mass_element_matrix": (None, [INT, INT, P, D, INT, P]),
         "orc_lumped_mass": (I64, [I64, P, P, INT, P]),
+        "orc_eig_jacobi": (None, [INT, P, INT, P, P]),
+        "orc_lanczos_eigs": (INT, [I64, P, P, P, INT, INT, INT, P, P]),
         "orc_topopt_quad_form": (D, [P, P, P]),
         "orc_topopt_filter": (None, [I64, I64, D, P, D, P]),
         "orc_topopt_oc": (None, [I64, P, P, D, D, P]),
@@ -470,3 +472,27 @@ def topopt_simp(nx, ny, h, e0, nu, vol_frac, penal, radius, max_iter, move_limit
     if rc != 0:
         raise SolveError(_SOLVE_MSG[rc])
     return rho, comp, it.value
+
+
+# ---- Lanczos eigenpairs (crates/tpt-fem-eigen/src/lib.rs) ---------------------
+SMALLEST, LARGEST = 0, 1
+
+
+def eig_jacobi(a, max_sweeps=200):
+    """jacobi (:170-218) -> (eigenvalues, eigenvectors as columns)."""
+    a = f64(a).copy()
+    m = a.shape[0]
+    eig, q = np.zeros(m), np.zeros((m, m))
+    lib().orc_eig_jacobi(m, _p(a), max_sweeps, _p(eig), _p(q))
+    return eig, q
+
+
+def lanczos_eigs(csr, num, which, lanczos_dim):
+    """lanczos_eigs(coo, num, which, lanczos_dim) (:223-268) on coo.to_csr() -> [(lambda, vec)]."""
+    n = csr.nrows
+    ev, vec = np.zeros(max(num, 1)), np.zeros((max(num, 1), n))
+    k = lib().orc_lanczos_eigs(n, _p(csr.row_ptrs), _p(csr.col_ind), _p(csr.values), num, which, lanczos_dim,
+                               _p(ev), _p(vec))
+    if k < 0:
+        raise ValueError("lanczos_dim must be >= 1")
+    return [(ev[i], vec[i].copy()) for i in range(k)]

@@ -1170,3 +1170,126 @@ int orc_topopt_simp(int64_t nx, int64_t ny, double h, double e0, double nu, doub
   free(coords); free(conn); free(rho); free(raw); free(sens); free(u); free(r); free(c); free(v);
   return rc;
 }
+
+
+/* ======================================================================== *
+ * Lanczos eigenpairs: crates/tpt-fem-eigen/src/lib.rs.                      *
+ * ======================================================================== */
+/* dot / norm (:15-21): Iterator::sum folds from -0.0.                      */
+static double eig_dot(int64_t n, const double *a, const double *b) {
+  double s = -0.0;
+  for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+/* matvec (:24-35): row sums from 0.0 in CSR order; y has nrows entries.    */
+void orc_eig_matvec(int64_t n, const int64_t *rp, const int64_t *ci, const double *v,
+                    const double *x, double *y) {
+  for (int64_t r = 0; r < n; r++) {
+    double s = 0.0;
+    for (int64_t c = rp[r]; c < rp[r + 1]; c++) s += v[c] * x[ci[c]];
+    y[r] = s;
+  }
+}
+
+/* jacobi (:170-218): cyclic Jacobi on a dense symmetric m x m matrix a
+ * (row-major, overwritten), eigenvectors as the columns of q.              */
+void orc_eig_jacobi(int m, double *a, int max_sweeps, double *eig, double *q) {
+  for (int i = 0; i < m * m; i++) q[i] = 0.0;
+  for (int i = 0; i < m; i++) q[i * m + i] = 1.0;
+  for (int sw = 0; sw < max_sweeps; sw++) {
+    int changed = 0;
+    for (int p = 0; p < m; p++)
+      for (int r = p + 1; r < m; r++) {
+        if (!(fabs(a[p * m + r]) > 1e-12)) continue;
+        changed = 1;
+        const double theta = (a[r * m + r] - a[p * m + p]) / (2.0 * a[p * m + r]);
+        const double t = theta >= 0.0 ? 1.0 / (theta + sqrt(1.0 + theta * theta))
+                                      : 1.0 / (theta - sqrt(1.0 + theta * theta));
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        const double s = t * c;
+        for (int i = 0; i < m; i++) {
+          const double aip = a[i * m + p], aiq = a[i * m + r];
+          a[i * m + p] = c * aip - s * aiq;
+          a[i * m + r] = s * aip + c * aiq;
+        }
+        for (int i = 0; i < m; i++) {
+          const double api = a[p * m + i], aqi = a[r * m + i];
+          a[p * m + i] = c * api - s * aqi;
+          a[r * m + i] = s * api + c * aqi;
+        }
+        for (int i = 0; i < m; i++) {
+          const double vip = q[i * m + p], viq = q[i * m + r];
+          q[i * m + p] = c * vip - s * viq;
+          q[i * m + r] = s * vip + c * viq;
+        }
+      }
+    if (!changed) break;
+  }
+  for (int i = 0; i < m; i++) eig[i] = a[i * m + i];
+}
+
+static const double *g_eig;
+static int cmp_eig(const void *pa, const void *pb) {
+  const int a = *(const int *)pa, b = *(const int *)pb;
+  const double x = g_eig[a], y = g_eig[b];
+  if (x < y) return -1;
+  if (x > y) return 1;
+  return a < b ? -1 : (a > b); /* stable; NaN compares Equal */
+}
+
+/* lanczos (:135-167) + lanczos_eigs (:223-268) on a square CSR of n rows.
+ * which 0 = Smallest, 1 = Largest.  Returns the number of pairs (<= num);
+ * evecs gets them row by row (n values each).                              */
+int orc_lanczos_eigs(int64_t n, const int64_t *rp, const int64_t *ci, const double *v, int num,
+                     int which, int lanczos_dim, double *evals, double *evecs) {
+  const int md = lanczos_dim;
+  if (md < 1 || n < 1) return -1;
+  double *basis = (double *)calloc((size_t)md * (size_t)n, sizeof(double));
+  double *w = (double *)malloc(sizeof(double) * (size_t)n);
+  double *alpha = (double *)malloc(sizeof(double) * md), *beta = (double *)malloc(sizeof(double) * md);
+  basis[0] = 1.0; /* e_0, normalised by its norm 1 */
+  int m = 1, na = 0;
+  for (int j = 0; j < md; j++) {
+    const double *vj = basis + (size_t)j * n;
+    orc_eig_matvec(n, rp, ci, v, vj, w);
+    const double a = eig_dot(n, w, vj);
+    for (int k = 0; k <= j; k++) {
+      const double *vk = basis + (size_t)k * n;
+      const double p = eig_dot(n, w, vk);
+      for (int64_t d = 0; d < n; d++) w[d] -= p * vk[d];
+    }
+    alpha[na++] = a;
+    if (j + 1 == md) break;
+    const double b = sqrt(eig_dot(n, w, w));
+    if (b < 1e-12) break;
+    beta[j] = b;
+    double *vn = basis + (size_t)(j + 1) * n;
+    for (int64_t d = 0; d < n; d++) vn[d] = w[d] / b;
+    m++;
+  }
+  double *t = (double *)calloc((size_t)m * m, sizeof(double)), *q = (double *)malloc(sizeof(double) * m * m);
+  double *eig = (double *)malloc(sizeof(double) * m);
+  int *idx = (int *)malloc(sizeof(int) * m);
+  for (int i = 0; i < m; i++) t[i * m + i] = alpha[i];
+  for (int i = 0; i + 1 < m; i++) t[i * m + i + 1] = t[(i + 1) * m + i] = beta[i];
+  orc_eig_jacobi(m, t, 200, eig, q);
+  for (int i = 0; i < m; i++) idx[i] = i;
+  g_eig = eig;
+  qsort(idx, (size_t)m, sizeof(int), cmp_eig);
+  const int take = num < m ? num : m;
+  for (int s = 0; s < take; s++) {
+    const int i = which == 0 ? idx[s] : idx[m - 1 - s];
+    double *vec = evecs + (size_t)s * n;
+    for (int64_t d = 0; d < n; d++) vec[d] = 0.0;
+    for (int r = 0; r < m; r++) {
+      const double c = q[r * m + i];
+      for (int64_t d = 0; d < n; d++) vec[d] += c * basis[(size_t)r * n + d];
+    }
+    const double nv = sqrt(eig_dot(n, vec, vec));
+    for (int64_t d = 0; d < n; d++) vec[d] /= nv;
+    evals[s] = eig[i];
+  }
+  free(basis); free(w); free(alpha); free(beta); free(t); free(q); free(eig); free(idx);
+  return take;
+}

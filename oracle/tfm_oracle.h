@@ -148,4 +148,11 @@ void orc_loguniform(int64_t n, double lo_exp10, double hi_exp10, uint64_t seed,
 #ifdef __cplusplus
 }
 #endif
+
+/* ---- Lanczos eigenpairs (crates/tpt-fem-eigen/src/lib.rs) ---- */
+void orc_eig_matvec(int64_t n, const int64_t *rp, const int64_t *ci, const double *v,
+                    const double *x, double *y);
+void orc_eig_jacobi(int m, double *a, int max_sweeps, double *eig, double *q);
+int orc_lanczos_eigs(int64_t n, const int64_t *rp, const int64_t *ci, const double *v, int num,
+                     int which, int lanczos_dim, double *evals, double *evecs);
 #endif

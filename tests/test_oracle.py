@@ -444,3 +444,61 @@ def test_mass_matrix_total_and_lumped():
     per_node = np.bincount(rows // 3, weights=csr.values)
     np.testing.assert_allclose(lump.vals, np.repeat(per_node / 3, 3), rtol=1e-14)
     assert np.array_equal(lump.rows, np.arange(len(c) * 3))
+
+
+# ---- Lanczos eigenpairs (crates/tpt-fem-eigen/src/lib.rs) -------------------------
+def _laplace_1d(n):
+    r, c, v = [], [], []
+    for i in range(n):
+        r.append(i); c.append(i); v.append(2.0)
+        if i + 1 < n:
+            r += [i, i + 1]; c += [i + 1, i]; v += [-1.0, -1.0]
+    return O.Coo(r, c, v).to_csr()
+
+
+def test_lanczos_reference_kat_and_closed_form():
+    """lanczos_1d_laplacian (:595-623): the extreme eigenvalues of the n = 10
+    discrete Laplacian, 2 - 2 cos(k pi / 11), to 1e-6 (and in fact 1e-13);
+    with a full basis every eigenvalue and unit eigenvector is recovered."""
+    n = 10
+    a = _laplace_1d(n)
+    lo = O.lanczos_eigs(a, 1, O.SMALLEST, n)[0][0]
+    hi = O.lanczos_eigs(a, 1, O.LARGEST, n)[0][0]
+    assert abs(lo - (2 - 2 * np.cos(np.pi / 11))) < 1e-13
+    assert abs(hi - (2 - 2 * np.cos(np.pi * 10 / 11))) < 1e-13
+    pairs = O.lanczos_eigs(a, n, O.SMALLEST, n)
+    k = np.arange(1, n + 1)
+    np.testing.assert_allclose([p[0] for p in pairs], 2 - 2 * np.cos(k * np.pi / 11), atol=1e-13)
+    dense = np.diag(np.full(n, 2.0)) - np.eye(n, k=1) - np.eye(n, k=-1)
+    for lam, vec in pairs:
+        assert abs(np.linalg.norm(vec) - 1) < 1e-14 and np.linalg.norm(dense @ vec - lam * vec) < 1e-12
+
+
+def test_jacobi_matches_lapack():
+    """jacobi (:170-218) against LAPACK's symmetric eigensolver; A Q = Q diag."""
+    rng = np.random.default_rng(17)
+    for m in (1, 2, 5, 12):
+        b = rng.standard_normal((m, m))
+        a = b + b.T
+        eig, q = O.eig_jacobi(a)
+        np.testing.assert_allclose(np.sort(eig), np.linalg.eigvalsh(a), atol=1e-12)
+        np.testing.assert_allclose(a @ q, q * eig, atol=1e-11)
+
+
+def test_lanczos_partial_basis_and_invariant_subspace():
+    """A partial basis converges the ends of the spectrum first (Ritz values
+    bracketed by the true ones); a diagonal matrix is an invariant subspace of
+    e_0, so the basis stops after one vector and one pair comes back."""
+    c, e = O.box_mesh([0, 0, 0], [1, 1, 1], (4, 4, 4))
+    a = O.assemble(c, e, O.make_op(O.TET, 3, 0, coef=1.0), 1).to_csr()
+    dense = np.zeros((a.nrows, a.nrows))
+    for r in range(a.nrows):
+        dense[r, a.col_ind[a.row_ptrs[r]:a.row_ptrs[r + 1]]] = a.values[a.row_ptrs[r]:a.row_ptrs[r + 1]]
+    # the pure Neumann Laplacian is singular: its smallest eigenvalue is 0
+    w = np.linalg.eigvalsh(dense)
+    hi = O.lanczos_eigs(a, 3, O.LARGEST, 40)
+    assert abs(hi[0][0] - w[-1]) < 1e-9 * w[-1] and hi[0][0] <= w[-1] * (1 + 1e-14)
+    assert hi[0][0] >= hi[1][0] >= hi[2][0]
+    d = O.Coo([0, 1, 2], [0, 1, 2], [3.0, 1.0, 2.0]).to_csr()
+    pairs = O.lanczos_eigs(d, 3, O.SMALLEST, 3)
+    assert len(pairs) == 1 and pairs[0][0] == 3.0

@@ -130,6 +130,7 @@ _SIGS = {
     "tfm_topopt_simp": (INT, [P, C.POINTER(TopOptParams), P, I64, P, P, C.POINTER(SolveOpts), P, P, P, P]),
     "tfm_topopt_simp_batch": (INT, [P, I64, C.POINTER(TopOptParams), P, P, I64, P, P, C.POINTER(SolveOpts), P, P,
                                     P, P]),
+    "tfm_lanczos_eigs": (INT, [P, INT, INT, INT, P, P, P]),
     "tfm_solve_poisson": (INT, [P, P, D, INT, D, I64, P, P, C.POINTER(SolveOpts), P]),
     "tfm_solve_elasticity": (INT, [P, P, INT, D, D, INT, P, I64, P, P, C.POINTER(SolveOpts), P]),
 }
@@ -555,3 +556,18 @@ def topopt_simp(ctx, nx, ny, h, e0, nu, vol_frac, penal, filter_radius, max_iter
     if batched:
         return TopOptResult(rho, comp, its.astype(np.int64), stats)
     return TopOptResult(rho[0], comp[0], int(its[0]), stats[0])
+
+
+EIG_SMALLEST, EIG_LARGEST = 0, 1
+
+
+def lanczos_eigs(matrix, num, which, lanczos_dim, vectors=True):
+    """lanczos_eigs(coo, num, which, lanczos_dim) (crates/tpt-fem-eigen/src/lib.rs:223-268)
+    on a device Matrix -> [(lambda, vec)] (vec None when vectors is False)."""
+    n = matrix.nrows
+    ev = np.zeros(max(num, 1))
+    vec = np.zeros((max(num, 1), n)) if vectors else None
+    cnt = INT(0)
+    matrix.ctx.check(_lib.tfm_lanczos_eigs(matrix.h, num, which, lanczos_dim, _ptr(ev), _ptr(vec),
+                                           C.byref(cnt)))
+    return [(float(ev[i]), vec[i].copy() if vectors else None) for i in range(cnt.value)]
