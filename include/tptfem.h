@@ -28,6 +28,7 @@ extern "C" {
 #define TFM_E_HIP (-5)           /* HIP runtime error                          */
 #define TFM_E_UNSUPPORTED (-6)   /* cell/operator combination not built        */
 #define TFM_E_MODEL (-7)         /* ElasticityError::ModelDimMismatch (elasticity/src/lib.rs:28-35) */
+#define TFM_E_CFL (-8)           /* DynamicError::CflViolation (dynamic/src/lib.rs:46, 58-61) */
 
 /* CellType order: crates/tpt-fem-mesh/src/lib.rs:39-52 (P1 cells). */
 enum { TFM_LINE = 0, TFM_TRI = 1, TFM_QUAD = 2, TFM_TET = 3, TFM_HEX = 4 };
@@ -241,6 +242,27 @@ enum { TFM_EIG_SMALLEST = 0, TFM_EIG_LARGEST = 1 }; /* EigWhich (eigen/src/lib.r
  * size), the basis stopping early at an invariant subspace (beta < 1e-12). */
 int tfm_lanczos_eigs(const tfm_matrix *A, int num, int which, int lanczos_dim, double *evals,
                      double *evecs, int *count);
+
+/* ---- time integration (crates/tpt-fem-dynamic/src/lib.rs) --------------- */
+/* CachedSystem (dynamic/src/lib.rs:133-166) is the three device matrices M, C,
+ * K (C may be the empty 1 x 1 CSR); every column must index a DOF (< n).
+ * The load is f(t_k) = fscale[k] * f (fscale NULL: 1), k = 0..nsteps;
+ * history: host [nsteps + 1][n] displacements, row 0 = u0. */
+/* central_difference (dynamic/src/lib.rs:289-377): lumped (row-sum) mass, the
+ * CFL guard 2 / sqrt(max_i sum_j |K_ij| / m_i) -- TFM_E_CFL with *critical
+ * set when dt exceeds it -- then the explicit steps, bit-identical. */
+int tfm_central_difference(const tfm_matrix *M, const tfm_matrix *C, const tfm_matrix *K, int64_t n,
+                           const double *u0, const double *v0, const double *f,
+                           const double *fscale, double dt, int nsteps, double *history,
+                           double *critical);
+/* newmark (dynamic/src/lib.rs:194-263): a0 from M a0 = f0 - C v0 - K u0 and
+ * every step's K_hat u = rhs solved by FP64 PCG (opts; NULL: rtol 1e-12),
+ * warm-started from the previous step, where the reference factors densely.
+ * stats (may be NULL): PCG iterations and device seconds over the steps. */
+int tfm_newmark(const tfm_matrix *M, const tfm_matrix *C, const tfm_matrix *K, int64_t n,
+                const double *u0, const double *v0, const double *f, const double *fscale,
+                double dt, double beta, double gamma, int nsteps, const tfm_solve_opts *opts,
+                double *history, tfm_solve_stats *stats);
 
 /* ---- drivers ------------------------------------------------------------ */
 /* solve_poisson (thermal/src/lib.rs:206-240) with constant source f, no

@@ -54,6 +54,10 @@ class OrcOp(C.Structure):
     ]
 
 
+class OrcCsr(C.Structure):
+    _fields_ = [("nrows", C.c_int64), ("rp", C.c_void_p), ("ci", C.c_void_p), ("v", C.c_void_p)]
+
+
 def _declare(L):
     sig = {
         "orc_gauss_legendre": (INT, [INT, P, P]),
@@ -82,6 +86,8 @@ def _declare(L):
         "orc_lumped_mass": (I64, [I64, P, P, INT, P]),
         "orc_eig_jacobi": (None, [INT, P, INT, P, P]),
         "orc_lanczos_eigs": (INT, [I64, P, P, P, INT, INT, INT, P, P]),
+        "orc_central_difference": (INT, [I64, OrcCsr, OrcCsr, OrcCsr, P, P, P, P, D, I64, P, P]),
+        "orc_newmark": (INT, [I64, OrcCsr, OrcCsr, OrcCsr, P, P, P, P, D, D, D, I64, P]),
         "orc_topopt_quad_form": (D, [P, P, P]),
         "orc_topopt_filter": (None, [I64, I64, D, P, D, P]),
         "orc_topopt_oc": (None, [I64, P, P, D, D, P]),
@@ -496,3 +502,47 @@ def lanczos_eigs(csr, num, which, lanczos_dim):
     if k < 0:
         raise ValueError("lanczos_dim must be >= 1")
     return [(ev[i], vec[i].copy()) for i in range(k)]
+
+
+# ---- time integration (crates/tpt-fem-dynamic/src/lib.rs) ----------------------
+class CflViolation(RuntimeError):
+    """DynamicError::CflViolation (dynamic/src/lib.rs:46, 58-61)."""
+
+    def __init__(self, dt, critical):
+        super().__init__(f"central-difference timestep {dt:e} exceeds the critical (CFL) limit {critical:e}")
+        self.dt, self.critical = dt, critical
+
+
+def _orc_csr(csr, keep):
+    rp, ci, v = i64(csr.row_ptrs), i64(csr.col_ind), f64(csr.values)
+    keep += [rp, ci, v]
+    return OrcCsr(csr.nrows, rp.ctypes.data, ci.ctypes.data, v.ctypes.data)
+
+
+def central_difference(m, c, k, u0, v0, F, dt, nsteps, fscale=None):
+    """central_difference (:289-377) on CSR operators with f(t_k) = fscale[k] * F -> history [nsteps+1, n]."""
+    keep = []
+    u0, v0, F = f64(u0), f64(v0), f64(F)
+    fs = None if fscale is None else f64(fscale)
+    n = len(u0)
+    hist = np.zeros((nsteps + 1, n))
+    crit = D(0.0)
+    rc = lib().orc_central_difference(n, _orc_csr(m, keep), _orc_csr(c, keep), _orc_csr(k, keep), _p(u0), _p(v0),
+                                      _p(F), _p(fs), dt, nsteps, _p(hist), C.byref(crit))
+    if rc == 1:
+        raise CflViolation(dt, crit.value)
+    return hist
+
+
+def newmark(m, c, k, u0, v0, F, dt, beta, gamma, nsteps, fscale=None):
+    """newmark (:194-263) with dense LU solves -> history [nsteps+1, n]."""
+    keep = []
+    u0, v0, F = f64(u0), f64(v0), f64(F)
+    fs = None if fscale is None else f64(fscale)
+    n = len(u0)
+    hist = np.zeros((nsteps + 1, n))
+    rc = lib().orc_newmark(n, _orc_csr(m, keep), _orc_csr(c, keep), _orc_csr(k, keep), _p(u0), _p(v0), _p(F),
+                           _p(fs), dt, beta, gamma, nsteps, _p(hist))
+    if rc != 0:
+        raise SolveError("sparse numeric factorization/solve failed: singular matrix")
+    return hist

@@ -1293,3 +1293,134 @@ int orc_lanczos_eigs(int64_t n, const int64_t *rp, const int64_t *ci, const doub
   free(basis); free(w); free(alpha); free(beta); free(t); free(q); free(eig); free(idx);
   return take;
 }
+
+
+/* ======================================================================== *
+ * Time integration: crates/tpt-fem-dynamic/src/lib.rs.  Operators are given *
+ * as CSR (CachedSystem::new's to_csr of the Coo inputs, :145-151); the load *
+ * is f(t_k) = fscale[k] * F (fscale NULL: 1).                               *
+ * ======================================================================== */
+/* Csr::matvec (crates/tpt-fem-sparse/src/lib.rs:195-205) into y[0..n).     */
+static void dyn_matvec(const orc_csr *a, int64_t n, const double *x, double *y) {
+  for (int64_t r = 0; r < n; r++) {
+    double s = 0.0;
+    if (r < a->nrows)
+      for (int64_t c = a->rp[r]; c < a->rp[r + 1]; c++) s += a->v[c] * x[a->ci[c]];
+    y[r] = s;
+  }
+}
+
+static double dyn_scale(const double *fscale, int64_t k) { return fscale ? fscale[k] : 1.0; }
+
+/* central_difference (:289-377).  Returns 0, or 1 for a CFL violation with
+ * *critical set; history gets (nsteps + 1) rows of n.                      */
+int orc_central_difference(int64_t n, orc_csr M, orc_csr D, orc_csr K, const double *u0,
+                           const double *v0, const double *F, const double *fscale, double dt,
+                           int64_t nsteps, double *history, double *critical) {
+  double *md = (double *)malloc(sizeof(double) * (size_t)n), *dv = (double *)malloc(sizeof(double) * (size_t)n);
+  double *ku = (double *)malloc(sizeof(double) * (size_t)n), *vh = (double *)malloc(sizeof(double) * (size_t)n);
+  double *u = (double *)malloc(sizeof(double) * (size_t)n), *a = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int64_t r = 0; r < n; r++) {
+    double s = 0.0;
+    for (int64_t c = M.rp[r]; c < M.rp[r + 1]; c++) s += M.v[c];
+    md[r] = s;
+  }
+  double rho = 0.0;
+  int has_mass = 0;
+  for (int64_t i = 0; i < n; i++)
+    if (md[i] > 0.0) {
+      double rs = 0.0;
+      for (int64_t c = K.rp[i]; c < K.rp[i + 1]; c++) rs += fabs(K.v[c]);
+      rho = fmax(rho, rs / md[i]);
+      has_mass = 1;
+    }
+  const double safe = has_mass && rho > 0.0 ? 2.0 / sqrt(rho) : INFINITY;
+  *critical = safe;
+  int rc = 0;
+  if (dt > safe) {
+    rc = 1;
+  } else {
+    dyn_matvec(&D, n, v0, dv);
+    dyn_matvec(&K, n, u0, ku);
+    const double s0 = dyn_scale(fscale, 0);
+    for (int64_t i = 0; i < n; i++) a[i] = (s0 * F[i] - dv[i] - ku[i]) / md[i];
+    for (int64_t i = 0; i < n; i++) {
+      u[i] = u0[i];
+      vh[i] = v0[i] - 0.5 * dt * a[i];
+      history[i] = u[i];
+    }
+    for (int64_t step = 1; step <= nsteps; step++) {
+      const double sk = dyn_scale(fscale, step);
+      dyn_matvec(&D, n, vh, dv);
+      dyn_matvec(&K, n, u, ku);
+      for (int64_t i = 0; i < n; i++) a[i] = (sk * F[i] - dv[i] - ku[i]) / md[i];
+      for (int64_t i = 0; i < n; i++) vh[i] += dt * a[i];
+      for (int64_t i = 0; i < n; i++) u[i] += dt * vh[i];
+      memcpy(history + (size_t)step * n, u, sizeof(double) * (size_t)n);
+    }
+  }
+  free(md); free(dv); free(ku); free(vh); free(u); free(a);
+  return rc;
+}
+
+/* Dense copy of a CSR's first n rows / columns (column-major, summed).      */
+static void dyn_dense(const orc_csr *a, int64_t n, double s, double *out) {
+  for (int64_t r = 0; r < n && r < a->nrows; r++)
+    for (int64_t c = a->rp[r]; c < a->rp[r + 1]; c++)
+      if (a->ci[c] < n) out[r + a->ci[c] * n] += a->v[c] * s;
+}
+
+/* newmark (:194-263) with dense LU solves, as the reference.  K_hat is the
+ * Coo sum K + M/(b dt^2) + C g/(b dt) collapsed in that insertion order
+ * (:221-229).  Returns 0, or -1 when a solve hits a zero pivot.            */
+int orc_newmark(int64_t n, orc_csr M, orc_csr D, orc_csr K, const double *u0, const double *v0,
+                const double *F, const double *fscale, double dt, double beta, double gamma,
+                int64_t nsteps, double *history) {
+  const double b = beta, g = gamma;
+  const size_t nn = (size_t)n;
+  double *dv = (double *)malloc(sizeof(double) * nn), *ku = (double *)malloc(sizeof(double) * nn);
+  double *a = (double *)malloc(sizeof(double) * nn), *u = (double *)malloc(sizeof(double) * nn);
+  double *v = (double *)malloc(sizeof(double) * nn), *pm = (double *)malloc(sizeof(double) * nn);
+  double *pc = (double *)malloc(sizeof(double) * nn), *rhs = (double *)malloc(sizeof(double) * nn);
+  double *mp = (double *)malloc(sizeof(double) * nn), *dp = (double *)malloc(sizeof(double) * nn);
+  double *dense = (double *)malloc(sizeof(double) * nn * nn), *kh = (double *)calloc(nn * nn, sizeof(double));
+  int rc = 0;
+  dyn_matvec(&D, n, v0, dv);
+  dyn_matvec(&K, n, u0, ku);
+  const double s0 = dyn_scale(fscale, 0);
+  for (int64_t i = 0; i < n; i++) a[i] = s0 * F[i] - dv[i] - ku[i];
+  memset(dense, 0, sizeof(double) * nn * nn);
+  dyn_dense(&M, n, 1.0, dense);
+  if (orc_dense_solve(n, dense, a)) rc = -1;
+  /* K_hat: each entry sums K's, then M's scaled, then C's scaled values */
+  dyn_dense(&K, n, 1.0, kh);
+  dyn_dense(&M, n, 1.0 / (b * dt * dt), kh);
+  dyn_dense(&D, n, g / (b * dt), kh);
+  for (int64_t i = 0; i < n; i++) {
+    u[i] = u0[i];
+    v[i] = v0[i];
+    history[i] = u[i];
+  }
+  for (int64_t step = 1; step <= nsteps && !rc; step++) {
+    const double sk = dyn_scale(fscale, step);
+    for (int64_t i = 0; i < n; i++) {
+      pm[i] = u[i] / (b * dt * dt) + v[i] / (b * dt) + (0.5 / b - 1.0) * a[i];
+      pc[i] = g / (b * dt) * u[i] + (g / b - 1.0) * v[i] + dt * (g / (2.0 * b) - 1.0) * a[i];
+    }
+    dyn_matvec(&M, n, pm, mp);
+    dyn_matvec(&D, n, pc, dp);
+    for (int64_t i = 0; i < n; i++) rhs[i] = sk * F[i] + mp[i] + dp[i];
+    memcpy(dense, kh, sizeof(double) * nn * nn);
+    if (orc_dense_solve(n, dense, rhs)) { rc = -1; break; }
+    for (int64_t i = 0; i < n; i++) {
+      const double an = (rhs[i] - u[i]) / (b * dt * dt) - v[i] / (b * dt) - (0.5 / b - 1.0) * a[i];
+      v[i] = v[i] + dt * ((1.0 - g) * a[i] + g * an);
+      a[i] = an;
+      u[i] = rhs[i];
+    }
+    memcpy(history + (size_t)step * n, u, sizeof(double) * nn);
+  }
+  free(dv); free(ku); free(a); free(u); free(v); free(pm); free(pc); free(rhs); free(mp); free(dp);
+  free(dense); free(kh);
+  return rc;
+}

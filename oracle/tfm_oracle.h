@@ -155,4 +155,13 @@ void orc_eig_matvec(int64_t n, const int64_t *rp, const int64_t *ci, const doubl
 void orc_eig_jacobi(int m, double *a, int max_sweeps, double *eig, double *q);
 int orc_lanczos_eigs(int64_t n, const int64_t *rp, const int64_t *ci, const double *v, int num,
                      int which, int lanczos_dim, double *evals, double *evecs);
+
+/* ---- time integration (crates/tpt-fem-dynamic/src/lib.rs) ---- */
+typedef struct { int64_t nrows; const int64_t *rp, *ci; const double *v; } orc_csr;
+int orc_central_difference(int64_t n, orc_csr M, orc_csr D, orc_csr K, const double *u0,
+                           const double *v0, const double *F, const double *fscale, double dt,
+                           int64_t nsteps, double *history, double *critical);
+int orc_newmark(int64_t n, orc_csr M, orc_csr D, orc_csr K, const double *u0, const double *v0,
+                const double *F, const double *fscale, double dt, double beta, double gamma,
+                int64_t nsteps, double *history);
 #endif

@@ -502,3 +502,47 @@ def test_lanczos_partial_basis_and_invariant_subspace():
     d = O.Coo([0, 1, 2], [0, 1, 2], [3.0, 1.0, 2.0]).to_csr()
     pairs = O.lanczos_eigs(d, 3, O.SMALLEST, 3)
     assert len(pairs) == 1 and pairs[0][0] == 3.0
+
+
+# ---- time integration (crates/tpt-fem-dynamic/src/lib.rs) -------------------------
+def _sdof(mass, stiff):
+    return (O.Coo([0], [0], [mass]).to_csr(), O.Coo([], [], []).to_csr(), O.Coo([0], [0], [stiff]).to_csr())
+
+
+def test_dynamic_reference_tests():
+    """The reference's three tests (:507-550) on the oracle."""
+    m, c, k = _sdof(1.0, 4.0)
+    h = O.newmark(m, c, k, [1.0], [0.0], [0.0], 0.005, 0.25, 0.5, 400)
+    assert abs(h[400][0] - np.cos(4.0)) < 5e-3
+    with pytest.raises(O.CflViolation):
+        O.central_difference(m, c, k, [1.0], [0.0], [0.0], 2.0, 10)
+    m, c, k = _sdof(2.0, 8.0)
+    h = O.newmark(m, c, k, [1.0], [0.0], [0.0], 0.002, 0.25, 0.5, 500)
+    for w in range(1, len(h) - 1):
+        vm = (h[w + 1][0] - h[w - 1][0]) / (2 * 0.002)
+        assert abs(0.5 * 2 * vm * vm + 0.5 * 8 * h[w][0] ** 2 - 4.0) < 1e-2
+
+
+def test_dynamic_discrete_closed_forms():
+    """Exact discrete solutions of the undamped oscillator from u0 = 1, v0 = 0:
+    central difference u_k = cos(k t) with cos t = 1 - w^2 dt^2 / 2, and the
+    average-acceleration Newmark (the trapezoidal rule) u_k = cos(k t') with
+    tan(t'/2) = w dt / 2; and for two coupled DOFs the same per mode of the
+    generalized eigenproblem (scipy)."""
+    w, dt, N = 3.0, 0.05, 80
+    m, c, k = _sdof(2.0, 2.0 * w * w)
+    kk = np.arange(N + 1)
+    cd = O.central_difference(m, c, k, [1.0], [0.0], [0.0], dt, N)[:, 0]
+    np.testing.assert_allclose(cd, np.cos(kk * np.arccos(1 - w * w * dt * dt / 2)), atol=1e-12)
+    nm = O.newmark(m, c, k, [1.0], [0.0], [0.0], dt, 0.25, 0.5, N)[:, 0]
+    np.testing.assert_allclose(nm, np.cos(kk * 2 * np.arctan(w * dt / 2)), atol=1e-12)
+    Md = np.array([[2.0, 0.5], [0.5, 1.0]])
+    Kd = np.array([[6.0, -2.0], [-2.0, 4.0]])
+    tocsr = lambda A: O.Coo(*np.nonzero(A), A[np.nonzero(A)]).to_csr()
+    u0 = np.array([1.0, -0.5])
+    lam, phi = linalg.eigh(Kd, Md)  # phi^T M phi = I
+    q0 = phi.T @ Md @ u0
+    th = 2 * np.arctan(np.sqrt(lam) * dt / 2)
+    want = np.array([phi @ (q0 * np.cos(j * th)) for j in kk])
+    got = O.newmark(tocsr(Md), O.Coo([], [], []).to_csr(), tocsr(Kd), u0, [0.0, 0.0], [0.0, 0.0], dt, 0.25, 0.5, N)
+    np.testing.assert_allclose(got, want, atol=1e-12)

@@ -908,6 +908,25 @@ int bsolver_refresh(BatchSolver *b) {
 }
 
 const double *bsolver_x(const BatchSolver *b) { return b->x; }
+double *bsolver_x_mut(BatchSolver *b) { return b->x; }
+
+// f[i] = fixed[i] ? 0 : d[i] (one right-hand side, already on the device).
+__global__ void kb_rhs_dev(int64_t n, const double *__restrict__ d, const unsigned char *__restrict__ fixed,
+                           double *__restrict__ f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = (fixed && fixed[i]) ? 0.0 : d[i];
+}
+
+int bsolver_set_rhs_dev(BatchSolver *b, const double *d_rhs) {
+  tfm_system *s = b->s;
+  if (b->NR != 1) return fail(s->ctx, TFM_E_INVALID, "device right-hand side needs one rhs per problem");
+  const int64_t n = s->ndof;
+  if (n > 0)
+    hipLaunchKernelGGL(kb_rhs_dev, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->ctx->stream, n, d_rhs,
+                       (const unsigned char *)s->d_fixed, b->f);
+  TFM_CHECK_HIP(s->ctx, hipGetLastError());
+  return TFM_OK;
+}
 
 template <int NR>
 static int bsolver_run_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_stats *stats, double *fu) {

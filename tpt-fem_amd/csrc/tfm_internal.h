@@ -193,6 +193,8 @@ int bsolver_set_rhs(BatchSolver *b, const double *rhs);
 int bsolver_refresh(BatchSolver *b);
 int bsolver_run(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_stats *stats, double *fu);
 const double *bsolver_x(const BatchSolver *b);  // [n_dof][NR], fixed DOFs 0
+double *bsolver_x_mut(BatchSolver *b);          // the same, to seed a warm start
+int bsolver_set_rhs_dev(BatchSolver *b, const double *d_rhs);  // NR = 1, device [n_dof]
 void bsolver_destroy(BatchSolver *b);
 
 // Loads (tfm_loads.hip): device rhs of n_nodes*dpn doubles, accumulated.

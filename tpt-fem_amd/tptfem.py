@@ -20,8 +20,8 @@ if not os.path.exists(LIB_PATH):
 
 _lib = C.CDLL(LIB_PATH)
 
-OK, E_INVALID, E_NUMERIC, E_NOT_CONVERGED, E_BREAKDOWN, E_HIP, E_UNSUPPORTED, E_MODEL = (
-    0, -1, -2, -3, -4, -5, -6, -7)
+OK, E_INVALID, E_NUMERIC, E_NOT_CONVERGED, E_BREAKDOWN, E_HIP, E_UNSUPPORTED, E_MODEL, E_CFL = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8)
 LINE, TRI, QUAD, TET, HEX = 0, 1, 2, 3, 4
 BAR, PLANE_STRESS, PLANE_STRAIN, CONTINUUM3D = 0, 1, 2, 3
 OP_POISSON, OP_ELASTICITY, OP_SIMP, OP_MASS = 0, 1, 2, 3
@@ -131,6 +131,8 @@ _SIGS = {
     "tfm_topopt_simp_batch": (INT, [P, I64, C.POINTER(TopOptParams), P, P, I64, P, P, C.POINTER(SolveOpts), P, P,
                                     P, P]),
     "tfm_lanczos_eigs": (INT, [P, INT, INT, INT, P, P, P]),
+    "tfm_central_difference": (INT, [P, P, P, I64, P, P, P, P, D, INT, P, P]),
+    "tfm_newmark": (INT, [P, P, P, I64, P, P, P, P, D, D, D, INT, C.POINTER(SolveOpts), P, P]),
     "tfm_solve_poisson": (INT, [P, P, D, INT, D, I64, P, P, C.POINTER(SolveOpts), P]),
     "tfm_solve_elasticity": (INT, [P, P, INT, D, D, INT, P, I64, P, P, C.POINTER(SolveOpts), P]),
 }
@@ -571,3 +573,41 @@ def lanczos_eigs(matrix, num, which, lanczos_dim, vectors=True):
     matrix.ctx.check(_lib.tfm_lanczos_eigs(matrix.h, num, which, lanczos_dim, _ptr(ev), _ptr(vec),
                                            C.byref(cnt)))
     return [(float(ev[i]), vec[i].copy() if vectors else None) for i in range(cnt.value)]
+
+
+class CflViolation(RuntimeError):
+    """DynamicError::CflViolation (crates/tpt-fem-dynamic/src/lib.rs:46, 58-61)."""
+
+    def __init__(self, msg, critical):
+        super().__init__(msg)
+        self.critical = critical
+
+
+def central_difference(m, c, k, u0, v0, f, dt, nsteps, fscale=None):
+    """central_difference (crates/tpt-fem-dynamic/src/lib.rs:289-377) on device matrices,
+    f(t_k) = fscale[k] * f -> history [nsteps+1, n]."""
+    u0, v0, f = _f64(u0), _f64(v0), _f64(f)
+    fs = None if fscale is None else _f64(fscale)
+    n = len(u0)
+    hist = np.zeros((nsteps + 1, n))
+    crit = D(0.0)
+    rc = _lib.tfm_central_difference(m.h, c.h, k.h, n, _ptr(u0), _ptr(v0), _ptr(f), _ptr(fs), dt, nsteps,
+                                     _ptr(hist), C.byref(crit))
+    if rc == E_CFL:
+        raise CflViolation((_lib.tfm_last_error(m.ctx.h) or b"").decode(), crit.value)
+    m.ctx.check(rc)
+    return hist
+
+
+def newmark(m, c, k, u0, v0, f, dt, beta, gamma, nsteps, fscale=None, rtol=1e-12, max_iter=100000):
+    """newmark (crates/tpt-fem-dynamic/src/lib.rs:194-263), PCG per step -> history [nsteps+1, n]."""
+    u0, v0, f = _f64(u0), _f64(v0), _f64(f)
+    fs = None if fscale is None else _f64(fscale)
+    n = len(u0)
+    hist = np.zeros((nsteps + 1, n))
+    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=max_iter, inner_rtol=0.3, max_outer=60)
+    st = SolveStats()
+    m.ctx.check(_lib.tfm_newmark(m.h, c.h, k.h, n, _ptr(u0), _ptr(v0), _ptr(f), _ptr(fs), dt, beta, gamma, nsteps,
+                                 C.byref(o), _ptr(hist), C.byref(st)))
+    m.stats = st
+    return hist
