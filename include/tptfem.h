@@ -136,6 +136,15 @@ int tfm_load_neumann(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, double g, doub
 int tfm_load_plane_traction(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, int axis,
                             double coord, double tol, const double *t, double *rhs);
 
+/* to_csr() of assemble(mesh, dpn, op) with apply_robin(mesh, dpn, h, coo)'s
+ * triplets appended (assembly/src/lib.rs:674-753): every boundary face adds
+ * w * measure * h * N_k * N_l on the component diagonal of its node pairs,
+ * summed after the element contributions.  h is constant, or -- axis >= 0 --
+ * h on the band |x[axis] - coord| <= tol and 0 elsewhere (zero entries are
+ * still created, as the reference pushes them). */
+int tfm_assemble_robin(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dofs_per_node,
+                       double h, int axis, double coord, double tol, tfm_matrix **out);
+
 /* ---- reduced systems (assembly/src/lib.rs:489-559) ----------------------- */
 enum { TFM_SYS_FP64 = 1, TFM_SYS_FP32 = 2 };
 /* reduce_system(assemble(mesh, dpn, op), rhs, bcs) without forming the full K:
@@ -270,6 +279,12 @@ int tfm_newmark(const tfm_matrix *M, const tfm_matrix *C, const tfm_matrix *K, i
 int tfm_solve_poisson(tfm_ctx *ctx, const tfm_mesh *mesh, double kappa, int quad_order,
                       double f, int64_t n_bc, const int64_t *bc_dof, const double *bc_val,
                       const tfm_solve_opts *opts, double *u);
+/* solve_poisson with Neumann and Robin terms (thermal/src/lib.rs:206-240):
+ * constant flux *neumann_g and coefficient *robin_h (NULL: None). */
+int tfm_solve_poisson_nr(tfm_ctx *ctx, const tfm_mesh *mesh, double kappa, int quad_order, double f,
+                         const double *neumann_g, const double *robin_h, int64_t n_bc,
+                         const int64_t *bc_dof, const double *bc_val, const tfm_solve_opts *opts,
+                         double *u);
 /* solve_elasticity (elasticity/src/lib.rs:355-381) with constant body force. */
 int tfm_solve_elasticity(tfm_ctx *ctx, const tfm_mesh *mesh, int model, double young,
                          double nu, int quad_order, const double *body, int64_t n_bc,

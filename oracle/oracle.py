@@ -88,6 +88,7 @@ def _declare(L):
         "orc_lanczos_eigs": (INT, [I64, P, P, P, INT, INT, INT, P, P]),
         "orc_central_difference": (INT, [I64, OrcCsr, OrcCsr, OrcCsr, P, P, P, P, D, I64, P, P]),
         "orc_newmark": (INT, [I64, OrcCsr, OrcCsr, OrcCsr, P, P, P, P, D, D, D, I64, P]),
+        "orc_apply_robin": (I64, [I64, INT, P, I64, INT, P, INT, D, INT, D, D, INT, P, P, P]),
         "orc_topopt_quad_form": (D, [P, P, P]),
         "orc_topopt_filter": (None, [I64, I64, D, P, D, P]),
         "orc_topopt_oc": (None, [I64, P, P, D, D, P]),
@@ -546,3 +547,21 @@ def newmark(m, c, k, u0, v0, F, dt, beta, gamma, nsteps, fscale=None):
     if rc != 0:
         raise SolveError("sparse numeric factorization/solve failed: singular matrix")
     return hist
+
+
+def apply_robin(coords, cell, conn, dpn, h, axis=-1, coord=0.0, tol=0.0, order=2):
+    """apply_robin_order (crates/tpt-fem-assembly/src/lib.rs:684-753) -> the appended triplets as a Coo."""
+    coords, conn = f64(coords), i64(conn)
+    fn = {LINE: 1, TRI: 2, QUAD: 2, TET: 3, HEX: 4}[cell]
+    cap = max(1, conn.shape[0] * 6 * 9 * fn * fn * dpn)
+    r, c, v = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(cap)
+    t = lib().orc_apply_robin(coords.shape[0], coords.shape[1], _p(coords), conn.shape[0], cell, _p(conn), dpn, h,
+                              axis, coord, tol, order, _p(r), _p(c), _p(v))
+    return Coo(r[:t], c[:t], v[:t])
+
+
+def assemble_robin(coords, cell, conn, op, dpn, h, axis=-1, coord=0.0, tol=0.0):
+    """assemble(...) then apply_robin(...) on the same Coo (thermal/src/lib.rs:219-237 composition)."""
+    a = assemble(coords, conn, op, dpn)
+    b = apply_robin(coords, cell, conn, dpn, h, axis, coord, tol)
+    return Coo(np.concatenate([a.rows, b.rows]), np.concatenate([a.cols, b.cols]), np.concatenate([a.vals, b.vals]))

@@ -940,6 +940,75 @@ static void neumann_pass(int64_t nnodes, int pdim, const double *coords, int64_t
   free(be); free(bf);
 }
 
+/* apply_robin_order: crates/tpt-fem-assembly/src/lib.rs:684-753.  Appends,
+ * for every boundary face (boundary_faces order), quadrature point, face node
+ * pair (k, l) and component c, the triplet (n_k*dpn + c, n_l*dpn + c,
+ * w * measure * h * N_k * N_l) -- no zero skip -- with h the constant hconst,
+ * or hconst on |x[axis] - coord| <= tol and 0 elsewhere when axis >= 0.
+ * Returns the number of triplets written.                                   */
+int64_t orc_apply_robin(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                        const int64_t *conn, int dpn, double hconst, int axis, double coord,
+                        double tol, int order, int64_t *rows, int64_t *cols, double *vals) {
+  (void)nnodes;
+  int nf, nn = orc_cell_nodes(cell);
+  const face_def *F = faces_of(cell, &nf);
+  int64_t *be = (int64_t *)malloc(sizeof(int64_t) * (nelem * nf + 1));
+  int64_t *bf = (int64_t *)malloc(sizeof(int64_t) * (nelem * nf + 1));
+  int64_t nb = orc_boundary_faces(nelem, cell, conn, be, bf);
+  int64_t t = 0;
+  for (int64_t q = 0; q < nb; q++) {
+    const int64_t e = be[q];
+    const face_def *fd = &F[bf[q]];
+    if (fd->nn == 1) continue; /* point faces of Line2 cells, as in the Neumann pass */
+    double phys[8][3];
+    for (int a = 0; a < nn; a++)
+      for (int d = 0; d < pdim; d++) phys[a][d] = coords[conn[e * nn + a] * pdim + d];
+    double qp[125 * 3], qw[125];
+    int nq = 0;
+    face_quad(fd->fcell, order, qp, qw, &nq);
+    const int fdim = orc_cell_dim(fd->fcell);
+    for (int iq = 0; iq < nq; iq++) {
+      double ns[8], ng[8 * 3], xp[3] = {0, 0, 0}, tang[2][3] = {{0, 0, 0}, {0, 0, 0}};
+      orc_shape(fd->fcell, qp + 3 * iq, ns);
+      orc_grad(fd->fcell, qp + 3 * iq, ng);
+      for (int k = 0; k < fd->nn; k++)
+        for (int d = 0; d < pdim; d++) xp[d] += ns[k] * phys[fd->idx[k]][d];
+      for (int m = 0; m < fdim; m++)
+        for (int k = 0; k < fd->nn; k++)
+          for (int d = 0; d < pdim; d++) tang[m][d] += ng[k * fdim + m] * phys[fd->idx[k]][d];
+      double measure;
+      if (fdim == 1) {
+        double s2 = -0.0;
+        for (int d = 0; d < pdim; d++) s2 = s2 + tang[0][d] * tang[0][d];
+        measure = sqrt(s2);
+      } else {
+        const double c0 = tang[0][1] * tang[1][2] - tang[0][2] * tang[1][1];
+        const double c1 = tang[0][2] * tang[1][0] - tang[0][0] * tang[1][2];
+        const double c2 = tang[0][0] * tang[1][1] - tang[0][1] * tang[1][0];
+        double s2 = -0.0;
+        s2 = s2 + c0 * c0;
+        s2 = s2 + c1 * c1;
+        s2 = s2 + c2 * c2;
+        measure = sqrt(s2);
+      }
+      double h = hconst;
+      if (axis >= 0 && !(fabs(xp[axis] - coord) <= tol)) h = 0.0;
+      for (int k = 0; k < fd->nn; k++)
+        for (int l = 0; l < fd->nn; l++) {
+          const int64_t bi = conn[e * nn + fd->idx[k]] * dpn, bj = conn[e * nn + fd->idx[l]] * dpn;
+          for (int c = 0; c < dpn; c++) {
+            rows[t] = bi + c;
+            cols[t] = bj + c;
+            vals[t] = qw[iq] * measure * h * ns[k] * ns[l];
+            t++;
+          }
+        }
+    }
+  }
+  free(be); free(bf);
+  return t;
+}
+
 void orc_apply_neumann(int64_t nnodes, int pdim, const double *coords, int64_t nelem,
                        int cell, const int64_t *conn, int dpn, double g, int order,
                        double *rhs) {

@@ -164,4 +164,9 @@ int orc_central_difference(int64_t n, orc_csr M, orc_csr D, orc_csr K, const dou
 int orc_newmark(int64_t n, orc_csr M, orc_csr D, orc_csr K, const double *u0, const double *v0,
                 const double *F, const double *fscale, double dt, double beta, double gamma,
                 int64_t nsteps, double *history);
+
+/* ---- Robin terms (crates/tpt-fem-assembly/src/lib.rs:684-753) ---- */
+int64_t orc_apply_robin(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                        const int64_t *conn, int dpn, double hconst, int axis, double coord,
+                        double tol, int order, int64_t *rows, int64_t *cols, double *vals);
 #endif

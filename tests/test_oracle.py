@@ -546,3 +546,26 @@ def test_dynamic_discrete_closed_forms():
     want = np.array([phi @ (q0 * np.cos(j * th)) for j in kk])
     got = O.newmark(tocsr(Md), O.Coo([], [], []).to_csr(), tocsr(Kd), u0, [0.0, 0.0], [0.0, 0.0], dt, 0.25, 0.5, N)
     np.testing.assert_allclose(got, want, atol=1e-12)
+
+
+# ---- Robin terms (crates/tpt-fem-assembly/src/lib.rs:684-753) ----------------------
+def test_robin_reference_kat_and_face_mass():
+    """robin_conserves_total_measure (:834-851): h = 1 on the unit right
+    triangle sums to its perimeter 2 + sqrt(2).  On a tet box each boundary
+    face contributes the exact P1 face mass h A / 12 (1 + d_kl) (the Degree2
+    rule integrates it exactly), the whole boundary h * area, and a band on
+    x = 1 only that face's share."""
+    tc, te = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]), np.array([[0, 1, 2]])
+    assert abs(O.apply_robin(tc, O.TRI, te, 1, 1.0).vals.sum() - (2 + np.sqrt(2))) < 1e-12
+    c, e = O.box_mesh([0, 0, 0], [1.0, 2.0, 1.5], (2, 2, 3))
+    r = O.apply_robin(c, O.TET, e, 1, 2.0)
+    area = 2 * (1.0 * 2.0 + 1.0 * 1.5 + 2.0 * 1.5)
+    assert abs(r.vals.sum() - 2.0 * area) < 1e-12
+    # the first boundary face: 3 points x 9 node pairs, consecutive
+    f0 = r.vals[:27].reshape(3, 3, 3).sum(0)
+    rows0 = r.rows[:9].reshape(3, 3)[:, 0]
+    X = c[rows0]
+    A = 0.5 * np.linalg.norm(np.cross(X[1] - X[0], X[2] - X[0]))
+    np.testing.assert_allclose(f0, 2.0 * A / 12 * (1 + np.eye(3)), rtol=1e-14)
+    band = O.apply_robin(c, O.TET, e, 1, 2.0, axis=0, coord=1.0, tol=1e-9)
+    assert len(band.vals) == len(r.vals) and abs(band.vals.sum() - 2.0 * 2.0 * 1.5) < 1e-12

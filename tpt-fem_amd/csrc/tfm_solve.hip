@@ -1480,6 +1480,34 @@ extern "C" int tfm_solve_poisson(tfm_ctx *ctx, const tfm_mesh *mesh, double kapp
   return rc;
 }
 
+extern "C" int tfm_solve_poisson_nr(tfm_ctx *ctx, const tfm_mesh *mesh, double kappa, int quad_order, double f,
+                                    const double *neumann_g, const double *robin_h, int64_t n_bc,
+                                    const int64_t *bc_dof, const double *bc_val, const tfm_solve_opts *opts,
+                                    double *u) {
+  if (!ctx || !mesh || !u) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  std::vector<double> rhs((size_t)mesh->n_nodes, 0.0);
+  if ((rc = tfm_load_source(ctx, mesh, quad_order, f, rhs.data()))) return rc;
+  if (neumann_g && (rc = tfm_load_neumann(ctx, mesh, 1, *neumann_g, rhs.data()))) return rc;
+  tfm_operator op{};
+  op.kind = TFM_OP_POISSON;
+  op.quad_order = quad_order;
+  op.coef = kappa;
+  tfm_matrix *K = nullptr;
+  rc = robin_h ? tfm_assemble_robin(ctx, mesh, &op, 1, *robin_h, -1, 0.0, 0.0, &K) : tfm_assemble(ctx, mesh, &op, 1, &K);
+  if (rc) return rc;
+  tfm_system *s = nullptr;
+  rc = tfm_system_from_matrix(ctx, K, mesh->n_nodes, rhs.data(), n_bc, bc_dof, bc_val, &s);
+  tfm_matrix_destroy(K);
+  if (rc) return rc;
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 40};
+  if (opts) o = *opts;
+  rc = tfm_system_solve(s, &o, u, nullptr);
+  tfm_system_destroy(s);
+  return rc;
+}
+
 extern "C" int tfm_solve_elasticity(tfm_ctx *ctx, const tfm_mesh *mesh, int model, double young, double nu,
                                     int quad_order, const double *body, int64_t n_bc, const int64_t *bc_dof,
                                     const double *bc_val, const tfm_solve_opts *opts, double *u) {

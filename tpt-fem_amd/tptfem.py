@@ -134,6 +134,8 @@ _SIGS = {
     "tfm_central_difference": (INT, [P, P, P, I64, P, P, P, P, D, INT, P, P]),
     "tfm_newmark": (INT, [P, P, P, I64, P, P, P, P, D, D, D, INT, C.POINTER(SolveOpts), P, P]),
     "tfm_solve_poisson": (INT, [P, P, D, INT, D, I64, P, P, C.POINTER(SolveOpts), P]),
+    "tfm_assemble_robin": (INT, [P, P, C.POINTER(Operator), INT, D, INT, D, D, PP]),
+    "tfm_solve_poisson_nr": (INT, [P, P, D, INT, D, P, P, I64, P, P, C.POINTER(SolveOpts), P]),
     "tfm_solve_elasticity": (INT, [P, P, INT, D, D, INT, P, I64, P, P, C.POINTER(SolveOpts), P]),
 }
 for _name, (_res, _args) in _SIGS.items():
@@ -611,3 +613,24 @@ def newmark(m, c, k, u0, v0, f, dt, beta, gamma, nsteps, fscale=None, rtol=1e-12
                                  C.byref(o), _ptr(hist), C.byref(st)))
     m.stats = st
     return hist
+
+
+def assemble_robin(mesh, dofs_per_node, op, h, axis=-1, coord=0.0, tol=0.0):
+    """to_csr() of assemble(...) followed by apply_robin(mesh, dpn, h, coo)
+    (crates/tpt-fem-assembly/src/lib.rs:674-753), h constant or on a plane band."""
+    out = C.c_void_p()
+    mesh.ctx.check(_lib.tfm_assemble_robin(mesh.ctx.h, mesh.h, C.byref(op), dofs_per_node, h, axis, coord, tol,
+                                           C.byref(out)))
+    return Matrix(mesh.ctx, out)
+
+
+def solve_poisson_nr(mesh, kappa, quad_order, f, bcs, neumann=None, robin=None, rtol=1e-13):
+    """solve_poisson (crates/tpt-fem-thermal/src/lib.rs:206-240) with constant Neumann flux and Robin coefficient."""
+    nb, bd, bv = _bc_arrays(bcs)
+    u = np.zeros(mesh.n_nodes)
+    g = None if neumann is None else _f64([neumann])
+    hh = None if robin is None else _f64([robin])
+    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
+    mesh.ctx.check(_lib.tfm_solve_poisson_nr(mesh.ctx.h, mesh.h, kappa, quad_order, f, _ptr(g), _ptr(hh), nb,
+                                             _ptr(bd), _ptr(bv), C.byref(o), _ptr(u)))
+    return u
