@@ -273,6 +273,15 @@ int tfm_newmark(const tfm_matrix *M, const tfm_matrix *C, const tfm_matrix *K, i
                 double dt, double beta, double gamma, int nsteps, const tfm_solve_opts *opts,
                 double *history, tfm_solve_stats *stats);
 
+/* ---- results ------------------------------------------------------------- */
+/* write_vtk / write_vtk_with_data (io-vtk/src/lib.rs:117-183): the mesh and
+ * n_fields per-node scalar fields (values[f]: host, n_nodes; names without
+ * blanks) as the legacy BINARY (big-endian) unstructured-grid file vtkio
+ * exports; the byte-swapped blocks are built on the device.  I/O failures
+ * return TFM_E_INVALID with VtkError::Io's text. */
+int tfm_write_vtk(const tfm_mesh *mesh, int n_fields, const char *const *names,
+                  const double *const *values, const char *path);
+
 /* ---- drivers ------------------------------------------------------------ */
 /* solve_poisson (thermal/src/lib.rs:206-240) with constant source f, no
  * Neumann/Robin terms. */

@@ -565,3 +565,29 @@ def assemble_robin(coords, cell, conn, op, dpn, h, axis=-1, coord=0.0, tol=0.0):
     a = assemble(coords, conn, op, dpn)
     b = apply_robin(coords, cell, conn, dpn, h, axis, coord, tol)
     return Coo(np.concatenate([a.rows, b.rows]), np.concatenate([a.cols, b.cols]), np.concatenate([a.vals, b.vals]))
+
+
+# ---- legacy binary VTK (crates/tpt-fem-io-vtk/src/lib.rs:117-170) --------------
+VTK_TYPE = {LINE: 3, TRI: 5, QUAD: 9, TET: 10, HEX: 12}  # vtk_cell_type (:101-114), P1 cells
+
+
+def vtk_legacy_bytes(coords, cell, conn, fields=(), title="tpt-fem mesh"):
+    """write_vtk_with_data's file (mesh_to_vtk :117-170 exported by vtkio as
+    legacy BINARY, big endian): points padded to 3 components as f64, cells as
+    u32 (count then node ids), types as i32, one SCALARS block per field."""
+    coords, conn = f64(coords), i64(conn)
+    n, ne = coords.shape[0], conn.shape[0]
+    pts = np.zeros((n, 3))
+    pts[:, :coords.shape[1]] = coords
+    nn = conn.shape[1]
+    verts = np.concatenate([np.full((ne, 1), nn), conn], axis=1).astype('>u4').ravel()
+    out = [f"# vtk DataFile Version 4.1\n{title}\nBINARY\n\nDATASET UNSTRUCTURED_GRID\nPOINTS {n} double\n".encode(),
+           pts.astype('>f8').tobytes(),
+           f"\n\nCELLS {ne} {ne * (nn + 1)}\n".encode(), verts.tobytes(),
+           f"\n\nCELL_TYPES {ne}\n".encode(), np.full(ne, VTK_TYPE[cell], '>i4').tobytes(),
+           f"\n\nPOINT_DATA {n}\n".encode()]
+    for name, vals in fields:
+        out += [f"\nSCALARS {name} double 1\nLOOKUP_TABLE default\n".encode(), f64(vals).astype('>f8').tobytes(),
+                b"\n"]
+    out.append(f"\nCELL_DATA {ne}\n\n".encode())
+    return b"".join(out)

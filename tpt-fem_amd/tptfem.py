@@ -131,6 +131,7 @@ _SIGS = {
     "tfm_topopt_simp_batch": (INT, [P, I64, C.POINTER(TopOptParams), P, P, I64, P, P, C.POINTER(SolveOpts), P, P,
                                     P, P]),
     "tfm_lanczos_eigs": (INT, [P, INT, INT, INT, P, P, P]),
+    "tfm_write_vtk": (INT, [P, INT, P, P, C.c_char_p]),
     "tfm_central_difference": (INT, [P, P, P, I64, P, P, P, P, D, INT, P, P]),
     "tfm_newmark": (INT, [P, P, P, I64, P, P, P, P, D, D, D, INT, C.POINTER(SolveOpts), P, P]),
     "tfm_solve_poisson": (INT, [P, P, D, INT, D, I64, P, P, C.POINTER(SolveOpts), P]),
@@ -634,3 +635,16 @@ def solve_poisson_nr(mesh, kappa, quad_order, f, bcs, neumann=None, robin=None, 
     mesh.ctx.check(_lib.tfm_solve_poisson_nr(mesh.ctx.h, mesh.h, kappa, quad_order, f, _ptr(g), _ptr(hh), nb,
                                              _ptr(bd), _ptr(bv), C.byref(o), _ptr(u)))
     return u
+
+
+def write_vtk(mesh, path, fields=()):
+    """write_vtk / write_vtk_with_data (crates/tpt-fem-io-vtk/src/lib.rs:117-183):
+    fields is a sequence of (name, per-node values)."""
+    names = (C.c_char_p * max(1, len(fields)))(*[f[0].encode() for f in fields])
+    arrs = [_f64(f[1]) for f in fields]
+    for a in arrs:
+        if len(a) != mesh.n_nodes:
+            raise ValueError("a field needs one value per node")
+    vals = (C.c_void_p * max(1, len(fields)))(*[a.ctypes.data for a in arrs])
+    mesh.ctx.check(_lib.tfm_write_vtk(mesh.h, len(fields), C.cast(names, C.c_void_p), C.cast(vals, C.c_void_p),
+                                      str(path).encode()))
