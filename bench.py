@@ -148,7 +148,10 @@ def bench_batch(args, T, log):
     log(f"assembly fp64 {times['fp64']:.3f} ms, fp32 {times['fp32']:.3f} ms")
     s.assemble(None, 0)
     solves = {}
-    for mode, name, rtol in ((T.SOLVE_PCG64, "plain_fp64", 1e-12), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", 1e-10)):
+    modes = [(T.SOLVE_PCG64, "plain_fp64", 1e-12), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", 1e-10)]
+    if args.workload == "c5":
+        modes.append((T.SOLVE_PCG64_MG, "multigrid_fp64", 1e-12))
+    for mode, name, rtol in modes:
         u = s.solve_batch(off, rhs=f, nrhs=nrhs, mode=mode, rtol=rtol, max_iter=200000, max_outer=400)
         st = s.batch_stats
         rec = {"rtol": rtol, "s": st[0][3], "max_iters": max(x[0] for x in st),
@@ -185,7 +188,8 @@ def bench_simp(args, T, log):
     desc = (f"{P} SIMP cantilever designs, Grid {nx}x{ny} nodes ({(nx - 1) * (ny - 1)} quads), point load at a "
             f"seeded right-edge node, vol 0.5, p 3, R 2, move 0.2, {iters} OC iterations")
     out = {}
-    for mode, name, rtol in ((T.SOLVE_PCG64, "plain_fp64", 1e-10), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", 1e-10)):
+    for mode, name, rtol in ((T.SOLVE_PCG64, "plain_fp64", 1e-10), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", 1e-10),
+                             (T.SOLVE_PCG64_MG, "multigrid_fp64", 1e-10)):
         T.topopt_simp(ctx, nx, ny, 1.0, 1.0, 0.3, 0.5, 3.0, 2.0, 1, 0.2, F[:4], bcs, mode=mode, rtol=rtol)  # warm-up
         ctx.synchronize()
         t0 = time.perf_counter()

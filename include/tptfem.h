@@ -167,7 +167,10 @@ int tfm_system_info(const tfm_system *sys, int64_t *n_free, int64_t *nnz, int64_
 int tfm_system_download(const tfm_system *sys, int64_t *row_ptrs, int64_t *col_ind,
                         double *values, double *fred, int64_t *free_dofs);
 
-enum { TFM_SOLVE_PCG64 = 0, TFM_SOLVE_MIXED = 1 };
+/* TFM_SOLVE_PCG64_MG: FP64 CG preconditioned by a geometric multigrid
+   V-cycle; batched SIMP systems on a tfm_mesh_grid_batch mesh only (one
+   problem per design), TFM_E_UNSUPPORTED elsewhere. */
+enum { TFM_SOLVE_PCG64 = 0, TFM_SOLVE_MIXED = 1, TFM_SOLVE_PCG64_MG = 2 };
 typedef struct {
   int mode;           /* TFM_SOLVE_*                                          */
   double rtol;        /* stop when ||f - K u|| <= rtol * ||f|| (FP64 residual) */

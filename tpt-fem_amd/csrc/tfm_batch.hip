@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "tfm_bsell.h"
 #include "tfm_elem.h"
@@ -120,8 +121,8 @@ __device__ __forceinline__ void rhs_sum(double x, double (&out)[NR]) {
     if ((int)(threadIdx.x % NR) == k) out[k] = x;
 }
 
-// z = Dinv r, both direction buffers zero, partial r.z and r.r into the
-// parity-1 slots (they play "iteration -1").
+// z = Dinv r (dinv null: z is given), both direction buffers zero, partial
+// r.z and r.r into the parity-1 slots (they play "iteration -1").
 template <typename VT, int NR>
 __global__ __launch_bounds__(kBT) void kbx_init(BView V, const VT *__restrict__ r, VT *__restrict__ z,
                                                 VT *__restrict__ p0, VT *__restrict__ p1, const VT *__restrict__ dinv) {
@@ -129,8 +130,8 @@ __global__ __launch_bounds__(kBT) void kbx_init(BView V, const VT *__restrict__ 
   blk_rows(V, blockIdx.x, r0, r1);
   double rz = 0.0, rr = 0.0;
   for (int i = r0 * NR + threadIdx.x; i < r1 * NR; i += kBT) {
-    const VT zi = dinv[i / NR] * r[i];
-    z[i] = zi;
+    const VT zi = dinv ? dinv[i / NR] * r[i] : z[i];
+    if (dinv) z[i] = zi;
     p0[i] = 0;
     p1[i] = 0;
     rz += (double)r[i] * (double)zi;
@@ -234,7 +235,8 @@ __global__ __launch_bounds__(kBT) void kbx_dir(BView V, const VT *__restrict__ b
 }
 
 // alpha = r.z / p.q per segment; x += alpha p, r -= alpha q, z = Dinv r on
-// the active segments; partial r.z, r.r.  Block 0 advances the counter.
+// the active segments; partial r.z, r.r.  Block 0 advances the counter.  With
+// dinv null z is left to a separate preconditioner and kbx_rz.
 template <typename VT, typename XT, int NR>
 __global__ __launch_bounds__(kBT) void kbx_update(BView V, XT *__restrict__ x, VT *__restrict__ r,
                                                   VT *__restrict__ z, const VT *__restrict__ p,
@@ -282,9 +284,11 @@ __global__ __launch_bounds__(kBT) void kbx_update(BView V, XT *__restrict__ x, V
       x[i] += (XT)(a * (double)p[i]);
       const VT ri = r[i] - af * q[i];
       r[i] = ri;
-      const VT zi = dinv[i / NR] * ri;
-      z[i] = zi;
-      zr += (double)ri * (double)zi;
+      if (dinv) {
+        const VT zi = dinv[i / NR] * ri;
+        z[i] = zi;
+        zr += (double)ri * (double)zi;
+      }
       rr += (double)ri * (double)ri;
     }
   }
@@ -293,6 +297,19 @@ __global__ __launch_bounds__(kBT) void kbx_update(BView V, XT *__restrict__ x, V
   rhs_sum<NR>(rr, c2);
   put_part<NR>(V, e ? Q_RZ1 : Q_RZ0, c1);
   put_part<NR>(V, e ? Q_RR1 : Q_RR0, c2);
+}
+
+// Partial r.z into `slot` after an external preconditioner wrote z.
+template <int NR>
+__global__ __launch_bounds__(kBT) void kbx_rz(BView V, const double *__restrict__ r, const double *__restrict__ z,
+                                              int slot) {
+  int r0, r1;
+  blk_rows(V, blockIdx.x, r0, r1);
+  double s = 0.0;
+  for (int i = r0 * NR + threadIdx.x; i < r1 * NR; i += kBT) s += r[i] * z[i];
+  double a[NR];
+  rhs_sum<NR>(s, a);
+  put_part<NR>(V, slot, a);
 }
 
 // FP64 residual r = f - K u (CSR, fixed order) over the block's rows and
@@ -609,13 +626,21 @@ static void bb_dir(const BBatch &B, const BView &V, hipStream_t st, const VT *bv
 // iteration, 16 iterations per graph replay.
 template <typename VT, typename XT, int NR>
 static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p0, VT *p1, VT *q, const VT *dinv,
-                XT *x, double rtol, int max_iter, const int *flags, std::vector<Seg> &hs) {
+                XT *x, double rtol, int max_iter, const int *flags, std::vector<Seg> &hs, const MgData *mg = nullptr) {
   hipStream_t st = ctx->stream;
   const BView V = B.view(max_iter, rtol * rtol);
   const int nseg = B.P * B.NR;
   dim3 g(B.nblk), t(kBT);
+  // multigrid (FP64 only): z = M r by a V-cycle instead of Dinv r
+  const VT *jac = dinv;
+  if constexpr (std::is_same<VT, double>::value) {
+    if (mg) {
+      jac = nullptr;
+      mg_apply(mg, r, z, st);
+    }
+  }
   hipLaunchKernelGGL(kb_reset, dim3((nseg + 255) / 256), dim3(256), 0, st, B.seg, nseg, flags, B.it, 1);
-  hipLaunchKernelGGL((kbx_init<VT, NR>), g, t, 0, st, V, r, z, p0, p1, dinv);
+  hipLaunchKernelGGL((kbx_init<VT, NR>), g, t, 0, st, V, r, z, p0, p1, jac);
   TFM_CHECK_HIP(ctx, hipGetLastError());
   const int CH = 16;  // iterations per replay (even: buffer parity is static)
   VT *pb[2] = {p0, p1};
@@ -623,7 +648,13 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
     for (int k = 0; k < CH; k++) {
       const int e = k & 1;
       bb_dir<VT, NR>(B, V, st, bv, z, pb[1 - e], pb[e], q, e);
-      hipLaunchKernelGGL((kbx_update<VT, XT, NR>), g, t, 0, st, V, x, r, z, pb[e], q, dinv, e);
+      hipLaunchKernelGGL((kbx_update<VT, XT, NR>), g, t, 0, st, V, x, r, z, pb[e], q, jac, e);
+      if constexpr (std::is_same<VT, double>::value) {
+        if (mg) {
+          mg_apply(mg, r, z, st);
+          hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, e ? Q_RZ1 : Q_RZ0);
+        }
+      }
     }
   };
   const bool use_graph = !getenv("TFM_NO_GRAPH");
@@ -813,6 +844,7 @@ struct BatchSolver {
   float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *d32 = nullptr,
         *x32 = nullptr;
   int *bad = nullptr, *odone = nullptr;
+  MgData *mg = nullptr;  // multigrid preconditioner (TFM_SOLVE_PCG64_MG)
 };
 
 void bsolver_destroy(BatchSolver *b) {
@@ -821,13 +853,27 @@ void bsolver_destroy(BatchSolver *b) {
   dfree(b->f); dfree(b->x); dfree(b->r64); dfree(b->d64); dfree(b->z64); dfree(b->p64); dfree(b->p64b);
   dfree(b->q64); dfree(b->r32); dfree(b->z32); dfree(b->p32); dfree(b->p32b); dfree(b->q32); dfree(b->d32);
   dfree(b->x32); dfree(b->bad); dfree(b->odone);
+  mg_destroy(b->mg);
   delete b;
 }
 
-int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, bool mixed, BatchSolver **out) {
+int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, int mode, BatchSolver **out) {
   tfm_ctx *ctx = s->ctx;
   *out = nullptr;
   if (NR != 1 && NR != 8) return fail(ctx, TFM_E_UNSUPPORTED, "batched solve supports 1 or 8 right-hand sides");
+  if (mode != TFM_SOLVE_PCG64 && mode != TFM_SOLVE_MIXED && mode != TFM_SOLVE_PCG64_MG)
+    return fail(ctx, TFM_E_INVALID, "unknown solve mode");
+  const bool mixed = mode == TFM_SOLVE_MIXED;
+  if (mode == TFM_SOLVE_PCG64_MG) {
+    // the V-cycle knows one layout: the SIMP operator on P concatenated
+    // grids, problem p owning design p's 2 nx ny DOFs
+    const tfm_mesh *m = s->has_plan ? s->plan.mesh : nullptr;
+    bool ok = m && s->plan.op.kind == TFM_OP_SIMP && s->dpn == 2 && m->grid_P == P && s->plan.d_rho;
+    for (int p = 0; ok && p <= P; p++) ok = dof_off[p] == (int64_t)p * 2 * m->grid_nx * m->grid_ny;
+    if (!ok)
+      return fail(ctx, TFM_E_UNSUPPORTED,
+                  "multigrid solve needs a SIMP system on a grid_batch mesh, one problem per design");
+  }
   if (mixed && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values");
   if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values");
   auto *b = new BatchSolver();
@@ -849,6 +895,8 @@ int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, bool mi
              (rc = dalloc(ctx, &b->p32, nv)) || (rc = dalloc(ctx, &b->p32b, nv)) ||
              (rc = dalloc(ctx, &b->q32, nv)) || (rc = dalloc(ctx, &b->d32, n)) || (rc = dalloc(ctx, &b->x32, nv)));
   }
+  if (!rc && mode == TFM_SOLVE_PCG64_MG)
+    rc = mg_create(ctx, P, (int)s->plan.mesh->grid_nx, (int)s->plan.mesh->grid_ny, NR, s->plan.ke0, &b->mg);
   if (!rc && hipMemsetAsync(b->x, 0, sizeof(double) * nv, ctx->stream) != hipSuccess)
     rc = fail(ctx, TFM_E_HIP, "hipMemsetAsync failed");
   if (rc) {
@@ -898,6 +946,8 @@ int bsolver_refresh(BatchSolver *b) {
   if (b->mixed)
     hipLaunchKernelGGL(kb_dinv<float>, dim3(g1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v32, s->d_fixed, b->d32,
                        b->bad);
+  if (b->mg && (rc = mg_update(b->mg, s->plan.d_rho, s->plan.op.penal, s->plan.op.e0, s->plan.op.emin, s->d_fixed)))
+    return rc;
   int hbad = 0;
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hbad, b->bad, sizeof(int), hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
@@ -950,7 +1000,7 @@ static int bsolver_run_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, tf
                        (float *)nullptr, (const int *)nullptr, (const int *)nullptr);
     hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
     rc = bpcg<double, double, NR>(ctx, B, B.v64, b->r64, b->z64, b->p64, b->p64b, b->q64, b->d64, b->x, o.rtol,
-                                  o.max_iter, nullptr, hs);
+                                  o.max_iter, nullptr, hs, b->mg);
     for (int q = 0; q < nseg; q++) {
       iters[q] = hs[q].iters;
       brk[q] = hs[q].breakdown;
@@ -1106,6 +1156,7 @@ extern "C" int tfm_mesh_grid_batch(tfm_ctx *ctx, int64_t P, int64_t nx, int64_t 
   if (nn >= INT32_MAX || ne * 8 >= INT32_MAX) return fail(ctx, TFM_E_UNSUPPORTED, "batch too large for int32 indexing");
   tfm_mesh *m = new tfm_mesh();
   m->ctx = ctx; m->cell = TFM_QUAD; m->nn = 4; m->pdim = 2; m->dim = 2; m->n_nodes = nn; m->n_elem = ne;
+  m->grid_P = P; m->grid_nx = nx; m->grid_ny = ny;
   if ((rc = dalloc(ctx, &m->d_coords, nn * 2)) || (rc = dalloc(ctx, &m->d_conn, ne * 4))) {
     tfm_mesh_destroy(m);
     return rc;
@@ -1136,7 +1187,7 @@ extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *d
   // A segment's rows must not couple to another problem's rows (block
   // diagonal): the kernels reduce per problem and would silently mix them.
   BatchSolver *b = nullptr;
-  if ((rc = bsolver_create(s, (int)P, dof_off, nrhs, o.mode == TFM_SOLVE_MIXED, &b))) return rc;
+  if ((rc = bsolver_create(s, (int)P, dof_off, nrhs, o.mode, &b))) return rc;
   if ((rc = bsolver_set_rhs(b, rhs)) || (rc = bsolver_refresh(b))) {
     bsolver_destroy(b);
     return rc;

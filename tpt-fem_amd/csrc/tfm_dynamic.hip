@@ -315,7 +315,7 @@ extern "C" int tfm_newmark(const tfm_matrix *M, const tfm_matrix *C, const tfm_m
   std::vector<double> zero(n, 0.0);
   const int64_t off[2] = {0, n};
   if ((rc = tfm_system_from_matrix(ctx, Kh, n, zero.data(), 0, nullptr, nullptr, &sk)) ||
-      (rc = bsolver_create(sk, 1, off, 1, false, &bs)) || (rc = bsolver_refresh(bs))) {
+      (rc = bsolver_create(sk, 1, off, 1, TFM_SOLVE_PCG64, &bs)) || (rc = bsolver_refresh(bs))) {
     cleanup();
     return rc;
   }

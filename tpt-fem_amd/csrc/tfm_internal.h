@@ -50,6 +50,8 @@ struct tfm_mesh {
   int *d_n2e = nullptr;
   int max_adj = 0;
   std::vector<int> h_n2e_ptr;  // kept for tile scheduling
+  // tfm_mesh_grid_batch: P copies of an nx x ny node grid (0 otherwise)
+  int64_t grid_P = 0, grid_nx = 0, grid_ny = 0;
 };
 
 struct tfm_matrix {
@@ -188,7 +190,10 @@ void bsell_destroy(BSell *b);
 // (1 or 8) right-hand sides each; the device solution stays in place between
 // runs, so a run can warm-start from the previous one.
 struct BatchSolver;
-int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, bool mixed, BatchSolver **out);
+// mode: TFM_SOLVE_PCG64 (Jacobi), TFM_SOLVE_MIXED (FP32 + FP64 refinement) or
+// TFM_SOLVE_PCG64_MG (multigrid V-cycle; SIMP systems on a tfm_mesh_grid_batch
+// mesh with one problem per design).
+int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, int mode, BatchSolver **out);
 int bsolver_set_rhs(BatchSolver *b, const double *rhs);
 int bsolver_refresh(BatchSolver *b);
 int bsolver_run(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_stats *stats, double *fu);
@@ -196,6 +201,14 @@ const double *bsolver_x(const BatchSolver *b);  // [n_dof][NR], fixed DOFs 0
 double *bsolver_x_mut(BatchSolver *b);          // the same, to seed a warm start
 int bsolver_set_rhs_dev(BatchSolver *b, const double *d_rhs);  // NR = 1, device [n_dof]
 void bsolver_destroy(BatchSolver *b);
+
+// Geometric multigrid for P batched topopt grids (tfm_mg.hip).
+struct MgData;
+int mg_levels(int nx, int ny, std::vector<int> *dims);
+int mg_create(tfm_ctx *ctx, int P, int nx, int ny, int NR, const double *ke0, MgData **out);
+int mg_update(MgData *mg, const double *d_rho, double penal, double e0, double emin, const unsigned char *d_fixed);
+void mg_apply(const MgData *mg, const double *r, double *z, hipStream_t st);  // z = M r, [n_dof][NR]
+void mg_destroy(MgData *mg);
 
 // Loads (tfm_loads.hip): device rhs of n_nodes*dpn doubles, accumulated.
 int load_source_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double f, double *d_rhs);

@@ -1274,6 +1274,13 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
   tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 1e-5, 40};
   if (opts) o = *opts;
   if (o.max_iter <= 0) o.max_iter = 20000;
+  if (o.mode != TFM_SOLVE_PCG64 && o.mode != TFM_SOLVE_MIXED && o.mode != TFM_SOLVE_PCG64_MG)
+    return fail(ctx, TFM_E_INVALID, "unknown solve mode");
+  if (o.mode == TFM_SOLVE_PCG64_MG) {
+    // one SIMP design on a grid: the batched solver with a single problem
+    const int64_t off[2] = {0, s->ndof};
+    return tfm_system_solve_batch(s, 1, off, 1, nullptr, &o, u, stats);
+  }
   const int n = (int)s->ndof;
   const bool mixed = o.mode == TFM_SOLVE_MIXED;
   // No free DOF: the reference's solve gets the 1 x 1 CSR of an empty

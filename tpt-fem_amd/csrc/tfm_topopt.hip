@@ -202,7 +202,7 @@ extern "C" int tfm_topopt_simp_batch(tfm_ctx *ctx, int64_t P, const tfm_topopt_p
   if ((rc = tfm_mesh_grid_batch(ctx, P, nx, ny, prm->h, &mesh)) ||
       (rc = tfm_system_create(ctx, mesh, &op, 2, P * n_bc, dofs.data(), vals.data(), f,
                               TFM_SYS_FP64 | (mixed ? TFM_SYS_FP32 : 0), &sys)) ||
-      (rc = bsolver_create(sys, (int)P, off.data(), 1, mixed, &bs)) || (rc = bsolver_set_rhs(bs, nullptr)) ||
+      (rc = bsolver_create(sys, (int)P, off.data(), 1, o.mode, &bs)) || (rc = bsolver_set_rhs(bs, nullptr)) ||
       (rc = dalloc(ctx, &d_raw, P * ne)) || (rc = dalloc(ctx, &d_sens, P * ne)) || (rc = dalloc(ctx, &d_vf, P)) ||
       (rc = dalloc(ctx, &d_ke0, 64))) {
     cleanup();

@@ -27,6 +27,9 @@ BAR, PLANE_STRESS, PLANE_STRAIN, CONTINUUM3D = 0, 1, 2, 3
 OP_POISSON, OP_ELASTICITY, OP_SIMP, OP_MASS = 0, 1, 2, 3
 SYS_FP64, SYS_FP32 = 1, 2
 SOLVE_PCG64, SOLVE_MIXED = 0, 1
+# FP64 CG with a geometric-multigrid V-cycle: batched SIMP systems on a
+# Mesh.grid_batch mesh (one problem per design) only
+SOLVE_PCG64_MG = 2
 NODES = {LINE: 2, TRI: 3, QUAD: 4, TET: 4, HEX: 8}
 DIMS = {LINE: 1, TRI: 2, QUAD: 2, TET: 3, HEX: 3}
 
