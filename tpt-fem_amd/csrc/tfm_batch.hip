@@ -15,12 +15,14 @@
 // precision form wraps FP32 segments in per-segment FP64 refinement.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <type_traits>
 
 #include "tfm_bsell.h"
 #include "tfm_elem.h"
 #include "tfm_internal.h"
+#include "tfm_mg.h"
 
 namespace tfm {
 
@@ -35,6 +37,7 @@ enum { Q_PQ = 0, Q_RZ0, Q_RZ1, Q_RR0, Q_RR1, Q_RES, Q_FF, Q_FU, Q_N };
 struct Seg {
   double rr, rr0;          // latest recurrence ||r||^2 and the initial one
   double rt, ff, fu;       // refinement residual: ||f - K u||^2, ||f||^2, f.u
+  double beta;             // multigrid mode: this iteration's beta
   int done, iters, breakdown, pad;
 };
 
@@ -232,6 +235,95 @@ __global__ __launch_bounds__(kBT) void kbx_dir(BView V, const VT *__restrict__ b
     }
   }
   put_part<NR>(V, Q_PQ, dot);
+}
+
+// Multigrid mode splits kbx_dir in three.  kbx_beta, one block per problem:
+// the stop test and beta of each of its segments (into the Seg).
+template <int NR>
+__global__ __launch_bounds__(kBT) void kbx_beta(BView V, int e) {
+  const int pr = blockIdx.x;
+  Seg *S = V.seg + (size_t)pr * NR;
+  const int it = *V.it;
+  double rr[NR], rzn[NR], rzo[NR];
+  prob_sum<NR>(V, e ? Q_RR0 : Q_RR1, pr, rr);
+  prob_sum<NR>(V, e ? Q_RZ0 : Q_RZ1, pr, rzn);
+  prob_sum<NR>(V, e ? Q_RZ1 : Q_RZ0, pr, rzo);
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < NR; k++) {
+    if (S[k].done) continue;
+    const bool fresh = S[k].rr0 < 0.0;
+    const double rr0 = fresh ? rr[k] : S[k].rr0;
+    const bool d = rr0 == 0.0 || rr[k] <= V.rtol2 * rr0 || it >= V.max_it;
+    S[k].beta = (it > 0 && !d && rzo[k] > 0.0) ? rzn[k] / rzo[k] : 0.0;
+    if (fresh) S[k].rr0 = rr[k];
+    S[k].rr = rr[k];
+    if (d) {
+      S[k].done = 1;
+      S[k].iters = it;
+    }
+  }
+}
+
+// kbx_dir_p: p = z + beta pold on the block's rows.
+template <int NR>
+__global__ __launch_bounds__(kBT) void kbx_dir_p(BView V, const double *__restrict__ z,
+                                                 const double *__restrict__ pold, double *__restrict__ pnew) {
+  const int b = blockIdx.x, pr = V.blk_prob[b];
+  const Seg *S = V.seg + (size_t)pr * NR;
+  __shared__ int s_any;
+  __shared__ double s_beta[NR];
+  if (threadIdx.x == 0) {
+    int a = 0;
+    for (int kk = 0; kk < NR; kk++) a |= !S[kk].done;
+    s_any = a;
+  }
+  if (threadIdx.x < NR) s_beta[threadIdx.x] = S[threadIdx.x].done ? 0.0 : S[threadIdx.x].beta;
+  __syncthreads();
+  if (!s_any) return;  // uniform per block
+  const double bk = s_beta[threadIdx.x % NR];
+  int r0, r1;
+  blk_rows(V, b, r0, r1);
+  for (int i = r0 * NR + threadIdx.x; i < r1 * NR; i += kBT) pnew[i] = z[i] + bk * pold[i];
+}
+
+// kbx_q_mg: q = K p matrix-free on SIMP grids (element moduli times the unit
+// stiffness) and the partial p.q; rows where `mask` (the finest Jacobi
+// diagonal inverse) is 0 -- the fixed DOFs -- get q = 0.  A block's rows are
+// whole nodes of one design.
+template <int NR>
+__global__ __launch_bounds__(kBT) void kbx_q_mg(BView V, MgOp o, const double *__restrict__ mask,
+                                                const double *__restrict__ pv, double *__restrict__ q) {
+  const int b = blockIdx.x, pr = V.blk_prob[b];
+  const Seg *S = V.seg + (size_t)pr * NR;
+  __shared__ int s_any;
+  if (threadIdx.x == 0) {
+    int a = 0;
+    for (int kk = 0; kk < NR; kk++) a |= !S[kk].done;
+    s_any = a;
+  }
+  __syncthreads();
+  if (!s_any) return;  // uniform per block, like kbx_dir_p's exit
+  const int k = threadIdx.x % NR;
+  int r0, r1;
+  blk_rows(V, b, r0, r1);
+  const int nn = o.nx * o.ny;
+  double dot = 0.0;
+  for (int idx = (r0 >> 1) * NR + threadIdx.x; idx < (r1 >> 1) * NR; idx += kBT) {
+    const int node = idx / NR;
+    const int p = node / nn, n = node - p * nn;
+    double y[2];
+    apply_node<true>(o, p, n % o.nx, n / o.nx, [&](int d) { return pv[d * NR + k]; }, y);
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const int g = 2 * node + a;
+      const double qa = mask[g] == 0.0 ? 0.0 : y[a];
+      q[g * NR + k] = qa;
+      dot += pv[g * NR + k] * qa;
+    }
+  }
+  double c[NR];
+  rhs_sum<NR>(dot, c);
+  put_part<NR>(V, Q_PQ, c);
 }
 
 // alpha = r.z / p.q per segment; x += alpha p, r -= alpha q, z = Dinv r on
@@ -631,27 +723,51 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
   const BView V = B.view(max_iter, rtol * rtol);
   const int nseg = B.P * B.NR;
   dim3 g(B.nblk), t(kBT);
-  // multigrid (FP64 only): z = M r by a V-cycle instead of Dinv r
+  hipLaunchKernelGGL(kb_reset, dim3((nseg + 255) / 256), dim3(256), 0, st, B.seg, nseg, flags, B.it, 1);
+  // multigrid (FP64 only): z = M r by a V-cycle instead of Dinv r, skipping
+  // the segments already done; the update kernels write z = w Dinv r (the
+  // V-cycle's first sweep) and kbx_rz replaces their r.z partials; K p is
+  // taken matrix-free
+  static_assert(sizeof(Seg) % sizeof(int) == 0, "Seg stride in ints");
+  const int *sdone = reinterpret_cast<const int *>(reinterpret_cast<const char *>(B.seg) + offsetof(Seg, done));
+  const int sstride = (int)(sizeof(Seg) / sizeof(int));
   const VT *jac = dinv;
+  MgOp mop{};
+  const double *mmask = nullptr;
   if constexpr (std::is_same<VT, double>::value) {
     if (mg) {
-      jac = nullptr;
-      mg_apply(mg, r, z, st);
+      jac = mg_sweep_weights(mg);
+      mop = mg_fine_op(mg);
+      mmask = mg_fine_dinv(mg);
     }
   }
-  hipLaunchKernelGGL(kb_reset, dim3((nseg + 255) / 256), dim3(256), 0, st, B.seg, nseg, flags, B.it, 1);
   hipLaunchKernelGGL((kbx_init<VT, NR>), g, t, 0, st, V, r, z, p0, p1, jac);
+  if constexpr (std::is_same<VT, double>::value) {
+    if (mg) {
+      mg_apply(mg, r, z, st, sdone, sstride);
+      hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, Q_RZ1);
+    }
+  }
   TFM_CHECK_HIP(ctx, hipGetLastError());
   const int CH = 16;  // iterations per replay (even: buffer parity is static)
   VT *pb[2] = {p0, p1};
   auto chunk = [&]() {
     for (int k = 0; k < CH; k++) {
       const int e = k & 1;
-      bb_dir<VT, NR>(B, V, st, bv, z, pb[1 - e], pb[e], q, e);
+      if constexpr (std::is_same<VT, double>::value) {
+        if (mg) {
+          hipLaunchKernelGGL((kbx_beta<NR>), dim3(B.P), t, 0, st, V, e);
+          hipLaunchKernelGGL((kbx_dir_p<NR>), g, t, 0, st, V, (const double *)z, (const double *)pb[1 - e], pb[e]);
+          hipLaunchKernelGGL((kbx_q_mg<NR>), g, t, 0, st, V, mop, mmask, (const double *)pb[e], q);
+        } else
+          bb_dir<VT, NR>(B, V, st, bv, z, pb[1 - e], pb[e], q, e);
+      } else {
+        bb_dir<VT, NR>(B, V, st, bv, z, pb[1 - e], pb[e], q, e);
+      }
       hipLaunchKernelGGL((kbx_update<VT, XT, NR>), g, t, 0, st, V, x, r, z, pb[e], q, jac, e);
       if constexpr (std::is_same<VT, double>::value) {
         if (mg) {
-          mg_apply(mg, r, z, st);
+          mg_apply(mg, r, z, st, sdone, sstride);
           hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, e ? Q_RZ1 : Q_RZ0);
         }
       }

@@ -15,10 +15,11 @@
 // each coarse operator is the Galerkin product P^T A P of the level below,
 // formed element by element (K_E = sum over E's children of P_c^T K_c P_c,
 // P_c the bilinear interpolation inside E), so soft and stiff regions keep
-// their coarse stiffness.  Smoothing: kMgNu weighted-Jacobi sweeps before and
-// after the coarse correction, which keeps the V-cycle symmetric positive
-// definite; coarsest level: the dense inverse per design, formed on the GPU
-// by an in-place Gauss-Jordan sweep in LDS.
+// their coarse stiffness.  Smoothing: one weighted-Jacobi sweep before and
+// one after the coarse correction, which keeps the V-cycle symmetric positive
+// definite (the pre-sweep from zero is fused into the residual); coarsest
+// level: the dense inverse per design, formed on the GPU by an in-place
+// Gauss-Jordan sweep in LDS.
 //
 // Vectors follow the batched solver's layout: [P * 2 * nodes][NR], NR right-
 // hand sides interleaved; one thread per (node, rhs) with the rhs fastest.
@@ -27,11 +28,11 @@
 #include <vector>
 
 #include "tfm_internal.h"
+#include "tfm_mg.h"
 
 namespace tfm {
 
 constexpr double kMgOmega = 0.6;  // Jacobi weight; lambda_max(D^-1 K) ~ 2.8 for Q4 plane stress
-constexpr int kMgNu = 1;          // sweeps before and after the coarse correction
 constexpr int kMgB = 256;
 constexpr int kMgMaxC = 90;       // coarsest DOFs per design: the dense matrix fills 64 KB of LDS
 
@@ -52,26 +53,9 @@ struct MgData {
   double *d_E = nullptr;     // [P][ne0] element Young's moduli
   const unsigned char *d_fixed = nullptr;  // [P * ndof0] (borrowed; null: nothing fixed)
   double *d_inv = nullptr;   // [P][nc][nc] inverse of the coarsest operator
+  double *d_wd0 = nullptr;   // [P * ndof0] w Dinv of level 0 (the first sweep's weights)
   int nc = 0;
 };
-
-// Operator of one level as the kernels see it.
-struct MgOp {
-  int nx, ny;
-  int64_t ndof, ne;
-  const double *E, *ke0;          // level 0
-  const unsigned char *fixed;     // level 0 (may be null)
-  const double *K;                // levels >= 1 (null at level 0)
-};
-
-// Local index of node (i, j) in element (ei, ej) (Grid::new order: bottom-
-// left, bottom-right, top-right, top-left) and the inverse map.
-__device__ __forceinline__ int elem_local(int i, int j, int ei, int ej) {
-  const int di = i - ei, dj = j - ej;
-  return dj == 0 ? di : 3 - di;
-}
-__device__ __forceinline__ int local_di(int l) { return l == 1 || l == 2; }
-__device__ __forceinline__ int local_dj(int l) { return l >= 2; }
 
 __device__ __forceinline__ bool is_fixed(const MgOp &o, int64_t g) { return o.fixed && o.fixed[g]; }
 
@@ -88,34 +72,27 @@ __device__ __forceinline__ double ke_entry(const MgOp &o, int p, int ei, int ej,
   return o.E[e] * o.ke0[r8 * 8 + c8];
 }
 
-// (A x) at node (i, j) of design p, both components, right-hand side k.
+// Segments (design p, rhs k) a V-cycle skips: done[(p * NR + k) * stride]
+// non-zero (the batched CG's per-segment stop flags); done null skips none.
+struct MgAct {
+  const int *done;
+  int stride;
+};
 template <int NR>
-__device__ __forceinline__ void apply_node(const MgOp &o, int p, int i, int j, int k, const double *__restrict__ x,
-                                           double (&y)[2]) {
-  const int nex = o.nx - 1, ney = o.ny - 1;
-  y[0] = y[1] = 0.0;
-  const int64_t base = (int64_t)p * o.ndof;
-  for (int ej = j - 1; ej <= j; ej++) {
-    if (ej < 0 || ej >= ney) continue;
-    for (int ei = i - 1; ei <= i; ei++) {
-      if (ei < 0 || ei >= nex) continue;
-      const int ln = elem_local(i, j, ei, ej);
-      const int64_t e = (int64_t)p * o.ne + (int64_t)ej * nex + ei;
-      const double *Ke = o.K ? o.K + e * 64 : o.ke0;
-      const double s = o.K ? 1.0 : o.E[e];
-#pragma unroll
-      for (int lm = 0; lm < 4; lm++) {
-        const int64_t nd = base + 2 * ((int64_t)(ej + local_dj(lm)) * o.nx + ei + local_di(lm));
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-          if (is_fixed(o, nd + b)) continue;
-          const double xv = s * x[(nd + b) * NR + k];
-          y[0] += Ke[(2 * ln) * 8 + 2 * lm + b] * xv;
-          y[1] += Ke[(2 * ln + 1) * 8 + 2 * lm + b] * xv;
-        }
-      }
-    }
-  }
+__device__ __forceinline__ bool skipped(const MgAct &a, int p, int k) {
+  return a.done && a.done[(p * NR + k) * a.stride];
+}
+
+// Thread -> (design, node, rhs) of a level, rhs fastest; false past the end.
+template <int NR>
+__device__ __forceinline__ bool node_thread(int P, int nn, int &p, int &n, int &k) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P * nn * NR) return false;
+  k = t % NR;
+  const int pn = t / NR;
+  p = pn / nn;
+  n = pn - p * nn;
+  return true;
 }
 
 __global__ void km_young(int64_t n, const double *__restrict__ rho, double penal, double e0, double emin,
@@ -124,8 +101,9 @@ __global__ void km_young(int64_t n, const double *__restrict__ rho, double penal
   if (t < n) E[t] = emin + pow(rho[t], penal) * (e0 - emin);
 }
 
-// Jacobi diagonal inverse of a level (0 at fixed or zero-diagonal DOFs).
-__global__ void km_dinv(int P, MgOp o, double *__restrict__ dinv) {
+// Jacobi diagonal inverse of a level (0 at fixed or zero-diagonal DOFs) and,
+// when wd is given, the first sweep's weights w Dinv.
+__global__ void km_dinv(int P, MgOp o, double *__restrict__ dinv, double *__restrict__ wd) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nn = (int64_t)o.nx * o.ny;
   if (t >= P * nn) return;
@@ -142,48 +120,43 @@ __global__ void km_dinv(int P, MgOp o, double *__restrict__ dinv) {
   }
   for (int a = 0; a < 2; a++) {
     const int64_t g = (int64_t)p * o.ndof + 2 * n + a;
-    dinv[g] = (!is_fixed(o, g) && d[a] > 0.0) ? 1.0 / d[a] : 0.0;
+    const double di = (!is_fixed(o, g) && d[a] > 0.0) ? 1.0 / d[a] : 0.0;
+    dinv[g] = di;
+    if (wd) wd[g] = kMgOmega * di;
   }
 }
 
-// One weighted-Jacobi sweep xo = xi + w Dinv (b - A xi); xi == nullptr means
-// xi = 0 (no product is formed).
-template <int NR>
-__global__ __launch_bounds__(kMgB) void km_smooth(int P, MgOp o, const double *__restrict__ dinv,
-                                                  const double *__restrict__ b, const double *__restrict__ xi,
-                                                  double *__restrict__ xo) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nn = (int64_t)o.nx * o.ny;
-  if (t >= P * nn * NR) return;
-  const int k = (int)(t % NR);
-  const int64_t pn = t / NR;
-  const int p = (int)(pn / nn), n = (int)(pn % nn);
-  double y[2] = {0.0, 0.0};
-  if (xi) apply_node<NR>(o, p, n % o.nx, n / o.nx, k, xi, y);
-#pragma unroll
-  for (int a = 0; a < 2; a++) {
-    const int64_t g = (int64_t)p * o.ndof + 2 * n + a;
-    const double x0 = xi ? xi[g * NR + k] : 0.0;
-    xo[g * NR + k] = x0 + kMgOmega * dinv[g] * (b[g * NR + k] - y[a]);
-  }
-}
-
-// r = b - A x (0 at fixed DOFs).
-template <int NR>
-__global__ __launch_bounds__(kMgB) void km_resid(int P, MgOp o, const double *__restrict__ b,
-                                                 const double *__restrict__ x, double *__restrict__ r) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nn = (int64_t)o.nx * o.ny;
-  if (t >= P * nn * NR) return;
-  const int k = (int)(t % NR);
-  const int64_t pn = t / NR;
-  const int p = (int)(pn / nn), n = (int)(pn % nn);
+// Descent with one pre-smoothing sweep from zero, x1 = w Dinv b (formed by
+// whoever wrote b): r = b - A x1.  Rows with a zero Jacobi weight (fixed
+// DOFs, or DOFs no basis function reaches) get r = 0.
+template <int NR, bool FINE>
+__global__ __launch_bounds__(kMgB) void km_resid0(int P, MgOp o, const double *__restrict__ dinv,
+                                                  const double *__restrict__ b, const double *__restrict__ x1,
+                                                  double *__restrict__ r, MgAct act) {
+  int p, n, k;
+  if (!node_thread<NR>(P, o.nx * o.ny, p, n, k) || skipped<NR>(act, p, k)) return;
   double y[2];
-  apply_node<NR>(o, p, n % o.nx, n / o.nx, k, x, y);
+  apply_node<FINE>(o, p, n % o.nx, n / o.nx, [&](int d) { return x1[d * NR + k]; }, y);
 #pragma unroll
   for (int a = 0; a < 2; a++) {
-    const int64_t g = (int64_t)p * o.ndof + 2 * n + a;
-    r[g * NR + k] = is_fixed(o, g) ? 0.0 : b[g * NR + k] - y[a];
+    const int g = p * (int)o.ndof + 2 * n + a;
+    r[g * NR + k] = dinv[g] == 0.0 ? 0.0 : b[g * NR + k] - y[a];
+  }
+}
+
+// Post-smoothing sweep: out = y + w Dinv (b - A y).
+template <int NR, bool FINE>
+__global__ __launch_bounds__(kMgB) void km_smooth(int P, MgOp o, const double *__restrict__ dinv,
+                                                  const double *__restrict__ b, const double *__restrict__ yv,
+                                                  double *__restrict__ out, MgAct act) {
+  int p, n, k;
+  if (!node_thread<NR>(P, o.nx * o.ny, p, n, k) || skipped<NR>(act, p, k)) return;
+  double y[2];
+  apply_node<FINE>(o, p, n % o.nx, n / o.nx, [&](int d) { return yv[d * NR + k]; }, y);
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    const int g = p * (int)o.ndof + 2 * n + a;
+    out[g * NR + k] = yv[g * NR + k] + kMgOmega * dinv[g] * (b[g * NR + k] - y[a]);
   }
 }
 
@@ -218,17 +191,15 @@ __device__ __forceinline__ double hat1(int i, int I, int n, int co) {
   return I0 == I ? 1.0 - w1 : (I1 == I ? w1 : 0.0);
 }
 
-// Restriction (transpose of the interpolation): bc = P^T rf.
+// Restriction (transpose of the interpolation): bc = P^T rf, and x1c = w
+// Dinv_c bc when the coarse level smooths (x1c non-null).
 template <int NR>
-__global__ __launch_bounds__(kMgB) void km_restrict(int P, int nxf, int nyf, int64_t ndf, int nxc, int nyc,
-                                                    int64_t ndc, int cox, int coy, const double *__restrict__ rf,
-                                                    double *__restrict__ bc) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nn = (int64_t)nxc * nyc;
-  if (t >= P * nn * NR) return;
-  const int k = (int)(t % NR);
-  const int64_t pn = t / NR;
-  const int p = (int)(pn / nn), n = (int)(pn % nn);
+__global__ __launch_bounds__(kMgB) void km_restrict(int P, int nxf, int nyf, int ndf, int nxc, int nyc, int ndc,
+                                                    int cox, int coy, const double *__restrict__ rf,
+                                                    double *__restrict__ bc, const double *__restrict__ dinvc,
+                                                    double *__restrict__ x1c, MgAct act) {
+  int p, n, k;
+  if (!node_thread<NR>(P, nxc * nyc, p, n, k) || skipped<NR>(act, p, k)) return;
   const int I = n % nxc, J = n / nxc;
   const int X = coarse_pos(I, nxf - 1, cox), Y = coarse_pos(J, nyf - 1, coy);
   double acc[2] = {0.0, 0.0};
@@ -242,45 +213,48 @@ __global__ __launch_bounds__(kMgB) void km_restrict(int P, int nxf, int nyf, int
       if (i < 0 || i >= nxf) continue;
       const double w = wy * hat1(i, I, nxf - 1, cox);
       if (w == 0.0) continue;
-      const int64_t g = (int64_t)p * ndf + 2 * ((int64_t)j * nxf + i);
+      const int g = p * ndf + 2 * (j * nxf + i);
       acc[0] += w * rf[g * NR + k];
       acc[1] += w * rf[(g + 1) * NR + k];
     }
   }
-  const int64_t g = (int64_t)p * ndc + 2 * n;
+  const int g = p * ndc + 2 * n;
   bc[g * NR + k] = acc[0];
   bc[(g + 1) * NR + k] = acc[1];
+  if (x1c) {  // the coarse level's pre-smoothed start w Dinv b
+    x1c[g * NR + k] = kMgOmega * dinvc[g] * acc[0];
+    x1c[(g + 1) * NR + k] = kMgOmega * dinvc[g + 1] * acc[1];
+  }
 }
 
-// xf += P xc; fixed fine DOFs are left alone (they stay 0).
+// Coarse correction on top of the pre-smoothed x1: y = x1 + P xc, 0 where
+// the Jacobi weight is 0 (fixed DOFs).
 template <int NR>
-__global__ __launch_bounds__(kMgB) void km_prolong(int P, int nxf, int nyf, int64_t ndf, int nxc, int64_t ndc,
-                                                   int cox, int coy, const unsigned char *__restrict__ fixed,
-                                                   const double *__restrict__ xc, double *__restrict__ xf) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nn = (int64_t)nxf * nyf;
-  if (t >= P * nn * NR) return;
-  const int k = (int)(t % NR);
-  const int64_t pn = t / NR;
-  const int p = (int)(pn / nn), n = (int)(pn % nn);
+__global__ __launch_bounds__(kMgB) void km_prolong0(int P, int nxf, int nyf, int ndf, int nxc, int ndc, int cox,
+                                                    int coy, const double *__restrict__ dinv,
+                                                    const double *__restrict__ x1, const double *__restrict__ xc,
+                                                    double *__restrict__ y, MgAct act) {
+  int p, n, k;
+  if (!node_thread<NR>(P, nxf * nyf, p, n, k) || skipped<NR>(act, p, k)) return;
   const int i = n % nxf, j = n / nxf;
   int I0, I1, J0, J1;
   double wx1, wy1;
   interp1(i, nxf - 1, cox, I0, I1, wx1);
   interp1(j, nyf - 1, coy, J0, J1, wy1);
   const double wx0 = 1.0 - wx1, wy0 = 1.0 - wy1;
-  const int64_t cb = (int64_t)p * ndc;
-  const int64_t c00 = cb + 2 * ((int64_t)J0 * nxc + I0), c10 = cb + 2 * ((int64_t)J0 * nxc + I1);
-  const int64_t c01 = cb + 2 * ((int64_t)J1 * nxc + I0), c11 = cb + 2 * ((int64_t)J1 * nxc + I1);
+  const int cb = p * ndc;
+  const int c00 = cb + 2 * (J0 * nxc + I0), c10 = cb + 2 * (J0 * nxc + I1);
+  const int c01 = cb + 2 * (J1 * nxc + I0), c11 = cb + 2 * (J1 * nxc + I1);
 #pragma unroll
   for (int a = 0; a < 2; a++) {
-    const int64_t g = (int64_t)p * ndf + 2 * n + a;
-    if (fixed && fixed[g]) continue;
+    const int g = p * ndf + 2 * n + a;
+    const double d = dinv[g];
     const double v = wy0 * (wx0 * xc[(c00 + a) * NR + k] + wx1 * xc[(c10 + a) * NR + k]) +
                      wy1 * (wx0 * xc[(c01 + a) * NR + k] + wx1 * xc[(c11 + a) * NR + k]);
-    xf[g * NR + k] += v;
+    y[g * NR + k] = d == 0.0 ? 0.0 : x1[g * NR + k] + v;
   }
 }
+
 
 // Galerkin coarse element matrices, one thread per (design, coarse element,
 // entry): K_E[r][c] = sum over E's children ch and child nodes l, m of
@@ -380,18 +354,17 @@ __global__ __launch_bounds__(256) void km_coarse_inv(MgOp o, int nc, double *__r
 // is symmetric, so row r is read as column r (coalesced).
 template <int NR>
 __global__ __launch_bounds__(kMgB) void km_coarse_solve(int P, int nc, const double *__restrict__ inv,
-                                                        const double *__restrict__ b, double *__restrict__ x) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)P * nc * NR) return;
-  const int k = (int)(t % NR);
-  const int64_t pr = t / NR;
-  const int p = (int)(pr / nc), r = (int)(pr % nc);
+                                                        const double *__restrict__ b, double *__restrict__ x,
+                                                        MgAct act) {
+  int p, r, k;
+  if (!node_thread<NR>(P, nc, p, r, k) || skipped<NR>(act, p, k)) return;
   const double *Ai = inv + (int64_t)p * nc * nc;
-  const double *bp = b + (int64_t)p * nc * NR;
+  const double *bp = b + p * nc * NR;
   double s = 0.0;
-  for (int c = 0; c < nc; c++) s += Ai[(int64_t)c * nc + r] * bp[(int64_t)c * NR + k];
-  x[pr * NR + k] = s;
+  for (int c = 0; c < nc; c++) s += Ai[c * nc + r] * bp[c * NR + k];
+  x[(p * nc + r) * NR + k] = s;
 }
+
 
 static unsigned mg_grid(int64_t n) { return (unsigned)((n + kMgB - 1) / kMgB); }
 
@@ -417,6 +390,7 @@ void mg_destroy(MgData *mg) {
   dfree(mg->d_ke0);
   dfree(mg->d_E);
   dfree(mg->d_inv);
+  dfree(mg->d_wd0);
   delete mg;
 }
 
@@ -454,6 +428,10 @@ int mg_create(tfm_ctx *ctx, int P, int nx, int ny, int NR, const double *ke0, Mg
     mg->lv.push_back(v);
   }
   mg->nc = (int)mg->lv.back().ndof;
+  if ((int64_t)P * mg->lv[0].ndof * NR >= INT32_MAX || (int64_t)P * mg->lv[0].ne * 64 >= INT32_MAX) {
+    delete mg;
+    return fail(ctx, TFM_E_UNSUPPORTED, "multigrid: batch too large for int32 indexing");
+  }
   int rc = TFM_OK;
   for (int l = 0; l <= mg->L && !rc; l++) {
     MgLevel &v = mg->lv[l];
@@ -462,11 +440,12 @@ int mg_create(tfm_ctx *ctx, int P, int nx, int ny, int NR, const double *ke0, Mg
     if (l > 0 && ((rc = dalloc(ctx, &v.b, nv)) || (rc = dalloc(ctx, &v.x, nv)) ||
                   (rc = dalloc(ctx, &v.K, (size_t)P * v.ne * 64))))
       break;
-    if (l < mg->L && ((rc = dalloc(ctx, &v.dinv, (size_t)P * v.ndof)) || (rc = dalloc(ctx, &v.x2, nv)) ||
-                      (rc = dalloc(ctx, &v.r, nv))))
-      break;
+    // the Jacobi diagonal of level 0 also masks the batched CG's product
+    if ((l < mg->L || l == 0) && (rc = dalloc(ctx, &v.dinv, (size_t)P * v.ndof))) break;
+    if (l < mg->L && ((rc = dalloc(ctx, &v.x2, nv)) || (rc = dalloc(ctx, &v.r, nv)))) break;
   }
   if (!rc && !(rc = dalloc(ctx, &mg->d_ke0, 64)) && !(rc = dalloc(ctx, &mg->d_E, (size_t)P * mg->lv[0].ne)) &&
+      !(rc = dalloc(ctx, &mg->d_wd0, (size_t)P * mg->lv[0].ndof)) &&
       !(rc = dalloc(ctx, &mg->d_inv, (size_t)P * mg->nc * mg->nc)))
     if (hipMemcpy(mg->d_ke0, ke0, sizeof(double) * 64, hipMemcpyHostToDevice) != hipSuccess)
       rc = fail(ctx, TFM_E_HIP, "multigrid: upload failed");
@@ -494,63 +473,66 @@ int mg_update(MgData *mg, const double *d_rho, double penal, double e0, double e
       hipLaunchKernelGGL(km_galerkin, dim3(mg_grid((int64_t)P * v.ne * 64)), dim3(kMgB), 0, st, P, f, v.nx, v.ny,
                          v.cox, v.coy, v.K);
     }
-    if (l < mg->L)
+    if (l < mg->L || l == 0)
       hipLaunchKernelGGL(km_dinv, dim3(mg_grid((int64_t)P * v.nx * v.ny)), dim3(kMgB), 0, st, P, level_op(mg, l),
-                         v.dinv);
+                         v.dinv, l == 0 ? mg->d_wd0 : nullptr);
   }
   hipLaunchKernelGGL(km_coarse_inv, dim3((unsigned)P), dim3(256), 0, st, level_op(mg, mg->L), mg->nc, mg->d_inv);
   TFM_CHECK_HIP(ctx, hipGetLastError());
   return TFM_OK;
 }
 
-// z = M r, one V-cycle.  Per level, the 2 kMgNu sweeps alternate between two
-// buffers so the last lands in the level's output (z at level 0, x2 below).
+// z = M r, one V-cycle with one Jacobi sweep before and one after the coarse
+// correction.  On entry z holds w D_0^-1 r (the caller forms it with the
+// residual; see mg_sweep_weights).  Level l (b_0 = r, x1_l = w D_l^-1 b_l):
+// r_l = b_l - A_l x1_l, b_{l+1} = P^T r_l (with x1_{l+1}, in x); coarsest
+// x_L = A_L^-1 b_L; back up, y_l = x1_l + P x_{l+1} (in x2) and x_l = y_l +
+// w D_l^-1 (b_l - A_l y_l) (z at level 0, x below).
 template <int NR>
-static void mg_vcycle(const MgData *mg, const double *r0, double *z0, hipStream_t st) {
+static void mg_vcycle(const MgData *mg, const double *r0, double *z0, hipStream_t st, MgAct act) {
   const int P = mg->P, L = mg->L;
-  auto bufs = [&](int l, double *(&bf)[2]) {
-    const MgLevel &v = mg->lv[l];
-    bf[0] = l == 0 ? v.x2 : v.x;
-    bf[1] = l == 0 ? z0 : v.x2;
-  };
   auto rhs = [&](int l) -> const double * { return l == 0 ? r0 : mg->lv[l].b; };
+  auto sol = [&](int l) -> double * { return l == 0 ? z0 : mg->lv[l].x; };
   for (int l = 0; l < L; l++) {
     const MgLevel &v = mg->lv[l], &c = mg->lv[l + 1];
-    const MgOp o = level_op(mg, l);
-    double *bf[2];
-    bufs(l, bf);
     const unsigned g = mg_grid((int64_t)P * v.nx * v.ny * NR);
-    hipLaunchKernelGGL((km_smooth<NR>), dim3(g), dim3(kMgB), 0, st, P, o, (const double *)v.dinv, rhs(l),
-                       (const double *)nullptr, bf[0]);
-    for (int s = 1; s < kMgNu; s++)
-      hipLaunchKernelGGL((km_smooth<NR>), dim3(g), dim3(kMgB), 0, st, P, o, (const double *)v.dinv, rhs(l),
-                         (const double *)bf[(s - 1) & 1], bf[s & 1]);
-    hipLaunchKernelGGL((km_resid<NR>), dim3(g), dim3(kMgB), 0, st, P, o, rhs(l),
-                       (const double *)bf[(kMgNu - 1) & 1], v.r);
+    if (l == 0)
+      hipLaunchKernelGGL((km_resid0<NR, true>), dim3(g), dim3(kMgB), 0, st, P, level_op(mg, l),
+                         (const double *)v.dinv, rhs(l), (const double *)sol(l), v.r, act);
+    else
+      hipLaunchKernelGGL((km_resid0<NR, false>), dim3(g), dim3(kMgB), 0, st, P, level_op(mg, l),
+                         (const double *)v.dinv, rhs(l), (const double *)sol(l), v.r, act);
+    const bool smooths = l + 1 < L;
     hipLaunchKernelGGL((km_restrict<NR>), dim3(mg_grid((int64_t)P * c.nx * c.ny * NR)), dim3(kMgB), 0, st, P, v.nx,
-                       v.ny, v.ndof, c.nx, c.ny, c.ndof, c.cox, c.coy, (const double *)v.r, c.b);
+                       v.ny, (int)v.ndof, c.nx, c.ny, (int)c.ndof, c.cox, c.coy, (const double *)v.r, c.b,
+                       (const double *)(smooths ? c.dinv : nullptr), smooths ? c.x : nullptr, act);
   }
-  const MgLevel &cl = mg->lv[L];
   hipLaunchKernelGGL((km_coarse_solve<NR>), dim3(mg_grid((int64_t)P * mg->nc * NR)), dim3(kMgB), 0, st, P, mg->nc,
-                     (const double *)mg->d_inv, rhs(L), L == 0 ? z0 : cl.x);
+                     (const double *)mg->d_inv, rhs(L), sol(L), act);
   for (int l = L - 1; l >= 0; l--) {
     const MgLevel &v = mg->lv[l], &c = mg->lv[l + 1];
-    const MgOp o = level_op(mg, l);
-    double *bf[2];
-    bufs(l, bf);
-    const double *xc = l + 1 == L ? c.x : c.x2;
     const unsigned g = mg_grid((int64_t)P * v.nx * v.ny * NR);
-    hipLaunchKernelGGL((km_prolong<NR>), dim3(g), dim3(kMgB), 0, st, P, v.nx, v.ny, v.ndof, c.nx, c.ndof, c.cox,
-                       c.coy, o.fixed, xc, bf[(kMgNu - 1) & 1]);
-    for (int s = kMgNu; s < 2 * kMgNu; s++)
-      hipLaunchKernelGGL((km_smooth<NR>), dim3(g), dim3(kMgB), 0, st, P, o, (const double *)v.dinv, rhs(l),
-                         (const double *)bf[(s - 1) & 1], bf[s & 1]);
+    hipLaunchKernelGGL((km_prolong0<NR>), dim3(g), dim3(kMgB), 0, st, P, v.nx, v.ny, (int)v.ndof, c.nx, (int)c.ndof,
+                       c.cox, c.coy, (const double *)v.dinv, (const double *)sol(l), (const double *)c.x, v.x2, act);
+    if (l == 0)
+      hipLaunchKernelGGL((km_smooth<NR, true>), dim3(g), dim3(kMgB), 0, st, P, level_op(mg, l),
+                         (const double *)v.dinv, rhs(l), (const double *)v.x2, sol(l), act);
+    else
+      hipLaunchKernelGGL((km_smooth<NR, false>), dim3(g), dim3(kMgB), 0, st, P, level_op(mg, l),
+                         (const double *)v.dinv, rhs(l), (const double *)v.x2, sol(l), act);
   }
 }
 
-void mg_apply(const MgData *mg, const double *r, double *z, hipStream_t st) {
-  if (mg->NR == 1) mg_vcycle<1>(mg, r, z, st);
-  else mg_vcycle<8>(mg, r, z, st);
+void mg_apply(const MgData *mg, const double *r, double *z, hipStream_t st, const int *done, int stride) {
+  const MgAct act{done, stride};
+  if (mg->NR == 1) mg_vcycle<1>(mg, r, z, st, act);
+  else mg_vcycle<8>(mg, r, z, st, act);
 }
+
+const double *mg_sweep_weights(const MgData *mg) { return mg->L > 0 ? mg->d_wd0 : nullptr; }
+
+MgOp mg_fine_op(const MgData *mg) { return level_op(mg, 0); }
+
+const double *mg_fine_dinv(const MgData *mg) { return mg->lv[0].dinv; }
 
 }  // namespace tfm
