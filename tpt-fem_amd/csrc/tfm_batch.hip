@@ -442,6 +442,73 @@ __global__ __launch_bounds__(kBT) void kbx_resid64(BView V, const int *__restric
   put_part<NR>(V, Q_FU, d);
 }
 
+// kbx_resid64 on the node-blocked sliced copy (one lane per block row, NR
+// right-hand sides at once), the same outputs; empty (fixed) rows have zero
+// blocks and f = 0 there, so r = 0 as in the CSR form.
+template <int BS, int NR>
+__global__ __launch_bounds__(kBT) void kbx_resid_bs(BView V, const double *__restrict__ bv,
+                                                    const double *__restrict__ u, const double *__restrict__ f,
+                                                    double *__restrict__ r, float *__restrict__ r32,
+                                                    const int *__restrict__ keep, const int *__restrict__ sel) {
+  const int b = blockIdx.x, pr = V.blk_prob[b];
+  constexpr int BB = BS * BS;
+  const int lane = threadIdx.x & 63;
+  const int s = V.blk_s0[b] + (threadIdx.x >> 6);
+  double rr[NR], ff[NR], fu[NR];
+  bool kept[NR], w32[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    rr[k] = ff[k] = fu[k] = 0.0;
+    kept[k] = keep && keep[pr * NR + k];
+    w32[k] = r32 && (!sel || sel[pr * NR + k]);
+  }
+  if (s < V.blk_s1[b]) {
+    const int br = V.sb[s] + lane;
+    const int o = V.soff[s], w = (V.soff[s + 1] - o) >> 6;
+    const int *cp = V.bcol + o + lane;
+    const double *vp = bv + (int64_t)o * BB + lane;
+    double acc[BS][NR];
+#pragma unroll
+    for (int i = 0; i < BS; i++)
+#pragma unroll
+      for (int k = 0; k < NR; k++) acc[i][k] = 0.0;
+    for (int kk = 0; kk < w; kk++) {
+      const int64_t c = (int64_t)cp[kk * 64] * BS * NR;
+      double xv[BS][NR];
+#pragma unroll
+      for (int j = 0; j < BS; j++)
+#pragma unroll
+        for (int k = 0; k < NR; k++) xv[j][k] = u[c + j * NR + k];
+      const double *vk = vp + (int64_t)kk * (64 * BB);
+#pragma unroll
+      for (int i = 0; i < BS; i++)
+#pragma unroll
+        for (int j = 0; j < BS; j++) {
+          const double a = vk[(i * BS + j) * 64];
+#pragma unroll
+          for (int k = 0; k < NR; k++) acc[i][k] += a * xv[j][k];
+        }
+    }
+    if (br < V.sb[s + 1]) {
+#pragma unroll
+      for (int i = 0; i < BS; i++)
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+          const int64_t idx = ((int64_t)br * BS + i) * NR + k;
+          const double fi = f[idx], ri = fi - acc[i][k];
+          r[idx] = kept[k] ? 0.0 : ri;
+          if (w32[k]) r32[idx] = kept[k] ? 0.0f : (float)ri;
+          rr[k] += ri * ri;
+          ff[k] += fi * fi;
+          fu[k] += fi * u[idx];
+        }
+    }
+  }
+  put_part<NR>(V, Q_RES, rr);
+  put_part<NR>(V, Q_FF, ff);
+  put_part<NR>(V, Q_FU, fu);
+}
+
 // One block per problem: the refinement residual totals into its segments.
 template <int NR>
 __global__ __launch_bounds__(kBT) void kbx_resid_finish(BView V) {
@@ -712,13 +779,46 @@ static void bb_dir(const BBatch &B, const BView &V, hipStream_t st, const VT *bv
   }
 }
 
+// FP64 residual r = f - K u and its partials: on the sliced copy when it
+// holds FP64 values, else on the CSR.
+template <int NR>
+static void bb_resid(const tfm_system *s, const BBatch &B, const BView &V, hipStream_t st, const double *u,
+                     const double *f, double *r, float *r32, const int *keep, const int *sel) {
+  dim3 g(B.nblk), t(kBT);
+  if (!B.v64) {
+    hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, u, f, r, r32, keep, sel);
+    return;
+  }
+  switch (B.bs) {
+  case 3: hipLaunchKernelGGL((kbx_resid_bs<3, NR>), g, t, 0, st, V, (const double *)B.v64, u, f, r, r32, keep, sel); break;
+  case 2: hipLaunchKernelGGL((kbx_resid_bs<2, NR>), g, t, 0, st, V, (const double *)B.v64, u, f, r, r32, keep, sel); break;
+  default: hipLaunchKernelGGL((kbx_resid_bs<1, NR>), g, t, 0, st, V, (const double *)B.v64, u, f, r, r32, keep, sel); break;
+  }
+}
+
 // Batched PCG from r (x accumulates).  Segments flagged in `flags` are left
 // alone; the others stop at ||r|| <= rtol ||f|| (recurrence residual; the
 // segments' ||f||^2 must be in place) or max_iter.  Two kernels per
 // iteration, 16 iterations per graph replay.
+// A captured chunk of iterations, kept between runs of one solver: its
+// kernels' arguments change only with the stop rule.
+struct GraphCache {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int max_it = 0;
+  double rtol2 = 0.0;
+  void clear() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
 template <typename VT, typename XT, int NR>
 static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p0, VT *p1, VT *q, const VT *dinv,
-                XT *x, double rtol, int max_iter, const int *flags, std::vector<Seg> &hs, const MgData *mg = nullptr) {
+                XT *x, double rtol, int max_iter, const int *flags, std::vector<Seg> &hs, const MgData *mg = nullptr,
+                GraphCache *gc = nullptr) {
   hipStream_t st = ctx->stream;
   const BView V = B.view(max_iter, rtol * rtol);
   const int nseg = B.P * B.NR;
@@ -774,14 +874,18 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
     }
   };
   const bool use_graph = !getenv("TFM_NO_GRAPH");
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  if (use_graph) {
+  GraphCache local;
+  GraphCache *G = gc ? gc : &local;
+  if (use_graph && (!G->exec || G->max_it != V.max_it || G->rtol2 != V.rtol2)) {
+    G->clear();
     TFM_CHECK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     chunk();
-    TFM_CHECK_HIP(ctx, hipStreamEndCapture(st, &graph));
-    TFM_CHECK_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    TFM_CHECK_HIP(ctx, hipStreamEndCapture(st, &G->graph));
+    TFM_CHECK_HIP(ctx, hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+    G->max_it = V.max_it;
+    G->rtol2 = V.rtol2;
   }
+  hipGraphExec_t exec = G->exec;
   hs.resize(nseg);
   int rc = TFM_OK;
   for (int it = 0;; it += CH) {
@@ -797,8 +901,7 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
     for (int q2 = 0; q2 < nseg; q2++) all &= hs[q2].done != 0;
     if (all || it > max_iter + 2 * CH) break;  // the kernels stop every segment at max_iter
   }
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
+  local.clear();
   return rc;
 }
 
@@ -828,7 +931,7 @@ static int bmixed(tfm_system *s, const BBatch &B, const BView &V, const double *
   std::vector<double> rr_ref(nseg), best(nseg);
   std::vector<int> since(nseg, 0), hsel(nseg, 0), hrst(nseg, 0), nbrk(nseg, 0);
   auto residual = [&](const int *sl) -> int {
-    hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, x, f, r64, r32, odone, sl);
+    bb_resid<NR>(s, B, V, st, x, f, r64, r32, odone, sl);
     hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
     TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
     TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
@@ -961,6 +1064,7 @@ struct BatchSolver {
         *x32 = nullptr;
   int *bad = nullptr, *odone = nullptr;
   MgData *mg = nullptr;  // multigrid preconditioner (TFM_SOLVE_PCG64_MG)
+  GraphCache graph;      // the plain CG's captured chunk
 };
 
 void bsolver_destroy(BatchSolver *b) {
@@ -970,6 +1074,7 @@ void bsolver_destroy(BatchSolver *b) {
   dfree(b->q64); dfree(b->r32); dfree(b->z32); dfree(b->p32); dfree(b->p32b); dfree(b->q32); dfree(b->d32);
   dfree(b->x32); dfree(b->bad); dfree(b->odone);
   mg_destroy(b->mg);
+  b->graph.clear();
   delete b;
 }
 
@@ -1053,7 +1158,9 @@ int bsolver_refresh(BatchSolver *b) {
   tfm_ctx *ctx = s->ctx;
   hipStream_t st = ctx->stream;
   const int n = (int)s->ndof;
-  int rc = b->mixed ? bb_values<float>(s, b->B, s->d_v32, &b->B.v32) : bb_values<double>(s, b->B, s->d_v64, &b->B.v64);
+  // FP64 sliced values serve the plain CG and every mode's FP64 residual
+  int rc = bb_values<double>(s, b->B, s->d_v64, &b->B.v64);
+  if (!rc && b->mixed) rc = bb_values<float>(s, b->B, s->d_v32, &b->B.v32);
   if (rc) return rc;
   TFM_CHECK_HIP(ctx, hipMemsetAsync(b->bad, 0, sizeof(int), st));
   const unsigned g1 = (unsigned)((n + 255) / 256);
@@ -1112,11 +1219,10 @@ static int bsolver_run_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, tf
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev0, st));
   if (!b->mixed) {
     // r = f - K x (x = 0 or the warm start), and ||f||^2 as the stop reference
-    hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, b->x, b->f, b->r64,
-                       (float *)nullptr, (const int *)nullptr, (const int *)nullptr);
+    bb_resid<NR>(s, B, V, st, b->x, b->f, b->r64, nullptr, nullptr, nullptr);
     hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
     rc = bpcg<double, double, NR>(ctx, B, B.v64, b->r64, b->z64, b->p64, b->p64b, b->q64, b->d64, b->x, o.rtol,
-                                  o.max_iter, nullptr, hs, b->mg);
+                                  o.max_iter, nullptr, hs, b->mg, &b->graph);
     for (int q = 0; q < nseg; q++) {
       iters[q] = hs[q].iters;
       brk[q] = hs[q].breakdown;
@@ -1129,8 +1235,7 @@ static int bsolver_run_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, tf
   }
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, st));
   // final FP64 residual certificate of every segment
-  hipLaunchKernelGGL((kbx_resid64<NR>), g, t, 0, st, V, s->d_rp, s->d_ci, s->d_v64, b->x, b->f, b->r64,
-                     (float *)nullptr, (const int *)nullptr, (const int *)nullptr);
+  bb_resid<NR>(s, B, V, st, b->x, b->f, b->r64, nullptr, nullptr, nullptr);
   hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));

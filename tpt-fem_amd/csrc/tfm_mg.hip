@@ -273,6 +273,7 @@ __global__ void km_galerkin(int P, MgOp f, int nxc, int nyc, int cox, int coy, d
   const int x0 = coarse_pos(EI, f.nx - 1, cox), x1 = coarse_pos(EI + 1, f.nx - 1, cox);
   const int y0 = coarse_pos(EJ, f.ny - 1, coy), y1 = coarse_pos(EJ + 1, f.ny - 1, coy);
   const double sx = 1.0 / (double)(x1 - x0), sy = 1.0 / (double)(y1 - y0);
+  const bool fine = f.K == nullptr;
   double acc = 0.0;
   for (int ej = y0; ej < y1; ej++)
     for (int ei = x0; ei < x1; ei++) {
@@ -283,12 +284,27 @@ __global__ void km_galerkin(int P, MgOp f, int nxc, int nyc, int cox, int coy, d
         wr[l] = w[mr];
         wc[l] = w[mc];
       }
+      const int64_t e = (int64_t)p * f.ne + (int64_t)ej * (f.nx - 1) + ei;
+      // a level-0 child is E_e Ke_unit with its fixed rows and columns zero:
+      // its modulus and the fixed flags of its row (a) and column (b) DOFs
+      double sc = 1.0;
+      bool fr[4] = {false, false, false, false}, fc[4] = {false, false, false, false};
+      if (fine) {
+        sc = f.E[e];
+        const int64_t base = (int64_t)p * f.ndof;
+        for (int l = 0; l < 4; l++) {
+          const int64_t g = base + 2 * ((int64_t)(ej + local_dj(l)) * f.nx + ei + local_di(l));
+          fr[l] = is_fixed(f, g + a);
+          fc[l] = is_fixed(f, g + b);
+        }
+      }
+      const double *Kk = fine ? f.ke0 : f.K + e * 64;
       for (int l = 0; l < 4; l++) {
-        if (wr[l] == 0.0) continue;
-        double s = 0.0;
+        if (wr[l] == 0.0 || fr[l]) continue;
+        double row = 0.0;
         for (int m = 0; m < 4; m++)
-          if (wc[m] != 0.0) s += ke_entry(f, p, ei, ej, 2 * l + a, 2 * m + b) * wc[m];
-        acc += wr[l] * s;
+          if (wc[m] != 0.0 && !fc[m]) row += Kk[(2 * l + a) * 8 + 2 * m + b] * wc[m];
+        acc += wr[l] * sc * row;
       }
     }
   Kc[pe * 64 + rc] = acc;
