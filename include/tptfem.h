@@ -225,7 +225,8 @@ typedef struct {
  * {densities, compliance, iterations} (:140-148).  f: host, 2*nx*ny; bcs as
  * (dof, value) pairs with value 0 (other values: TFM_E_UNSUPPORTED).  Each
  * state equation K(rho) u = f is solved on the GPU by the PCG of opts (NULL:
- * FP64 PCG to rtol 1e-12) instead of the dense LU, warm-started from the
+ * FP64 CG with the multigrid V-cycle, TFM_SOLVE_PCG64_MG, to rtol 1e-12)
+ * instead of the dense LU, warm-started from the
  * previous iteration.  densities: (nx-1)(ny-1); compliance: max_iter + 1
  * (index 0 the uniform start); iterations: max_iter; stats (may be NULL):
  * PCG iterations and device seconds summed over the solves, worst residual. */

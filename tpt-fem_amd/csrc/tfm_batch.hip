@@ -289,7 +289,8 @@ __global__ __launch_bounds__(kBT) void kbx_dir_p(BView V, const double *__restri
 // kbx_q_mg: q = K p matrix-free on SIMP grids (element moduli times the unit
 // stiffness) and the partial p.q; rows where `mask` (the finest Jacobi
 // diagonal inverse) is 0 -- the fixed DOFs -- get q = 0.  A block's rows are
-// whole nodes of one design.
+// whole nodes of one design.  (Forming p here from z and pold at every
+// neighbour instead of in kbx_dir_p doubles the loads and measured slower.)
 template <int NR>
 __global__ __launch_bounds__(kBT) void kbx_q_mg(BView V, MgOp o, const double *__restrict__ mask,
                                                 const double *__restrict__ pv, double *__restrict__ q) {
@@ -324,6 +325,44 @@ __global__ __launch_bounds__(kBT) void kbx_q_mg(BView V, MgOp o, const double *_
   double c[NR];
   rhs_sum<NR>(dot, c);
   put_part<NR>(V, Q_PQ, c);
+}
+
+// kbx_smooth_rz: the V-cycle's last step on the finest level, z = y + w Dinv
+// (r - K y) (y the coarse-corrected start), fused with the partial r.z of
+// the CG; the segments already done are skipped.
+template <int NR>
+__global__ __launch_bounds__(kBT) void kbx_smooth_rz(BView V, MgOp o, const double *__restrict__ dinv,
+                                                     const double *__restrict__ r, const double *__restrict__ yv,
+                                                     double *__restrict__ z, int slot) {
+  const int b = blockIdx.x, pr = V.blk_prob[b];
+  const Seg *S = V.seg + (size_t)pr * NR;
+  __shared__ int s_done[NR];
+  if (threadIdx.x < NR) s_done[threadIdx.x] = S[threadIdx.x].done;
+  __syncthreads();
+  const int k = threadIdx.x % NR;
+  const bool act = !s_done[k];
+  int r0, r1;
+  blk_rows(V, b, r0, r1);
+  const int nn = o.nx * o.ny;
+  double dot = 0.0;
+  if (act)
+    for (int idx = (r0 >> 1) * NR + threadIdx.x; idx < (r1 >> 1) * NR; idx += kBT) {
+      const int node = idx / NR;
+      const int p = node / nn, n = node - p * nn;
+      double y[2];
+      apply_node<true>(o, p, n % o.nx, n / o.nx, [&](int d) { return yv[d * NR + k]; }, y);
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int g = 2 * node + a;
+        const double ra = r[g * NR + k];
+        const double za = yv[g * NR + k] + kMgOmega * dinv[g] * (ra - y[a]);
+        z[g * NR + k] = za;
+        dot += ra * za;
+      }
+    }
+  double c[NR];
+  rhs_sum<NR>(dot, c);
+  put_part<NR>(V, slot, c);
 }
 
 // alpha = r.z / p.q per segment; x += alpha p, r -= alpha q, z = Dinv r on
@@ -826,8 +865,8 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
   hipLaunchKernelGGL(kb_reset, dim3((nseg + 255) / 256), dim3(256), 0, st, B.seg, nseg, flags, B.it, 1);
   // multigrid (FP64 only): z = M r by a V-cycle instead of Dinv r, skipping
   // the segments already done; the update kernels write z = w Dinv r (the
-  // V-cycle's first sweep) and kbx_rz replaces their r.z partials; K p is
-  // taken matrix-free
+  // V-cycle's first sweep) and the closing fine sweep replaces their r.z
+  // partials; beta has its own kernel and K p is taken matrix-free
   static_assert(sizeof(Seg) % sizeof(int) == 0, "Seg stride in ints");
   const int *sdone = reinterpret_cast<const int *>(reinterpret_cast<const char *>(B.seg) + offsetof(Seg, done));
   const int sstride = (int)(sizeof(Seg) / sizeof(int));
@@ -841,15 +880,24 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
       mmask = mg_fine_dinv(mg);
     }
   }
-  hipLaunchKernelGGL((kbx_init<VT, NR>), g, t, 0, st, V, r, z, p0, p1, jac);
-  if constexpr (std::is_same<VT, double>::value) {
-    if (mg) {
-      mg_apply(mg, r, z, st, sdone, sstride);
-      hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, Q_RZ1);
+  // z = M r and the partial r.z into `slot`: the V-cycle up to the finest
+  // level's last sweep, which runs fused with the dot product
+  auto precond = [&](int slot) {
+    if constexpr (std::is_same<VT, double>::value) {
+      const double *yv = mg_apply(mg, r, z, st, sdone, sstride, false);
+      if (yv)
+        hipLaunchKernelGGL((kbx_smooth_rz<NR>), g, t, 0, st, V, mop, mmask, (const double *)r, yv,
+                           (double *)z, slot);
+      else
+        hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, slot);
     }
-  }
+  };
+  hipLaunchKernelGGL((kbx_init<VT, NR>), g, t, 0, st, V, r, z, p0, p1, jac);
+  if (mg) precond(Q_RZ1);
   TFM_CHECK_HIP(ctx, hipGetLastError());
-  const int CH = 16;  // iterations per replay (even: buffer parity is static)
+  // iterations per replay (even: buffer parity is static); multigrid runs
+  // need few iterations, so shorter replays waste fewer past convergence
+  const int CH = mg ? 8 : 16;
   VT *pb[2] = {p0, p1};
   auto chunk = [&]() {
     for (int k = 0; k < CH; k++) {
@@ -866,10 +914,7 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
       }
       hipLaunchKernelGGL((kbx_update<VT, XT, NR>), g, t, 0, st, V, x, r, z, pb[e], q, jac, e);
       if constexpr (std::is_same<VT, double>::value) {
-        if (mg) {
-          mg_apply(mg, r, z, st, sdone, sstride);
-          hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, e ? Q_RZ1 : Q_RZ0);
-        }
+        if (mg) precond(e ? Q_RZ1 : Q_RZ0);
       }
     }
   };

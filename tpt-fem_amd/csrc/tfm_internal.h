@@ -208,8 +208,11 @@ int mg_levels(int nx, int ny, std::vector<int> *dims);
 int mg_create(tfm_ctx *ctx, int P, int nx, int ny, int NR, const double *ke0, MgData **out);
 int mg_update(MgData *mg, const double *d_rho, double penal, double e0, double emin, const unsigned char *d_fixed);
 // z = M r ([n_dof][NR]); segment (p, k) is skipped when done[(p NR + k) stride] != 0
-// on entry z holds mg_sweep_weights .* r (when those exist)
-void mg_apply(const MgData *mg, const double *r, double *z, hipStream_t st, const int *done, int stride);
+// on entry z holds mg_sweep_weights .* r (when those exist).  last false:
+// the finest level's closing sweep z = y + w Dinv (r - K y) is left to the
+// caller, who gets y back (null when the grid has one level: z is complete).
+const double *mg_apply(const MgData *mg, const double *r, double *z, hipStream_t st, const int *done, int stride,
+                       bool last = true);
 const double *mg_sweep_weights(const MgData *mg);  // [n_dof] w Dinv of the finest level, or null
 const double *mg_fine_dinv(const MgData *mg);      // [n_dof] finest Jacobi diagonal inverse (0: fixed)
 struct MgOp;

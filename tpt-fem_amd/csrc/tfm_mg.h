@@ -14,6 +14,8 @@
 
 namespace tfm {
 
+constexpr double kMgOmega = 0.6;  // Jacobi weight; lambda_max(D^-1 K) ~ 2.8 for Q4 plane stress
+
 // Operator of one level.
 struct MgOp {
   int nx, ny;

@@ -32,7 +32,6 @@
 
 namespace tfm {
 
-constexpr double kMgOmega = 0.6;  // Jacobi weight; lambda_max(D^-1 K) ~ 2.8 for Q4 plane stress
 constexpr int kMgB = 256;
 constexpr int kMgMaxC = 90;       // coarsest DOFs per design: the dense matrix fills 64 KB of LDS
 
@@ -505,7 +504,7 @@ int mg_update(MgData *mg, const double *d_rho, double penal, double e0, double e
 // x_L = A_L^-1 b_L; back up, y_l = x1_l + P x_{l+1} (in x2) and x_l = y_l +
 // w D_l^-1 (b_l - A_l y_l) (z at level 0, x below).
 template <int NR>
-static void mg_vcycle(const MgData *mg, const double *r0, double *z0, hipStream_t st, MgAct act) {
+static void mg_vcycle(const MgData *mg, const double *r0, double *z0, hipStream_t st, MgAct act, bool last) {
   const int P = mg->P, L = mg->L;
   auto rhs = [&](int l) -> const double * { return l == 0 ? r0 : mg->lv[l].b; };
   auto sol = [&](int l) -> double * { return l == 0 ? z0 : mg->lv[l].x; };
@@ -530,19 +529,22 @@ static void mg_vcycle(const MgData *mg, const double *r0, double *z0, hipStream_
     const unsigned g = mg_grid((int64_t)P * v.nx * v.ny * NR);
     hipLaunchKernelGGL((km_prolong0<NR>), dim3(g), dim3(kMgB), 0, st, P, v.nx, v.ny, (int)v.ndof, c.nx, (int)c.ndof,
                        c.cox, c.coy, (const double *)v.dinv, (const double *)sol(l), (const double *)c.x, v.x2, act);
-    if (l == 0)
-      hipLaunchKernelGGL((km_smooth<NR, true>), dim3(g), dim3(kMgB), 0, st, P, level_op(mg, l),
-                         (const double *)v.dinv, rhs(l), (const double *)v.x2, sol(l), act);
-    else
+    if (l == 0) {
+      if (last)
+        hipLaunchKernelGGL((km_smooth<NR, true>), dim3(g), dim3(kMgB), 0, st, P, level_op(mg, l),
+                           (const double *)v.dinv, rhs(l), (const double *)v.x2, sol(l), act);
+    } else
       hipLaunchKernelGGL((km_smooth<NR, false>), dim3(g), dim3(kMgB), 0, st, P, level_op(mg, l),
                          (const double *)v.dinv, rhs(l), (const double *)v.x2, sol(l), act);
   }
 }
 
-void mg_apply(const MgData *mg, const double *r, double *z, hipStream_t st, const int *done, int stride) {
+const double *mg_apply(const MgData *mg, const double *r, double *z, hipStream_t st, const int *done, int stride,
+                       bool last) {
   const MgAct act{done, stride};
-  if (mg->NR == 1) mg_vcycle<1>(mg, r, z, st, act);
-  else mg_vcycle<8>(mg, r, z, st, act);
+  if (mg->NR == 1) mg_vcycle<1>(mg, r, z, st, act, last);
+  else mg_vcycle<8>(mg, r, z, st, act, last);
+  return (!last && mg->L > 0) ? mg->lv[0].x2 : nullptr;
 }
 
 const double *mg_sweep_weights(const MgData *mg) { return mg->L > 0 ? mg->d_wd0 : nullptr; }

@@ -538,8 +538,12 @@ class TopOptResult:
 
 
 def topopt_simp(ctx, nx, ny, h, e0, nu, vol_frac, penal, filter_radius, max_iter, move_limit, f, bcs,
-                mode=SOLVE_PCG64, rtol=1e-12, pcg_max_iter=100000, inner_rtol=0.3):
+                mode=SOLVE_PCG64_MG, rtol=1e-12, pcg_max_iter=100000, inner_rtol=0.3):
     """topopt_simp(params, f, bcs) (crates/tpt-fem-topopt/src/lib.rs:342-385) on the GPU.
+
+    The state equations are solved by FP64 CG with the multigrid V-cycle by
+    default (mode SOLVE_PCG64 / SOLVE_MIXED select Jacobi PCG / the precision
+    form).
 
     With f of shape (P, 2*nx*ny) (and vol_frac a scalar or P values) the P
     designs run as one batch; the result's arrays then gain a leading P axis."""
