@@ -151,6 +151,7 @@ def bench_batch(args, T, log):
     modes = [(T.SOLVE_PCG64, "plain_fp64", 1e-12), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", 1e-10)]
     if args.workload == "c5":
         modes.append((T.SOLVE_PCG64_MG, "multigrid_fp64", 1e-12))
+    modes = [m for m in modes if m[1].split("_")[0] in args.solvers]
     for mode, name, rtol in modes:
         u = s.solve_batch(off, rhs=f, nrhs=nrhs, mode=mode, rtol=rtol, max_iter=200000, max_outer=400)
         st = s.batch_stats
@@ -189,8 +190,9 @@ def bench_simp(args, T, log):
     desc = (f"{P} SIMP cantilever designs, Grid {nx}x{ny} nodes ({(nx - 1) * (ny - 1)} quads), point load at a "
             f"seeded right-edge node, vol 0.5, p 3, R 2, move 0.2, {iters} OC iterations")
     out = {}
-    for mode, name, rtol in ((T.SOLVE_PCG64, "plain_fp64", 1e-10), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", 1e-10),
-                             (T.SOLVE_PCG64_MG, "multigrid_fp64", 1e-10)):
+    modes = ((T.SOLVE_PCG64, "plain_fp64", 1e-10), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", 1e-10),
+             (T.SOLVE_PCG64_MG, "multigrid_fp64", 1e-10))
+    for mode, name, rtol in [m for m in modes if m[1].split("_")[0] in args.solvers]:
         T.topopt_simp(ctx, nx, ny, 1.0, 1.0, 0.3, 0.5, 3.0, 2.0, 1, 0.2, F[:4], bcs, mode=mode, rtol=rtol)  # warm-up
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -206,9 +208,11 @@ def bench_simp(args, T, log):
                "volume_error_max": float(np.abs(r.densities.sum(axis=1) - 0.5 * r.densities.shape[1]).max())}
         out[name] = rec
         log(f"simp {name}: {rec}")
+    vname = "multigrid_fp64" if "multigrid_fp64" in out else next(iter(out))
     line = {"metric": "SIMP OC iterations (all designs) per second", "workload": "simp", "config": {"workload": desc},
-            "value": iters / out["multigrid_fp64"]["wall_s"], "unit": "OC iterations/s",
-            "value_solver": "multigrid_fp64 (the default state solver)", "loops": out}
+            "value": iters / out[vname]["wall_s"], "unit": "OC iterations/s",
+            "value_solver": vname + (" (the default state solver)" if vname == "multigrid_fp64" else ""),
+            "loops": out}
     if not args.skip_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -245,6 +249,8 @@ def main():
                     help="c3: the headline single problem; c4 / c5: the batching forms; simp: the batched SIMP "
                          "loop (own JSON lines)")
     ap.add_argument("--simp-iters", type=int, default=10, help="OC iterations of the simp workload")
+    ap.add_argument("--solvers", nargs="+", choices=["plain", "mixed", "multigrid"],
+                    default=["plain", "mixed", "multigrid"], help="state solvers of the c4 / c5 / simp workloads")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run five 1 GiB device-to-device copies (scale for HBM byte counters)")
     args = ap.parse_args()
