@@ -698,6 +698,8 @@ struct BBatch {
   double *part = nullptr;
   Seg *seg = nullptr;
   int *it = nullptr;
+  Seg *hseg = nullptr;         // pinned host copy of seg, read while the next chunk runs
+  hipEvent_t ev_seg = nullptr;  // recorded after each copy into hseg
   BView view(int max_it, double rtol2) const {
     return BView{nblk, bs, blk_prob, blk_s0, blk_s1, prob_blk0, prob_nblk, sb, soff, bcol, part, seg, it,
                  max_it, rtol2};
@@ -708,6 +710,10 @@ static void bb_free(BBatch &B) {
   dfree(B.blk_prob); dfree(B.blk_s0); dfree(B.blk_s1); dfree(B.prob_blk0); dfree(B.prob_nblk);
   dfree(B.sb); dfree(B.soff); dfree(B.bcol); dfree(B.map); dfree(B.v64); dfree(B.v32);
   dfree(B.part); dfree(B.seg); dfree(B.it);
+  if (B.hseg) (void)hipHostFree(B.hseg);
+  if (B.ev_seg) (void)hipEventDestroy(B.ev_seg);
+  B.hseg = nullptr;
+  B.ev_seg = nullptr;
 }
 
 template <int BS>
@@ -763,6 +769,12 @@ static int bb_setup(tfm_system *s, int P, const int64_t *dof_off, int NR, BBatch
     dfree(cnt);
     dfree(w);
     return rc;
+  }
+  if (hipHostMalloc((void **)&B.hseg, sizeof(Seg) * P * NR, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&B.ev_seg, hipEventDisableTiming) != hipSuccess) {
+    dfree(cnt);
+    dfree(w);
+    return fail(ctx, TFM_E_HIP, "pinned segment buffer allocation failed");
   }
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(B.blk_prob, bp.data(), sizeof(int) * B.nblk, hipMemcpyHostToDevice, st));
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(B.blk_s0, b0.data(), sizeof(int) * B.nblk, hipMemcpyHostToDevice, st));
@@ -933,19 +945,35 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
   hipGraphExec_t exec = G->exec;
   hs.resize(nseg);
   int rc = TFM_OK;
-  for (int it = 0;; it += CH) {
+  // Chunk k + 1 is queued before the host reads the segments' state after
+  // chunk k (pinned copy + event), so the host's check and the next graph
+  // launch overlap the GPU's work; a chunk past convergence only runs
+  // kernels that exit at once.
+  auto launch = [&]() -> bool {
     if (!use_graph) {
       chunk();
-    } else if (hipGraphLaunch(exec, st) != hipSuccess) {
+      return true;
+    }
+    return hipGraphLaunch(exec, st) == hipSuccess;
+  };
+  auto snapshot = [&]() -> bool {
+    return hipMemcpyAsync(B.hseg, B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipEventRecord(B.ev_seg, st) == hipSuccess;
+  };
+  if (!launch() || !snapshot()) rc = fail(ctx, TFM_E_HIP, "batched PCG launch failed");
+  for (int it = CH; !rc; it += CH) {
+    if (!launch()) {
       rc = fail(ctx, TFM_E_HIP, "hipGraphLaunch failed");
       break;
     }
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
-    TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+    TFM_CHECK_HIP(ctx, hipEventSynchronize(B.ev_seg));
     bool all = true;
-    for (int q2 = 0; q2 < nseg; q2++) all &= hs[q2].done != 0;
+    for (int q2 = 0; q2 < nseg; q2++) all &= B.hseg[q2].done != 0;
     if (all || it > max_iter + 2 * CH) break;  // the kernels stop every segment at max_iter
+    if (!snapshot()) rc = fail(ctx, TFM_E_HIP, "hipMemcpyAsync failed");
   }
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
   local.clear();
   return rc;
 }
