@@ -138,3 +138,86 @@ def test_single_system_solve_with_mg(ctx):
     assert np.linalg.norm(u - ur) <= 1e-9 * np.linalg.norm(ur) and s.stats.iterations < 200
     s.close()
     m.close()
+
+
+# ---- box meshes (3-D multigrid) ---------------------------------------------
+
+def box_system(ctx, n, dpn, seed, jitter=0.2, faces=((0, 0.0),)):
+    """A jittered box_mesh system: Poisson (dpn 1, kappa_e = exp(N(0,1)),
+    f = 1) or elasticity (dpn 3, E_e log-uniform [0.1, 10], nu 0.3, traction
+    (0, 0, -1e-3) on x = 1), DOFs on the listed (axis, coord) faces fixed;
+    also returns the oracle's system for the direct solve."""
+    import tptfem as T
+
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], n)
+    if jitter:
+        m.jitter(n, jitter, seed)
+    coords, conn = m.download()
+    rng = np.random.default_rng(seed)
+    fixed = sorted({int(v) for axis, val in faces for v in O.nodes_on_plane(coords, axis, val, 1e-9)})
+    bcs = [(dpn * v + c, 0.0) for v in fixed for c in range(dpn)]
+    if dpn == 1:
+        coef = np.exp(rng.standard_normal(len(conn)))
+        op = T.Operator.poisson(1.0, 2, kappa_e=coef)
+        rhs = O.poisson_rhs(coords, O.TET, conn, 1.0)
+        oop = O.make_op(O.TET, 3, 0, coef_e=coef)
+    else:
+        coef = 10.0 ** rng.uniform(-1, 1, len(conn))
+        op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=coef)
+        rhs = O.plane_traction(coords, O.TET, conn, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
+        oop = O.make_op(O.TET, 3, 1, coef_e=coef, nu=0.3)
+    s = T.System.create(m, op, dpn, bcs, rhs, flags=T.SYS_FP64)
+    return m, s, (coords, conn, oop, rhs, bcs)
+
+
+@pytest.mark.parametrize("case", [((2, 2, 2), 3), ((6, 5, 4), 1), ((9, 7, 4), 1), ((6, 6, 6), 3), ((7, 5, 9), 3)])
+def test_mg_box_matches_direct(ctx, case):
+    """One level (the coarse inverse solves the system), even and odd
+    coarsening on all three axes, Poisson and elasticity: the multigrid CG
+    solution equals the oracle's dense-LU solve to 1e-9."""
+    import tptfem as T
+
+    n, dpn = case
+    m, s, (coords, conn, oop, rhs, bcs) = box_system(ctx, n, dpn, seed=sum(n) * dpn)
+    u = s.solve(T.SOLVE_PCG64_MG, rtol=1e-12, max_iter=2000)
+    ur = O.solve_with_dirichlet(O.assemble(coords, conn, oop, dpn), rhs, bcs)
+    assert np.linalg.norm(u - ur) <= 1e-9 * np.linalg.norm(ur), n
+    assert s.stats.rel_residual <= 1e-11
+    s.close()
+    m.close()
+
+
+def test_mg_box_iterations_do_not_grow(ctx):
+    """Jacobi PCG needs ~2x the iterations per refinement; the V-cycle keeps
+    the count flat (elasticity, 8^3 and 16^3 jittered boxes)."""
+    import tptfem as T
+
+    its = []
+    for n in (8, 16):
+        m, s, _ = box_system(ctx, (n, n, n), 3, seed=n)
+        uj = s.solve(T.SOLVE_PCG64, rtol=1e-10, max_iter=20000)
+        itj = s.stats.iterations
+        um = s.solve(T.SOLVE_PCG64_MG, rtol=1e-10, max_iter=2000)
+        itm = s.stats.iterations
+        assert np.linalg.norm(um - uj) <= 1e-8 * np.linalg.norm(uj)
+        its.append((itm, itj))
+        s.close()
+        m.close()
+    (m8, j8), (m16, j16) = its
+    assert m16 <= 60 and m16 <= m8 + 16 and 4 * m16 < j16, its
+
+
+def test_mg_box_needs_box_mesh(ctx):
+    """A box system built from host arrays carries no node grid: refused."""
+    import tptfem as T
+
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], (3, 3, 3))
+    m = T.Mesh.from_arrays(ctx, T.TET, coords, conn)
+    bcs = [(int(v), 0.0) for v in O.nodes_on_plane(coords, 0, 0.0, 1e-9)]
+    s = T.System.create(m, T.Operator.poisson(1.0, 2), 1, bcs, O.poisson_rhs(coords, O.TET, conn, 1.0),
+                        flags=T.SYS_FP64)
+    with pytest.raises(T.TfmError) as ei:
+        s.solve(T.SOLVE_PCG64_MG)
+    assert ei.value.code == T.E_UNSUPPORTED
+    s.close()
+    m.close()

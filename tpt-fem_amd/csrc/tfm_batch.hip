@@ -548,6 +548,88 @@ __global__ __launch_bounds__(kBT) void kbx_resid_bs(BView V, const double *__res
   put_part<NR>(V, Q_FU, fu);
 }
 
+// Multigrid on a box system (one problem, one rhs), level 0 on the sliced
+// copy.  kbx_mg3_resid: r0 = r - K z (z the V-cycle's first sweep).
+template <int BS>
+__global__ __launch_bounds__(kBT) void kbx_mg3_resid(BView V, const double *__restrict__ bv,
+                                                     const double *__restrict__ r, const double *__restrict__ z,
+                                                     double *__restrict__ r0) {
+  const int b = blockIdx.x;
+  if (V.seg[V.blk_prob[b]].done) return;
+  constexpr int BB = BS * BS;
+  const int lane = threadIdx.x & 63;
+  const int s = V.blk_s0[b] + (threadIdx.x >> 6);
+  if (s >= V.blk_s1[b]) return;
+  const int br = V.sb[s] + lane;
+  const int o = V.soff[s], w = (V.soff[s + 1] - o) >> 6;
+  const int *cp = V.bcol + o + lane;
+  const double *vp = bv + (int64_t)o * BB + lane;
+  double acc[BS];
+#pragma unroll
+  for (int i = 0; i < BS; i++) acc[i] = 0.0;
+  for (int kk = 0; kk < w; kk++) {
+    const int64_t c = (int64_t)cp[kk * 64] * BS;
+    double xv[BS];
+#pragma unroll
+    for (int j = 0; j < BS; j++) xv[j] = z[c + j];
+    const double *vk = vp + (int64_t)kk * (64 * BB);
+#pragma unroll
+    for (int i = 0; i < BS; i++)
+#pragma unroll
+      for (int j = 0; j < BS; j++) acc[i] += vk[(i * BS + j) * 64] * xv[j];
+  }
+  if (br < V.sb[s + 1])
+#pragma unroll
+    for (int i = 0; i < BS; i++) {
+      const int64_t idx = (int64_t)br * BS + i;
+      r0[idx] = r[idx] - acc[i];
+    }
+}
+
+// kbx_mg3_smooth: the V-cycle's closing sweep on level 0, z = y + w Dinv
+// (r - K y), fused with the CG's r.z partial sums into `slot`.
+template <int BS>
+__global__ __launch_bounds__(kBT) void kbx_mg3_smooth(BView V, const double *__restrict__ bv,
+                                                      const double *__restrict__ r, const double *__restrict__ y,
+                                                      const double *__restrict__ dinv, double *__restrict__ z,
+                                                      int slot) {
+  const int b = blockIdx.x;
+  if (V.seg[V.blk_prob[b]].done) return;  // uniform per block
+  constexpr int BB = BS * BS;
+  const int lane = threadIdx.x & 63;
+  const int s = V.blk_s0[b] + (threadIdx.x >> 6);
+  double dot[1] = {0.0};
+  if (s < V.blk_s1[b]) {
+    const int br = V.sb[s] + lane;
+    const int o = V.soff[s], w = (V.soff[s + 1] - o) >> 6;
+    const int *cp = V.bcol + o + lane;
+    const double *vp = bv + (int64_t)o * BB + lane;
+    double acc[BS];
+#pragma unroll
+    for (int i = 0; i < BS; i++) acc[i] = 0.0;
+    for (int kk = 0; kk < w; kk++) {
+      const int64_t c = (int64_t)cp[kk * 64] * BS;
+      double xv[BS];
+#pragma unroll
+      for (int j = 0; j < BS; j++) xv[j] = y[c + j];
+      const double *vk = vp + (int64_t)kk * (64 * BB);
+#pragma unroll
+      for (int i = 0; i < BS; i++)
+#pragma unroll
+        for (int j = 0; j < BS; j++) acc[i] += vk[(i * BS + j) * 64] * xv[j];
+    }
+    if (br < V.sb[s + 1])
+#pragma unroll
+      for (int i = 0; i < BS; i++) {
+        const int64_t idx = (int64_t)br * BS + i;
+        const double zi = y[idx] + kMg3Omega * dinv[idx] * (r[idx] - acc[i]);
+        z[idx] = zi;
+        dot[0] += r[idx] * zi;
+      }
+  }
+  put_part<1>(V, slot, dot);
+}
+
 // One block per problem: the refinement residual totals into its segments.
 template <int NR>
 __global__ __launch_bounds__(kBT) void kbx_resid_finish(BView V) {
@@ -869,7 +951,7 @@ struct GraphCache {
 template <typename VT, typename XT, int NR>
 static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p0, VT *p1, VT *q, const VT *dinv,
                 XT *x, double rtol, int max_iter, const int *flags, std::vector<Seg> &hs, const MgData *mg = nullptr,
-                GraphCache *gc = nullptr) {
+                GraphCache *gc = nullptr, const Mg3Data *mg3 = nullptr) {
   hipStream_t st = ctx->stream;
   const BView V = B.view(max_iter, rtol * rtol);
   const int nseg = B.P * B.NR;
@@ -891,6 +973,7 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
       mop = mg_fine_op(mg);
       mmask = mg_fine_dinv(mg);
     }
+    if (mg3) jac = mg3_sweep_weights(mg3);
   }
   // z = M r and the partial r.z into `slot`: the V-cycle up to the finest
   // level's last sweep, which runs fused with the dot product
@@ -904,12 +987,34 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
         hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, slot);
     }
   };
+  // the same on a box system: level 0 on the sliced copy around mg3_cycle
+  auto precond3 = [&](int slot) {
+    if constexpr (std::is_same<VT, double>::value && NR == 1) {
+      if (mg3_depth(mg3) == 0) {
+        mg3_direct(mg3, r, z, st, sdone);
+        hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, slot);
+        return;
+      }
+      double *r0 = mg3_fine_r(mg3);
+      const double *d0 = mg3_fine_dinv(mg3);
+      switch (B.bs) {
+      case 3: hipLaunchKernelGGL((kbx_mg3_resid<3>), g, t, 0, st, V, bv, r, z, r0); break;
+      default: hipLaunchKernelGGL((kbx_mg3_resid<1>), g, t, 0, st, V, bv, r, z, r0); break;
+      }
+      const double *yv = mg3_cycle(mg3, z, st, sdone);
+      switch (B.bs) {
+      case 3: hipLaunchKernelGGL((kbx_mg3_smooth<3>), g, t, 0, st, V, bv, r, yv, d0, z, slot); break;
+      default: hipLaunchKernelGGL((kbx_mg3_smooth<1>), g, t, 0, st, V, bv, r, yv, d0, z, slot); break;
+      }
+    }
+  };
   hipLaunchKernelGGL((kbx_init<VT, NR>), g, t, 0, st, V, r, z, p0, p1, jac);
   if (mg) precond(Q_RZ1);
+  if (mg3) precond3(Q_RZ1);
   TFM_CHECK_HIP(ctx, hipGetLastError());
   // iterations per replay (even: buffer parity is static); multigrid runs
   // need few iterations, so shorter replays waste fewer past convergence
-  const int CH = mg ? 8 : 16;
+  const int CH = (mg || mg3) ? 8 : 16;
   VT *pb[2] = {p0, p1};
   auto chunk = [&]() {
     for (int k = 0; k < CH; k++) {
@@ -927,6 +1032,7 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
       hipLaunchKernelGGL((kbx_update<VT, XT, NR>), g, t, 0, st, V, x, r, z, pb[e], q, jac, e);
       if constexpr (std::is_same<VT, double>::value) {
         if (mg) precond(e ? Q_RZ1 : Q_RZ0);
+        if (mg3) precond3(e ? Q_RZ1 : Q_RZ0);
       }
     }
   };
@@ -1136,8 +1242,9 @@ struct BatchSolver {
   float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *d32 = nullptr,
         *x32 = nullptr;
   int *bad = nullptr, *odone = nullptr;
-  MgData *mg = nullptr;  // multigrid preconditioner (TFM_SOLVE_PCG64_MG)
-  GraphCache graph;      // the plain CG's captured chunk
+  MgData *mg = nullptr;    // multigrid preconditioner (TFM_SOLVE_PCG64_MG), SIMP grids
+  Mg3Data *mg3 = nullptr;  // the same for one system on a box mesh
+  GraphCache graph;        // the plain CG's captured chunk
 };
 
 void bsolver_destroy(BatchSolver *b) {
@@ -1147,6 +1254,7 @@ void bsolver_destroy(BatchSolver *b) {
   dfree(b->q64); dfree(b->r32); dfree(b->z32); dfree(b->p32); dfree(b->p32b); dfree(b->q32); dfree(b->d32);
   dfree(b->x32); dfree(b->bad); dfree(b->odone);
   mg_destroy(b->mg);
+  mg3_destroy(b->mg3);
   b->graph.clear();
   delete b;
 }
@@ -1158,15 +1266,21 @@ int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, int mod
   if (mode != TFM_SOLVE_PCG64 && mode != TFM_SOLVE_MIXED && mode != TFM_SOLVE_PCG64_MG)
     return fail(ctx, TFM_E_INVALID, "unknown solve mode");
   const bool mixed = mode == TFM_SOLVE_MIXED;
+  bool mg2d = false, mg3d = false;
   if (mode == TFM_SOLVE_PCG64_MG) {
-    // the V-cycle knows one layout: the SIMP operator on P concatenated
-    // grids, problem p owning design p's 2 nx ny DOFs
+    // the V-cycles know two layouts: the SIMP operator on P concatenated
+    // grids, problem p owning design p's 2 nx ny DOFs; and one system with
+    // 1 or 3 DOFs per node on a box mesh, one right-hand side
     const tfm_mesh *m = s->has_plan ? s->plan.mesh : nullptr;
-    bool ok = m && s->plan.op.kind == TFM_OP_SIMP && s->dpn == 2 && m->grid_P == P && s->plan.d_rho;
-    for (int p = 0; ok && p <= P; p++) ok = dof_off[p] == (int64_t)p * 2 * m->grid_nx * m->grid_ny;
-    if (!ok)
+    mg2d = m && s->plan.op.kind == TFM_OP_SIMP && s->dpn == 2 && m->grid_P == P && s->plan.d_rho;
+    for (int p = 0; mg2d && p <= P; p++) mg2d = dof_off[p] == (int64_t)p * 2 * m->grid_nx * m->grid_ny;
+    mg3d = !mg2d && m && m->box_n[0] > 0 && P == 1 && NR == 1 && (s->dpn == 1 || s->dpn == 3) &&
+           dof_off[0] == 0 && dof_off[1] == s->ndof &&
+           s->ndof == (int64_t)s->dpn * (m->box_n[0] + 1) * (m->box_n[1] + 1) * (m->box_n[2] + 1);
+    if (!mg2d && !mg3d)
       return fail(ctx, TFM_E_UNSUPPORTED,
-                  "multigrid solve needs a SIMP system on a grid_batch mesh, one problem per design");
+                  "multigrid solve needs a SIMP system on a grid_batch mesh (one problem per design) or one "
+                  "system on a box mesh with one right-hand side");
   }
   if (mixed && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values");
   if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values");
@@ -1189,8 +1303,13 @@ int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, int mod
              (rc = dalloc(ctx, &b->p32, nv)) || (rc = dalloc(ctx, &b->p32b, nv)) ||
              (rc = dalloc(ctx, &b->q32, nv)) || (rc = dalloc(ctx, &b->d32, n)) || (rc = dalloc(ctx, &b->x32, nv)));
   }
-  if (!rc && mode == TFM_SOLVE_PCG64_MG)
+  if (!rc && mg2d)
     rc = mg_create(ctx, P, (int)s->plan.mesh->grid_nx, (int)s->plan.mesh->grid_ny, NR, s->plan.ke0, &b->mg);
+  if (!rc && mg3d) {
+    const int64_t *bn = s->plan.mesh->box_n;
+    const int n3[3] = {(int)bn[0] + 1, (int)bn[1] + 1, (int)bn[2] + 1};
+    rc = mg3_create(ctx, n3, s->dpn, &b->mg3);
+  }
   if (!rc && hipMemsetAsync(b->x, 0, sizeof(double) * nv, ctx->stream) != hipSuccess)
     rc = fail(ctx, TFM_E_HIP, "hipMemsetAsync failed");
   if (rc) {
@@ -1244,6 +1363,7 @@ int bsolver_refresh(BatchSolver *b) {
                        b->bad);
   if (b->mg && (rc = mg_update(b->mg, s->plan.d_rho, s->plan.op.penal, s->plan.op.e0, s->plan.op.emin, s->d_fixed)))
     return rc;
+  if (b->mg3 && (rc = mg3_update(b->mg3, s->d_rp, s->d_ci, s->d_v64))) return rc;
   int hbad = 0;
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hbad, b->bad, sizeof(int), hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
@@ -1295,7 +1415,7 @@ static int bsolver_run_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, tf
     bb_resid<NR>(s, B, V, st, b->x, b->f, b->r64, nullptr, nullptr, nullptr);
     hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
     rc = bpcg<double, double, NR>(ctx, B, B.v64, b->r64, b->z64, b->p64, b->p64b, b->q64, b->d64, b->x, o.rtol,
-                                  o.max_iter, nullptr, hs, b->mg, &b->graph);
+                                  o.max_iter, nullptr, hs, b->mg, &b->graph, b->mg3);
     for (int q = 0; q < nseg; q++) {
       iters[q] = hs[q].iters;
       brk[q] = hs[q].breakdown;

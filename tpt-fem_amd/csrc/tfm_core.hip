@@ -398,6 +398,9 @@ extern "C" int tfm_mesh_box(tfm_ctx *ctx, const double mn[3], const double mx[3]
     tfm_mesh_destroy(m);
     return fail(ctx, TFM_E_HIP, "box_mesh kernel failed");
   }
+  m->box_n[0] = nx;
+  m->box_n[1] = ny;
+  m->box_n[2] = nz;
   *out = m;
   return TFM_OK;
 }

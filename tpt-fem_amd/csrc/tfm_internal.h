@@ -52,6 +52,8 @@ struct tfm_mesh {
   std::vector<int> h_n2e_ptr;  // kept for tile scheduling
   // tfm_mesh_grid_batch: P copies of an nx x ny node grid (0 otherwise)
   int64_t grid_P = 0, grid_nx = 0, grid_ny = 0;
+  // tfm_mesh_box: cells per axis (0 otherwise; jitter keeps the node grid)
+  int64_t box_n[3] = {0, 0, 0};
 };
 
 struct tfm_matrix {
@@ -217,6 +219,22 @@ const double *mg_sweep_weights(const MgData *mg);  // [n_dof] w Dinv of the fine
 const double *mg_fine_dinv(const MgData *mg);      // [n_dof] finest Jacobi diagonal inverse (0: fixed)
 struct MgOp;
 MgOp mg_fine_op(const MgData *mg);
+
+// Geometric multigrid for one system on a box mesh (tfm_mg3.hip), one rhs.
+// The caller runs level 0 on its own matrix copy: z = mg3_sweep_weights .* r,
+// mg3_fine_r = r - K z, y = mg3_cycle(z), then z = y + w Dinv (r - K y) with
+// w = kMg3Omega and Dinv = mg3_fine_dinv; with depth 0, mg3_direct.
+struct Mg3Data;
+int mg3_levels(const int n[3], int dpn, std::vector<int> *dims);
+int mg3_create(tfm_ctx *ctx, const int n[3], int dpn, Mg3Data **out);  // n: nodes per axis
+int mg3_update(Mg3Data *mg, const int *rp, const int *ci, const double *v);
+const double *mg3_cycle(const Mg3Data *mg, const double *z, hipStream_t st, const int *done);
+void mg3_direct(const Mg3Data *mg, const double *r, double *z, hipStream_t st, const int *done);
+int mg3_depth(const Mg3Data *mg);
+double *mg3_fine_r(const Mg3Data *mg);
+const double *mg3_fine_dinv(const Mg3Data *mg);
+const double *mg3_sweep_weights(const Mg3Data *mg);
+void mg3_destroy(Mg3Data *mg);
 void mg_destroy(MgData *mg);
 
 // Loads (tfm_loads.hip): device rhs of n_nodes*dpn doubles, accumulated.

@@ -14,7 +14,8 @@
 
 namespace tfm {
 
-constexpr double kMgOmega = 0.6;  // Jacobi weight; lambda_max(D^-1 K) ~ 2.8 for Q4 plane stress
+constexpr double kMgOmega = 0.6;   // Jacobi weight; lambda_max(D^-1 K) ~ 2.8 for Q4 plane stress
+constexpr double kMg3Omega = 0.5;  // box meshes; lambda_max(D^-1 K) ~ 3.1-3.3 for P1-tet elasticity
 
 // Operator of one level.
 struct MgOp {
@@ -77,6 +78,90 @@ __device__ __forceinline__ void apply_node(const MgOp &o, int p, int i, int j, c
     const double s = FINE ? o.E[e] : 1.0;
     y[0] += s * t0;
     y[1] += s * t1;
+  }
+}
+
+
+// Segments (design p, rhs k) a V-cycle skips: done[(p * NR + k) * stride]
+// non-zero (the batched CG's per-segment stop flags); done null skips none.
+struct MgAct {
+  const int *done;
+  int stride;
+};
+template <int NR>
+__device__ __forceinline__ bool skipped(const MgAct &a, int p, int k) {
+  return a.done && a.done[(p * NR + k) * a.stride];
+}
+
+// One axis of the interpolation.  A coarsened axis of n fine elements has
+// coarse nodes at fine positions X(I) = min(2 I, n), I = 0..ceil(n / 2); a
+// fine node either coincides with a coarse node or is the midpoint of two.
+__device__ __forceinline__ int coarse_pos(int I, int n, int co) { return co ? min(2 * I, n) : I; }
+
+__device__ __forceinline__ void interp1(int i, int n, int co, int &I0, int &I1, double &w1) {
+  if (!co) {
+    I0 = I1 = i;
+    w1 = 0.0;
+  } else if (i == n) {
+    I0 = I1 = (n + 1) / 2;
+    w1 = 0.0;
+  } else if ((i & 1) == 0) {
+    I0 = I1 = i / 2;
+    w1 = 0.0;
+  } else {
+    I0 = (i - 1) / 2;
+    I1 = I0 + 1;
+    w1 = 0.5;
+  }
+}
+
+// Weight of coarse node I in the interpolation at fine node i.
+__device__ __forceinline__ double hat1(int i, int I, int n, int co) {
+  int I0, I1;
+  double w1;
+  interp1(i, n, co, I0, I1, w1);
+  if (I0 == I1) return I0 == I ? 1.0 : 0.0;
+  return I0 == I ? 1.0 - w1 : (I1 == I ? w1 : 0.0);
+}
+
+// In-place inverse of the SPD nc x nc matrix A in LDS by the Gauss-Jordan
+// sweep (no pivoting), the whole block cooperating; d0 holds A's diagonal.
+// A DOF whose pivot vanishes relative to its diagonal -- one no coarse basis
+// function reaches -- gets a zero row and column.  Ends with a barrier.
+__device__ __forceinline__ void gj_invert_lds(double *A, const double *d0, int nc) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int k = 0; k < nc; k++) {
+    __syncthreads();
+    const double piv = A[k * nc + k];
+    const bool bad = !(d0[k] > 0.0) || !(piv > 1e-13 * d0[k]);
+    __syncthreads();
+    if (bad) {
+      for (int q = tid; q < nc; q += nt) {
+        A[k * nc + q] = 0.0;
+        A[q * nc + k] = 0.0;
+      }
+      continue;
+    }
+    for (int q = tid; q < nc; q += nt)
+      if (q != k) A[k * nc + q] /= piv;
+    __syncthreads();
+    for (int q = tid; q < nc * nc; q += nt) {
+      const int r = q / nc, c = q % nc;
+      if (r != k && c != k) A[q] -= A[r * nc + k] * A[k * nc + c];
+    }
+    __syncthreads();
+    for (int q = tid; q < nc; q += nt)
+      if (q != k) A[q * nc + k] = -A[q * nc + k] / piv;
+    if (tid == 0) A[k * nc + k] = 1.0 / piv;
+  }
+  __syncthreads();
+}
+
+// Stores the symmetrised inverse (A + A^T) / 2.
+__device__ __forceinline__ void gj_store_sym(const double *A, int nc, double *__restrict__ out) {
+  for (int q = threadIdx.x; q < nc * nc; q += blockDim.x) {
+    const int r = q / nc, c = q % nc;
+    out[q] = 0.5 * (A[q] + A[c * nc + r]);
   }
 }
 

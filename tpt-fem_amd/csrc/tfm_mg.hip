@@ -71,17 +71,6 @@ __device__ __forceinline__ double ke_entry(const MgOp &o, int p, int ei, int ej,
   return o.E[e] * o.ke0[r8 * 8 + c8];
 }
 
-// Segments (design p, rhs k) a V-cycle skips: done[(p * NR + k) * stride]
-// non-zero (the batched CG's per-segment stop flags); done null skips none.
-struct MgAct {
-  const int *done;
-  int stride;
-};
-template <int NR>
-__device__ __forceinline__ bool skipped(const MgAct &a, int p, int k) {
-  return a.done && a.done[(p * NR + k) * a.stride];
-}
-
 // Thread -> (design, node, rhs) of a level, rhs fastest; false past the end.
 template <int NR>
 __device__ __forceinline__ bool node_thread(int P, int nn, int &p, int &n, int &k) {
@@ -157,37 +146,6 @@ __global__ __launch_bounds__(kMgB) void km_smooth(int P, MgOp o, const double *_
     const int g = p * (int)o.ndof + 2 * n + a;
     out[g * NR + k] = yv[g * NR + k] + kMgOmega * dinv[g] * (b[g * NR + k] - y[a]);
   }
-}
-
-// One axis of the interpolation.  A coarsened axis of n fine elements has
-// coarse nodes at fine positions X(I) = min(2 I, n), I = 0..ceil(n / 2); a
-// fine node either coincides with a coarse node or is the midpoint of two.
-__device__ __forceinline__ int coarse_pos(int I, int n, int co) { return co ? min(2 * I, n) : I; }
-
-__device__ __forceinline__ void interp1(int i, int n, int co, int &I0, int &I1, double &w1) {
-  if (!co) {
-    I0 = I1 = i;
-    w1 = 0.0;
-  } else if (i == n) {
-    I0 = I1 = (n + 1) / 2;
-    w1 = 0.0;
-  } else if ((i & 1) == 0) {
-    I0 = I1 = i / 2;
-    w1 = 0.0;
-  } else {
-    I0 = (i - 1) / 2;
-    I1 = I0 + 1;
-    w1 = 0.5;
-  }
-}
-
-// Weight of coarse node I in the interpolation at fine node i.
-__device__ __forceinline__ double hat1(int i, int I, int n, int co) {
-  int I0, I1;
-  double w1;
-  interp1(i, n, co, I0, I1, w1);
-  if (I0 == I1) return I0 == I ? 1.0 : 0.0;
-  return I0 == I ? 1.0 - w1 : (I1 == I ? w1 : 0.0);
 }
 
 // Restriction (transpose of the interpolation): bc = P^T rf, and x1c = w
@@ -333,36 +291,8 @@ __global__ __launch_bounds__(256) void km_coarse_inv(MgOp o, int nc, double *__r
     A[q] = s;
     if (r == c) d0[r] = s;
   }
-  for (int k = 0; k < nc; k++) {
-    __syncthreads();
-    const double piv = A[k * nc + k];
-    const bool bad = !(d0[k] > 0.0) || !(piv > 1e-13 * d0[k]);
-    __syncthreads();
-    if (bad) {
-      for (int q = tid; q < nc; q += 256) {
-        A[k * nc + q] = 0.0;
-        A[q * nc + k] = 0.0;
-      }
-      continue;
-    }
-    for (int q = tid; q < nc; q += 256)
-      if (q != k) A[k * nc + q] /= piv;
-    __syncthreads();
-    for (int q = tid; q < nc * nc; q += 256) {
-      const int r = q / nc, c = q % nc;
-      if (r != k && c != k) A[q] -= A[r * nc + k] * A[k * nc + c];
-    }
-    __syncthreads();
-    for (int q = tid; q < nc; q += 256)
-      if (q != k) A[q * nc + k] = -A[q * nc + k] / piv;
-    if (tid == 0) A[k * nc + k] = 1.0 / piv;
-  }
-  __syncthreads();
-  double *out = inv + (int64_t)p * nc * nc;
-  for (int q = tid; q < nc * nc; q += 256) {
-    const int r = q / nc, c = q % nc;
-    out[q] = 0.5 * (A[q] + A[c * nc + r]);
-  }
+  gj_invert_lds(A, d0, nc);
+  gj_store_sym(A, nc, inv + (int64_t)p * nc * nc);
 }
 
 // Coarsest solve x = A^-1 b, one thread per (design, row, rhs); the inverse
