@@ -1327,6 +1327,10 @@ int bsolver_set_rhs(BatchSolver *b, const double *rhs) {
   tfm_ctx *ctx = s->ctx;
   const int64_t n = s->ndof;
   const int NR = b->NR;
+  if (!rhs && NR == 1) {  // the system's own rhs is already 0 at fixed DOFs
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(b->f, s->d_f, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    return TFM_OK;
+  }
   std::vector<unsigned char> fx((size_t)n, 0);
   TFM_CHECK_HIP(ctx, hipMemcpy(fx.data(), s->d_fixed, n, hipMemcpyDeviceToHost));
   std::vector<double> hf((size_t)n * NR, 0.0);
@@ -1363,7 +1367,7 @@ int bsolver_refresh(BatchSolver *b) {
                        b->bad);
   if (b->mg && (rc = mg_update(b->mg, s->plan.d_rho, s->plan.op.penal, s->plan.op.e0, s->plan.op.emin, s->d_fixed)))
     return rc;
-  if (b->mg3 && (rc = mg3_update(b->mg3, s->d_rp, s->d_ci, s->d_v64))) return rc;
+  if (b->mg3 && (rc = mg3_update(b->mg3, s->d_rp, s->d_ci, s->d_v64, b->d64))) return rc;
   int hbad = 0;
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hbad, b->bad, sizeof(int), hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
@@ -1612,6 +1616,14 @@ extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *d
     return rc;
   }
   const int64_t n = s->ndof;
+  if (nrhs == 1) {
+    // x is 0 at the fixed DOFs: u = x, then the Dirichlet values in bcs order
+    hipError_t e = hipMemcpy(u, b->x, sizeof(double) * n, hipMemcpyDeviceToHost);
+    bsolver_destroy(b);
+    if (e != hipSuccess) return fail(ctx, TFM_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    for (size_t q = 0; q < s->bc_dof.size(); q++) u[s->bc_dof[q]] = s->bc_val[q];
+    return rc;
+  }
   std::vector<double> hx((size_t)n * nrhs);
   hipError_t e = hipMemcpy(hx.data(), b->x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost);
   bsolver_destroy(b);

@@ -227,7 +227,7 @@ MgOp mg_fine_op(const MgData *mg);
 struct Mg3Data;
 int mg3_levels(const int n[3], int dpn, std::vector<int> *dims);
 int mg3_create(tfm_ctx *ctx, const int n[3], int dpn, Mg3Data **out);  // n: nodes per axis
-int mg3_update(Mg3Data *mg, const int *rp, const int *ci, const double *v);
+int mg3_update(Mg3Data *mg, const int *rp, const int *ci, const double *v, const double *dinv0);
 const double *mg3_cycle(const Mg3Data *mg, const double *z, hipStream_t st, const int *done);
 void mg3_direct(const Mg3Data *mg, const double *r, double *z, hipStream_t st, const int *done);
 int mg3_depth(const Mg3Data *mg);

@@ -63,18 +63,10 @@ __device__ __forceinline__ int thread_id() { return blockIdx.x * blockDim.x + th
 
 __device__ __forceinline__ bool mg3_skip(const MgAct &a) { return a.done && a.done[0]; }
 
-// Jacobi diagonal inverse of level 0 from its CSR rows (0 at empty, fixed
-// rows) and the first sweep's weights w Dinv.
-__global__ void km3_diag0(int nrows, const int *__restrict__ rp, const int *__restrict__ ci,
-                          const double *__restrict__ v, double *__restrict__ dinv, double *__restrict__ wd) {
+// The first sweep's weights w Dinv of level 0.
+__global__ void km3_weights(int n, const double *__restrict__ dinv, double *__restrict__ wd) {
   const int g = thread_id();
-  if (g >= nrows) return;
-  double d = 0.0;
-  for (int e = rp[g]; e < rp[g + 1]; e++)
-    if (ci[e] == g) d = v[e];
-  const double di = d > 0.0 ? 1.0 / d : 0.0;
-  dinv[g] = di;
-  wd[g] = kMg3Omega * di;
+  if (g < n) wd[g] = kMg3Omega * dinv[g];
 }
 
 // Jacobi diagonal inverse of a stencil level.
@@ -442,7 +434,8 @@ int mg3_create(tfm_ctx *ctx, const int n[3], int dpn, Mg3Data **out) {
 template <int DPN>
 static void mg3_update_t(Mg3Data *mg, const int *rp, const int *ci, const double *v, hipStream_t st) {
   const int nrows = mg->lv[0].nn * DPN;
-  hipLaunchKernelGGL(km3_diag0, dim3(g3grid(nrows)), dim3(kMg3B), 0, st, nrows, rp, ci, v, mg->lv[0].dinv, mg->d_wd0);
+  hipLaunchKernelGGL(km3_weights, dim3(g3grid(nrows)), dim3(kMg3B), 0, st, nrows, (const double *)mg->lv[0].dinv,
+                     mg->d_wd0);
   for (int l = 1; l <= mg->L; l++) {
     const Mg3Level &f = mg->lv[l - 1], &c = mg->lv[l];
     const unsigned gg = g3grid((int64_t)c.nn * DPN, kMg3GB);
@@ -462,10 +455,13 @@ static void mg3_update_t(Mg3Data *mg, const int *rp, const int *ci, const double
                      (const double *)(mg->L > 0 ? cl.S : nullptr), rp, ci, v, mg->nc, mg->d_inv);
 }
 
-// New matrix values (level 0's CSR, fixed rows empty): every level's
-// operator, Jacobi diagonal and the coarsest inverse.
-int mg3_update(Mg3Data *mg, const int *rp, const int *ci, const double *v) {
+// New matrix values (level 0's CSR, fixed rows empty, and its Jacobi
+// diagonal inverse, 0 at fixed DOFs): every level's operator, Jacobi
+// diagonal and the coarsest inverse.
+int mg3_update(Mg3Data *mg, const int *rp, const int *ci, const double *v, const double *dinv0) {
   hipStream_t st = mg->ctx->stream;
+  TFM_CHECK_HIP(mg->ctx, hipMemcpyAsync(mg->lv[0].dinv, dinv0, sizeof(double) * mg->lv[0].nn * mg->dpn,
+                                        hipMemcpyDeviceToDevice, st));
   if (mg->dpn == 3) mg3_update_t<3>(mg, rp, ci, v, st);
   else mg3_update_t<1>(mg, rp, ci, v, st);
   TFM_CHECK_HIP(mg->ctx, hipGetLastError());
