@@ -154,11 +154,26 @@ __global__ __launch_bounds__(kMg3GB) void km3_galerkin(G3 f, G3 c, int co0, int 
         const double wi = w0 * w1 * w2;
         const int fn = (i0 * f.n1 + i1) * f.n2 + i2;
         if (FROM_CSR) {
+          // a box mesh couples a node only to nodes at most one step away per
+          // axis, so with >= 3 nodes on the two fastest axes the column's
+          // offset (dx n1 n2 + dy n2 + dz) decodes without division
+          const bool fast = f.n1 >= 3 && f.n2 >= 3;
           const int row = fn * DPN + a;
           for (int e = rp[row]; e < rp[row + 1]; e++) {
             const int col = ci[e], gn = col / DPN, b = col - gn * DPN;
             int g0, g1, g2;
-            node3(gn, f, g0, g1, g2);
+            if (fast) {
+              int dd = gn - fn;
+              const int dx = dd > f.n2 + 1 ? 1 : (dd < -(f.n2 + 1) ? -1 : 0);
+              dd -= dx * f.n1 * f.n2;
+              const int dy = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
+              dd -= dy * f.n2;
+              g0 = i0 + dx;
+              g1 = i1 + dy;
+              g2 = i2 + dd;
+            } else {
+              node3(gn, f, g0, g1, g2);
+            }
             gal_add<DPN>(acc, g0, g1, g2, b, wi * v[e], f, co, I0, I1, I2);
           }
         } else {
