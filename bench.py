@@ -358,9 +358,18 @@ def main():
             "sweep": {"plain_fp64": tp, "mixed_fp32_fp64refine": tm},
         })
         if "multigrid" in args.solvers:
-            # FP64 CG with the box-mesh multigrid V-cycle (set-up -- Galerkin
-            # levels, coarse inverse -- in wall_s, not in s)
+            # FP64 CG with the box-mesh multigrid V-cycle.  Its first call
+            # builds the solver kept on the system (sliced copy, level
+            # storage), as the reference solve above built the other forms'
+            # sliced copy; every timed call still refreshes the values and the
+            # Galerkin hierarchy (in wall_s, not in s).
+            ctx.synchronize()
+            w0 = time.perf_counter()
+            sysm.solve(T.SOLVE_PCG64_MG, rtol=1e-2, max_iter=1000)
+            first_wall = time.perf_counter() - w0
             rg, tg = to_target(T.SOLVE_PCG64_MG, "multigrid")
+            if rg:
+                rg["first_call_wall_s"] = first_wall
             solve["multigrid_fp64"] = rg
             solve["sweep"]["multigrid_fp64"] = tg
             if rp and rg:

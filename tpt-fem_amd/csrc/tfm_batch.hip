@@ -1463,6 +1463,24 @@ int bsolver_run(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_st
   return b->NR == 1 ? bsolver_run_nr<1>(b, o, warm, stats, fu) : bsolver_run_nr<8>(b, o, warm, stats, fu);
 }
 
+int system_solve_mg(tfm_system *s, const tfm_solve_opts &o, double *u, tfm_solve_stats *stats) {
+  tfm_ctx *ctx = s->ctx;
+  int rc;
+  if (!s->mg_solver) {
+    const int64_t off[2] = {0, s->ndof};
+    if ((rc = bsolver_create(s, 1, off, 1, TFM_SOLVE_PCG64_MG, &s->mg_solver))) return rc;
+  }
+  BatchSolver *b = s->mg_solver;
+  if ((rc = bsolver_set_rhs(b, nullptr)) || (rc = bsolver_refresh(b))) return rc;
+  rc = bsolver_run(b, o, false, stats, nullptr);
+  if (rc && rc != TFM_E_NOT_CONVERGED) return rc;
+  // x is 0 at the fixed DOFs: u = x, then the Dirichlet values in bcs order
+  const hipError_t e = hipMemcpy(u, b->x, sizeof(double) * s->ndof, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  for (size_t q = 0; q < s->bc_dof.size(); q++) u[s->bc_dof[q]] = s->bc_val[q];
+  return rc;
+}
+
 }  // namespace tfm
 
 using namespace tfm;

@@ -203,6 +203,9 @@ const double *bsolver_x(const BatchSolver *b);  // [n_dof][NR], fixed DOFs 0
 double *bsolver_x_mut(BatchSolver *b);          // the same, to seed a warm start
 int bsolver_set_rhs_dev(BatchSolver *b, const double *d_rhs);  // NR = 1, device [n_dof]
 void bsolver_destroy(BatchSolver *b);
+// tfm_system_solve in the multigrid mode: one problem, the system's own rhs;
+// the solver (sliced copy, hierarchy) is kept on the system for later solves.
+int system_solve_mg(tfm_system *s, const tfm_solve_opts &o, double *u, tfm_solve_stats *stats);
 
 // Geometric multigrid for P batched topopt grids (tfm_mg.hip).
 struct MgData;
@@ -265,4 +268,5 @@ struct tfm_system {
   std::vector<int64_t> free_dofs;
   bool owns_pattern = false;
   tfm::BSell bsell;           // solver copy, built on the first solve
+  tfm::BatchSolver *mg_solver = nullptr;  // multigrid solver of tfm_system_solve, built on its first use
 };

@@ -1277,9 +1277,10 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
   if (o.mode != TFM_SOLVE_PCG64 && o.mode != TFM_SOLVE_MIXED && o.mode != TFM_SOLVE_PCG64_MG)
     return fail(ctx, TFM_E_INVALID, "unknown solve mode");
   if (o.mode == TFM_SOLVE_PCG64_MG) {
-    // one SIMP design on a grid: the batched solver with a single problem
-    const int64_t off[2] = {0, s->ndof};
-    return tfm_system_solve_batch(s, 1, off, 1, nullptr, &o, u, stats);
+    if (s->n_free == 0)
+      return fail(ctx, TFM_E_NUMERIC,
+                  "sparse numeric factorization/solve failed: rhs length 0 does not match matrix dimension 1");
+    return system_solve_mg(s, o, u, stats);  // the batched solver with one problem, kept on the system
   }
   const int n = (int)s->ndof;
   const bool mixed = o.mode == TFM_SOLVE_MIXED;
@@ -1459,6 +1460,7 @@ extern "C" void tfm_system_destroy(tfm_system *s) {
   dfree(s->d_v64);
   dfree(s->d_v32);
   bsell_destroy(&s->bsell);
+  bsolver_destroy(s->mg_solver);
   dfree(s->d_f);
   delete s;
 }
