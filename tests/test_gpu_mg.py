@@ -221,3 +221,24 @@ def test_mg_box_needs_box_mesh(ctx):
     assert ei.value.code == T.E_UNSUPPORTED
     s.close()
     m.close()
+
+
+def test_solve_drivers_with_mg(ctx):
+    """solve_poisson and solve_elasticity on a box mesh with the multigrid
+    mode: the reference's drivers' results (oracle dense LU) to 1e-10."""
+    import tptfem as T
+
+    n = (6, 5, 7)
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], n)
+    coords, conn = m.download()
+    faces = [(a, v) for a in range(3) for v in (0.0, 1.0)]
+    bcs = [(int(i), 0.0) for a, v in faces for i in O.nodes_on_plane(coords, a, v, 1e-9)]
+    u = T.solve_poisson(m, 1.0, 2, 1.0, bcs, rtol=1e-13, mode=T.SOLVE_PCG64_MG)
+    ur = O.solve_poisson(coords, O.TET, conn, 1.0, 2, 1.0, bcs)
+    assert np.abs(u - ur).max() <= 1e-10 * np.abs(ur).max()
+    clamp = [(int(i) * 3 + c, 0.0) for i in O.nodes_on_plane(coords, 0, 0.0, 1e-9) for c in range(3)]
+    u = T.solve_elasticity(m, T.CONTINUUM3D, 1.0, 0.3, 2, [0.0, 0.0, -1.0], clamp, rtol=1e-13,
+                           mode=T.SOLVE_PCG64_MG)
+    ue = T.solve_elasticity(m, T.CONTINUUM3D, 1.0, 0.3, 2, [0.0, 0.0, -1.0], clamp, rtol=1e-13)
+    assert np.abs(u - ue).max() <= 1e-10 * np.abs(ue).max()
+    m.close()

@@ -509,22 +509,26 @@ def solve_with_dirichlet(matrix, rhs, bcs, rtol=1e-13):
         s.close()
 
 
-def solve_poisson(mesh, kappa, quad_order, f, bcs, rtol=1e-13):
-    """solve_poisson (crates/tpt-fem-thermal/src/lib.rs:206-240), constant source, no Neumann/Robin."""
+def solve_poisson(mesh, kappa, quad_order, f, bcs, rtol=1e-13, mode=SOLVE_PCG64):
+    """solve_poisson (crates/tpt-fem-thermal/src/lib.rs:206-240), constant source, no Neumann/Robin.
+
+    mode SOLVE_PCG64_MG solves with the multigrid CG (meshes from Mesh.box)."""
     nb, bd, bv = _bc_arrays(bcs)
     u = np.zeros(mesh.n_nodes)
-    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
+    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
     mesh.ctx.check(_lib.tfm_solve_poisson(mesh.ctx.h, mesh.h, kappa, quad_order, f, nb, _ptr(bd), _ptr(bv),
                                           C.byref(o), _ptr(u)))
     return u
 
 
-def solve_elasticity(mesh, model, young, nu, quad_order, body, bcs, rtol=1e-13):
-    """solve_elasticity (crates/tpt-fem-elasticity/src/lib.rs:355-381), constant body force."""
+def solve_elasticity(mesh, model, young, nu, quad_order, body, bcs, rtol=1e-13, mode=SOLVE_PCG64):
+    """solve_elasticity (crates/tpt-fem-elasticity/src/lib.rs:355-381), constant body force.
+
+    mode SOLVE_PCG64_MG solves with the multigrid CG (meshes from Mesh.box)."""
     nb, bd, bv = _bc_arrays(bcs)
     u = np.zeros(mesh.n_nodes * mesh.dim)
     b = _f64(list(body) + [0.0] * (3 - len(body)))
-    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
+    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
     mesh.ctx.check(_lib.tfm_solve_elasticity(mesh.ctx.h, mesh.h, model, young, nu, quad_order, _ptr(b), nb,
                                              _ptr(bd), _ptr(bv), C.byref(o), _ptr(u)))
     return u
