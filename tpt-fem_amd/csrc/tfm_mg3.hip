@@ -29,7 +29,7 @@ namespace tfm {
 
 constexpr int kMg3B = 256;
 constexpr int kMg3MaxC = 90;  // coarsest DOFs: the dense matrix fills 64 KB of LDS
-constexpr int kMg3GB = 64;    // Galerkin: (coarse node, row) items per block
+constexpr int kMg3GB = 256;   // Galerkin: (coarse node, row, column component) items per block
 
 struct Mg3Level {
   int n[3] = {0, 0, 0};   // nodes per axis; node id (i n[1] + j) n[2] + k
@@ -79,59 +79,52 @@ __global__ void km3_dinv(int nn, const double *__restrict__ S, double *__restric
   dinv[t] = d > 0.0 ? 1.0 / d : 0.0;
 }
 
-// Adds w * val to the Galerkin row of coarse node (I0, I1, I2), column
-// component b, for the fine node (g0, g1, g2): every interpolation parent of
-// the fine node takes its share, at its stencil offset from the coarse node.
-template <int DPN>
-__device__ __forceinline__ void gal_add(double *acc, int g0, int g1, int g2, int b, double w, const G3 &f,
+// Adds w to the Galerkin row of coarse node (I0, I1, I2) for the fine node
+// (g0, g1, g2): every interpolation parent of the fine node takes its share
+// at its stencil offset d from the coarse node, acc[d].  The fine node is at
+// most two fine steps from I's position, so each parent is at most one
+// coarse step from I; a single parent appears twice, the second time with
+// weight 0, so all eight terms are added without branching.
+__device__ __forceinline__ void gal_add(double *acc, int g0, int g1, int g2, double w, const G3 &f,
                                         const int (&co)[3], int I0, int I1, int I2) {
-  int P0[2], P1[2], P2[2];
-  double W0[2], W1[2], W2[2];
-  double w1;
-  interp1(g0, f.n0 - 1, co[0], P0[0], P0[1], w1);
-  W0[0] = 1.0 - w1;
-  W0[1] = w1;
-  interp1(g1, f.n1 - 1, co[1], P1[0], P1[1], w1);
-  W1[0] = 1.0 - w1;
-  W1[1] = w1;
-  interp1(g2, f.n2 - 1, co[2], P2[0], P2[1], w1);
-  W2[0] = 1.0 - w1;
-  W2[1] = w1;
-  for (int u = 0; u < 2; u++) {
-    if (W0[u] == 0.0) continue;
-    const int d0 = P0[u] - I0;
-    if (d0 < -1 || d0 > 1) continue;
-    for (int v = 0; v < 2; v++) {
-      if (W1[v] == 0.0) continue;
-      const int d1 = P1[v] - I1;
-      if (d1 < -1 || d1 > 1) continue;
-      for (int s = 0; s < 2; s++) {
-        if (W2[s] == 0.0) continue;
-        const int d2 = P2[s] - I2;
-        if (d2 < -1 || d2 > 1) continue;
-        acc[((d0 + 1) * 9 + (d1 + 1) * 3 + d2 + 1) * DPN + b] += w * W0[u] * W1[v] * W2[s];
-      }
-    }
-  }
+  int A0, B0, A1, B1, A2, B2;
+  double x0, x1, x2;
+  interp1(g0, f.n0 - 1, co[0], A0, B0, x0);
+  interp1(g1, f.n1 - 1, co[1], A1, B1, x1);
+  interp1(g2, f.n2 - 1, co[2], A2, B2, x2);
+  const int o0[2] = {(A0 - I0 + 1) * 9, (B0 - I0 + 1) * 9};
+  const int o1[2] = {(A1 - I1 + 1) * 3, (B1 - I1 + 1) * 3};
+  const int o2[2] = {A2 - I2 + 1, B2 - I2 + 1};
+  const double v0[2] = {w * (1.0 - x0), w * x0};
+  const double v1[2] = {1.0 - x1, x1};
+  const double v2[2] = {1.0 - x2, x2};
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int q = 0; q < 2; q++) acc[o0[u] + o1[t] + o2[q]] += v0[u] * v1[t] * v2[q];
 }
 
-// Galerkin coarse stencil, one thread per (coarse node I, row a): row a of
-// every block of I's 27-point stencil, sum over the fine nodes i of I's
-// support (weight w_iI) of w_iI A_i,a,: P, accumulated in the thread's LDS
-// slice.  FROM_CSR: the fine operator is level 0's CSR; else a stencil level.
+// Galerkin coarse stencil, one thread per (coarse node I, row a, column
+// component b): entry (a, b) of every block of I's 27-point stencil, sum over
+// the fine nodes i of I's support (weight w_iI) of w_iI A_(i,a),(:,b) P,
+// accumulated in the thread's LDS slice.  FROM_CSR: the fine operator is
+// level 0's CSR; else a stencil level.
 template <int DPN, bool FROM_CSR>
 __global__ __launch_bounds__(kMg3GB) void km3_galerkin(G3 f, G3 c, int co0, int co1, int co2,
                                                        const int *__restrict__ rp, const int *__restrict__ ci,
                                                        const double *__restrict__ v, const double *__restrict__ Sf,
                                                        double *__restrict__ Sc) {
-  __shared__ double s_acc[kMg3GB][27 * DPN];
+  __shared__ double s_acc[kMg3GB][27];
   double *acc = s_acc[threadIdx.x];
   const int item = thread_id();
   const int ncn = c.n0 * c.n1 * c.n2;
-  if (item >= ncn * DPN) return;  // no barrier below: each thread owns its slice
-  const int I = item / DPN, a = item % DPN;
+  if (item >= ncn * DPN * DPN) return;  // no barrier below: each thread owns its slice
+  const int b = item % DPN, ia = item / DPN;
+  const int I = ia / DPN, a = ia % DPN;
   const int co[3] = {co0, co1, co2};
-  for (int q = 0; q < 27 * DPN; q++) acc[q] = 0.0;
+  for (int q = 0; q < 27; q++) acc[q] = 0.0;
   int I0, I1, I2;
   node3(I, c, I0, I1, I2);
   const int X0 = coarse_pos(I0, f.n0 - 1, co0), X1 = coarse_pos(I1, f.n1 - 1, co1),
@@ -160,36 +153,35 @@ __global__ __launch_bounds__(kMg3GB) void km3_galerkin(G3 f, G3 c, int co0, int 
           const bool fast = f.n1 >= 3 && f.n2 >= 3;
           const int row = fn * DPN + a;
           for (int e = rp[row]; e < rp[row + 1]; e++) {
-            const int col = ci[e], gn = col / DPN, b = col - gn * DPN;
+            const int col = ci[e], gn = col / DPN;
+            if (col - gn * DPN != b) continue;
             int g0, g1, g2;
             if (fast) {
               int dd = gn - fn;
-              const int dx = dd > f.n2 + 1 ? 1 : (dd < -(f.n2 + 1) ? -1 : 0);
-              dd -= dx * f.n1 * f.n2;
-              const int dy = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
-              dd -= dy * f.n2;
-              g0 = i0 + dx;
-              g1 = i1 + dy;
+              const int ox = dd > f.n2 + 1 ? 1 : (dd < -(f.n2 + 1) ? -1 : 0);
+              dd -= ox * f.n1 * f.n2;
+              const int oy = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
+              dd -= oy * f.n2;
+              g0 = i0 + ox;
+              g1 = i1 + oy;
               g2 = i2 + dd;
             } else {
               node3(gn, f, g0, g1, g2);
             }
-            gal_add<DPN>(acc, g0, g1, g2, b, wi * v[e], f, co, I0, I1, I2);
+            gal_add(acc, g0, g1, g2, wi * v[e], f, co, I0, I1, I2);
           }
         } else {
           for (int d = 0; d < 27; d++) {
             const int g0 = i0 + d / 9 - 1, g1 = i1 + (d / 3) % 3 - 1, g2 = i2 + d % 3 - 1;
             if (g0 < 0 || g0 >= f.n0 || g1 < 0 || g1 >= f.n1 || g2 < 0 || g2 >= f.n2) continue;
-            const double *blk = Sf + ((int64_t)fn * 27 + d) * DPN * DPN + a * DPN;
-            for (int b = 0; b < DPN; b++)
-              if (blk[b] != 0.0) gal_add<DPN>(acc, g0, g1, g2, b, wi * blk[b], f, co, I0, I1, I2);
+            const double val = Sf[((int64_t)fn * 27 + d) * DPN * DPN + a * DPN + b];
+            if (val != 0.0) gal_add(acc, g0, g1, g2, wi * val, f, co, I0, I1, I2);
           }
         }
       }
     }
   }
-  for (int d = 0; d < 27; d++)
-    for (int b = 0; b < DPN; b++) Sc[((int64_t)I * 27 + d) * DPN * DPN + a * DPN + b] = acc[d * DPN + b];
+  for (int d = 0; d < 27; d++) Sc[((int64_t)I * 27 + d) * DPN * DPN + a * DPN + b] = acc[d];
 }
 
 // y = S x at node n (all components); neighbours off the grid have zero blocks.
@@ -453,7 +445,7 @@ static void mg3_update_t(Mg3Data *mg, const int *rp, const int *ci, const double
                      mg->d_wd0);
   for (int l = 1; l <= mg->L; l++) {
     const Mg3Level &f = mg->lv[l - 1], &c = mg->lv[l];
-    const unsigned gg = g3grid((int64_t)c.nn * DPN, kMg3GB);
+    const unsigned gg = g3grid((int64_t)c.nn * DPN * DPN, kMg3GB);
     if (l == 1)
       hipLaunchKernelGGL((km3_galerkin<DPN, true>), dim3(gg), dim3(kMg3GB), 0, st, g3(f), g3(c), c.co[0], c.co[1],
                          c.co[2], rp, ci, v, (const double *)nullptr, c.S);
