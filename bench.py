@@ -262,7 +262,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
+              "(sharding a problem over several GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
