@@ -142,7 +142,7 @@ def test_single_system_solve_with_mg(ctx):
 
 # ---- box meshes (3-D multigrid) ---------------------------------------------
 
-def box_system(ctx, n, dpn, seed, jitter=0.2, faces=((0, 0.0),)):
+def box_system(ctx, n, dpn, seed, jitter=0.2, faces=((0, 0.0),), flags=None):
     """A jittered box_mesh system: Poisson (dpn 1, kappa_e = exp(N(0,1)),
     f = 1) or elasticity (dpn 3, E_e log-uniform [0.1, 10], nu 0.3, traction
     (0, 0, -1e-3) on x = 1), DOFs on the listed (axis, coord) faces fixed;
@@ -166,7 +166,7 @@ def box_system(ctx, n, dpn, seed, jitter=0.2, faces=((0, 0.0),)):
         op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=coef)
         rhs = O.plane_traction(coords, O.TET, conn, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
         oop = O.make_op(O.TET, 3, 1, coef_e=coef, nu=0.3)
-    s = T.System.create(m, op, dpn, bcs, rhs, flags=T.SYS_FP64)
+    s = T.System.create(m, op, dpn, bcs, rhs, flags=T.SYS_FP64 if flags is None else flags)
     return m, s, (coords, conn, oop, rhs, bcs)
 
 
@@ -241,4 +241,20 @@ def test_solve_drivers_with_mg(ctx):
                            mode=T.SOLVE_PCG64_MG)
     ue = T.solve_elasticity(m, T.CONTINUUM3D, 1.0, 0.3, 2, [0.0, 0.0, -1.0], clamp, rtol=1e-13)
     assert np.abs(u - ue).max() <= 1e-10 * np.abs(ue).max()
+    m.close()
+
+
+@pytest.mark.parametrize("case", [((6, 6, 6), 3), ((9, 7, 4), 1)])
+def test_mg_box_with_fp32_values(ctx, case):
+    """A system that also holds FP32 values runs the V-cycle's fine passes on
+    them; the CG stays FP64, so the solution is still the direct one to 1e-9."""
+    import tptfem as T
+
+    n, dpn = case
+    m, s, (coords, conn, oop, rhs, bcs) = box_system(ctx, n, dpn, seed=5 * dpn, flags=T.SYS_FP64 | T.SYS_FP32)
+    u = s.solve(T.SOLVE_PCG64_MG, rtol=1e-12, max_iter=2000)
+    ur = O.solve_with_dirichlet(O.assemble(coords, conn, oop, dpn), rhs, bcs)
+    assert np.linalg.norm(u - ur) <= 1e-9 * np.linalg.norm(ur)
+    assert s.stats.iterations <= 60
+    s.close()
     m.close()

@@ -549,9 +549,13 @@ __global__ __launch_bounds__(kBT) void kbx_resid_bs(BView V, const double *__res
 }
 
 // Multigrid on a box system (one problem, one rhs), level 0 on the sliced
-// copy.  kbx_mg3_resid: r0 = r - K z (z the V-cycle's first sweep).
-template <int BS>
-__global__ __launch_bounds__(kBT) void kbx_mg3_resid(BView V, const double *__restrict__ bv,
+// copy -- its FP32 values when the system has them (MT float: the V-cycle
+// only preconditions, so its fine-level products may use the rounded
+// matrix; both sweeps use the same one, which keeps the V-cycle symmetric;
+// the CG's own product stays FP64).  kbx_mg3_resid: r0 = r - K z (z the
+// V-cycle's first sweep).
+template <int BS, typename MT>
+__global__ __launch_bounds__(kBT) void kbx_mg3_resid(BView V, const MT *__restrict__ bv,
                                                      const double *__restrict__ r, const double *__restrict__ z,
                                                      double *__restrict__ r0) {
   const int b = blockIdx.x;
@@ -563,7 +567,7 @@ __global__ __launch_bounds__(kBT) void kbx_mg3_resid(BView V, const double *__re
   const int br = V.sb[s] + lane;
   const int o = V.soff[s], w = (V.soff[s + 1] - o) >> 6;
   const int *cp = V.bcol + o + lane;
-  const double *vp = bv + (int64_t)o * BB + lane;
+  const MT *vp = bv + (int64_t)o * BB + lane;
   double acc[BS];
 #pragma unroll
   for (int i = 0; i < BS; i++) acc[i] = 0.0;
@@ -572,11 +576,11 @@ __global__ __launch_bounds__(kBT) void kbx_mg3_resid(BView V, const double *__re
     double xv[BS];
 #pragma unroll
     for (int j = 0; j < BS; j++) xv[j] = z[c + j];
-    const double *vk = vp + (int64_t)kk * (64 * BB);
+    const MT *vk = vp + (int64_t)kk * (64 * BB);
 #pragma unroll
     for (int i = 0; i < BS; i++)
 #pragma unroll
-      for (int j = 0; j < BS; j++) acc[i] += vk[(i * BS + j) * 64] * xv[j];
+      for (int j = 0; j < BS; j++) acc[i] += (double)vk[(i * BS + j) * 64] * xv[j];
   }
   if (br < V.sb[s + 1])
 #pragma unroll
@@ -588,8 +592,8 @@ __global__ __launch_bounds__(kBT) void kbx_mg3_resid(BView V, const double *__re
 
 // kbx_mg3_smooth: the V-cycle's closing sweep on level 0, z = y + w Dinv
 // (r - K y), fused with the CG's r.z partial sums into `slot`.
-template <int BS>
-__global__ __launch_bounds__(kBT) void kbx_mg3_smooth(BView V, const double *__restrict__ bv,
+template <int BS, typename MT>
+__global__ __launch_bounds__(kBT) void kbx_mg3_smooth(BView V, const MT *__restrict__ bv,
                                                       const double *__restrict__ r, const double *__restrict__ y,
                                                       const double *__restrict__ dinv, double *__restrict__ z,
                                                       int slot) {
@@ -603,7 +607,7 @@ __global__ __launch_bounds__(kBT) void kbx_mg3_smooth(BView V, const double *__r
     const int br = V.sb[s] + lane;
     const int o = V.soff[s], w = (V.soff[s + 1] - o) >> 6;
     const int *cp = V.bcol + o + lane;
-    const double *vp = bv + (int64_t)o * BB + lane;
+    const MT *vp = bv + (int64_t)o * BB + lane;
     double acc[BS];
 #pragma unroll
     for (int i = 0; i < BS; i++) acc[i] = 0.0;
@@ -612,11 +616,11 @@ __global__ __launch_bounds__(kBT) void kbx_mg3_smooth(BView V, const double *__r
       double xv[BS];
 #pragma unroll
       for (int j = 0; j < BS; j++) xv[j] = y[c + j];
-      const double *vk = vp + (int64_t)kk * (64 * BB);
+      const MT *vk = vp + (int64_t)kk * (64 * BB);
 #pragma unroll
       for (int i = 0; i < BS; i++)
 #pragma unroll
-        for (int j = 0; j < BS; j++) acc[i] += vk[(i * BS + j) * 64] * xv[j];
+        for (int j = 0; j < BS; j++) acc[i] += (double)vk[(i * BS + j) * 64] * xv[j];
     }
     if (br < V.sb[s + 1])
 #pragma unroll
@@ -997,14 +1001,21 @@ static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p
       }
       double *r0 = mg3_fine_r(mg3);
       const double *d0 = mg3_fine_dinv(mg3);
-      switch (B.bs) {
-      case 3: hipLaunchKernelGGL((kbx_mg3_resid<3>), g, t, 0, st, V, bv, r, z, r0); break;
-      default: hipLaunchKernelGGL((kbx_mg3_resid<1>), g, t, 0, st, V, bv, r, z, r0); break;
+      const float *b32 = B.v32;
+      if (b32) {
+        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_resid<3, float>), g, t, 0, st, V, b32, r, z, r0);
+        else hipLaunchKernelGGL((kbx_mg3_resid<1, float>), g, t, 0, st, V, b32, r, z, r0);
+      } else {
+        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_resid<3, double>), g, t, 0, st, V, bv, r, z, r0);
+        else hipLaunchKernelGGL((kbx_mg3_resid<1, double>), g, t, 0, st, V, bv, r, z, r0);
       }
       const double *yv = mg3_cycle(mg3, z, st, sdone);
-      switch (B.bs) {
-      case 3: hipLaunchKernelGGL((kbx_mg3_smooth<3>), g, t, 0, st, V, bv, r, yv, d0, z, slot); break;
-      default: hipLaunchKernelGGL((kbx_mg3_smooth<1>), g, t, 0, st, V, bv, r, yv, d0, z, slot); break;
+      if (b32) {
+        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_smooth<3, float>), g, t, 0, st, V, b32, r, yv, d0, z, slot);
+        else hipLaunchKernelGGL((kbx_mg3_smooth<1, float>), g, t, 0, st, V, b32, r, yv, d0, z, slot);
+      } else {
+        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_smooth<3, double>), g, t, 0, st, V, bv, r, yv, d0, z, slot);
+        else hipLaunchKernelGGL((kbx_mg3_smooth<1, double>), g, t, 0, st, V, bv, r, yv, d0, z, slot);
       }
     }
   };
@@ -1356,7 +1367,7 @@ int bsolver_refresh(BatchSolver *b) {
   const int n = (int)s->ndof;
   // FP64 sliced values serve the plain CG and every mode's FP64 residual
   int rc = bb_values<double>(s, b->B, s->d_v64, &b->B.v64);
-  if (!rc && b->mixed) rc = bb_values<float>(s, b->B, s->d_v32, &b->B.v32);
+  if (!rc && (b->mixed || (b->mg3 && s->d_v32))) rc = bb_values<float>(s, b->B, s->d_v32, &b->B.v32);
   if (rc) return rc;
   TFM_CHECK_HIP(ctx, hipMemsetAsync(b->bad, 0, sizeof(int), st));
   const unsigned g1 = (unsigned)((n + 255) / 256);
