@@ -121,6 +121,35 @@ def test_elasticity_tet_rigid_modes_symmetry_and_closed_form():
     np.testing.assert_allclose(K, vol * B.T @ D @ B, rtol=1e-11, atol=1e-13)
 
 
+def test_four_equal_contributions_sum_is_one_multiplication():
+    """The GPU tet kernels write 4 c where the reference adds a P1 tet's four
+    equal quadrature contributions, ((c + c) + c) + c: bitwise equal in binary
+    round-to-nearest-even.  Checked on every half-precision value (the
+    argument does not depend on the format; numpy's float16 sums here are
+    exact in float32, then rounded once), on random doubles and on the doubles
+    where fl(3c) is a tie (significand = 2 mod 4, c's significand >= 4/3)."""
+    h = np.arange(1 << 16, dtype=np.uint16).view(np.float16)
+    h = h[np.isfinite(h)]
+    with np.errstate(over="ignore"):
+        s = ((h + h) + h) + h
+        np.testing.assert_array_equal(s.view(np.uint16), (h * np.float16(4)).view(np.uint16))
+    rng = np.random.default_rng(7)
+    bits = rng.integers(0, 1 << 63, 2_000_000, dtype=np.int64).astype(np.uint64)
+    sig = rng.integers(1 << 50, 1 << 52, 500_000, dtype=np.int64).astype(np.uint64)
+    sig = (sig & ~np.uint64(3)) | np.uint64(2)  # significand = 2 mod 4
+    sig = sig[sig >= np.uint64((1 << 52) // 3)]  # c's significand >= 4/3
+    ties = sig | (rng.integers(1, 2046, sig.size, dtype=np.int64).astype(np.uint64) << np.uint64(52))
+    c = np.concatenate([bits, ties, bits | np.uint64(1 << 63)]).view(np.float64)
+    c = c[np.isfinite(c)]
+    with np.errstate(over="ignore"):
+        s = ((c + c) + c) + c
+        np.testing.assert_array_equal(s.view(np.uint64), (4.0 * c).view(np.uint64))
+    t = ties.view(np.float64)  # fl(3 t) really is a tie: 3 M needs two bits more
+    M = (sig & np.uint64((1 << 52) - 1)) | np.uint64(1 << 52)
+    assert np.all((np.uint64(3) * M) % np.uint64(4) == np.uint64(2)) and np.all(np.uint64(3) * M >= np.uint64(1 << 54))
+    assert t.size > 100_000
+
+
 def test_q4_simp_unit_stiffness_matches_textbook():
     nu = 0.3
     k = np.array([1 / 2 - nu / 6, 1 / 8 + nu / 8, -1 / 4 - nu / 12, -1 / 8 + 3 * nu / 8, -1 / 4 + nu / 12,

@@ -19,8 +19,8 @@
 // reference's duplicate-summation order (Coo::to_csr) -- and writes them at
 // the planned positions.  No hashing, searching or atomics.
 //
-// Entry arithmetic is the reference's (tfm_elem.h: bdb3 and four equal
-// quadrature contributions); this translation unit is built without FMA
+// Entry arithmetic is the reference's (tfm_elem.h: bdb3, and quad4 for the
+// sum of the four equal quadrature contributions); this translation unit is built without FMA
 // contraction.
 #include <hip/hip_runtime.h>
 
@@ -117,7 +117,7 @@ __device__ __forceinline__ void tet_row(const TetPair<DPN, VT> &pr, int a, VT *o
       for (int lm = 0; lm < 4; lm++) {
         const double dot = pr.gl[0] * pr.g[lm][0] + pr.gl[1] * pr.g[lm][1] + pr.gl[2] * pr.g[lm][2];
         const double c = pr.k0 * dot * pr.k1;
-        out[lm * stride] = c + c + c + c;
+        out[lm * stride] = quad4(c);
       }
     } else {
 #pragma unroll
@@ -125,7 +125,7 @@ __device__ __forceinline__ void tet_row(const TetPair<DPN, VT> &pr, int a, VT *o
 #pragma unroll
         for (int b = 0; b < 3; b++) {
           const double c = pr.k0 * bdb3(pr.gl, pr.g[lm], a, b, pr.D);
-          out[(lm * 3 + b) * stride] = c + c + c + c;
+          out[(lm * 3 + b) * stride] = quad4(c);
         }
     }
   } else {

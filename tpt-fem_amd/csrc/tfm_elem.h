@@ -87,6 +87,16 @@ __device__ __forceinline__ DConst make_d(int model, double e, double nu) {
   return D;
 }
 
+// The reference's sum of four equal quadrature contributions, ((c + c) + c)
+// + c from 0.0, as one multiplication: in binary round-to-nearest-even the
+// two are bitwise equal.  c + c = 2c and fl(2c + c) = fl(3c) = 4c - d; 4c is
+// representable and |d| <= ulp(3c) / 2, which is <= ulp(4c) / 4 when 3c lies
+// below 4c's binade and <= ulp(4c) / 2 otherwise, so fl(4c - d) = 4c except
+// at a tie, which needs c's integer significand M = 2 (mod 4) and then
+// rounds to 4c, whose significand M is even.  (+0.0 + c only differs for c =
+// -0.0, an entry assemble() skips either way.)
+__device__ __forceinline__ double quad4(double c) { return 4.0 * c; }
+
 // s(a,b) = sum_s sum_t B_i[s][a] D[s][t] B_j[t][b], non-zero terms in the
 // reference loop order (s outer, t inner), B/D of b_matrix/constitutive.
 __device__ __forceinline__ double bdb3(const double *gi, const double *gj, int a, int b,
@@ -239,7 +249,7 @@ __device__ __forceinline__ void rowblock_f64(const double (&X)[Cell<CELL>::NN][3
 
   if constexpr (CELL == TFM_TET) {
     // Four identical quadrature contributions (constant gradients, equal
-    // weights): the reference adds c four times starting from 0.0.
+    // weights): the reference adds c four times starting from 0.0 (quad4).
     double g[4][3];
     double det = fabs(map_tet(X, g));
     const double w = 1.0 / 24.0;
@@ -249,7 +259,7 @@ __device__ __forceinline__ void rowblock_f64(const double (&X)[Cell<CELL>::NN][3
       for (int j = 0; j < 4; j++) {
         double dot = g[ln][0] * g[j][0] + g[ln][1] * g[j][1] + g[ln][2] * g[j][2];
         double c = wk * dot * det;
-        out[j] = c + c + c + c;
+        out[j] = quad4(c);
       }
     } else {
       double wd = w * det;
@@ -260,7 +270,7 @@ __device__ __forceinline__ void rowblock_f64(const double (&X)[Cell<CELL>::NN][3
 #pragma unroll
           for (int b = 0; b < 3; b++) {
             double c = wd * bdb3(g[ln], g[j], a, b, D);
-            out[a * W + j * 3 + b] = c + c + c + c;
+            out[a * W + j * 3 + b] = quad4(c);
           }
     }
     return;
