@@ -38,6 +38,36 @@ def asm_bytes(n_nodes, n_elem, nnz, n_free, sv, per_elem_coef=True):
     return 24 * n_nodes + 16 * n_elem + (8 * n_elem if per_elem_coef else 0) + (sv + 4) * nnz + 4 * (n_free + 1)
 
 
+def host_cpu():
+    """The host's CPU model and logical CPU count (for the CPU baseline record)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
+class one_core:
+    """Pins the calling thread to one core (the lowest it may run on) for the
+    CPU baseline, as the single-threaded reference would run; restores the
+    previous affinity on exit."""
+
+    def __enter__(self):
+        self.prev = os.sched_getaffinity(0)
+        self.core = min(self.prev)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.prev)
+        return False
+
+
 def cpu_baseline(n, seed=7):
     """Single-core oracle (restated reference CPU path): assemble triplets with
     the exact-zero skip, Coo::to_csr and reduce_system, on an n^3 sample of
@@ -51,10 +81,11 @@ def cpu_baseline(n, seed=7):
     bcs = [(int(i) * 3 + k, 0.0) for i in O.nodes_on_plane(coords, 0, 0.0, 1e-9) for k in range(3)]
     rhs = np.zeros(len(coords) * 3)
     op = O.make_op(O.TET, 3, 1, coef_e=ye, nu=0.3)
-    t0 = time.perf_counter()
-    coo = O.assemble(coords, conn, op, 3)
-    O.reduce_system(coo, rhs, bcs)
-    dt = time.perf_counter() - t0
+    with one_core():
+        t0 = time.perf_counter()
+        coo = O.assemble(coords, conn, op, 3)
+        O.reduce_system(coo, rhs, bcs)
+        dt = time.perf_counter() - t0
     return len(conn) / dt, len(conn), dt
 
 
@@ -222,11 +253,12 @@ def bench_simp(args, T, log):
 
         cn, cm, ci = 49, 25, 5
         f1, b1 = O.cantilever_load(cn, cm, 1.0)
-        t0 = time.perf_counter()
-        O.topopt_simp(cn, cm, 1.0, 1.0, 0.3, 0.5, 3.0, 2.4, ci, 0.2, f1, b1)
-        dt = time.perf_counter() - t0
+        with one_core():
+            t0 = time.perf_counter()
+            O.topopt_simp(cn, cm, 1.0, 1.0, 0.3, 0.5, 3.0, 2.4, ci, 0.2, f1, b1)
+            dt = time.perf_counter() - t0
         line["cpu_baseline"] = {"value": ci / dt, "unit": "OC iterations/s of ONE design", "cores": 1,
-                                "kind": "port",
+                                "kind": "port", "host": host_cpu(),
                                 "sample": f"oracle topopt_simp (dense LU, as the reference), 1 design Grid {cn}x{cm}, "
                                           f"{ci} OC iterations; the 257x129 grid needs a 34.9 GB dense LU per solve"}
     print(json.dumps(line), flush=True)
@@ -378,7 +410,7 @@ def main():
     cpu = None
     if not args.skip_cpu:
         eps, ne, dt = cpu_baseline(args.cpu_n)
-        cpu = {"value": eps, "unit": "elements/s", "cores": 1, "kind": "port",
+        cpu = {"value": eps, "unit": "elements/s", "cores": 1, "kind": "port", "host": host_cpu(),
                "sample": f"oracle assemble+to_csr+reduce_system, {args.cpu_n}^3 jittered elasticity "
                          f"({ne} tets, {dt:.1f}s)"}
         log(f"cpu baseline {eps:.0f} elements/s")
