@@ -12,6 +12,11 @@
 // present entry and a presence mask of its DPN columns (columns of fixed DOFs
 // and entries assemble() skipped as exact zeros are absent).
 //
+// The plan stores each tile's row blocks as items in the order the numeric
+// pass takes them (the tile's diagonal blocks first), so a tile reads its
+// items straight from its first-item index: no per-node tables, prefix sums
+// or barriers before the element math starts.
+//
 // Numeric pass: tiles of consecutive nodes (<= kTetTP pairs).  One lane per
 // (node, element) pair computes the element map once and the pair's whole row
 // block (4 x DPN x DPN entries) into LDS; then one lane per (row, block) adds
@@ -144,49 +149,27 @@ __device__ __forceinline__ void tet_row(const TetPair<DPN, VT> &pr, int a, VT *o
   }
 }
 
-// Plan entries of one row block of a tile, for all DPN rows.
+// Plan entries of one item (row block), for all DPN rows.
 struct TetBlk {
   unsigned mask;
   unsigned long long lmb;
   int kb;             // first pair of the block's node within the tile
   int pos[3];         // per row: CSR position of the first present entry
-  unsigned pm[3];     // per row: presence mask of the DPN columns
+  unsigned pm;        // per row a: presence mask of the DPN columns in bits 4a..4a+3
 };
 
-// Item it of a tile: first the diagonal blocks of its nodes (stored first in
-// each node's block list; s_dpre counts them), then the off-diagonal blocks
-// (s_opre counts them).  Grouping the diagonal blocks -- 24 contributions
-// against 4-6 -- keeps the long summation loops in the tile's first wave.
-__device__ __forceinline__ void tet_item_map(int it, int nt, const int *s_bptr, const int *s_dpre,
-                                             const int *s_opre, int &t, int &blk) {
-  const int nd = s_dpre[nt];
-  t = 0;
-  if (it < nd) {
-    while (t + 1 < nt && s_dpre[t + 1] <= it) t++;
-    blk = s_bptr[t];
-  } else {
-    const int j = it - nd;
-    while (t + 1 < nt && s_opre[t + 1] <= j) t++;
-    blk = s_bptr[t] + (s_dpre[t + 1] - s_dpre[t]) + (j - s_opre[t]);
-  }
-}
-
 template <int DPN>
-__device__ __forceinline__ TetBlk tet_blk(int blk, int t, const int *s_bptr, const int *s_nptr,
-                                          const unsigned *__restrict__ bmask,
-                                          const unsigned long long *__restrict__ blmb,
-                                          const int *__restrict__ ipos, const unsigned char *__restrict__ ipm) {
+__device__ __forceinline__ TetBlk tet_blk(int64_t item, int64_t nitems, const unsigned *__restrict__ mask,
+                                          const unsigned long long *__restrict__ lmb, const int *__restrict__ pos,
+                                          const unsigned *__restrict__ pmk) {
   TetBlk B;
-  const int nb = s_bptr[t + 1] - s_bptr[t], k = blk - s_bptr[t];
-  B.mask = bmask[blk];
-  B.lmb = blmb[blk];
-  B.kb = s_nptr[t];
+  B.mask = mask[item];
+  B.lmb = lmb[item];
+  const unsigned w = pmk[item];
+  B.kb = (int)(w >> 16);
+  B.pm = w & 0xfffu;
 #pragma unroll
-  for (int a = 0; a < DPN; a++) {
-    const int64_t gi = (int64_t)DPN * s_bptr[t] + a * nb + k;
-    B.pos[a] = ipos[gi];
-    B.pm[a] = ipm[gi];
-  }
+  for (int a = 0; a < DPN; a++) B.pos[a] = pos[a * nitems + item];
   return B;
 }
 
@@ -225,13 +208,10 @@ __device__ __forceinline__ void tet_sum(const VT *s_c, const TetBlk &B, VT (&acc
 template <int DPN, typename VT>
 __device__ __forceinline__ void tet_write(const TetBlk &B, int a, const VT (&acc)[DPN], VT *__restrict__ vals) {
   int q = B.pos[0];
-  unsigned pm = B.pm[0];
 #pragma unroll
   for (int i = 1; i < DPN; i++)  // selects: a dynamic index would use scratch memory
-    if (a == i) {
-      q = B.pos[i];
-      pm = B.pm[i];
-    }
+    if (a == i) q = B.pos[i];
+  const unsigned pm = (B.pm >> (4 * a)) & 0xfu;
 #pragma unroll
   for (int b = 0; b < DPN; b++)
     if ((pm >> b) & 1u) vals[q++] = acc[b];
@@ -239,32 +219,19 @@ __device__ __forceinline__ void tet_write(const TetBlk &B, int a, const VT (&acc
 
 template <int DPN, typename VT>
 __global__ __launch_bounds__(kTetTP, 5) void k_asm_tet(const double *__restrict__ coords, const int *__restrict__ conn,
-                                                     const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
-                                                     const int *__restrict__ tiles, const int *__restrict__ tpa,
-                                                     int ntiles, OpParams P, const int *__restrict__ bptr,
-                                                     const unsigned char *__restrict__ hasdiag,
-                                                     const unsigned *__restrict__ bmask,
-                                                     const unsigned long long *__restrict__ blmb,
-                                                     const int *__restrict__ ipos,
-                                                     const unsigned char *__restrict__ ipm, VT *__restrict__ vals) {
+                                                     const int *__restrict__ n2e, const int *__restrict__ tpa,
+                                                     const int *__restrict__ tib, int ntiles, OpParams P,
+                                                     const unsigned *__restrict__ imask,
+                                                     const unsigned long long *__restrict__ ilmb,
+                                                     const int *__restrict__ ipos, const unsigned *__restrict__ ipmk,
+                                                     int64_t nitems, VT *__restrict__ vals) {
   __shared__ VT s_c[4 * DPN * kTetTP];  // one DOF row of every pair: s_c[(lm * DPN + b) * kTetTP + pair]
-  __shared__ int s_nptr[kTetNodes + 1];
-  __shared__ int s_bptr[kTetNodes + 1];
-  __shared__ int s_dpre[kTetNodes + 1];  // diagonal blocks of the tile's nodes before node t
-  __shared__ int s_opre[kTetNodes + 1];  // off-diagonal blocks before node t
-  __shared__ int s_hd[kTetNodes];
 
   const int tid = threadIdx.x;
   const int tile = xcd_item(blockIdx.x, ntiles);  // neighbouring tiles share an XCD's L2
   if (tile >= ntiles) return;
-  const int n0 = tiles[tile], n1 = tiles[tile + 1];
   const int pa = tpa[tile], np = tpa[tile + 1] - pa;
-  const int nt = n1 - n0;
-  if (tid <= nt) {
-    s_nptr[tid] = n2e_ptr[n0 + tid] - pa;
-    s_bptr[tid] = bptr[n0 + tid];
-  }
-  if (tid < nt) s_hd[tid] = hasdiag[n0 + tid];
+  const int ib = tib[tile], nblk = tib[tile + 1] - ib;
 
   // element loads, one (node, element) pair per lane (unconditional: inactive
   // lanes read pair 0 of the tile, or of the mesh when the tile's nodes have
@@ -280,29 +247,10 @@ __global__ __launch_bounds__(kTetTP, 5) void k_asm_tet(const double *__restrict_
     for (int p = 0; p < 3; p++) X[a][p] = coords[(int64_t)nd * 3 + p];
   }
   const double coef = P.coef_e ? P.coef_e[e] : P.coef;
-  __syncthreads();  // s_nptr / s_bptr / s_hd (waits on LDS only: the loads stay in flight)
-  if (tid == 0) {
-    int d = 0, o = 0;
-    for (int t = 0; t < nt; t++) {
-      s_dpre[t] = d;
-      s_opre[t] = o;
-      d += s_hd[t];
-      o += s_bptr[t + 1] - s_bptr[t] - s_hd[t];
-    }
-    s_dpre[nt] = d;
-    s_opre[nt] = o;
-  }
-  __syncthreads();
 
-  // the first round's plan entries, fetched while the element math runs
-  const int b0 = s_bptr[0];
-  const int nblk = s_bptr[nt] - b0;
+  // the first round's items, fetched while the element math runs
   TetBlk B0{};
-  if (tid < nblk) {
-    int t, blk;
-    tet_item_map(tid, nt, s_bptr, s_dpre, s_opre, t, blk);
-    B0 = tet_blk<DPN>(blk, t, s_bptr, s_nptr, bmask, blmb, ipos, ipm);
-  }
+  if (tid < nblk) B0 = tet_blk<DPN>(ib + tid, nitems, imask, ilmb, ipos, ipmk);
 
   TetPair<DPN, VT> pr;
   tet_setup<DPN, VT>(X, ln, coef, P, pr);
@@ -318,9 +266,7 @@ __global__ __launch_bounds__(kTetTP, 5) void k_asm_tet(const double *__restrict_
       tet_write<DPN, VT>(B0, a, acc, vals);
     }
     for (int it = tid + kTetTP; it < nblk; it += kTetTP) {
-      int t, blk;
-      tet_item_map(it, nt, s_bptr, s_dpre, s_opre, t, blk);
-      const TetBlk B = tet_blk<DPN>(blk, t, s_bptr, s_nptr, bmask, blmb, ipos, ipm);
+      const TetBlk B = tet_blk<DPN>(ib + it, nitems, imask, ilmb, ipos, ipmk);
       tet_sum<DPN, VT>(s_c, B, acc);
       tet_write<DPN, VT>(B, a, acc, vals);
     }
@@ -415,6 +361,48 @@ __global__ void k_tet_plan_fill(int N, const int *__restrict__ rp, const int *__
   }
 }
 
+// Item records in the numeric pass's order, one thread per tile: the tile's
+// diagonal blocks (first in each node's block list) in node order, then the
+// off-diagonal blocks in node and column order.  A tile's items occupy the
+// block range of its nodes, [bptr[n0], bptr[n1]).
+template <int DPN>
+__global__ void k_tet_plan_items(int ntiles, const int *__restrict__ tiles, const int *__restrict__ tpa,
+                                 const int *__restrict__ n2e_ptr, const int *__restrict__ bptr,
+                                 const unsigned char *__restrict__ hasdiag, const unsigned *__restrict__ bmask,
+                                 const unsigned long long *__restrict__ blmb, const int *__restrict__ bpos,
+                                 const unsigned char *__restrict__ bpm, int64_t nitems, int *__restrict__ tib,
+                                 unsigned *__restrict__ imask, unsigned long long *__restrict__ ilmb,
+                                 int *__restrict__ ipos, unsigned *__restrict__ ipmk) {
+  const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tile >= ntiles) return;
+  const int n0 = tiles[tile], n1 = tiles[tile + 1];
+  const int b0 = bptr[n0];
+  tib[tile] = b0;
+  if (tile == ntiles - 1) tib[ntiles] = bptr[n1];
+  int nd = 0;
+  for (int n = n0; n < n1; n++) nd += hasdiag[n];
+  int d = 0, o = 0;
+  for (int n = n0; n < n1; n++) {
+    const int bb = bptr[n], nb = bptr[n + 1] - bb, hd = hasdiag[n];
+    const unsigned kb = (unsigned)(n2e_ptr[n] - tpa[tile]);
+    for (int k = 0; k < nb; k++) {
+      const int64_t item = b0 + (hd && k == 0 ? d : nd + o + (k - hd));
+      imask[item] = bmask[bb + k];
+      ilmb[item] = blmb[bb + k];
+      unsigned w = kb << 16;
+#pragma unroll
+      for (int a = 0; a < DPN; a++) {
+        const int64_t gi = (int64_t)DPN * bb + a * nb + k;
+        ipos[a * nitems + item] = bpos[gi];
+        w |= (unsigned)bpm[gi] << (4 * a);
+      }
+      ipmk[item] = w;
+    }
+    d += hd;
+    o += nb - hd;
+  }
+}
+
 // Tile schedule: consecutive nodes with <= kTetTP pairs and <= kTetNodes nodes.
 static int tet_tiles(tfm_ctx *ctx, const tfm_mesh *mesh, TetPlan *tp) {
   const int64_t N = mesh->n_nodes;
@@ -445,48 +433,67 @@ int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp
   hipStream_t st = ctx->stream;
   int rc;
   if ((rc = tet_tiles(ctx, mesh, tp))) return rc;
-  int *cnt = nullptr, *err = nullptr;
-  if ((rc = dalloc(ctx, &cnt, N)) || (rc = dalloc(ctx, &tp->d_bptr, N + 1)) || (rc = dalloc(ctx, &err, 1)) ||
-      (rc = dalloc(ctx, &tp->d_hasdiag, N))) {
+  // per-node block lists (temporaries), then the item records
+  int *cnt = nullptr, *err = nullptr, *bptr = nullptr, *bpos = nullptr;
+  unsigned char *hasdiag = nullptr, *bpm = nullptr;
+  unsigned *bmask = nullptr;
+  unsigned long long *blmb = nullptr;
+  auto cleanup = [&]() {
     dfree(cnt);
     dfree(err);
+    dfree(bptr);
+    dfree(hasdiag);
+    dfree(bmask);
+    dfree(blmb);
+    dfree(bpos);
+    dfree(bpm);
+  };
+  if ((rc = dalloc(ctx, &cnt, N)) || (rc = dalloc(ctx, &bptr, N + 1)) || (rc = dalloc(ctx, &err, 1)) ||
+      (rc = dalloc(ctx, &hasdiag, N))) {
+    cleanup();
     return rc;
   }
   const unsigned g = (unsigned)((N + 255) / 256);
   if (N > 0) {
     if (dpn == 1)
-      hipLaunchKernelGGL(k_tet_plan_count<1>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, cnt, tp->d_hasdiag);
+      hipLaunchKernelGGL(k_tet_plan_count<1>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, cnt, hasdiag);
     else
-      hipLaunchKernelGGL(k_tet_plan_count<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, cnt, tp->d_hasdiag);
+      hipLaunchKernelGGL(k_tet_plan_count<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, cnt, hasdiag);
   }
   int64_t nblk = 0;
-  if ((rc = exclusive_scan_i32(ctx, cnt, tp->d_bptr, N, &nblk))) {
-    dfree(cnt);
-    dfree(err);
+  if ((rc = exclusive_scan_i32(ctx, cnt, bptr, N, &nblk))) {
+    cleanup();
     return rc;
   }
-  dfree(cnt);
-  tp->nblocks = nblk;
-  if ((rc = dalloc(ctx, &tp->d_bmask, nblk)) || (rc = dalloc(ctx, &tp->d_blmb, nblk)) ||
-      (rc = dalloc(ctx, &tp->d_ipos, nblk * dpn)) || (rc = dalloc(ctx, &tp->d_ipm, nblk * dpn))) {
-    dfree(err);
+  tp->nitems = nblk;
+  if ((rc = dalloc(ctx, &bmask, nblk)) || (rc = dalloc(ctx, &blmb, nblk)) || (rc = dalloc(ctx, &bpos, nblk * dpn)) ||
+      (rc = dalloc(ctx, &bpm, nblk * dpn)) || (rc = dalloc(ctx, &tp->d_tib, (int64_t)tp->ntiles + 1)) ||
+      (rc = dalloc(ctx, &tp->d_mask, nblk)) || (rc = dalloc(ctx, &tp->d_lmb, nblk)) ||
+      (rc = dalloc(ctx, &tp->d_pos, nblk * dpn)) || (rc = dalloc(ctx, &tp->d_pmk, nblk))) {
+    cleanup();
     return rc;
   }
   TFM_CHECK_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), st));
   if (N > 0) {
-    if (dpn == 1)
+    const unsigned gt = (unsigned)((tp->ntiles + 255) / 256);
+    if (dpn == 1) {
       hipLaunchKernelGGL(k_tet_plan_fill<1>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_conn,
-                         mesh->d_n2e_ptr, mesh->d_n2e, tp->d_bptr, tp->d_hasdiag, tp->d_bmask, tp->d_blmb,
-                         tp->d_ipos, tp->d_ipm, err);
-    else
+                         mesh->d_n2e_ptr, mesh->d_n2e, bptr, hasdiag, bmask, blmb, bpos, bpm, err);
+      hipLaunchKernelGGL(k_tet_plan_items<1>, dim3(gt), dim3(256), 0, st, tp->ntiles, tp->d_tiles, tp->d_tpa,
+                         mesh->d_n2e_ptr, bptr, hasdiag, bmask, blmb, bpos, bpm, nblk, tp->d_tib, tp->d_mask,
+                         tp->d_lmb, tp->d_pos, tp->d_pmk);
+    } else {
       hipLaunchKernelGGL(k_tet_plan_fill<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_conn,
-                         mesh->d_n2e_ptr, mesh->d_n2e, tp->d_bptr, tp->d_hasdiag, tp->d_bmask, tp->d_blmb,
-                         tp->d_ipos, tp->d_ipm, err);
+                         mesh->d_n2e_ptr, mesh->d_n2e, bptr, hasdiag, bmask, blmb, bpos, bpm, err);
+      hipLaunchKernelGGL(k_tet_plan_items<3>, dim3(gt), dim3(256), 0, st, tp->ntiles, tp->d_tiles, tp->d_tpa,
+                         mesh->d_n2e_ptr, bptr, hasdiag, bmask, blmb, bpos, bpm, nblk, tp->d_tib, tp->d_mask,
+                         tp->d_lmb, tp->d_pos, tp->d_pmk);
+    }
   }
   int herr = 0;
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
-  dfree(err);
+  cleanup();
   if (herr) return fail(ctx, TFM_E_UNSUPPORTED, "pattern not covered by the tet plan");
   return TFM_OK;
 }
@@ -494,12 +501,11 @@ int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp
 void tet_plan_destroy(TetPlan *tp) {
   dfree(tp->d_tiles);
   dfree(tp->d_tpa);
-  dfree(tp->d_bptr);
-  dfree(tp->d_hasdiag);
-  dfree(tp->d_bmask);
-  dfree(tp->d_blmb);
-  dfree(tp->d_ipos);
-  dfree(tp->d_ipm);
+  dfree(tp->d_tib);
+  dfree(tp->d_mask);
+  dfree(tp->d_lmb);
+  dfree(tp->d_pos);
+  dfree(tp->d_pmk);
   *tp = TetPlan{};
 }
 
@@ -508,9 +514,9 @@ int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const TetPlan &tp, 
   if (tp.ntiles == 0) return TFM_OK;
   hipStream_t st = ctx->stream;
   dim3 g(xcd_grid(tp.ntiles)), t(kTetTP);
-#define TFM_TET(D, VT, V)                                                                                           \
-  hipLaunchKernelGGL((k_asm_tet<D, VT>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e_ptr, mesh->d_n2e, \
-                     tp.d_tiles, tp.d_tpa, tp.ntiles, P, tp.d_bptr, tp.d_hasdiag, tp.d_bmask, tp.d_blmb, tp.d_ipos, tp.d_ipm, V)
+#define TFM_TET(D, VT, V)                                                                                        \
+  hipLaunchKernelGGL((k_asm_tet<D, VT>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e, tp.d_tpa, tp.d_tib, \
+                     tp.ntiles, P, tp.d_mask, tp.d_lmb, tp.d_pos, tp.d_pmk, tp.nitems, V)
   if (dpn == 1) {
     if (v64) TFM_TET(1, double, v64);
     if (v32) TFM_TET(1, float, v32);

@@ -103,27 +103,23 @@ inline int cell_dim(int cell) {
 // context stream; returns the total through *total (synchronises).
 int exclusive_scan_i32(tfm_ctx *ctx, const int *d_in, int *d_out, int64_t n, int64_t *total);
 
-// P1-tet assembly plan (tfm_asm_tet.hip), built once per pattern:
-//   tiles  int32 [ntiles + 1]          first node of each tile
-//   tpa    int32 [ntiles + 1]          first (node, element) pair of each tile
-//   bptr   int32 [n_nodes + 1]         first row block of each node (its diagonal block
-//                                      first when it has one, hasdiag u8 [n_nodes])
-//   bmask  u32   [nblocks]             contributing (node, element) pairs of a block
-//   blmb   u64   [nblocks]             2-bit local index of the column node per pair
-//   ipos   int32 [nblocks * dpn]       CSR position of a (row, block)'s first entry,
-//                                      items ordered (node, row a, block)
-//   ipm    u8    [nblocks * dpn]       presence mask of the block's dpn columns
+// P1-tet assembly plan (tfm_asm_tet.hip), built once per pattern: the tile
+// schedule and one item record per row block (n, m) of each tile's nodes
+// (28 bytes: mask u32, lmb u64, dpn x pos int32, pmk u32), stored in the
+// order the numeric pass takes them.
 struct TetPlan {
-  int *d_tiles = nullptr;
-  int *d_tpa = nullptr;
+  int *d_tiles = nullptr;  // first node of each tile (ntiles + 1)
+  int *d_tpa = nullptr;    // first (node, element) pair of each tile
+  int *d_tib = nullptr;    // first item of each tile
   int ntiles = 0;
-  int *d_bptr = nullptr;
-  unsigned char *d_hasdiag = nullptr;  // the node has a diagonal block (stored first)
-  unsigned *d_bmask = nullptr;
-  unsigned long long *d_blmb = nullptr;
-  int *d_ipos = nullptr;
-  unsigned char *d_ipm = nullptr;
-  int64_t nblocks = 0;
+  // One item per row block (n, m) of a tile's nodes, in tile order: the
+  // tile's diagonal blocks first, then the off-diagonal ones in node and
+  // column order.
+  unsigned *d_mask = nullptr;            // which of n's pairs contribute
+  unsigned long long *d_lmb = nullptr;   // 2-bit local index of m in each pair
+  int *d_pos = nullptr;                  // [DPN][nitems] CSR position of row a's first present entry
+  unsigned *d_pmk = nullptr;             // presence masks of the DPN rows (4 bits each) | first pair of n << 16
+  int64_t nitems = 0;
 };
 
 // Assembly engine (tfm_assemble.hip).
