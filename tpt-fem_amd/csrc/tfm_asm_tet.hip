@@ -39,8 +39,8 @@
 
 namespace tfm {
 
-constexpr int kTetTP = 128;    // pairs (and threads) per tile
-constexpr int kTetNodes = 16;  // nodes per tile
+constexpr int kTetTP = 256;    // pairs (and threads) per tile
+constexpr int kTetNodes = 32;  // nodes per tile
 
 // FP32 entry of the precision form: vol * B_i^T D B_j (structural zeros dropped).
 __device__ __forceinline__ float bdb3f(const float *gi, const float *gj, int a, int b, float m, float cnu,
