@@ -11,10 +11,11 @@
 // Levels: every axis with at least two elements is coarsened 2:1 (an odd
 // count leaves one coarse cell spanning a single fine cell), until at most
 // kMg3MaxC DOFs remain.  Level 0 is the assembled matrix itself (its products
-// run on the batched solver's sliced copy); level 1 is the Galerkin product
-// P^T A P with trilinear P, formed from the CSR rows; every coarser level the
-// Galerkin product of the level above.  With trilinear interpolation each
-// coarse operator is a 27-point stencil of dpn x dpn blocks, stored per node.
+// run on the batched solver's sliced copy); each refresh also writes its CSR
+// rows into the 27-point stencil form of the coarser levels, and every level
+// l >= 1 is the Galerkin product P^T A P of level l - 1 with trilinear P.
+// With trilinear interpolation each coarse operator is a 27-point stencil of
+// dpn x dpn blocks, stored per node.
 // Smoothing: one weighted-Jacobi sweep before and one after the coarse
 // correction (the V-cycle stays symmetric positive definite); coarsest: the
 // dense inverse formed in LDS.  One right-hand side.
@@ -35,7 +36,7 @@ struct Mg3Level {
   int n[3] = {0, 0, 0};   // nodes per axis; node id (i n[1] + j) n[2] + k
   int co[3] = {0, 0, 0};  // coarsened from the level above along each axis
   int nn = 0;
-  double *S = nullptr;     // levels >= 1: [nn][27][dpn][dpn], offset d = (dx + 1) 9 + (dy + 1) 3 + dz + 1
+  double *S = nullptr;     // [nn][27][dpn][dpn], offset d = (dx + 1) 9 + (dy + 1) 3 + dz + 1 (level 0: when L > 0)
   double *dinv = nullptr;  // [nn dpn]
   double *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr;  // [nn dpn]
 };
@@ -106,16 +107,70 @@ __device__ __forceinline__ void gal_add(double *acc, int g0, int g1, int g2, dou
       for (int q = 0; q < 2; q++) acc[o0[u] + o1[t] + o2[q]] += v0[u] * v1[t] * v2[q];
 }
 
+// Level 0's CSR rows in the stencil form: entry (a, b) of the block at the
+// column node's offset d from the row node (absent couplings and fixed DOFs'
+// empty rows give 0).  One wave per node: the lanes stride over the node's
+// DPN contiguous rows (coalesced reads), scatter into the node's 27 DPN^2
+// values in LDS, then store them contiguously.  A box mesh couples a node
+// only to nodes at most one step away per axis.
+constexpr int kMg3CsN = 4;  // nodes (waves) per block
+template <int DPN>
+__global__ __launch_bounds__(64 * kMg3CsN) void km3_csr_stencil(G3 f, const int *__restrict__ rp,
+                                                                const int *__restrict__ ci,
+                                                                const double *__restrict__ v, double *__restrict__ S) {
+  constexpr int NV = 27 * DPN * DPN;
+  __shared__ double s_blk[kMg3CsN][NV];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nn = f.n0 * f.n1 * f.n2;
+  const int fn = blockIdx.x * kMg3CsN + w;
+  double *blk = s_blk[w];
+  for (int q = lane; q < NV; q += 64) blk[q] = 0.0;
+  __syncthreads();
+  if (fn < nn) {
+    // with >= 3 nodes on the two fastest axes the offset dx n1 n2 + dy n2 + dz
+    // of a neighbour decodes without division
+    const bool fast = f.n1 >= 3 && f.n2 >= 3;
+    int i0, i1, i2;
+    node3(fn, f, i0, i1, i2);
+    int rb[DPN + 1];
+#pragma unroll
+    for (int a = 0; a <= DPN; a++) rb[a] = rp[fn * DPN + a];
+    for (int e = rb[0] + lane; e < rb[DPN]; e += 64) {
+      int a = 0;
+#pragma unroll
+      for (int q = 1; q < DPN; q++)
+        if (e >= rb[q]) a = q;
+      const int col = ci[e], gn = col / DPN, b = col - gn * DPN;
+      int d0, d1, d2;
+      if (fast) {
+        int dd = gn - fn;
+        d0 = dd > f.n2 + 1 ? 1 : (dd < -(f.n2 + 1) ? -1 : 0);
+        dd -= d0 * f.n1 * f.n2;
+        d1 = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
+        d2 = dd - d1 * f.n2;
+      } else {
+        int g0, g1, g2;
+        node3(gn, f, g0, g1, g2);
+        d0 = g0 - i0;
+        d1 = g1 - i1;
+        d2 = g2 - i2;
+      }
+      if (d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1) continue;
+      blk[((d0 + 1) * 9 + (d1 + 1) * 3 + d2 + 1) * DPN * DPN + a * DPN + b] = v[e];
+    }
+  }
+  __syncthreads();
+  if (fn < nn)
+    for (int q = lane; q < NV; q += 64) S[(int64_t)fn * NV + q] = blk[q];
+}
+
 // Galerkin coarse stencil, one thread per (coarse node I, row a, column
 // component b): entry (a, b) of every block of I's 27-point stencil, sum over
 // the fine nodes i of I's support (weight w_iI) of w_iI A_(i,a),(:,b) P,
-// accumulated in the thread's LDS slice.  FROM_CSR: the fine operator is
-// level 0's CSR; else a stencil level.
-template <int DPN, bool FROM_CSR>
+// accumulated in the thread's LDS slice; Sf is the fine level's stencil.
+template <int DPN>
 __global__ __launch_bounds__(kMg3GB) void km3_galerkin(G3 f, G3 c, int co0, int co1, int co2,
-                                                       const int *__restrict__ rp, const int *__restrict__ ci,
-                                                       const double *__restrict__ v, const double *__restrict__ Sf,
-                                                       double *__restrict__ Sc) {
+                                                       const double *__restrict__ Sf, double *__restrict__ Sc) {
   __shared__ double s_acc[kMg3GB][27];
   double *acc = s_acc[threadIdx.x];
   const int item = thread_id();
@@ -146,37 +201,11 @@ __global__ __launch_bounds__(kMg3GB) void km3_galerkin(G3 f, G3 c, int co0, int 
         if (w2 == 0.0) continue;
         const double wi = w0 * w1 * w2;
         const int fn = (i0 * f.n1 + i1) * f.n2 + i2;
-        if (FROM_CSR) {
-          // a box mesh couples a node only to nodes at most one step away per
-          // axis, so with >= 3 nodes on the two fastest axes the column's
-          // offset (dx n1 n2 + dy n2 + dz) decodes without division
-          const bool fast = f.n1 >= 3 && f.n2 >= 3;
-          const int row = fn * DPN + a;
-          for (int e = rp[row]; e < rp[row + 1]; e++) {
-            const int col = ci[e], gn = col / DPN;
-            if (col - gn * DPN != b) continue;
-            int g0, g1, g2;
-            if (fast) {
-              int dd = gn - fn;
-              const int ox = dd > f.n2 + 1 ? 1 : (dd < -(f.n2 + 1) ? -1 : 0);
-              dd -= ox * f.n1 * f.n2;
-              const int oy = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
-              dd -= oy * f.n2;
-              g0 = i0 + ox;
-              g1 = i1 + oy;
-              g2 = i2 + dd;
-            } else {
-              node3(gn, f, g0, g1, g2);
-            }
-            gal_add(acc, g0, g1, g2, wi * v[e], f, co, I0, I1, I2);
-          }
-        } else {
-          for (int d = 0; d < 27; d++) {
-            const int g0 = i0 + d / 9 - 1, g1 = i1 + (d / 3) % 3 - 1, g2 = i2 + d % 3 - 1;
-            if (g0 < 0 || g0 >= f.n0 || g1 < 0 || g1 >= f.n1 || g2 < 0 || g2 >= f.n2) continue;
-            const double val = Sf[((int64_t)fn * 27 + d) * DPN * DPN + a * DPN + b];
-            if (val != 0.0) gal_add(acc, g0, g1, g2, wi * val, f, co, I0, I1, I2);
-          }
+        for (int d = 0; d < 27; d++) {
+          const int g0 = i0 + d / 9 - 1, g1 = i1 + (d / 3) % 3 - 1, g2 = i2 + d % 3 - 1;
+          if (g0 < 0 || g0 >= f.n0 || g1 < 0 || g1 >= f.n1 || g2 < 0 || g2 >= f.n2) continue;
+          const double val = Sf[((int64_t)fn * 27 + d) * DPN * DPN + a * DPN + b];
+          if (val != 0.0) gal_add(acc, g0, g1, g2, wi * val, f, co, I0, I1, I2);
         }
       }
     }
@@ -422,9 +451,8 @@ int mg3_create(tfm_ctx *ctx, const int n[3], int dpn, Mg3Data **out) {
   for (int l = 0; l <= mg->L && !rc; l++) {
     Mg3Level &v = mg->lv[l];
     const size_t nv = (size_t)v.nn * dpn;
-    if (l > 0 && ((rc = dalloc(ctx, &v.S, nv * 27 * dpn)) || (rc = dalloc(ctx, &v.b, nv)) ||
-                  (rc = dalloc(ctx, &v.x, nv))))
-      break;
+    if ((l > 0 || mg->L > 0) && (rc = dalloc(ctx, &v.S, nv * 27 * dpn))) break;
+    if (l > 0 && ((rc = dalloc(ctx, &v.b, nv)) || (rc = dalloc(ctx, &v.x, nv)))) break;
     if ((l < mg->L || l == 0) && (rc = dalloc(ctx, &v.dinv, nv))) break;
     if (l < mg->L && ((rc = dalloc(ctx, &v.x2, nv)) || (rc = dalloc(ctx, &v.r, nv)))) break;
   }
@@ -439,20 +467,20 @@ int mg3_create(tfm_ctx *ctx, const int n[3], int dpn, Mg3Data **out) {
 }
 
 template <int DPN>
-static void mg3_update_t(Mg3Data *mg, const int *rp, const int *ci, const double *v, hipStream_t st) {
+static int mg3_update_t(Mg3Data *mg, const int *rp, const int *ci, const double *v, hipStream_t st) {
   const int nrows = mg->lv[0].nn * DPN;
   hipLaunchKernelGGL(km3_weights, dim3(g3grid(nrows)), dim3(kMg3B), 0, st, nrows, (const double *)mg->lv[0].dinv,
                      mg->d_wd0);
+  if (mg->L > 0) {
+    const Mg3Level &l0 = mg->lv[0];
+    hipLaunchKernelGGL((km3_csr_stencil<DPN>), dim3((unsigned)((l0.nn + kMg3CsN - 1) / kMg3CsN)),
+                       dim3(64 * kMg3CsN), 0, st, g3(l0), rp, ci, v, l0.S);
+  }
   for (int l = 1; l <= mg->L; l++) {
     const Mg3Level &f = mg->lv[l - 1], &c = mg->lv[l];
     const unsigned gg = g3grid((int64_t)c.nn * DPN * DPN, kMg3GB);
-    if (l == 1)
-      hipLaunchKernelGGL((km3_galerkin<DPN, true>), dim3(gg), dim3(kMg3GB), 0, st, g3(f), g3(c), c.co[0], c.co[1],
-                         c.co[2], rp, ci, v, (const double *)nullptr, c.S);
-    else
-      hipLaunchKernelGGL((km3_galerkin<DPN, false>), dim3(gg), dim3(kMg3GB), 0, st, g3(f), g3(c), c.co[0],
-                         c.co[1], c.co[2], (const int *)nullptr, (const int *)nullptr, (const double *)nullptr,
-                         (const double *)f.S, c.S);
+    hipLaunchKernelGGL((km3_galerkin<DPN>), dim3(gg), dim3(kMg3GB), 0, st, g3(f), g3(c), c.co[0], c.co[1], c.co[2],
+                       (const double *)f.S, c.S);
     if (l < mg->L)
       hipLaunchKernelGGL((km3_dinv<DPN>), dim3(g3grid((int64_t)c.nn * DPN)), dim3(kMg3B), 0, st, c.nn,
                          (const double *)c.S, c.dinv);
@@ -460,6 +488,7 @@ static void mg3_update_t(Mg3Data *mg, const int *rp, const int *ci, const double
   const Mg3Level &cl = mg->lv[mg->L];
   hipLaunchKernelGGL((km3_coarse_inv<DPN>), dim3(1), dim3(256), 0, st, g3(cl),
                      (const double *)(mg->L > 0 ? cl.S : nullptr), rp, ci, v, mg->nc, mg->d_inv);
+  return TFM_OK;
 }
 
 // New matrix values (level 0's CSR, fixed rows empty, and its Jacobi
@@ -469,8 +498,8 @@ int mg3_update(Mg3Data *mg, const int *rp, const int *ci, const double *v, const
   hipStream_t st = mg->ctx->stream;
   TFM_CHECK_HIP(mg->ctx, hipMemcpyAsync(mg->lv[0].dinv, dinv0, sizeof(double) * mg->lv[0].nn * mg->dpn,
                                         hipMemcpyDeviceToDevice, st));
-  if (mg->dpn == 3) mg3_update_t<3>(mg, rp, ci, v, st);
-  else mg3_update_t<1>(mg, rp, ci, v, st);
+  const int rc = mg->dpn == 3 ? mg3_update_t<3>(mg, rp, ci, v, st) : mg3_update_t<1>(mg, rp, ci, v, st);
+  if (rc) return rc;
   TFM_CHECK_HIP(mg->ctx, hipGetLastError());
   return TFM_OK;
 }
