@@ -18,15 +18,16 @@
 // or barriers before the element math starts.
 //
 // Numeric pass: tiles of consecutive nodes (<= kTetTP pairs).  One lane per
-// (node, element) pair computes the element map once and the pair's whole row
-// block (4 x DPN x DPN entries) into LDS; then one lane per (row, block) adds
-// the contributions of the block's pairs in ascending element order -- the
-// reference's duplicate-summation order (Coo::to_csr) -- and writes them at
-// the planned positions.  No hashing, searching or atomics.
+// (node, element) pair computes the element map once; then, one DOF row a at
+// a time, each pair writes its 4 x DPN entries of row a into LDS and one lane
+// per row block adds the contributions of the block's pairs in ascending
+// element order -- the reference's duplicate-summation order (Coo::to_csr)
+// -- and writes them at the planned positions.  No hashing, searching or
+// atomics.
 //
 // Entry arithmetic is the reference's (tfm_elem.h: bdb3, and quad4 for the
-// sum of the four equal quadrature contributions); this translation unit is built without FMA
-// contraction.
+// sum of the four equal quadrature contributions); this translation unit is
+// built without FMA contraction.
 #include <hip/hip_runtime.h>
 
 #include <climits>
