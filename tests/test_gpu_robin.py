@@ -64,7 +64,7 @@ def test_solve_poisson_with_neumann_and_robin(ctx):
     rhs = O.apply_neumann(c, O.TET, e, 1, 0.3, O.poisson_rhs(c, O.TET, e, 1.0))
     ur = O.solve_with_dirichlet(coo, rhs, bcs)
     assert np.abs(u - ur).max() <= 1e-10 * np.abs(ur).max()
-    # without Robin/Neumann the driver is solve_poisson
+    # without Robin/Neumann terms solve_poisson_nr reduces to solve_poisson
     u0 = T.solve_poisson_nr(m, 1.0, 2, 1.0, bcs)
     ur0 = O.solve_with_dirichlet(O.assemble(c, e, O.make_op(O.TET, 3, 0, coef=1.0), 1),
                                  O.poisson_rhs(c, O.TET, e, 1.0), bcs)
