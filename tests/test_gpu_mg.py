@@ -170,11 +170,14 @@ def box_system(ctx, n, dpn, seed, jitter=0.2, faces=((0, 0.0),), flags=None):
     return m, s, (coords, conn, oop, rhs, bcs)
 
 
-@pytest.mark.parametrize("case", [((2, 2, 2), 3), ((6, 5, 4), 1), ((9, 7, 4), 1), ((6, 6, 6), 3), ((7, 5, 9), 3)])
+@pytest.mark.parametrize("case", [((2, 2, 2), 3), ((6, 5, 4), 1), ((9, 7, 4), 1), ((6, 6, 6), 3), ((7, 5, 9), 3),
+                                  ((12, 1, 10), 1), ((10, 9, 1), 3)])
 def test_mg_box_matches_direct(ctx, case):
     """One level (the coarse inverse solves the system), even and odd
-    coarsening on all three axes, Poisson and elasticity: the multigrid CG
-    solution equals the oracle's dense-LU solve to 1e-9."""
+    coarsening on all three axes, an axis of one cell (never coarsened; with
+    two nodes on a fast axis the stencil conversion decodes neighbours by
+    division), Poisson and elasticity: the multigrid CG solution equals the
+    oracle's dense-LU solve to 1e-9."""
     import tptfem as T
 
     n, dpn = case
