@@ -277,7 +277,7 @@ def main():
                     help="residual tolerances tried, loosest first, until the error target is met")
     ap.add_argument("--inner-rtol", type=float, default=0.3,
                     help="FP32 residual reduction between FP64 residual replacements")
-    ap.add_argument("--cpu-n", type=int, default=32, help="cells per axis of the CPU-baseline sample")
+    ap.add_argument("--cpu-n", type=int, default=48, help="cells per axis of the CPU-baseline sample (about 10 s on one core)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve", action="store_true")
     ap.add_argument("--workload", choices=["c3", "c4", "c5", "simp"], default="c3",
