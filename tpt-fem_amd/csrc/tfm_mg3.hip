@@ -80,31 +80,27 @@ __global__ void km3_dinv(int nn, const double *__restrict__ S, double *__restric
   dinv[t] = d > 0.0 ? 1.0 / d : 0.0;
 }
 
-// Adds w to the Galerkin row of coarse node (I0, I1, I2) for the fine node
-// (g0, g1, g2): every interpolation parent of the fine node takes its share
-// at its stencil offset d from the coarse node, acc[d].  The fine node is at
-// most two fine steps from I's position, so each parent is at most one
-// coarse step from I; a single parent appears twice, the second time with
-// weight 0, so all eight terms are added without branching.
-__device__ __forceinline__ void gal_add(double *acc, int g0, int g1, int g2, double w, const G3 &f,
-                                        const int (&co)[3], int I0, int I1, int I2) {
-  int A0, B0, A1, B1, A2, B2;
-  double x0, x1, x2;
-  interp1(g0, f.n0 - 1, co[0], A0, B0, x0);
-  interp1(g1, f.n1 - 1, co[1], A1, B1, x1);
-  interp1(g2, f.n2 - 1, co[2], A2, B2, x2);
-  const int o0[2] = {(A0 - I0 + 1) * 9, (B0 - I0 + 1) * 9};
-  const int o1[2] = {(A1 - I1 + 1) * 3, (B1 - I1 + 1) * 3};
-  const int o2[2] = {A2 - I2 + 1, B2 - I2 + 1};
-  const double v0[2] = {w * (1.0 - x0), w * x0};
-  const double v1[2] = {1.0 - x1, x1};
-  const double v2[2] = {1.0 - x2, x2};
+// Interpolation parents of the fine positions i - 1, i, i + 1 along one axis,
+// as stencil offsets from the coarse node I (times the axis stride, 9, 3 or
+// 1) and weights.  The fine node i is at most one fine step from I's
+// position, so every parent is at most one coarse step from I; a single
+// parent appears twice, the second time with weight 0, so all eight terms of
+// a coupling are added without branching.
+struct GalAxis {
+  int o[3][2];
+  double w[3][2];
+};
+__device__ __forceinline__ void gal_axis(int i, int nel, int co, int I, int stride, GalAxis &p) {
 #pragma unroll
-  for (int u = 0; u < 2; u++)
-#pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-      for (int q = 0; q < 2; q++) acc[o0[u] + o1[t] + o2[q]] += v0[u] * v1[t] * v2[q];
+  for (int q = 0; q < 3; q++) {
+    int A, B;
+    double x;
+    interp1(i + q - 1, nel, co, A, B, x);
+    p.o[q][0] = (A - I + 1) * stride;
+    p.o[q][1] = (B - I + 1) * stride;
+    p.w[q][0] = 1.0 - x;
+    p.w[q][1] = x;
+  }
 }
 
 // Level 0's CSR rows in the stencil form: entry (a, b) of the block at the
@@ -178,7 +174,6 @@ __global__ __launch_bounds__(kMg3GB) void km3_galerkin(G3 f, G3 c, int co0, int 
   if (item >= ncn * DPN * DPN) return;  // no barrier below: each thread owns its slice
   const int b = item % DPN, ia = item / DPN;
   const int I = ia / DPN, a = ia % DPN;
-  const int co[3] = {co0, co1, co2};
   for (int q = 0; q < 27; q++) acc[q] = 0.0;
   int I0, I1, I2;
   node3(I, c, I0, I1, I2);
@@ -201,11 +196,25 @@ __global__ __launch_bounds__(kMg3GB) void km3_galerkin(G3 f, G3 c, int co0, int 
         if (w2 == 0.0) continue;
         const double wi = w0 * w1 * w2;
         const int fn = (i0 * f.n1 + i1) * f.n2 + i2;
+        GalAxis p0, p1, p2;  // parents of the neighbours i + d - 1 on each axis
+        gal_axis(i0, f.n0 - 1, co0, I0, 9, p0);
+        gal_axis(i1, f.n1 - 1, co1, I1, 3, p1);
+        gal_axis(i2, f.n2 - 1, co2, I2, 1, p2);
+#pragma unroll
         for (int d = 0; d < 27; d++) {
-          const int g0 = i0 + d / 9 - 1, g1 = i1 + (d / 3) % 3 - 1, g2 = i2 + d % 3 - 1;
+          const int q0 = d / 9, q1 = (d / 3) % 3, q2 = d % 3;
+          const int g0 = i0 + q0 - 1, g1 = i1 + q1 - 1, g2 = i2 + q2 - 1;
           if (g0 < 0 || g0 >= f.n0 || g1 < 0 || g1 >= f.n1 || g2 < 0 || g2 >= f.n2) continue;
           const double val = Sf[((int64_t)fn * 27 + d) * DPN * DPN + a * DPN + b];
-          if (val != 0.0) gal_add(acc, g0, g1, g2, wi * val, f, co, I0, I1, I2);
+          if (val == 0.0) continue;
+          const double w = wi * val;
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+#pragma unroll
+              for (int q = 0; q < 2; q++)
+                acc[p0.o[q0][u] + p1.o[q1][t] + p2.o[q2][q]] += w * p0.w[q0][u] * p1.w[q1][t] * p2.w[q2][q];
         }
       }
     }
