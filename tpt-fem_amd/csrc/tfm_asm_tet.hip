@@ -40,7 +40,7 @@
 
 namespace tfm {
 
-constexpr int kTetTP = 256;    // pairs (and threads) per tile
+constexpr int kTetTP = 192;    // pairs (and threads) per tile
 constexpr int kTetNodes = 32;  // nodes per tile
 
 // FP32 entry of the precision form: vol * B_i^T D B_j (structural zeros dropped).
