@@ -778,7 +778,7 @@ struct BBatch {
   int P = 0, NR = 1, bs = 1, nblk = 0, nslices = 0;
   int64_t nblkst = 0;
   int *blk_prob = nullptr, *blk_s0 = nullptr, *blk_s1 = nullptr, *prob_blk0 = nullptr, *prob_nblk = nullptr;
-  int *sb = nullptr, *soff = nullptr, *bcol = nullptr, *map = nullptr;
+  int *sb = nullptr, *soff = nullptr, *bcol = nullptr, *map = nullptr;  // map: slot -> CSR position (-1: none)
   double *v64 = nullptr;
   float *v32 = nullptr;
   double *part = nullptr;
@@ -887,7 +887,7 @@ static int bb_setup(tfm_system *s, int P, const int64_t *dof_off, int NR, BBatch
   default: bb_launch_build<1>(s, B, nullptr, nullptr, (int)nbr, st, 1); break;
   }
   TFM_CHECK_HIP(ctx, hipGetLastError());
-  return TFM_OK;
+  return bs_invert_map(ctx, &B.map, s->nnz, B.nblkst * bs * bs);  // B.map: slot -> CSR position
 }
 
 template <typename VT>
@@ -897,10 +897,9 @@ static int bb_values(tfm_system *s, BBatch &B, const VT *src, VT **dst) {
   const int64_t nv = B.nblkst * B.bs * B.bs;
   int rc = *dst ? TFM_OK : dalloc(ctx, dst, nv);
   if (rc) return rc;
-  TFM_CHECK_HIP(ctx, hipMemsetAsync(*dst, 0, sizeof(VT) * nv, st));
-  if (s->nnz)
-    hipLaunchKernelGGL((k_bs_values<VT, VT>), dim3((unsigned)((s->nnz + 255) / 256)), dim3(256), 0, st, s->nnz,
-                       B.map, src, *dst);
+  if (nv > 0)
+    hipLaunchKernelGGL((k_bs_values<VT, VT>), dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, nv, B.map, src,
+                       *dst);
   TFM_CHECK_HIP(ctx, hipGetLastError());
   return TFM_OK;
 }

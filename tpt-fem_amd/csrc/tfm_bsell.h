@@ -13,6 +13,8 @@
 #include <climits>
 #include <cstdint>
 
+#include "tfm_internal.h"
+
 namespace tfm {
 
 // Smallest column block among the current heads of a block row's BS sorted rows.
@@ -99,11 +101,38 @@ __global__ void k_bs_fill(int nbr, int nslices, const int *__restrict__ sb, cons
   for (; k < w; k++) bcol[o + k * 64 + lane] = 0;
 }
 
-// Values of the CSR into their slots (slots of absent entries stay zero).
-template <typename VT, typename ST>
-__global__ void k_bs_values(int64_t nnz, const int *__restrict__ map, const ST *__restrict__ src, VT *__restrict__ dst) {
+// Turns the fill's CSR -> slot map into the slot -> CSR map (slots of absent
+// entries keep -1, the caller's clear).
+static __global__ void k_bs_invert(int64_t nnz, const int *__restrict__ map, int *__restrict__ inv) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < nnz) dst[map[j]] = (VT)src[j];
+  if (j < nnz) inv[map[j]] = (int)j;
+}
+
+// Values of the CSR into their slots, one thread per slot (coalesced stores;
+// slots of absent entries get zero).
+template <typename VT, typename ST>
+__global__ void k_bs_values(int64_t nslots, const int *__restrict__ inv, const ST *__restrict__ src,
+                            VT *__restrict__ dst) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nslots) {
+    const int j = inv[q];
+    dst[q] = j >= 0 ? (VT)src[j] : (VT)0;
+  }
+}
+
+// Replaces *map (CSR -> slot, nnz entries) by its inverse (nslots entries).
+static inline int bs_invert_map(tfm_ctx *ctx, int **map, int64_t nnz, int64_t nslots) {
+  int *inv = nullptr;
+  int rc = dalloc(ctx, &inv, nslots > 0 ? nslots : 1);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(inv, 0xff, sizeof(int) * (nslots > 0 ? nslots : 1), st));
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_bs_invert, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, nnz, (const int *)*map, inv);
+  TFM_CHECK_HIP(ctx, hipGetLastError());
+  dfree(*map);  // waits for the inversion
+  *map = inv;
+  return TFM_OK;
 }
 
 }  // namespace tfm

@@ -177,7 +177,7 @@ struct BSell {
   int64_t nblk = 0, nnz = 0;
   int *d_soff = nullptr;
   int *d_bcol = nullptr;
-  int *d_map = nullptr;
+  int *d_map = nullptr;  // slot -> CSR position (-1: a padding or absent entry)
   double *d_v64 = nullptr;
   float *d_v32 = nullptr;
 };

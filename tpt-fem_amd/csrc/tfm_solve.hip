@@ -777,6 +777,10 @@ static int bsell_build(tfm_system *s) {
     break;
   }
   TFM_CHECK_HIP(ctx, hipGetLastError());
+  if ((rc = bs_invert_map(ctx, &B.d_map, s->nnz, nblk * bs * bs))) {
+    bsell_destroy(&B);
+    return rc;
+  }
   B.bs = bs;
   B.nbr = (int)nbr;
   B.nslices = (int)nsl;
@@ -797,10 +801,9 @@ static int bsell_values(tfm_system *s, const VT *src, VT **dst) {
   if (!*dst) {
     int rc = dalloc(ctx, dst, nv);
     if (rc) return rc;
-    TFM_CHECK_HIP(ctx, hipMemsetAsync(*dst, 0, sizeof(VT) * nv, st));
   }
-  hipLaunchKernelGGL((k_bs_values<VT, VT>), dim3((unsigned)((B.nnz + 255) / 256)), dim3(256), 0, st, B.nnz, B.d_map,
-                     src, *dst);
+  hipLaunchKernelGGL((k_bs_values<VT, VT>), dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, nv, B.d_map, src,
+                     *dst);
   TFM_CHECK_HIP(ctx, hipGetLastError());
   return TFM_OK;
 }
