@@ -77,6 +77,12 @@ int tfm_ctx_synchronize(tfm_ctx *ctx);
  * reference counterpart (measurement utility). */
 int tfm_ctx_timer_begin(tfm_ctx *ctx);
 int tfm_ctx_timer_end(tfm_ctx *ctx, double *ms);
+/* The library's device prefix sum of n non-negative int32 counts (host
+ * buffer), as every pattern build runs it: *total = their sum, or
+ * TFM_E_UNSUPPORTED when the sum reaches 2^31 - 1 (row pointers, slot and
+ * block indices are int32).  No reference counterpart (diagnostic of the
+ * range guard). */
+int tfm_debug_scan_total(tfm_ctx *ctx, const int32_t *counts, int64_t n, int64_t *total);
 
 /* ---- meshes (crates/tpt-fem-mesh/src/lib.rs:39-148) ---------------------- */
 /* Mesh with one cell type; coords node-major [n_nodes x pdim], conn
@@ -129,6 +135,38 @@ int tfm_load_source(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, double f
 /* rhs[node*dim+a] += elasticity_body_vector with constant b (elasticity/src/lib.rs:312-348). */
 int tfm_load_body(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, const double *b,
                   double *rhs);
+/* Closure-valued loads.  The reference's loads take closures: the source
+ * s(x) of poisson_source_vector (thermal/src/lib.rs:163-196), the body force
+ * b(x) of elasticity_body_vector (elasticity/src/lib.rs:312-348), the flux
+ * g(x, n) of apply_neumann_order and the coefficient h(x, n) of
+ * apply_robin_order (assembly/src/lib.rs:601-753).  Across the C ABI a
+ * closure becomes its values at the points the reference calls it at: the
+ * library hands out those points, the caller evaluates, the library
+ * integrates in the reference's order (bit-identical to a closure returning
+ * the same values).
+ * Element points: xq [n_elem][nq][pdim] (xq NULL: *nq only), positions formed
+ * as the reference forms them (x[d] = sum_k N_k x_k[d], node order). */
+int tfm_mesh_quad_points(const tfm_mesh *mesh, int quad_order, double *xq, int64_t *nq);
+/* rhs[node] += poisson_source_vector from per-point values sq [n_elem][nq]. */
+int tfm_load_source_qp(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, const double *sq,
+                       double *rhs);
+/* rhs[node*dim+a] += elasticity_body_vector from per-point values bq [n_elem][nq][dim]. */
+int tfm_load_body_qp(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, const double *bq,
+                     double *rhs);
+/* Boundary points: every boundary-face quadrature point (boundary_faces order,
+ * assembly/src/lib.rs:563-583, then the face rule of `order`) with its
+ * position and outward unit normal, x and normal [count][pdim] (either NULL:
+ * not written).  Point faces of Line2 cells have no points. */
+int tfm_mesh_boundary_points(const tfm_mesh *mesh, int order, double *x, double *normal,
+                             int64_t *count);
+/* apply_neumann_order(mesh, dpn, flux, rhs, order) from per-point values gq [count]. */
+int tfm_load_neumann_qp(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, int order, const double *gq,
+                        double *rhs);
+/* to_csr() of assemble(mesh, dpn, op) with apply_robin_order(mesh, dpn, coeff,
+ * coo, order)'s triplets from per-point values hq [count]. */
+int tfm_assemble_robin_qp(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
+                          int order, const double *hq, tfm_matrix **out);
+
 /* apply_neumann (assembly/src/lib.rs:591-667) with constant flux g tiled over dpn. */
 int tfm_load_neumann(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, double g, double *rhs);
 /* Traction t[dpn] on the plane x[axis] == coord (|.| <= tol): one apply_neumann
@@ -159,8 +197,16 @@ int tfm_system_from_matrix(tfm_ctx *ctx, const tfm_matrix *k, int64_t n, const d
                            tfm_system **out);
 /* Numeric re-assembly on the existing pattern (the hot path).  op may carry
  * new coefficients (coef_e, rho), or be NULL to reuse the device-resident
- * ones; the pattern must stay valid for it.  which: TFM_SYS_FP64 and/or
- * TFM_SYS_FP32 value arrays to refill (0 = every array the system holds). */
+ * ones.  which: TFM_SYS_FP64 and/or TFM_SYS_FP32 value arrays to refill (0 =
+ * every array the system holds).
+ * The pattern was built from the creation coefficients: an element whose
+ * coefficient was exactly 0 then has no entries in it, so giving it a
+ * non-zero coefficient returns TFM_E_INVALID (create the system again); a
+ * coefficient that becomes 0 leaves explicit zero entries.
+ * FP32 values last filled with older coefficients than the FP64 values are
+ * stale: TFM_SOLVE_MIXED refuses them (TFM_E_INVALID) and the multigrid CG
+ * runs its fine passes on the FP64 values instead; an FP32 pass with op NULL
+ * brings them up to date. */
 int tfm_system_assemble(tfm_system *sys, const tfm_operator *op, int which);
 int tfm_system_info(const tfm_system *sys, int64_t *n_free, int64_t *nnz, int64_t *n_dof);
 /* ReducedSystem {kred (as its to_csr()), fred, free}. */
@@ -298,6 +344,19 @@ int tfm_solve_poisson_nr(tfm_ctx *ctx, const tfm_mesh *mesh, double kappa, int q
                          const double *neumann_g, const double *robin_h, int64_t n_bc,
                          const int64_t *bc_dof, const double *bc_val, const tfm_solve_opts *opts,
                          double *u);
+/* solve_poisson (thermal/src/lib.rs:206-240) with closure-valued source,
+ * Neumann flux and Robin coefficient given at their points: sq [n_elem][nq]
+ * (tfm_mesh_quad_points), neumann_gq / robin_hq [count] at
+ * tfm_mesh_boundary_points(mesh, 2, ...) (FACE_QUAD_ORDER), NULL: None. */
+int tfm_solve_poisson_qp(tfm_ctx *ctx, const tfm_mesh *mesh, double kappa, int quad_order,
+                         const double *sq, const double *neumann_gq, const double *robin_hq,
+                         int64_t n_bc, const int64_t *bc_dof, const double *bc_val,
+                         const tfm_solve_opts *opts, double *u);
+/* solve_elasticity (elasticity/src/lib.rs:355-381) with the body-force
+ * closure's values bq [n_elem][nq][dim] at tfm_mesh_quad_points. */
+int tfm_solve_elasticity_qp(tfm_ctx *ctx, const tfm_mesh *mesh, int model, double young, double nu,
+                            int quad_order, const double *bq, int64_t n_bc, const int64_t *bc_dof,
+                            const double *bc_val, const tfm_solve_opts *opts, double *u);
 /* solve_elasticity (elasticity/src/lib.rs:355-381) with constant body force. */
 int tfm_solve_elasticity(tfm_ctx *ctx, const tfm_mesh *mesh, int model, double young,
                          double nu, int quad_order, const double *body, int64_t n_bc,

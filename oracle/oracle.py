@@ -97,6 +97,15 @@ def _declare(L):
         "orc_apply_neumann": (None, [I64, INT, P, I64, INT, P, INT, D, INT, P]),
         "orc_plane_traction": (None, [I64, INT, P, I64, INT, P, INT, INT, D, D, P, INT, P]),
         "orc_poisson_rhs": (None, [I64, INT, P, I64, INT, P, D, INT, P]),
+        "orc_qp_coords": (INT, [INT, INT, P, INT, P]),
+        "orc_boundary_points": (I64, [I64, INT, P, I64, INT, P, INT, P, P]),
+        "orc_apply_neumann_qp": (None, [I64, INT, P, I64, INT, P, INT, P, INT, P]),
+        "orc_apply_robin_qp": (I64, [I64, INT, P, I64, INT, P, INT, P, INT, P, P, P]),
+        "orc_poisson_source_vector_qp": (None, [INT, INT, P, P, INT, P]),
+        "orc_elasticity_body_vector": (None, [INT, INT, P, P, INT, P]),
+        "orc_body_rhs": (None, [I64, INT, P, I64, INT, P, P, INT, P]),
+        "orc_source_rhs_qp": (None, [I64, INT, P, I64, INT, P, P, INT, P]),
+        "orc_solve_elasticity": (INT, [I64, INT, P, I64, INT, P, INT, D, D, INT, P, I64, P, P, P]),
         "orc_solve_poisson": (INT, [I64, INT, P, I64, INT, P, D, INT, D, I64, P, P, P]),
         "orc_jitter_interior": (None, [I64, I64, I64, D, C.c_uint64, P]),
         "orc_loguniform": (None, [I64, D, D, C.c_uint64, P]),
@@ -185,6 +194,54 @@ def poisson_source_vector(cell, phys, f_const=1.0, order=2):
     f = np.zeros(NODES[cell])
     lib().orc_poisson_source_vector(cell, phys.shape[1], _p(phys), f_const, order, _p(f))
     return f
+
+
+def qp_coords(cell, phys, order=2):
+    """Quadrature-point positions a source/body closure is called at (nq x pdim)."""
+    phys = f64(phys)
+    x = np.zeros((125, phys.shape[1]))
+    nq = lib().orc_qp_coords(cell, phys.shape[1], _p(phys), order, _p(x))
+    return x[:nq].copy()
+
+
+def mesh_qp_coords(coords, cell, conn, order=2):
+    """qp_coords of every element: (nelem, nq, pdim)."""
+    coords, conn = f64(coords), i64(conn)
+    return np.array([qp_coords(cell, coords[c], order) for c in conn]).reshape(len(conn), -1, coords.shape[1])
+
+
+def poisson_source_vector_qp(cell, phys, sq, order=2):
+    """poisson_source_vector with the source closure's values at the quadrature points."""
+    phys, sq = f64(phys), f64(sq)
+    f = np.zeros(NODES[cell])
+    lib().orc_poisson_source_vector_qp(cell, phys.shape[1], _p(phys), _p(sq), order, _p(f))
+    return f
+
+
+def elasticity_body_vector(cell, phys, bq, order=2):
+    """elasticity_body_vector with the body-force closure's values bq (nq x dim)."""
+    phys, bq = f64(phys), f64(bq)
+    f = np.zeros(NODES[cell] * DIMS[cell])
+    lib().orc_elasticity_body_vector(cell, phys.shape[1], _p(phys), _p(bq), order, _p(f))
+    return f
+
+
+def body_rhs(coords, cell, conn, bq, order=2):
+    """solve_elasticity's rhs: body vectors scattered in element order; bq (nelem, nq, dim)."""
+    coords, conn, bq = f64(coords), i64(conn), f64(bq)
+    rhs = np.zeros(coords.shape[0] * DIMS[cell])
+    lib().orc_body_rhs(coords.shape[0], coords.shape[1], _p(coords), conn.shape[0], cell, _p(conn), _p(bq),
+                       order, _p(rhs))
+    return rhs
+
+
+def source_rhs_qp(coords, cell, conn, sq, order=2):
+    """solve_poisson's rhs with per-point source values sq (nelem, nq)."""
+    coords, conn, sq = f64(coords), i64(conn), f64(sq)
+    rhs = np.zeros(coords.shape[0])
+    lib().orc_source_rhs_qp(coords.shape[0], coords.shape[1], _p(coords), conn.shape[0], cell, _p(conn), _p(sq),
+                            order, _p(rhs))
+    return rhs
 
 
 def elasticity_element_matrix(cell, phys, model, young, nu, order=2):
@@ -431,6 +488,23 @@ def solve_poisson(coords, cell, conn, kappa, order, f_const, bcs):
     return u
 
 
+def solve_elasticity(coords, cell, conn, model, young, nu, order, bq, bcs):
+    """solve_elasticity (crates/tpt-fem-elasticity/src/lib.rs:355-381); bq (nelem, nq, dim)
+    holds the body-force closure's values at every element quadrature point."""
+    coords, conn, bq = f64(coords), i64(conn), f64(bq)
+    n = coords.shape[0] * DIMS[cell]
+    bd = i64([b[0] for b in bcs]) if bcs else np.zeros(1, np.int64)
+    bv = f64([b[1] for b in bcs]) if bcs else np.zeros(1)
+    u = np.zeros(n)
+    rc = lib().orc_solve_elasticity(coords.shape[0], coords.shape[1], _p(coords), conn.shape[0], cell, _p(conn),
+                                    model, young, nu, order, _p(bq), len(bcs), _p(bd), _p(bv), _p(u))
+    if rc == -5:
+        raise ValueError("model/dimension mismatch")
+    if rc != 0:
+        raise SolveError(_SOLVE_MSG[rc])
+    return u
+
+
 # ---- SIMP topology optimization (crates/tpt-fem-topopt/src/lib.rs) -------------
 def topopt_quad_form(ke0, u, nodes):
     """element_quad_form (:241-254)."""
@@ -557,6 +631,39 @@ def apply_robin(coords, cell, conn, dpn, h, axis=-1, coord=0.0, tol=0.0, order=2
     r, c, v = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(cap)
     t = lib().orc_apply_robin(coords.shape[0], coords.shape[1], _p(coords), conn.shape[0], cell, _p(conn), dpn, h,
                               axis, coord, tol, order, _p(r), _p(c), _p(v))
+    return Coo(r[:t], c[:t], v[:t])
+
+
+def boundary_points(coords, cell, conn, order=2):
+    """Every boundary-face quadrature point's position and outward unit normal --
+    the (x, n) apply_neumann_order / apply_robin_order pass to their closure --
+    in boundary_faces order, points in rule order: two (count, pdim) arrays."""
+    coords, conn = f64(coords), i64(conn)
+    pdim = coords.shape[1]
+    cap = max(1, conn.shape[0] * 6 * 25)
+    x, nrm = np.zeros((cap, pdim)), np.zeros((cap, pdim))
+    m = lib().orc_boundary_points(coords.shape[0], pdim, _p(coords), conn.shape[0], cell, _p(conn), order, _p(x),
+                                  _p(nrm))
+    return x[:m].copy(), nrm[:m].copy()
+
+
+def apply_neumann_qp(coords, cell, conn, dpn, gq, rhs, order=2):
+    """apply_neumann_order with the flux closure's values at boundary_points."""
+    coords, conn, gq = f64(coords), i64(conn), f64(gq)
+    rhs = f64(rhs).copy()
+    lib().orc_apply_neumann_qp(coords.shape[0], coords.shape[1], _p(coords), conn.shape[0], cell, _p(conn), dpn,
+                               _p(gq), order, _p(rhs))
+    return rhs
+
+
+def apply_robin_qp(coords, cell, conn, dpn, hq, order=2):
+    """apply_robin_order with the coefficient closure's values at boundary_points -> Coo."""
+    coords, conn, hq = f64(coords), i64(conn), f64(hq)
+    fn = {LINE: 1, TRI: 2, QUAD: 2, TET: 3, HEX: 4}[cell]
+    cap = max(1, conn.shape[0] * 6 * 25 * fn * fn * dpn)
+    r, c, v = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(cap)
+    t = lib().orc_apply_robin_qp(coords.shape[0], coords.shape[1], _p(coords), conn.shape[0], cell, _p(conn), dpn,
+                                 _p(hq), order, _p(r), _p(c), _p(v))
     return Coo(r[:t], c[:t], v[:t])
 
 

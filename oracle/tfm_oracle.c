@@ -283,6 +283,58 @@ void orc_poisson_source_vector(int cell, int pdim, const double *phys,
   }
 }
 
+/* Physical quadrature-point positions of one element, formed as
+ * poisson_source_vector and elasticity_body_vector form the argument of the
+ * closure (crates/tpt-fem-thermal/src/lib.rs:180-185,
+ * crates/tpt-fem-elasticity/src/lib.rs:331-336): x[d] = sum_k ns[k]*phys[k][d]
+ * accumulated from 0.0 in node order.  x: nq x pdim.  Returns nq.          */
+int orc_qp_coords(int cell, int pdim, const double *phys, int order, double *x) {
+  double qp[125 * 3], qw[125], ns[8];
+  int nn = orc_cell_nodes(cell);
+  int nq = orc_cell_quad(cell, order, qp, qw);
+  for (int q = 0; q < nq; q++) {
+    orc_shape(cell, qp + 3 * q, ns);
+    for (int d = 0; d < pdim; d++) x[q * pdim + d] = 0.0;
+    for (int k = 0; k < nn; k++)
+      for (int d = 0; d < pdim; d++) x[q * pdim + d] += ns[k] * phys[k * pdim + d];
+  }
+  return nq;
+}
+
+/* poisson_source_vector (crates/tpt-fem-thermal/src/lib.rs:163-196) with the
+ * closure's value s(x_q) given per quadrature point: f[i] += w*s*ns[i]*det. */
+void orc_poisson_source_vector_qp(int cell, int pdim, const double *phys, const double *sq, int order,
+                                  double *f) {
+  double qp[125 * 3], qw[125], ns[8], lg[8 * 3], jac[9], inv[9];
+  int nn = orc_cell_nodes(cell), dim = orc_cell_dim(cell);
+  int nq = orc_cell_quad(cell, order, qp, qw);
+  for (int i = 0; i < nn; i++) f[i] = 0.0;
+  for (int q = 0; q < nq; q++) {
+    orc_shape(cell, qp + 3 * q, ns);
+    orc_grad(cell, qp + 3 * q, lg);
+    double det = fabs(orc_map(nn, pdim, phys, dim, lg, jac, inv));
+    for (int i = 0; i < nn; i++) f[i] += qw[q] * sq[q] * ns[i] * det;
+  }
+}
+
+/* elasticity_body_vector (crates/tpt-fem-elasticity/src/lib.rs:312-348) with
+ * the closure's value b(x_q) (dim components) given per quadrature point:
+ * f[i*dim + a] += w * ns[i] * b[a] * det, points in rule order.             */
+void orc_elasticity_body_vector(int cell, int pdim, const double *phys, const double *bq, int order,
+                                double *f) {
+  double qp[125 * 3], qw[125], ns[8], lg[8 * 3], jac[9], inv[9];
+  int nn = orc_cell_nodes(cell), dim = orc_cell_dim(cell);
+  int nq = orc_cell_quad(cell, order, qp, qw);
+  for (int i = 0; i < nn * dim; i++) f[i] = 0.0;
+  for (int q = 0; q < nq; q++) {
+    orc_shape(cell, qp + 3 * q, ns);
+    orc_grad(cell, qp + 3 * q, lg);
+    double det = fabs(orc_map(nn, pdim, phys, dim, lg, jac, inv));
+    for (int i = 0; i < nn; i++)
+      for (int a = 0; a < dim; a++) f[i * dim + a] += qw[q] * ns[i] * bq[q * dim + a] * det;
+  }
+}
+
 /* constitutive: crates/tpt-fem-elasticity/src/lib.rs:197-238.             */
 static int constitutive(int model, double e, double nu, int dim, double d[6][6]) {
   memset(d, 0, sizeof(double) * 36);
@@ -860,10 +912,29 @@ static void face_quad(int fcell, int order, double *pts, double *wts, int *nq) {
 /* One scalar Neumann pass with flux g(x, n) = gconst if (axis < 0 or
  * |x[axis] - coord| <= tol) else 0.  comp < 0 tiles the load over all dpn
  * components (the reference's behaviour); comp >= 0 writes one component. */
+static void neumann_pass_ex(int64_t nnodes, int pdim, const double *coords, int64_t nelem,
+                            int cell, const int64_t *conn, int dpn, int comp, double gconst,
+                            int axis, double coord, double tol, int order, double *rhs,
+                            const double *gq, double *xout, double *nout, int64_t *npts);
 static void neumann_pass(int64_t nnodes, int pdim, const double *coords, int64_t nelem,
                          int cell, const int64_t *conn, int dpn, int comp, double gconst,
                          int axis, double coord, double tol, int order, double *rhs) {
+  neumann_pass_ex(nnodes, pdim, coords, nelem, cell, conn, dpn, comp, gconst, axis, coord, tol, order, rhs,
+                  NULL, NULL, NULL, NULL);
+}
+
+/* The same pass with the closure's value at every boundary-face quadrature
+ * point given in gq (boundary_faces order, then points in rule order) when
+ * gq != NULL; xout / nout (may be NULL) receive each point's position and
+ * outward unit normal -- the arguments apply_neumann_order passes to the
+ * closure (crates/tpt-fem-assembly/src/lib.rs:620-656) -- and *npts their
+ * count.  rhs may be NULL (points only).                                    */
+static void neumann_pass_ex(int64_t nnodes, int pdim, const double *coords, int64_t nelem,
+                            int cell, const int64_t *conn, int dpn, int comp, double gconst,
+                            int axis, double coord, double tol, int order, double *rhs,
+                            const double *gq, double *xout, double *nout, int64_t *npts) {
   (void)nnodes;
+  int64_t ip = 0;
   int nf, nn = orc_cell_nodes(cell);
   const face_def *F = faces_of(cell, &nf);
   int64_t *be = (int64_t *)malloc(sizeof(int64_t) * (nelem * nf + 1));
@@ -924,9 +995,20 @@ static void neumann_pass(int64_t nnodes, int pdim, const double *coords, int64_t
         double ln = measure;
         for (int d = 0; d < 3; d++) nrm[d] = c[d] / ln;
       }
-      (void)nrm;
-      double g = gconst;
-      if (axis >= 0 && !(fabs(xp[axis] - coord) <= tol)) g = 0.0;
+      /* outward: flip when the normal points against face centroid minus
+       * element centroid (:642-651) */
+      double dd = -0.0;
+      for (int d = 0; d < dim; d++) dd = dd + nrm[d] * (fcen[d] - ecen[d]);
+      if (dd < 0.0)
+        for (int d = 0; d < dim; d++) nrm[d] = -nrm[d];
+      if (xout)
+        for (int d = 0; d < dim; d++) xout[ip * dim + d] = xp[d];
+      if (nout)
+        for (int d = 0; d < dim; d++) nout[ip * dim + d] = nrm[d];
+      double g = gq ? gq[ip] : gconst;
+      ip++;
+      if (!gq && axis >= 0 && !(fabs(xp[axis] - coord) <= tol)) g = 0.0;
+      if (!rhs) continue;
       for (int k = 0; k < fd->nn; k++) {
         int64_t base = conn[e * nn + fd->idx[k]] * dpn;
         if (comp < 0) {
@@ -937,7 +1019,28 @@ static void neumann_pass(int64_t nnodes, int pdim, const double *coords, int64_t
       }
     }
   }
+  if (npts) *npts = ip;
   free(be); free(bf);
+}
+
+/* Positions and outward unit normals of every boundary-face quadrature point
+ * (the closure arguments of apply_neumann_order / apply_robin_order); x and
+ * nrm: count x pdim, either may be NULL.  Returns the count.              */
+int64_t orc_boundary_points(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                            const int64_t *conn, int order, double *x, double *nrm) {
+  int64_t np = 0;
+  neumann_pass_ex(nnodes, pdim, coords, nelem, cell, conn, 1, -1, 0.0, -1, 0.0, 0.0, order, NULL, NULL, x, nrm,
+                  &np);
+  return np;
+}
+
+/* apply_neumann_order (crates/tpt-fem-assembly/src/lib.rs:601-667) with the
+ * flux closure's value g(x_q, n_q) at every boundary point (gq, in
+ * orc_boundary_points order), tiled over dpn components.                   */
+void orc_apply_neumann_qp(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                          const int64_t *conn, int dpn, const double *gq, int order, double *rhs) {
+  neumann_pass_ex(nnodes, pdim, coords, nelem, cell, conn, dpn, -1, 0.0, -1, 0.0, 0.0, order, rhs, gq, NULL, NULL,
+                  NULL);
 }
 
 /* apply_robin_order: crates/tpt-fem-assembly/src/lib.rs:684-753.  Appends,
@@ -946,10 +1049,29 @@ static void neumann_pass(int64_t nnodes, int pdim, const double *coords, int64_t
  * w * measure * h * N_k * N_l) -- no zero skip -- with h the constant hconst,
  * or hconst on |x[axis] - coord| <= tol and 0 elsewhere when axis >= 0.
  * Returns the number of triplets written.                                   */
+static int64_t robin_pass(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                          const int64_t *conn, int dpn, double hconst, int axis, double coord, double tol,
+                          int order, const double *hq, int64_t *rows, int64_t *cols, double *vals);
 int64_t orc_apply_robin(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
                         const int64_t *conn, int dpn, double hconst, int axis, double coord,
                         double tol, int order, int64_t *rows, int64_t *cols, double *vals) {
+  return robin_pass(nnodes, pdim, coords, nelem, cell, conn, dpn, hconst, axis, coord, tol, order, NULL, rows, cols,
+                    vals);
+}
+
+/* apply_robin_order with the coefficient closure's value h(x_q, n_q) at every
+ * boundary point (hq, in orc_boundary_points order).                        */
+int64_t orc_apply_robin_qp(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                           const int64_t *conn, int dpn, const double *hq, int order, int64_t *rows,
+                           int64_t *cols, double *vals) {
+  return robin_pass(nnodes, pdim, coords, nelem, cell, conn, dpn, 0.0, -1, 0.0, 0.0, order, hq, rows, cols, vals);
+}
+
+static int64_t robin_pass(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                          const int64_t *conn, int dpn, double hconst, int axis, double coord, double tol,
+                          int order, const double *hq, int64_t *rows, int64_t *cols, double *vals) {
   (void)nnodes;
+  int64_t ip = 0;
   int nf, nn = orc_cell_nodes(cell);
   const face_def *F = faces_of(cell, &nf);
   int64_t *be = (int64_t *)malloc(sizeof(int64_t) * (nelem * nf + 1));
@@ -991,8 +1113,9 @@ int64_t orc_apply_robin(int64_t nnodes, int pdim, const double *coords, int64_t 
         s2 = s2 + c2 * c2;
         measure = sqrt(s2);
       }
-      double h = hconst;
-      if (axis >= 0 && !(fabs(xp[axis] - coord) <= tol)) h = 0.0;
+      double h = hq ? hq[ip] : hconst;
+      ip++;
+      if (!hq && axis >= 0 && !(fabs(xp[axis] - coord) <= tol)) h = 0.0;
       for (int k = 0; k < fd->nn; k++)
         for (int l = 0; l < fd->nn; l++) {
           const int64_t bi = conn[e * nn + fd->idx[k]] * dpn, bj = conn[e * nn + fd->idx[l]] * dpn;
@@ -1044,6 +1167,68 @@ void orc_poisson_rhs(int64_t nnodes, int pdim, const double *coords, int64_t nel
     orc_poisson_source_vector(cell, pdim, phys, f_const, order, f);
     for (int i = 0; i < nn; i++) rhs[conn[e * nn + i]] += f[i];
   }
+}
+
+/* solve_elasticity's body-force scatter (crates/tpt-fem-elasticity/src/lib.rs:370-379):
+ * rhs[node*dim + a] += f[i*dim + a], elements in order.  bq: nelem x nq x dim. */
+void orc_body_rhs(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                  const int64_t *conn, const double *bq, int order, double *rhs) {
+  double qp[125 * 3], qw[125];
+  int nn = orc_cell_nodes(cell), dim = orc_cell_dim(cell);
+  int nq = orc_cell_quad(cell, order, qp, qw);
+  double phys[8 * 3], f[8 * 3];
+  for (int64_t i = 0; i < nnodes * dim; i++) rhs[i] = 0.0;
+  for (int64_t e = 0; e < nelem; e++) {
+    for (int a = 0; a < nn; a++)
+      for (int p = 0; p < pdim; p++) phys[a * pdim + p] = coords[conn[e * nn + a] * pdim + p];
+    orc_elasticity_body_vector(cell, pdim, phys, bq + e * nq * dim, order, f);
+    for (int i = 0; i < nn; i++)
+      for (int a = 0; a < dim; a++) rhs[conn[e * nn + i] * dim + a] += f[i * dim + a];
+  }
+}
+
+/* solve_poisson's source scatter (crates/tpt-fem-thermal/src/lib.rs:223-230)
+ * with per-point source values sq: nelem x nq.                              */
+void orc_source_rhs_qp(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                       const int64_t *conn, const double *sq, int order, double *rhs) {
+  double qp[125 * 3], qw[125];
+  int nn = orc_cell_nodes(cell);
+  int nq = orc_cell_quad(cell, order, qp, qw);
+  double phys[8 * 3], f[8];
+  for (int64_t i = 0; i < nnodes; i++) rhs[i] = 0.0;
+  for (int64_t e = 0; e < nelem; e++) {
+    for (int a = 0; a < nn; a++)
+      for (int p = 0; p < pdim; p++) phys[a * pdim + p] = coords[conn[e * nn + a] * pdim + p];
+    orc_poisson_source_vector_qp(cell, pdim, phys, sq + e * nq, order, f);
+    for (int i = 0; i < nn; i++) rhs[conn[e * nn + i]] += f[i];
+  }
+}
+
+/* solve_elasticity (crates/tpt-fem-elasticity/src/lib.rs:355-381): try_assemble
+ * of elasticity_element_matrix, the body-force rhs, solve_with_dirichlet.
+ * Returns 0, -5 on a model/dimension mismatch, or the solver's status.      */
+int orc_solve_elasticity(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                         const int64_t *conn, int model, double young, double nu, int order,
+                         const double *bq, int64_t nbc, const int64_t *bc_dof, const double *bc_val,
+                         double *u) {
+  int nn = orc_cell_nodes(cell), dim = orc_cell_dim(cell);
+  int64_t cap = nelem * nn * nn * dim * dim;
+  int64_t *r = (int64_t *)malloc(sizeof(int64_t) * (cap ? cap : 1));
+  int64_t *c = (int64_t *)malloc(sizeof(int64_t) * (cap ? cap : 1));
+  double *v = (double *)malloc(sizeof(double) * (cap ? cap : 1));
+  double *rhs = (double *)malloc(sizeof(double) * (nnodes * dim > 0 ? nnodes * dim : 1));
+  orc_op op;
+  memset(&op, 0, sizeof op);
+  op.cell = cell; op.pdim = pdim; op.kind = 1; op.model = model; op.order = order;
+  op.coef = young; op.nu = nu;
+  int64_t t = orc_assemble(nnodes, coords, nelem, conn, &op, dim, r, c, v);
+  int rc = -5;
+  if (t >= 0) {
+    orc_body_rhs(nnodes, pdim, coords, nelem, cell, conn, bq, order, rhs);
+    rc = orc_solve_with_dirichlet(t, r, c, v, nnodes * dim, rhs, nbc, bc_dof, bc_val, u);
+  }
+  free(r); free(c); free(v); free(rhs);
+  return rc;
 }
 
 int orc_solve_poisson(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,

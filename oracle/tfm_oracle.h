@@ -45,6 +45,13 @@ void orc_poisson_element_matrix(int cell, int pdim, const double *phys,
                                 double kappa, int order, double *k);
 void orc_poisson_source_vector(int cell, int pdim, const double *phys,
                                double f_const, int order, double *f);
+/* Closure-valued loads: the quadrature-point positions a closure is called
+ * at, and the element vectors from the closure's per-point values. */
+int orc_qp_coords(int cell, int pdim, const double *phys, int order, double *x);
+void orc_poisson_source_vector_qp(int cell, int pdim, const double *phys, const double *sq, int order,
+                                  double *f);
+void orc_elasticity_body_vector(int cell, int pdim, const double *phys, const double *bq, int order,
+                                double *f);
 int orc_elasticity_element_matrix(int cell, int pdim, const double *phys,
                                   int model, double young, double nu,
                                   int order, double *k);
@@ -117,10 +124,28 @@ void orc_plane_traction(int64_t nnodes, int pdim, const double *coords,
                         int axis, double coord, double tol, const double *t,
                         int order, double *rhs);
 
+/* Closure-valued boundary terms: the points (position, outward unit normal)
+ * a flux / Robin closure is called at, and the passes from its values. */
+int64_t orc_boundary_points(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                            const int64_t *conn, int order, double *x, double *nrm);
+void orc_apply_neumann_qp(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                          const int64_t *conn, int dpn, const double *gq, int order, double *rhs);
+int64_t orc_apply_robin_qp(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                           const int64_t *conn, int dpn, const double *hq, int order, int64_t *rows,
+                           int64_t *cols, double *vals);
+
 /* ---- drivers ---- */
 void orc_poisson_rhs(int64_t nnodes, int pdim, const double *coords,
                      int64_t nelem, int cell, const int64_t *conn,
                      double f_const, int order, double *rhs);
+void orc_body_rhs(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                  const int64_t *conn, const double *bq, int order, double *rhs);
+void orc_source_rhs_qp(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                       const int64_t *conn, const double *sq, int order, double *rhs);
+int orc_solve_elasticity(int64_t nnodes, int pdim, const double *coords, int64_t nelem, int cell,
+                         const int64_t *conn, int model, double young, double nu, int order,
+                         const double *bq, int64_t nbc, const int64_t *bc_dof, const double *bc_val,
+                         double *u);
 int orc_solve_poisson(int64_t nnodes, int pdim, const double *coords,
                       int64_t nelem, int cell, const int64_t *conn,
                       double kappa, int order, double f_const, int64_t nbc,

@@ -242,8 +242,9 @@ def test_solve_drivers_with_mg(ctx):
     clamp = [(int(i) * 3 + c, 0.0) for i in O.nodes_on_plane(coords, 0, 0.0, 1e-9) for c in range(3)]
     u = T.solve_elasticity(m, T.CONTINUUM3D, 1.0, 0.3, 2, [0.0, 0.0, -1.0], clamp, rtol=1e-13,
                            mode=T.SOLVE_PCG64_MG)
-    ue = T.solve_elasticity(m, T.CONTINUUM3D, 1.0, 0.3, 2, [0.0, 0.0, -1.0], clamp, rtol=1e-13)
-    assert np.abs(u - ue).max() <= 1e-10 * np.abs(ue).max()
+    bq = np.tile([0.0, 0.0, -1.0], (len(conn), 4, 1))
+    ur = O.solve_elasticity(coords, O.TET, conn, O.CONT3D, 1.0, 0.3, 2, bq, clamp)
+    assert np.abs(u - ur).max() <= 1e-10 * np.abs(ur).max()
     m.close()
 
 

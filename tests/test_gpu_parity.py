@@ -253,8 +253,19 @@ def test_source_and_body_vectors_bitexact(ctx):
     qc, qe = O.grid(6, 4, 0.5)
     mq = T.Mesh.from_arrays(ctx, T.QUAD, qc, qe)
     assert np.array_equal(T.source_vector(mq, 1.0, 3), O.poisson_rhs(qc, O.QUAD, qe, 1.0, 3))
-    b = T.body_vector(m, [0.5, -1.0, 2.0])
-    assert abs(b[2::3].sum() - 2.0) < 1e-12 and abs(b[0::3].sum() - 0.5) < 1e-12
+    bvec = np.array([0.5, -1.0, 2.0])
+    b = T.body_vector(m, bvec)
+    assert np.array_equal(b, O.body_rhs(coords, O.TET, conn, np.tile(bvec, (len(conn), 4, 1))))
+    # Q4 (dim 2) and Hex8 (8 points at order 2, 27 at order 3)
+    qcj = perturb(qc, 0.05, 3)
+    mq = T.Mesh.from_arrays(ctx, T.QUAD, qcj, qe)
+    assert np.array_equal(T.body_vector(mq, bvec[:2], 2), O.body_rhs(qcj, O.QUAD, qe, np.tile(bvec[:2], (len(qe), 4, 1)), 2))
+    hc, he = hex_box((3, 2, 2))
+    hc = perturb(hc, 0.04, 8)
+    mh = T.Mesh.from_arrays(ctx, T.HEX, hc, he)
+    for order, nq in ((2, 8), (3, 27)):
+        assert np.array_equal(T.body_vector(mh, bvec, order),
+                              O.body_rhs(hc, O.HEX, he, np.tile(bvec, (len(he), nq, 1)), order))
 
 
 def test_plane_traction_and_neumann_match_oracle(ctx):

@@ -461,6 +461,22 @@ static OpParams make_params(const AsmPlan *pl, const double *d_rule, int nq, con
 int plan_set_coefficients(tfm_ctx *ctx, AsmPlan *pl, const tfm_operator *op) {
   const tfm_mesh *m = pl->mesh;
   int rc;
+  // The pattern holds no entries of an element whose coefficient was 0 when
+  // it was built (assemble() skipped them all): a re-assembly that gives such
+  // an element a non-zero coefficient would need entries the pattern lacks.
+  if (pl->d_rp && op->kind != TFM_OP_SIMP) {
+    auto coef = [&](int64_t e) { return op->coef_e ? op->coef_e[e] : op->coef; };
+    bool bad = false;
+    if (pl->zero_all)
+      for (int64_t e = 0; e < m->n_elem && !bad; e++) bad = coef(e) != 0.0;
+    else
+      for (int64_t e : pl->zero_elems)
+        if ((bad = coef(e) != 0.0)) break;
+    if (bad)
+      return fail(ctx, TFM_E_INVALID,
+                  "re-assembly gives a non-zero coefficient to an element that had coefficient 0 when the "
+                  "pattern was built (its entries are not in the pattern); create the system again");
+  }
   pl->op = *op;
   if (op->coef_e) {
     if (!pl->d_coef_e && (rc = dalloc(ctx, &pl->d_coef_e, m->n_elem))) return rc;
@@ -509,6 +525,14 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
   pl->ndof = mesh->n_nodes * dpn;
   if (pl->ndof >= INT32_MAX) return fail(ctx, TFM_E_UNSUPPORTED, "too many DOFs for int32 indexing");
   if ((rc = plan_set_coefficients(ctx, pl, op))) return rc;
+  if (op->kind != TFM_OP_SIMP) {
+    if (op->coef_e) {
+      for (int64_t e = 0; e < mesh->n_elem; e++)
+        if (op->coef_e[e] == 0.0) pl->zero_elems.push_back(e);
+    } else {
+      pl->zero_all = op->coef == 0.0 && mesh->n_elem > 0;
+    }
+  }
   if (h_fixed) {
     if ((rc = dalloc(ctx, &pl->d_fixed, pl->ndof))) return rc;
     TFM_CHECK_HIP(ctx, hipMemcpyAsync(pl->d_fixed, h_fixed, pl->ndof, hipMemcpyHostToDevice, ctx->stream));

@@ -1293,6 +1293,8 @@ int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, int mod
                   "system on a box mesh with one right-hand side");
   }
   if (mixed && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values");
+  if (mixed && !s->fp32_current())
+    return fail(ctx, TFM_E_INVALID, "mixed solve: the FP32 values predate the last FP64 re-assembly");
   if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values");
   auto *b = new BatchSolver();
   b->s = s;
@@ -1365,8 +1367,14 @@ int bsolver_refresh(BatchSolver *b) {
   hipStream_t st = ctx->stream;
   const int n = (int)s->ndof;
   // FP64 sliced values serve the plain CG and every mode's FP64 residual
+  if (b->mixed && !s->fp32_current())
+    return fail(ctx, TFM_E_INVALID, "mixed solve: the FP32 values predate the last FP64 re-assembly");
   int rc = bb_values<double>(s, b->B, s->d_v64, &b->B.v64);
-  if (!rc && (b->mixed || (b->mg3 && s->d_v32))) rc = bb_values<float>(s, b->B, s->d_v32, &b->B.v32);
+  // The box multigrid's two fine passes may read the FP32 copy, but only
+  // while it holds the same matrix as the FP64 values (same version);
+  // otherwise they run on the FP64 copy.
+  if (!rc && (b->mixed || (b->mg3 && s->fp32_current()))) rc = bb_values<float>(s, b->B, s->d_v32, &b->B.v32);
+  else if (!rc && b->mg3) dfree(b->B.v32);
   if (rc) return rc;
   TFM_CHECK_HIP(ctx, hipMemsetAsync(b->bad, 0, sizeof(int), st));
   const unsigned g1 = (unsigned)((n + 255) / 256);

@@ -42,25 +42,51 @@ int scratch(tfm_ctx *ctx, size_t bytes, void **out) {
   return TFM_OK;
 }
 
+// Sum of n non-negative int32 counts in int64 (integer addition, so the
+// atomic order does not matter): the range check must not trust a sum that
+// an int32 scan has already wrapped.
+__global__ void k_sum_i64(const int *__restrict__ in, int64_t n, unsigned long long *__restrict__ out,
+                          int *__restrict__ neg) {
+  long long s = 0;
+  int anyneg = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int v = in[i];
+    anyneg |= v < 0;
+    s += v;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (anyneg) atomicOr(neg, 1);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)s);
+}
+
 int exclusive_scan_i32(tfm_ctx *ctx, const int *d_in, int *d_out, int64_t n, int64_t *total) {
-  // out[0..n-1] = exclusive prefix, out[n] = total (int32; caller bounds n)
+  // out[0..n-1] = exclusive prefix, out[n] = total.  The total is formed in
+  // int64 first; the int32 scan only runs when every prefix fits.
   size_t tmp = 0;
   TFM_CHECK_HIP(ctx, rocprim::exclusive_scan(nullptr, tmp, d_in, d_out, 0, (size_t)n,
                                              rocprim::plus<int>(), ctx->stream));
   void *d_tmp = nullptr;
-  int rc = scratch(ctx, tmp + 16, &d_tmp);
+  const size_t off = (tmp + 255) / 256 * 256;
+  int rc = scratch(ctx, off + 16, &d_tmp);
   if (rc) return rc;
+  unsigned long long *d_sum = (unsigned long long *)((char *)d_tmp + off);
+  int *d_neg = (int *)(d_sum + 1);
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(d_sum, 0, 16, ctx->stream));
+  if (n > 0) {
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_sum_i64, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, d_in, n, d_sum, d_neg);
+    TFM_CHECK_HIP(ctx, hipGetLastError());
+  }
+  unsigned long long hsum[2] = {0, 0};
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(hsum, d_sum, 16, hipMemcpyDeviceToHost, ctx->stream));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if ((int)(hsum[1] & 0xffffffffu)) return fail(ctx, TFM_E_INVALID, "scan: negative count");
+  const int64_t t = (int64_t)hsum[0];
+  if (t < 0 || t >= (int64_t)INT32_MAX) return fail(ctx, TFM_E_UNSUPPORTED, "scan total exceeds int32 range");
   if (n > 0)
     TFM_CHECK_HIP(ctx, rocprim::exclusive_scan(d_tmp, tmp, d_in, d_out, 0, (size_t)n,
                                                rocprim::plus<int>(), ctx->stream));
-  int last_in = 0, last_out = 0;
-  if (n > 0) {
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(&last_in, d_in + n - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(&last_out, d_out + n - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  }
-  TFM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  int64_t t = (int64_t)last_in + (int64_t)last_out;
-  if (t > INT32_MAX) return fail(ctx, TFM_E_UNSUPPORTED, "scan total exceeds int32 range");
   int ti = (int)t;
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(d_out + n, &ti, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -292,6 +318,23 @@ extern "C" int tfm_ctx_timer_end(tfm_ctx *ctx, double *ms) {
   TFM_CHECK_HIP(ctx, hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
   *ms = f;
   return TFM_OK;
+}
+
+extern "C" int tfm_debug_scan_total(tfm_ctx *ctx, const int32_t *counts, int64_t n, int64_t *total) {
+  if (!ctx || !total || n < 0 || (n > 0 && !counts)) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  int *d_in = nullptr, *d_out = nullptr;
+  if ((rc = dalloc(ctx, &d_in, (size_t)n)) || (rc = dalloc(ctx, &d_out, (size_t)n + 1))) {
+    dfree(d_in);
+    return rc;
+  }
+  hipError_t e = n > 0 ? hipMemcpy(d_in, counts, sizeof(int) * (size_t)n, hipMemcpyHostToDevice) : hipSuccess;
+  if (e != hipSuccess) rc = fail(ctx, TFM_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  if (!rc) rc = exclusive_scan_i32(ctx, d_in, d_out, n, total);
+  dfree(d_in);
+  dfree(d_out);
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

@@ -148,6 +148,11 @@ struct AsmPlan {
   // P1-tet fast path (tfm_asm_tet.hip), when its plan could be built
   bool use_tet = false;
   TetPlan tet;
+  // Elements whose coefficient was exactly 0 when the pattern was built (all
+  // their entries were skipped, so they own no pattern entries); zero_all:
+  // the scalar coefficient was 0.  A re-assembly may not make them non-zero.
+  std::vector<int64_t> zero_elems;
+  bool zero_all = false;
 };
 struct OpParams;
 int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci, TetPlan *tp);
@@ -239,6 +244,18 @@ void mg_destroy(MgData *mg);
 // Loads (tfm_loads.hip): device rhs of n_nodes*dpn doubles, accumulated.
 int load_source_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double f, double *d_rhs);
 int load_body_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *b, double *d_rhs);
+// The same from the closure's values at every element quadrature point (host
+// arrays [n_elem][nq] and [n_elem][nq][dim]).
+int load_source_qp_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *h_sq, double *d_rhs);
+int load_body_qp_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *h_bq, double *d_rhs);
+int element_rule_points(tfm_ctx *ctx, int cell, int order, int *nq);
+// Neumann pass from the flux closure's values at every boundary-face point
+// (host array in tfm_mesh_boundary_points order), tiled over dpn components.
+int load_neumann_qp_dev(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int order, const double *h_gq, double *d_rhs);
+// to_csr(assemble ++ apply_robin): h constant (on an optional band) or, when
+// h_hq is given, the coefficient closure's values at every boundary point.
+int assemble_robin_dev(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn, double h, int axis,
+                       double coord, double tol, int order, const double *h_hq, tfm_matrix **out);
 // zeroed: d_rhs holds +0.0 everywhere, so faces that can only add +0.0 may be skipped.
 int load_neumann_dev(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int comp, double g, int axis,
                      double coord, double tol, double *d_rhs, bool zeroed);
@@ -265,4 +282,16 @@ struct tfm_system {
   bool owns_pattern = false;
   tfm::BSell bsell;           // solver copy, built on the first solve
   tfm::BatchSolver *mg_solver = nullptr;  // multigrid solver of tfm_system_solve, built on its first use
+  // Coefficient versions: `version` counts the coefficient sets the system
+  // has been assembled with; each numeric pass stamps the value arrays it
+  // wrote with the current one.  FP32 values describe the same matrix as
+  // the FP64 values only while ver32 == ver64 (an FP64-only re-assembly with
+  // new coefficients leaves them stale until an FP32 pass runs).
+  uint64_t version = 0, ver64 = 0, ver32 = 0;
+  void note_values(bool new_coefficients, bool w64, bool w32) {
+    if (new_coefficients || version == 0) ++version;
+    if (w64) ver64 = version;
+    if (w32) ver32 = version;
+  }
+  bool fp32_current() const { return d_v32 != nullptr && ver32 == ver64; }
 };

@@ -21,13 +21,15 @@ namespace tfm {
 
 int build_rule(int cell, int order, std::vector<double> &rule);
 
+// qv (optional): the closure's values at every element quadrature point,
+// [n_elem][nq][dpn], in place of the constant s_const / bvec.
 __global__ void k_load_source(const double *__restrict__ coords, int pdim, const int *__restrict__ conn,
                               int nn, int cell, const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
                               int64_t n_nodes, const double *__restrict__ rule, int nq, double s_const,
-                              const double *__restrict__ bvec, int dim, double *__restrict__ rhs) {
+                              const double *__restrict__ bvec, int dim, const double *__restrict__ qv, int body,
+                              double *__restrict__ rhs) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
-  const bool body = bvec != nullptr;
   const int dpn = body ? dim : 1;
   double acc[3];
   for (int a = 0; a < dpn; a++) acc[a] = rhs[n * dpn + a];
@@ -56,15 +58,36 @@ __global__ void k_load_source(const double *__restrict__ coords, int pdim, const
       default: { det = map_point<8, 3>(X, lg, g); break; }
       }
       det = fabs(det);
+      const double *qe = qv ? qv + ((int64_t)e * nq + q) * dpn : nullptr;
       if (body) {
-        for (int a = 0; a < dpn; a++) f[a] += w * ns[ln] * bvec[a] * det;
+        for (int a = 0; a < dpn; a++) f[a] += w * ns[ln] * (qe ? qe[a] : bvec[a]) * det;
       } else {
-        f[0] += w * s_const * ns[ln] * det;
+        f[0] += w * (qe ? qe[0] : s_const) * ns[ln] * det;
       }
     }
     for (int a = 0; a < dpn; a++) acc[a] += f[a];
   }
   for (int a = 0; a < dpn; a++) rhs[n * dpn + a] = acc[a];
+}
+
+// Physical positions of every element quadrature point, formed as the
+// reference forms a closure's argument (crates/tpt-fem-thermal/src/lib.rs:180-185,
+// crates/tpt-fem-elasticity/src/lib.rs:331-336): x[d] = sum_k ns[k] * X[k][d]
+// from 0.0 in node order.  One thread per (element, point); out
+// [n_elem][nq][pdim].
+__global__ void k_qp_coords(const double *__restrict__ coords, int pdim, const int *__restrict__ conn, int nn,
+                            int64_t n_elem, const double *__restrict__ rule, int nq, double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_elem * nq) return;
+  const int64_t e = t / nq;
+  const int q = (int)(t % nq);
+  const double *ns = rule + q * kRuleStride + 4;
+  double x[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < nn; k++) {
+    const int64_t nd = conn[e * nn + k];
+    for (int d = 0; d < pdim; d++) x[d] += ns[k] * coords[nd * pdim + d];
+  }
+  for (int d = 0; d < pdim; d++) out[t * pdim + d] = x[d];
 }
 
 // Boundary-face tables (crates/tpt-fem-assembly/src/lib.rs:73-117, P1 cells).
@@ -131,6 +154,105 @@ __device__ __forceinline__ void face_point(const FaceTab &FT, const double (&P)[
   }
 }
 
+// Outward unit normal of a face quadrature point, as apply_neumann_order /
+// apply_robin_order form the closure's second argument
+// (crates/tpt-fem-assembly/src/lib.rs:394-420 surface, :634-651): the in-plane
+// perpendicular (-t1, t0) of a Line face or the cross product of a surface
+// face's tangents, divided by its norm, flipped when it points against face
+// centroid minus element centroid (mean(): sum in node order, then divide).
+__device__ __forceinline__ void face_normal(const FaceTab &FT, const double (&P)[4][3], const double *ecen,
+                                            int pdim, const double *__restrict__ R, double (&nrm)[3]) {
+  const double *ng = R + 12;
+  const int fdim = FT.fcell == TFM_LINE ? 1 : 2;
+  double t[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int m = 0; m < fdim; m++)
+    for (int k = 0; k < FT.fn_; k++)
+      for (int d = 0; d < pdim; d++) t[m][d] += ng[k * fdim + m] * P[k][d];
+  nrm[0] = nrm[1] = nrm[2] = 0.0;
+  if (fdim == 1) {
+    nrm[0] = -t[0][1];
+    nrm[1] = t[0][0];
+    double s = nrm[0] * nrm[0];
+    for (int d = 1; d < pdim; d++) s = s + nrm[d] * nrm[d];
+    const double ln = sqrt(s);
+    for (int d = 0; d < pdim; d++) nrm[d] = nrm[d] / ln;
+  } else {
+    const double c[3] = {t[0][1] * t[1][2] - t[0][2] * t[1][1], t[0][2] * t[1][0] - t[0][0] * t[1][2],
+                         t[0][0] * t[1][1] - t[0][1] * t[1][0]};
+    const double ln = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    for (int d = 0; d < 3; d++) nrm[d] = c[d] / ln;
+  }
+  double fcen[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < FT.fn_; k++)
+    for (int d = 0; d < pdim; d++) fcen[d] += P[k][d];
+  for (int d = 0; d < pdim; d++) fcen[d] = fcen[d] / (double)FT.fn_;
+  double dd = nrm[0] * (fcen[0] - ecen[0]);
+  for (int d = 1; d < pdim; d++) dd = dd + nrm[d] * (fcen[d] - ecen[d]);
+  if (dd < 0.0)
+    for (int d = 0; d < pdim; d++) nrm[d] = -nrm[d];
+}
+
+// Which faces of each element lie on the boundary (bit fi of bmask[e]; point
+// faces of Line2 cells excluded), and how many: boundary_faces' order is
+// (element, face), so the index of face fi of element e among all boundary
+// faces is boff[e] + popcount(bmask[e] & ((1 << fi) - 1)).
+__global__ void k_bface_mask(const int *__restrict__ conn, int nn, int cell, const int *__restrict__ n2e_ptr,
+                             const int *__restrict__ n2e, int64_t n_elem, unsigned char *__restrict__ bmask,
+                             int *__restrict__ cnt) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  const FaceTab &FT = c_faces[cell];
+  int en[8];
+  for (int a = 0; a < nn; a++) en[a] = conn[e * nn + a];
+  unsigned m = 0;
+  if (FT.fn_ > 1)
+    for (int fi = 0; fi < FT.nf; fi++) {
+      const int n0 = en[FT.idx[fi][0]];
+      if (face_on_boundary(FT, fi, en, (int)e, nn, conn, n2e, n2e_ptr[n0], n2e_ptr[n0 + 1])) m |= 1u << fi;
+    }
+  bmask[e] = (unsigned char)m;
+  cnt[e] = __popc(m);
+}
+
+__device__ __forceinline__ int64_t bface_point(const unsigned char *bmask, const int *boff, int64_t e, int fi,
+                                               int fnq, int q) {
+  return ((int64_t)boff[e] + __popc((unsigned)bmask[e] & ((1u << fi) - 1u))) * fnq + q;
+}
+
+// Position and outward normal of every boundary-face quadrature point, one
+// thread per element.
+__global__ void k_bface_points(const double *__restrict__ coords, int pdim, const int *__restrict__ conn, int nn,
+                               int cell, int64_t n_elem, const unsigned char *__restrict__ bmask,
+                               const int *__restrict__ boff, const double *__restrict__ frule, int fnq,
+                               double *__restrict__ xout, double *__restrict__ nout) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem || bmask[e] == 0) return;
+  const FaceTab &FT = c_faces[cell];
+  int en[8];
+  for (int a = 0; a < nn; a++) en[a] = conn[e * nn + a];
+  double ecen[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < nn; a++)
+    for (int d = 0; d < pdim; d++) ecen[d] += coords[(int64_t)en[a] * pdim + d];
+  for (int d = 0; d < pdim; d++) ecen[d] = ecen[d] / (double)nn;
+  for (int fi = 0; fi < FT.nf; fi++) {
+    if (!((bmask[e] >> fi) & 1)) continue;
+    double P[4][3];
+    for (int k = 0; k < FT.fn_; k++)
+      for (int d = 0; d < 3; d++) P[k][d] = d < pdim ? coords[(int64_t)en[FT.idx[fi][k]] * pdim + d] : 0.0;
+    for (int q = 0; q < fnq; q++) {
+      double w, measure, xp[3], nrm[3];
+      const double *ns;
+      face_point(FT, P, pdim, frule + q * kRuleStride, w, ns, xp, measure);
+      face_normal(FT, P, ecen, pdim, frule + q * kRuleStride, nrm);
+      const int64_t ip = bface_point(bmask, boff, e, fi, fnq, q);
+      for (int d = 0; d < pdim; d++) {
+        if (xout) xout[ip * pdim + d] = xp[d];
+        if (nout) nout[ip * pdim + d] = nrm[d];
+      }
+    }
+  }
+}
+
 // One scalar Neumann pass: for every boundary face containing node n, in
 // (element, face) order, rhs[n*dpn + c] += w*measure*g*N_k at each face
 // quadrature point, with g = gconst, or -- when axis >= 0 -- gconst only at
@@ -147,7 +269,8 @@ __global__ void k_load_neumann(const double *__restrict__ coords, int pdim, cons
                                int cell, const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
                                int64_t n_nodes, const double *__restrict__ frule, int fnq, int dpn, int comp,
                                double gconst, int axis, double coord, double tol, int skip_far,
-                               double *__restrict__ rhs) {
+                               const double *__restrict__ gq, const unsigned char *__restrict__ bmask,
+                               const int *__restrict__ boff, double *__restrict__ rhs) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   const FaceTab &FT = c_faces[cell];
@@ -176,8 +299,8 @@ __global__ void k_load_neumann(const double *__restrict__ coords, int pdim, cons
         const double margin = 8.0 * 2.220446049250313e-16 * (fabs(lo) + fabs(hi)) + 1e-300;
         if (finite && (hi + margin < coord - tol || lo - margin > coord + tol)) continue;
       }
-      if (!face_on_boundary(FT, fi, en, e, nn, conn, n2e, pb, pe)) continue;
       if (FT.fn_ == 1) continue;  // point faces of Line2 cells: no surface measure
+      if (bmask ? !((bmask[e] >> fi) & 1) : !face_on_boundary(FT, fi, en, e, nn, conn, n2e, pb, pe)) continue;
       double P[4][3];
       for (int k = 0; k < FT.fn_; k++)
         for (int d = 0; d < 3; d++) P[k][d] = d < pdim ? coords[(int64_t)en[FT.idx[fi][k]] * pdim + d] : 0.0;
@@ -185,8 +308,8 @@ __global__ void k_load_neumann(const double *__restrict__ coords, int pdim, cons
         double w, measure, xp[3];
         const double *ns;
         face_point(FT, P, pdim, frule + q * kRuleStride, w, ns, xp, measure);
-        double g = gconst;
-        if (axis >= 0 && !(fabs(xp[axis] - coord) <= tol)) g = 0.0;
+        double g = gq ? gq[bface_point(bmask, boff, e, fi, fnq, q)] : gconst;
+        if (!gq && axis >= 0 && !(fabs(xp[axis] - coord) <= tol)) g = 0.0;
         double val = w * measure * g * ns[kself];
         touched = true;
         if (comp < 0) {
@@ -215,61 +338,171 @@ static int upload_rule(tfm_ctx *ctx, int cell, int order, double **d_rule, int *
 
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-int load_source_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double f, double *d_rhs) {
-  int rc;
-  if ((rc = ensure_adjacency(mesh))) return rc;
-  double *d_rule = nullptr;
-  int nq = 0;
-  if ((rc = upload_rule(ctx, mesh->cell, order, &d_rule, &nq))) return rc;
-  if (mesh->n_nodes > 0)
-    hipLaunchKernelGGL(k_load_source, dim3(grid1(mesh->n_nodes, 128)), dim3(128), 0, ctx->stream, mesh->d_coords,
-                       mesh->pdim, mesh->d_conn, mesh->nn, mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, mesh->n_nodes,
-                       d_rule, nq, f, (const double *)nullptr, mesh->dim, d_rhs);
-  hipError_t e = hipStreamSynchronize(ctx->stream);
-  dfree(d_rule);
-  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, "source-vector kernel failed");
+int element_rule_points(tfm_ctx *ctx, int cell, int order, int *nq) {
+  std::vector<double> rule;
+  int q = build_rule(cell, order, rule);
+  if (q <= 0) return fail(ctx, TFM_E_INVALID, "gauss_legendre: order must be in 1..=5");
+  *nq = q;
   return TFM_OK;
 }
 
-int load_body_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *b, double *d_rhs) {
+// Source (body = false) or body-force (body = true) load: constant values
+// (f, or b[dim]) or, when h_qv is given, the closure's values at every
+// element quadrature point (host, [n_elem][nq][1 or dim]).
+static int load_elem_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, bool body, double f, const double *b,
+                         const double *h_qv, double *d_rhs) {
   int rc;
   if ((rc = ensure_adjacency(mesh))) return rc;
-  double *d_rule = nullptr, *d_b = nullptr;
+  double *d_rule = nullptr, *d_b = nullptr, *d_qv = nullptr;
   int nq = 0;
   if ((rc = upload_rule(ctx, mesh->cell, order, &d_rule, &nq))) return rc;
-  if ((rc = dalloc(ctx, &d_b, 3))) return rc;
+  const int dpn = body ? mesh->dim : 1;
+  auto cleanup = [&]() {
+    dfree(d_rule);
+    dfree(d_b);
+    dfree(d_qv);
+  };
+  if ((rc = dalloc(ctx, &d_b, 3))) { cleanup(); return rc; }
   double hb[3] = {0, 0, 0};
-  for (int a = 0; a < mesh->dim; a++) hb[a] = b[a];
-  TFM_CHECK_HIP(ctx, hipMemcpyAsync(d_b, hb, sizeof hb, hipMemcpyHostToDevice, ctx->stream));
-  if (mesh->n_nodes > 0)
+  if (body && b)
+    for (int a = 0; a < mesh->dim; a++) hb[a] = b[a];
+  hipError_t e = hipMemcpyAsync(d_b, hb, sizeof hb, hipMemcpyHostToDevice, ctx->stream);
+  const size_t nv = (size_t)mesh->n_elem * nq * dpn;
+  if (e == hipSuccess && h_qv) {
+    if ((rc = dalloc(ctx, &d_qv, nv))) { cleanup(); return rc; }
+    e = hipMemcpyAsync(d_qv, h_qv, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream);
+  }
+  if (e == hipSuccess && mesh->n_nodes > 0)
     hipLaunchKernelGGL(k_load_source, dim3(grid1(mesh->n_nodes, 128)), dim3(128), 0, ctx->stream, mesh->d_coords,
                        mesh->pdim, mesh->d_conn, mesh->nn, mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, mesh->n_nodes,
-                       d_rule, nq, 0.0, d_b, mesh->dim, d_rhs);
+                       d_rule, nq, f, (const double *)d_b, mesh->dim, (const double *)d_qv, (int)body, d_rhs);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  cleanup();
+  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, std::string("element load kernel: ") + hipGetErrorString(e));
+  return TFM_OK;
+}
+
+int load_source_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double f, double *d_rhs) {
+  return load_elem_dev(ctx, mesh, order, false, f, nullptr, nullptr, d_rhs);
+}
+
+int load_body_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *b, double *d_rhs) {
+  return load_elem_dev(ctx, mesh, order, true, 0.0, b, nullptr, d_rhs);
+}
+
+int load_source_qp_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *h_sq, double *d_rhs) {
+  return load_elem_dev(ctx, mesh, order, false, 0.0, nullptr, h_sq, d_rhs);
+}
+
+int load_body_qp_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, const double *h_bq, double *d_rhs) {
+  return load_elem_dev(ctx, mesh, order, true, 0.0, nullptr, h_bq, d_rhs);
+}
+
+static int face_cell(int cell) { return cell == TFM_TET ? TFM_TRI : (cell == TFM_HEX ? TFM_QUAD : TFM_LINE); }
+
+// The boundary-face index of a mesh (k_bface_mask + scan): per element the
+// mask of its boundary faces and the index of its first one.
+struct BFaceIndex {
+  unsigned char *bmask = nullptr;
+  int *boff = nullptr;
+  int64_t nbf = 0;
+  ~BFaceIndex() {
+    dfree(bmask);
+    dfree(boff);
+  }
+};
+
+static int bface_index(tfm_ctx *ctx, tfm_mesh *mesh, BFaceIndex *bi) {
+  int rc;
+  if ((rc = ensure_adjacency(mesh))) return rc;
+  int *cnt = nullptr;
+  if ((rc = dalloc(ctx, &bi->bmask, mesh->n_elem)) || (rc = dalloc(ctx, &bi->boff, mesh->n_elem + 1)) ||
+      (rc = dalloc(ctx, &cnt, mesh->n_elem)))
+    return rc;
+  if (mesh->n_elem > 0)
+    hipLaunchKernelGGL(k_bface_mask, dim3(grid1(mesh->n_elem, 128)), dim3(128), 0, ctx->stream, mesh->d_conn,
+                       mesh->nn, mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, mesh->n_elem, bi->bmask, cnt);
+  rc = exclusive_scan_i32(ctx, cnt, bi->boff, mesh->n_elem, &bi->nbf);
+  dfree(cnt);
+  return rc;
+}
+
+// Neumann pass with the face rule of `order` (FACE_QUAD_ORDER = 2 in
+// apply_neumann) and, when h_gq is given, the flux closure's values at every
+// boundary point (host, in tfm_mesh_boundary_points order) in place of g.
+static int load_neumann_dev_ex(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int comp, double g, int axis, double coord,
+                               double tol, int order, const double *h_gq, double *d_rhs, bool zeroed) {
+  int rc;
+  if ((rc = ensure_adjacency(mesh))) return rc;
+  // face_quad (assembly/src/lib.rs:336-358): Line faces Gauss-Legendre,
+  // Tri faces the triangle rule, Quad faces the tensor rule.
+  double *d_rule = nullptr, *d_gq = nullptr;
+  int nq = 0;
+  if ((rc = upload_rule(ctx, face_cell(mesh->cell), order, &d_rule, &nq))) return rc;
+  BFaceIndex bi;
+  if (h_gq) {
+    if ((rc = bface_index(ctx, mesh, &bi)) || (rc = dalloc(ctx, &d_gq, (size_t)bi.nbf * nq))) {
+      dfree(d_rule);
+      return rc;
+    }
+    if (bi.nbf > 0 && hipMemcpyAsync(d_gq, h_gq, sizeof(double) * bi.nbf * nq, hipMemcpyHostToDevice,
+                                     ctx->stream) != hipSuccess) {
+      dfree(d_rule);
+      dfree(d_gq);
+      return fail(ctx, TFM_E_HIP, "flux upload failed");
+    }
+  }
+  if (mesh->n_nodes > 0)
+    hipLaunchKernelGGL(k_load_neumann, dim3(grid1(mesh->n_nodes, 128)), dim3(128), 0, ctx->stream, mesh->d_coords,
+                       mesh->pdim, mesh->d_conn, mesh->nn, mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, mesh->n_nodes,
+                       d_rule, nq, dpn, comp, g, axis, coord, tol,
+                       (int)(!h_gq && zeroed && axis >= 0 && axis < mesh->pdim), (const double *)d_gq,
+                       (const unsigned char *)(h_gq ? bi.bmask : nullptr), (const int *)(h_gq ? bi.boff : nullptr),
+                       d_rhs);
   hipError_t e = hipStreamSynchronize(ctx->stream);
   dfree(d_rule);
-  dfree(d_b);
-  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, "body-vector kernel failed");
+  dfree(d_gq);
+  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, "Neumann kernel failed");
   return TFM_OK;
 }
 
 int load_neumann_dev(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int comp, double g, int axis, double coord,
                      double tol, double *d_rhs, bool zeroed) {
+  return load_neumann_dev_ex(ctx, mesh, dpn, comp, g, axis, coord, tol, 2, nullptr, d_rhs, zeroed);
+}
+
+int load_neumann_qp_dev(tfm_ctx *ctx, tfm_mesh *mesh, int dpn, int order, const double *h_gq, double *d_rhs) {
+  return load_neumann_dev_ex(ctx, mesh, dpn, -1, 0.0, -1, 0.0, 0.0, order, h_gq, d_rhs, false);
+}
+
+int boundary_points_dev(tfm_ctx *ctx, tfm_mesh *mesh, int order, double *h_x, double *h_n, int64_t *count) {
   int rc;
-  if ((rc = ensure_adjacency(mesh))) return rc;
-  // face_quad (assembly/src/lib.rs:336-358) at FACE_QUAD_ORDER = 2: Line
-  // faces Gauss-Legendre 2, Tri faces the Degree2 rule, Quad faces 2x2.
-  int fcell = mesh->cell == TFM_TET ? TFM_TRI : (mesh->cell == TFM_HEX ? TFM_QUAD : TFM_LINE);
-  double *d_rule = nullptr;
+  double *d_rule = nullptr, *d_x = nullptr, *d_n = nullptr;
   int nq = 0;
-  if ((rc = upload_rule(ctx, fcell, 2, &d_rule, &nq))) return rc;
-  if (mesh->n_nodes > 0)
-    hipLaunchKernelGGL(k_load_neumann, dim3(grid1(mesh->n_nodes, 128)), dim3(128), 0, ctx->stream, mesh->d_coords,
-                       mesh->pdim, mesh->d_conn, mesh->nn, mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, mesh->n_nodes,
-                       d_rule, nq, dpn, comp, g, axis, coord, tol, (int)(zeroed && axis >= 0 && axis < mesh->pdim),
-                       d_rhs);
-  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if ((rc = upload_rule(ctx, face_cell(mesh->cell), order, &d_rule, &nq))) return rc;
+  BFaceIndex bi;
+  if ((rc = bface_index(ctx, mesh, &bi))) { dfree(d_rule); return rc; }
+  const int64_t np = bi.nbf * nq;
+  *count = np;
+  if ((h_x || h_n) && np > 0) {
+    if ((rc = dalloc(ctx, &d_x, (size_t)np * mesh->pdim)) || (rc = dalloc(ctx, &d_n, (size_t)np * mesh->pdim))) {
+      dfree(d_rule);
+      dfree(d_x);
+      return rc;
+    }
+    hipLaunchKernelGGL(k_bface_points, dim3(grid1(mesh->n_elem, 128)), dim3(128), 0, ctx->stream, mesh->d_coords,
+                       mesh->pdim, mesh->d_conn, mesh->nn, mesh->cell, mesh->n_elem, (const unsigned char *)bi.bmask,
+                       (const int *)bi.boff, (const double *)d_rule, nq, d_x, d_n);
+    hipError_t e = hipSuccess;
+    if (h_x) e = hipMemcpyAsync(h_x, d_x, sizeof(double) * np * mesh->pdim, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && h_n)
+      e = hipMemcpyAsync(h_n, d_n, sizeof(double) * np * mesh->pdim, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(d_x);
+    dfree(d_n);
+    if (e != hipSuccess) { dfree(d_rule); return fail(ctx, TFM_E_HIP, "boundary points failed"); }
+  }
   dfree(d_rule);
-  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, "Neumann kernel failed");
   return TFM_OK;
 }
 
@@ -289,7 +522,8 @@ template <int FILL>
 __global__ void k_robin(const double *__restrict__ coords, int pdim, const int *__restrict__ conn, int nn,
                         int cell, const int *__restrict__ n2e_ptr, const int *__restrict__ n2e, int64_t nrows,
                         int dpn, const double *__restrict__ frule, int fnq, double h, int axis, double coord,
-                        double tol, const int *__restrict__ krp, const int *__restrict__ kci,
+                        double tol, const double *__restrict__ hqv, const unsigned char *__restrict__ bmask,
+                        const int *__restrict__ boff, const int *__restrict__ krp, const int *__restrict__ kci,
                         const double *__restrict__ kv, int64_t knrows, int *__restrict__ cnt,
                         const int *__restrict__ orp, int *__restrict__ oci, double *__restrict__ ov,
                         int *__restrict__ maxcol, int *__restrict__ err) {
@@ -318,7 +552,7 @@ __global__ void k_robin(const double *__restrict__ coords, int pdim, const int *
       for (int k = 0; k < FT.fn_; k++)
         if (FT.idx[fi][k] == ln) kself = k;
       if (kself < 0 || FT.fn_ == 1) continue;
-      if (!face_on_boundary(FT, fi, en, e, nn, conn, n2e, pb, pe)) continue;
+      if (bmask ? !((bmask[e] >> fi) & 1) : !face_on_boundary(FT, fi, en, e, nn, conn, n2e, pb, pe)) continue;
       double P[4][3];
       for (int k = 0; k < FT.fn_; k++)
         for (int d = 0; d < 3; d++) P[k][d] = d < pdim ? coords[(int64_t)en[FT.idx[fi][k]] * pdim + d] : 0.0;
@@ -326,7 +560,8 @@ __global__ void k_robin(const double *__restrict__ coords, int pdim, const int *
         double w, measure, xp[3];
         const double *ns;
         face_point(FT, P, pdim, frule + q * kRuleStride, w, ns, xp, measure);
-        const double hq = (axis < 0 || fabs(xp[axis] - coord) <= tol) ? h : 0.0;
+        const double hq = hqv ? hqv[bface_point(bmask, boff, e, fi, fnq, q)]
+                              : ((axis < 0 || fabs(xp[axis] - coord) <= tol) ? h : 0.0);
         const double base = w * measure * hq * ns[kself];
         for (int l = 0; l < FT.fn_; l++) {
           const int col = en[FT.idx[fi][l]] * dpn + c;
@@ -366,7 +601,7 @@ __global__ void k_robin(const double *__restrict__ coords, int pdim, const int *
 
 // K = to_csr(assemble(...) ++ apply_robin(...)) as a new matrix.
 int assemble_robin_dev(tfm_ctx *ctx, const tfm_mesh *cmesh, const tfm_operator *op, int dpn, double h, int axis,
-                       double coord, double tol, tfm_matrix **out) {
+                       double coord, double tol, int order, const double *h_hq, tfm_matrix **out) {
   tfm_mesh *mesh = const_cast<tfm_mesh *>(cmesh);
   int rc;
   tfm_matrix *K = nullptr;
@@ -376,32 +611,48 @@ int assemble_robin_dev(tfm_ctx *ctx, const tfm_mesh *cmesh, const tfm_operator *
     return rc;
   }
   hipStream_t st = ctx->stream;
-  const int fcell = mesh->cell == TFM_TET ? TFM_TRI : (mesh->cell == TFM_HEX ? TFM_QUAD : TFM_LINE);
   const int64_t nrows = mesh->n_nodes * dpn;
-  double *d_rule = nullptr;
+  double *d_rule = nullptr, *d_hq = nullptr;
   int nq = 0, *cnt = nullptr, *aux = nullptr;
   tfm_matrix *R = new tfm_matrix();
   R->ctx = ctx;
+  BFaceIndex bi;
   auto cleanup = [&]() {
     dfree(d_rule);
+    dfree(d_hq);
     dfree(cnt);
     dfree(aux);
     tfm_matrix_destroy(K);
   };
-  if ((rc = upload_rule(ctx, fcell, 2, &d_rule, &nq)) || (rc = dalloc(ctx, &cnt, nrows)) ||
+  if ((rc = upload_rule(ctx, face_cell(mesh->cell), order, &d_rule, &nq)) || (rc = dalloc(ctx, &cnt, nrows)) ||
       (rc = dalloc(ctx, &aux, 3)) || (rc = dalloc(ctx, &R->d_rp, nrows + 1))) {
     cleanup();
     tfm_matrix_destroy(R);
     return rc;
   }
+  if (h_hq) {
+    if ((rc = bface_index(ctx, mesh, &bi)) || (rc = dalloc(ctx, &d_hq, (size_t)bi.nbf * nq))) {
+      cleanup();
+      tfm_matrix_destroy(R);
+      return rc;
+    }
+    if (bi.nbf > 0 && hipMemcpy(d_hq, h_hq, sizeof(double) * bi.nbf * nq, hipMemcpyHostToDevice) != hipSuccess) {
+      cleanup();
+      tfm_matrix_destroy(R);
+      return fail(ctx, TFM_E_HIP, "Robin coefficient upload failed");
+    }
+  }
+  const unsigned char *bm = h_hq ? bi.bmask : nullptr;
+  const int *bo = h_hq ? bi.boff : nullptr;
   const int init[3] = {-1, -1, 0};  // largest column, largest non-empty row, overflow flag
   TFM_CHECK_HIP(ctx, hipMemcpy(aux, init, sizeof init, hipMemcpyHostToDevice));
   const unsigned g = (unsigned)((nrows + 127) / 128);
   if (nrows > 0)
     hipLaunchKernelGGL(k_robin<0>, dim3(g), dim3(128), 0, st, mesh->d_coords, mesh->pdim, mesh->d_conn, mesh->nn,
                        mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, nrows, dpn, (const double *)d_rule, nq, h, axis,
-                       coord, tol, (const int *)K->d_rp, (const int *)K->d_ci, (const double *)K->d_v, K->nnz ? K->nrows : 0,
-                       cnt, (const int *)nullptr, (int *)nullptr, (double *)nullptr, aux, aux + 2);
+                       coord, tol, (const double *)d_hq, bm, bo, (const int *)K->d_rp, (const int *)K->d_ci,
+                       (const double *)K->d_v, K->nnz ? K->nrows : 0, cnt, (const int *)nullptr, (int *)nullptr,
+                       (double *)nullptr, aux, aux + 2);
   int64_t nnz = 0;
   if ((rc = exclusive_scan_i32(ctx, cnt, R->d_rp, nrows, &nnz))) {
     cleanup();
@@ -426,8 +677,9 @@ int assemble_robin_dev(tfm_ctx *ctx, const tfm_mesh *cmesh, const tfm_operator *
   if (nrows > 0 && nnz > 0)
     hipLaunchKernelGGL(k_robin<1>, dim3(g), dim3(128), 0, st, mesh->d_coords, mesh->pdim, mesh->d_conn, mesh->nn,
                        mesh->cell, mesh->d_n2e_ptr, mesh->d_n2e, nrows, dpn, (const double *)d_rule, nq, h, axis,
-                       coord, tol, (const int *)K->d_rp, (const int *)K->d_ci, (const double *)K->d_v, K->nnz ? K->nrows : 0,
-                       (int *)nullptr, (const int *)R->d_rp, R->d_ci, R->d_v, aux, aux + 2);
+                       coord, tol, (const double *)d_hq, bm, bo, (const int *)K->d_rp, (const int *)K->d_ci,
+                       (const double *)K->d_v, K->nnz ? K->nrows : 0, (int *)nullptr, (const int *)R->d_rp, R->d_ci,
+                       R->d_v, aux, aux + 2);
   hipError_t e = hipStreamSynchronize(st);
   cleanup();
   if (e != hipSuccess) {
@@ -448,7 +700,24 @@ extern "C" int tfm_assemble_robin(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_
   if (axis >= mesh->pdim) return fail(ctx, TFM_E_INVALID, "Robin band axis beyond the coordinates");
   int rc = set_device(ctx);
   if (rc) return rc;
-  return assemble_robin_dev(ctx, mesh, op, dpn, h, axis, coord, tol, out);
+  return assemble_robin_dev(ctx, mesh, op, dpn, h, axis, coord, tol, 2, nullptr, out);
+}
+
+extern "C" int tfm_assemble_robin_qp(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
+                                     int order, const double *hq, tfm_matrix **out) {
+  if (!ctx || !mesh || !op || !out || !hq || dpn < 1 || dpn > 3) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  return assemble_robin_dev(ctx, mesh, op, dpn, 0.0, -1, 0.0, 0.0, order, hq, out);
+}
+
+extern "C" int tfm_mesh_boundary_points(const tfm_mesh *mesh, int order, double *x, double *normal,
+                                        int64_t *count) {
+  if (!mesh || !count) return TFM_E_INVALID;
+  tfm_ctx *ctx = mesh->ctx;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  return boundary_points_dev(ctx, const_cast<tfm_mesh *>(mesh), order, x, normal, count);
 }
 
 static int host_rhs_roundtrip(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, double *rhs,
@@ -493,6 +762,55 @@ extern "C" int tfm_load_body(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order,
                             &a);
 }
 
+extern "C" int tfm_mesh_quad_points(const tfm_mesh *mesh, int quad_order, double *xq, int64_t *nq) {
+  if (!mesh || !nq) return TFM_E_INVALID;
+  tfm_ctx *ctx = mesh->ctx;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  int q = 0;
+  if ((rc = element_rule_points(ctx, mesh->cell, quad_order, &q))) return rc;
+  *nq = q;
+  if (!xq || mesh->n_elem == 0) return TFM_OK;
+  double *d_rule = nullptr, *d_x = nullptr;
+  if ((rc = upload_rule(ctx, mesh->cell, quad_order, &d_rule, &q))) return rc;
+  const int64_t nt = mesh->n_elem * q;
+  if ((rc = dalloc(ctx, &d_x, (size_t)nt * mesh->pdim))) { dfree(d_rule); return rc; }
+  hipLaunchKernelGGL(k_qp_coords, dim3(grid1(nt, 256)), dim3(256), 0, ctx->stream, mesh->d_coords, mesh->pdim,
+                     mesh->d_conn, mesh->nn, mesh->n_elem, d_rule, q, d_x);
+  hipError_t e = hipMemcpyAsync(xq, d_x, sizeof(double) * nt * mesh->pdim, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dfree(d_rule);
+  dfree(d_x);
+  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, std::string("quadrature points: ") + hipGetErrorString(e));
+  return TFM_OK;
+}
+
+struct QpArg { int order; const double *qv; };
+
+extern "C" int tfm_load_source_qp(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, const double *sq,
+                                  double *rhs) {
+  if (!ctx || !mesh || !rhs || (!sq && mesh->n_elem > 0)) return TFM_E_INVALID;
+  QpArg a{quad_order, sq};
+  return host_rhs_roundtrip(ctx, mesh, 1, rhs,
+                            [](tfm_ctx *c, tfm_mesh *m, double *d, const void *p) {
+                              const QpArg *s = (const QpArg *)p;
+                              return load_source_qp_dev(c, m, s->order, s->qv, d);
+                            },
+                            &a);
+}
+
+extern "C" int tfm_load_body_qp(tfm_ctx *ctx, const tfm_mesh *mesh, int quad_order, const double *bq,
+                                double *rhs) {
+  if (!ctx || !mesh || !rhs || (!bq && mesh->n_elem > 0)) return TFM_E_INVALID;
+  QpArg a{quad_order, bq};
+  return host_rhs_roundtrip(ctx, mesh, mesh->dim, rhs,
+                            [](tfm_ctx *c, tfm_mesh *m, double *d, const void *p) {
+                              const QpArg *s = (const QpArg *)p;
+                              return load_body_qp_dev(c, m, s->order, s->qv, d);
+                            },
+                            &a);
+}
+
 extern "C" int tfm_load_neumann(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, double g, double *rhs) {
   if (!ctx || !mesh || !rhs || dpn < 1 || dpn > 3) return TFM_E_INVALID;
   NeuArg a{dpn, -1, g, -1, 0.0, 0.0};
@@ -501,6 +819,20 @@ extern "C" int tfm_load_neumann(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, dou
                               const NeuArg *s = (const NeuArg *)p;
                               return load_neumann_dev(c, m, s->dpn, s->comp, s->g, s->axis, s->coord, s->tol, d,
                                                       false);
+                            },
+                            &a);
+}
+
+struct NeuQpArg { int dpn, order; const double *gq; };
+
+extern "C" int tfm_load_neumann_qp(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, int order, const double *gq,
+                                   double *rhs) {
+  if (!ctx || !mesh || !rhs || !gq || dpn < 1 || dpn > 3) return TFM_E_INVALID;
+  NeuQpArg a{dpn, order, gq};
+  return host_rhs_roundtrip(ctx, mesh, dpn, rhs,
+                            [](tfm_ctx *c, tfm_mesh *m, double *d, const void *p) {
+                              const NeuQpArg *s = (const NeuQpArg *)p;
+                              return load_neumann_qp_dev(c, m, s->dpn, s->order, s->gq, d);
                             },
                             &a);
 }

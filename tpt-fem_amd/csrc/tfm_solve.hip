@@ -1197,6 +1197,7 @@ extern "C" int tfm_system_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_o
   if ((flags & TFM_SYS_FP64) && (rc = dalloc(ctx, &s->d_v64, s->nnz))) { tfm_system_destroy(s); return rc; }
   if ((flags & TFM_SYS_FP32) && (rc = dalloc(ctx, &s->d_v32, s->nnz))) { tfm_system_destroy(s); return rc; }
   if ((rc = plan_numeric(ctx, &s->plan, s->d_v64, s->d_v32))) { tfm_system_destroy(s); return rc; }
+  s->note_values(true, s->d_v64 != nullptr, s->d_v32 != nullptr);
   if ((rc = dalloc(ctx, &s->d_f, s->ndof))) { tfm_system_destroy(s); return rc; }
   // Homogeneous Dirichlet values: rhs[f] - K[f,c] * 0 leaves every free entry
   // unchanged, so fred is rhs restricted to the free DOFs.
@@ -1224,8 +1225,11 @@ extern "C" int tfm_system_assemble(tfm_system *s, const tfm_operator *op, int wh
                                         sizeof(double) * 64, hipMemcpyHostToDevice, ctx->stream));
   }
   if (which == 0) which = TFM_SYS_FP64 | TFM_SYS_FP32;
-  return plan_numeric(ctx, &s->plan, (which & TFM_SYS_FP64) ? s->d_v64 : nullptr,
-                      (which & TFM_SYS_FP32) ? s->d_v32 : nullptr);
+  double *w64 = (which & TFM_SYS_FP64) ? s->d_v64 : nullptr;
+  float *w32 = (which & TFM_SYS_FP32) ? s->d_v32 : nullptr;
+  if ((rc = plan_numeric(ctx, &s->plan, w64, w32))) return rc;
+  s->note_values(op != nullptr, w64 != nullptr, w32 != nullptr);
+  return TFM_OK;
 }
 
 extern "C" int tfm_system_info(const tfm_system *s, int64_t *n_free, int64_t *nnz, int64_t *n_dof) {
@@ -1293,6 +1297,8 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
     return fail(ctx, TFM_E_NUMERIC,
                 "sparse numeric factorization/solve failed: rhs length 0 does not match matrix dimension 1");
   if (mixed && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values (TFM_SYS_FP32)");
+  if (mixed && !s->fp32_current())
+    return fail(ctx, TFM_E_INVALID, "mixed solve: the FP32 values predate the last FP64 re-assembly");
   if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values (TFM_SYS_FP64)");
   hipStream_t st = ctx->stream;
   // rows per CSR-stream chunk, from the longest row
@@ -1514,6 +1520,71 @@ extern "C" int tfm_solve_poisson_nr(tfm_ctx *ctx, const tfm_mesh *mesh, double k
   tfm_matrix_destroy(K);
   if (rc) return rc;
   tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 40};
+  if (opts) o = *opts;
+  rc = tfm_system_solve(s, &o, u, nullptr);
+  tfm_system_destroy(s);
+  return rc;
+}
+
+extern "C" int tfm_solve_poisson_qp(tfm_ctx *ctx, const tfm_mesh *mesh, double kappa, int quad_order,
+                                    const double *sq, const double *neumann_gq, const double *robin_hq, int64_t n_bc,
+                                    const int64_t *bc_dof, const double *bc_val, const tfm_solve_opts *opts,
+                                    double *u) {
+  if (!ctx || !mesh || !u || (!sq && mesh->n_elem > 0)) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  tfm_mesh *m = const_cast<tfm_mesh *>(mesh);
+  const int64_t n = mesh->n_nodes;
+  double *d_rhs = nullptr;
+  if ((rc = dalloc(ctx, &d_rhs, n))) return rc;
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(d_rhs, 0, sizeof(double) * std::max<int64_t>(n, 1), ctx->stream));
+  rc = load_source_qp_dev(ctx, m, quad_order, sq, d_rhs);
+  // apply_neumann / apply_robin use FACE_QUAD_ORDER = 2 (assembly/src/lib.rs:596, :679)
+  if (!rc && neumann_gq) rc = load_neumann_qp_dev(ctx, m, 1, 2, neumann_gq, d_rhs);
+  std::vector<double> rhs((size_t)n, 0.0);
+  if (!rc && n > 0 && hipMemcpy(rhs.data(), d_rhs, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(ctx, TFM_E_HIP, "rhs download failed");
+  dfree(d_rhs);
+  if (rc) return rc;
+  tfm_operator op{};
+  op.kind = TFM_OP_POISSON;
+  op.quad_order = quad_order;
+  op.coef = kappa;
+  tfm_system *s = nullptr;
+  if (robin_hq) {
+    tfm_matrix *K = nullptr;
+    if ((rc = assemble_robin_dev(ctx, mesh, &op, 1, 0.0, -1, 0.0, 0.0, 2, robin_hq, &K))) return rc;
+    rc = tfm_system_from_matrix(ctx, K, n, rhs.data(), n_bc, bc_dof, bc_val, &s);
+    tfm_matrix_destroy(K);
+  } else {
+    rc = tfm_system_create(ctx, mesh, &op, 1, n_bc, bc_dof, bc_val, rhs.data(), TFM_SYS_FP64, &s);
+  }
+  if (rc) return rc;
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 0};
+  if (opts) o = *opts;
+  rc = tfm_system_solve(s, &o, u, nullptr);
+  tfm_system_destroy(s);
+  return rc;
+}
+
+extern "C" int tfm_solve_elasticity_qp(tfm_ctx *ctx, const tfm_mesh *mesh, int model, double young, double nu,
+                                       int quad_order, const double *bq, int64_t n_bc, const int64_t *bc_dof,
+                                       const double *bc_val, const tfm_solve_opts *opts, double *u) {
+  if (!ctx || !mesh || !u || (!bq && mesh->n_elem > 0)) return TFM_E_INVALID;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  const int dim = mesh->dim;
+  std::vector<double> rhs((size_t)(mesh->n_nodes * dim), 0.0);
+  if ((rc = tfm_load_body_qp(ctx, mesh, quad_order, bq, rhs.data()))) return rc;
+  tfm_operator op{};
+  op.kind = TFM_OP_ELASTICITY;
+  op.model = model;
+  op.quad_order = quad_order;
+  op.coef = young;
+  op.nu = nu;
+  tfm_system *s = nullptr;
+  if ((rc = tfm_system_create(ctx, mesh, &op, dim, n_bc, bc_dof, bc_val, rhs.data(), TFM_SYS_FP64, &s))) return rc;
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 0};
   if (opts) o = *opts;
   rc = tfm_system_solve(s, &o, u, nullptr);
   tfm_system_destroy(s);

@@ -226,7 +226,8 @@ extern "C" int tfm_topopt_simp_batch(tfm_ctx *ctx, int64_t P, const tfm_topopt_p
     for (int64_t p = 0; p < P; p++) stats[p] = tfm_solve_stats{0, 0, 0.0, 0.0};
   for (int64_t k = 0; k < nc && !rc; k++) {
     // iteration 1 re-solves the uniform start (:356-367): same system, same u
-    if (k >= 2) rc = plan_numeric(ctx, &sys->plan, sys->d_v64, mixed ? sys->d_v32 : nullptr);
+    if (k >= 2 && !(rc = plan_numeric(ctx, &sys->plan, sys->d_v64, mixed ? sys->d_v32 : nullptr)))
+      sys->note_values(true, true, mixed && sys->d_v32);
     if (!rc && k != 1 && !(rc = bsolver_refresh(bs))) rc = bsolver_run(bs, o, k >= 2, ss.data(), fu.data());
     if (rc) break;
     for (int64_t p = 0; p < P; p++) {

@@ -101,6 +101,7 @@ _SIGS = {
     "tfm_ctx_synchronize": (INT, [P]),
     "tfm_ctx_timer_begin": (INT, [P]),
     "tfm_ctx_timer_end": (INT, [P, C.POINTER(D)]),
+    "tfm_debug_scan_total": (INT, [P, P, I64, P]),
     "tfm_mesh_create": (INT, [P, INT, I64, INT, P, I64, P, PP]),
     "tfm_mesh_box": (INT, [P, P, P, I64, I64, I64, PP]),
     "tfm_mesh_grid": (INT, [P, I64, I64, D, PP]),
@@ -119,6 +120,14 @@ _SIGS = {
     "tfm_load_source": (INT, [P, P, INT, D, P]),
     "tfm_load_body": (INT, [P, P, INT, P, P]),
     "tfm_load_neumann": (INT, [P, P, INT, D, P]),
+    "tfm_mesh_quad_points": (INT, [P, INT, P, P]),
+    "tfm_load_source_qp": (INT, [P, P, INT, P, P]),
+    "tfm_load_body_qp": (INT, [P, P, INT, P, P]),
+    "tfm_mesh_boundary_points": (INT, [P, INT, P, P, P]),
+    "tfm_load_neumann_qp": (INT, [P, P, INT, INT, P, P]),
+    "tfm_assemble_robin_qp": (INT, [P, P, C.POINTER(Operator), INT, INT, P, PP]),
+    "tfm_solve_poisson_qp": (INT, [P, P, D, INT, P, P, P, I64, P, P, C.POINTER(SolveOpts), P]),
+    "tfm_solve_elasticity_qp": (INT, [P, P, INT, D, D, INT, P, I64, P, P, C.POINTER(SolveOpts), P]),
     "tfm_load_plane_traction": (INT, [P, P, INT, INT, D, D, P, P]),
     "tfm_system_create": (INT, [P, P, C.POINTER(Operator), INT, I64, P, P, P, INT, PP]),
     "tfm_system_from_matrix": (INT, [P, P, I64, P, I64, P, P, PP]),
@@ -219,6 +228,13 @@ class Context:
         self.check(_lib.tfm_ctx_timer_end(self.h, C.byref(ms)))
         return ms.value
 
+    def scan_total(self, counts):
+        """The library's device prefix sum of int32 counts (its int32 range guard)."""
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        t = I64()
+        self.check(_lib.tfm_debug_scan_total(self.h, _ptr(c), len(c), C.byref(t)))
+        return t.value
+
 
 class Mesh:
     """Device-resident mesh (crates/tpt-fem-mesh/src/lib.rs:39-148)."""
@@ -278,6 +294,25 @@ class Mesh:
         conn = np.zeros((self.n_elem, NODES[self.cell]), dtype=np.int64)
         self.ctx.check(_lib.tfm_mesh_download(self.h, _ptr(coords), _ptr(conn)))
         return coords, conn
+
+    def quad_points(self, quad_order=2):
+        """Positions of every element quadrature point, (n_elem, nq, pdim): the
+        points the reference calls a source / body-force closure at."""
+        nq = I64()
+        self.ctx.check(_lib.tfm_mesh_quad_points(self.h, quad_order, None, C.byref(nq)))
+        x = np.zeros((max(1, self.n_elem), nq.value, self.pdim))
+        self.ctx.check(_lib.tfm_mesh_quad_points(self.h, quad_order, _ptr(x), C.byref(nq)))
+        return x[: self.n_elem]
+
+    def boundary_points(self, order=2):
+        """Every boundary-face quadrature point's position and outward unit normal,
+        two (count, pdim) arrays: the (x, n) a Neumann / Robin closure is called with."""
+        cnt = I64()
+        self.ctx.check(_lib.tfm_mesh_boundary_points(self.h, order, None, None, C.byref(cnt)))
+        x = np.zeros((max(1, cnt.value), self.pdim))
+        n = np.zeros((max(1, cnt.value), self.pdim))
+        self.ctx.check(_lib.tfm_mesh_boundary_points(self.h, order, _ptr(x), _ptr(n), C.byref(cnt)))
+        return x[: cnt.value], n[: cnt.value]
 
     def nodes_on_plane(self, axis, coord, tol):
         out = np.zeros(max(1, self.n_nodes), dtype=np.int64)
@@ -384,9 +419,46 @@ def body_vector(mesh, b, quad_order=2, rhs=None):
     return rhs
 
 
-def apply_neumann(mesh, dofs_per_node, g, rhs):
+def _values_at(fn, pts, ncomp, vectorized, normals=None):
+    """A closure's values at points (n, pdim) [and normals]: one call per point
+    as the reference makes them, or one call on the whole array (vectorized)."""
+    if vectorized:
+        v = fn(pts) if normals is None else fn(pts, normals)
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (len(pts),) + ((ncomp,) if ncomp > 1 else ())))
+    if normals is None:
+        vals = [fn(x) for x in pts]
+    else:
+        vals = [fn(x, n) for x, n in zip(pts, normals)]
+    out = np.asarray(vals, dtype=np.float64).reshape(len(pts), -1)
+    return np.ascontiguousarray(out[:, :ncomp] if ncomp > 1 else out[:, 0])
+
+
+def source_vector_qp(mesh, sq, quad_order=2, rhs=None):
+    """rhs += poisson_source_vector scattered, from the source's values sq (n_elem, nq)."""
+    rhs = np.zeros(mesh.n_nodes) if rhs is None else _f64(rhs).copy()
+    sq = _f64(sq)
+    mesh.ctx.check(_lib.tfm_load_source_qp(mesh.ctx.h, mesh.h, quad_order, _ptr(sq), _ptr(rhs)))
+    return rhs
+
+
+def body_vector_qp(mesh, bq, quad_order=2, rhs=None):
+    """rhs += elasticity_body_vector scattered, from the body force's values bq (n_elem, nq, dim)."""
+    rhs = np.zeros(mesh.n_nodes * mesh.dim) if rhs is None else _f64(rhs).copy()
+    bq = _f64(bq)
+    mesh.ctx.check(_lib.tfm_load_body_qp(mesh.ctx.h, mesh.h, quad_order, _ptr(bq), _ptr(rhs)))
+    return rhs
+
+
+def apply_neumann(mesh, dofs_per_node, g, rhs, order=2, vectorized=False):
+    """apply_neumann / apply_neumann_order (crates/tpt-fem-assembly/src/lib.rs:591-667):
+    g a constant flux, or a closure g(x, n) evaluated at the boundary points."""
     rhs = _f64(rhs).copy()
-    mesh.ctx.check(_lib.tfm_load_neumann(mesh.ctx.h, mesh.h, dofs_per_node, g, _ptr(rhs)))
+    if not callable(g) and order == 2:
+        mesh.ctx.check(_lib.tfm_load_neumann(mesh.ctx.h, mesh.h, dofs_per_node, g, _ptr(rhs)))
+        return rhs
+    x, n = mesh.boundary_points(order)
+    gq = _values_at(g, x, 1, vectorized, n) if callable(g) else np.full(len(x), float(g))
+    mesh.ctx.check(_lib.tfm_load_neumann_qp(mesh.ctx.h, mesh.h, dofs_per_node, order, _ptr(gq), _ptr(rhs)))
     return rhs
 
 
@@ -509,26 +581,59 @@ def solve_with_dirichlet(matrix, rhs, bcs, rtol=1e-13):
         s.close()
 
 
-def solve_poisson(mesh, kappa, quad_order, f, bcs, rtol=1e-13, mode=SOLVE_PCG64):
-    """solve_poisson (crates/tpt-fem-thermal/src/lib.rs:206-240), constant source, no Neumann/Robin.
+def solve_poisson(mesh, kappa, quad_order, f, bcs, rtol=1e-13, mode=SOLVE_PCG64, neumann=None, robin=None,
+                  vectorized=False):
+    """solve_poisson (crates/tpt-fem-thermal/src/lib.rs:206-240).
 
-    mode SOLVE_PCG64_MG solves with the multigrid CG (meshes from Mesh.box)."""
+    f is a constant source or a closure s(x); neumann and robin are None or
+    closures g(x, n) / h(x, n), evaluated at the points the reference calls
+    them at (Mesh.quad_points, Mesh.boundary_points) -- one call per point, or
+    one call on the whole (m, pdim) arrays with vectorized=True.  mode
+    SOLVE_PCG64_MG solves with the multigrid CG (meshes from Mesh.box,
+    constant source, no Neumann/Robin terms)."""
     nb, bd, bv = _bc_arrays(bcs)
     u = np.zeros(mesh.n_nodes)
-    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
-    mesh.ctx.check(_lib.tfm_solve_poisson(mesh.ctx.h, mesh.h, kappa, quad_order, f, nb, _ptr(bd), _ptr(bv),
-                                          C.byref(o), _ptr(u)))
+    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=0)
+    if not callable(f) and neumann is None and robin is None:
+        mesh.ctx.check(_lib.tfm_solve_poisson(mesh.ctx.h, mesh.h, kappa, quad_order, f, nb, _ptr(bd), _ptr(bv),
+                                              C.byref(o), _ptr(u)))
+        return u
+    xq = mesh.quad_points(quad_order)
+    nq = xq.shape[1]
+    if callable(f):
+        sq = _values_at(f, xq.reshape(-1, mesh.pdim), 1, vectorized).reshape(mesh.n_elem, nq)
+    else:
+        sq = np.full((mesh.n_elem, nq), float(f))
+    gq = hq = None
+    if neumann is not None or robin is not None:
+        xb, nrm = mesh.boundary_points(2)
+        if neumann is not None:
+            gq = _values_at(neumann, xb, 1, vectorized, nrm)
+        if robin is not None:
+            hq = _values_at(robin, xb, 1, vectorized, nrm)
+    mesh.ctx.check(_lib.tfm_solve_poisson_qp(mesh.ctx.h, mesh.h, kappa, quad_order, _ptr(sq), _ptr(gq), _ptr(hq),
+                                             nb, _ptr(bd), _ptr(bv), C.byref(o), _ptr(u)))
     return u
 
 
-def solve_elasticity(mesh, model, young, nu, quad_order, body, bcs, rtol=1e-13, mode=SOLVE_PCG64):
-    """solve_elasticity (crates/tpt-fem-elasticity/src/lib.rs:355-381), constant body force.
+def solve_elasticity(mesh, model, young, nu, quad_order, body, bcs, rtol=1e-13, mode=SOLVE_PCG64,
+                     vectorized=False):
+    """solve_elasticity (crates/tpt-fem-elasticity/src/lib.rs:355-381).
 
-    mode SOLVE_PCG64_MG solves with the multigrid CG (meshes from Mesh.box)."""
+    body is a constant vector or a closure b(x) returning dim components,
+    evaluated at Mesh.quad_points.  mode SOLVE_PCG64_MG solves with the
+    multigrid CG (meshes from Mesh.box, constant body force)."""
     nb, bd, bv = _bc_arrays(bcs)
     u = np.zeros(mesh.n_nodes * mesh.dim)
+    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=0)
+    if callable(body):
+        xq = mesh.quad_points(quad_order)
+        bq = _values_at(body, xq.reshape(-1, mesh.pdim), mesh.dim, vectorized)
+        bq = np.ascontiguousarray(bq.reshape(mesh.n_elem, xq.shape[1], mesh.dim))
+        mesh.ctx.check(_lib.tfm_solve_elasticity_qp(mesh.ctx.h, mesh.h, model, young, nu, quad_order, _ptr(bq), nb,
+                                                    _ptr(bd), _ptr(bv), C.byref(o), _ptr(u)))
+        return u
     b = _f64(list(body) + [0.0] * (3 - len(body)))
-    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
     mesh.ctx.check(_lib.tfm_solve_elasticity(mesh.ctx.h, mesh.h, model, young, nu, quad_order, _ptr(b), nb,
                                              _ptr(bd), _ptr(bv), C.byref(o), _ptr(u)))
     return u
@@ -627,10 +732,17 @@ def newmark(m, c, k, u0, v0, f, dt, beta, gamma, nsteps, fscale=None, rtol=1e-12
     return hist
 
 
-def assemble_robin(mesh, dofs_per_node, op, h, axis=-1, coord=0.0, tol=0.0):
+def assemble_robin(mesh, dofs_per_node, op, h, axis=-1, coord=0.0, tol=0.0, order=2, vectorized=False):
     """to_csr() of assemble(...) followed by apply_robin(mesh, dpn, h, coo)
-    (crates/tpt-fem-assembly/src/lib.rs:674-753), h constant or on a plane band."""
+    (crates/tpt-fem-assembly/src/lib.rs:674-753): h constant (or on a plane
+    band), or a closure h(x, n) evaluated at the boundary points of `order`."""
     out = C.c_void_p()
+    if callable(h) or order != 2:
+        x, n = mesh.boundary_points(order)
+        hq = _values_at(h, x, 1, vectorized, n) if callable(h) else np.full(len(x), float(h))
+        mesh.ctx.check(_lib.tfm_assemble_robin_qp(mesh.ctx.h, mesh.h, C.byref(op), dofs_per_node, order, _ptr(hq),
+                                                  C.byref(out)))
+        return Matrix(mesh.ctx, out)
     mesh.ctx.check(_lib.tfm_assemble_robin(mesh.ctx.h, mesh.h, C.byref(op), dofs_per_node, h, axis, coord, tol,
                                            C.byref(out)))
     return Matrix(mesh.ctx, out)
