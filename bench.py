@@ -184,7 +184,7 @@ def bench_batch(args, T, log):
         modes.append((T.SOLVE_PCG64_MG, "multigrid_fp64", 1e-12))
     modes = [m for m in modes if m[1].split("_")[0] in args.solvers]
     for mode, name, rtol in modes:
-        u = s.solve_batch(off, rhs=f, nrhs=nrhs, mode=mode, rtol=rtol, max_iter=200000, max_outer=400)
+        u = s.solve_batch(off, rhs=f, nrhs=nrhs, mode=mode, rtol=rtol, max_iter=200000)
         st = s.batch_stats
         rec = {"rtol": rtol, "s": st[0][3], "max_iters": max(x[0] for x in st),
                "max_rel_res": max(x[2] for x in st)}
@@ -367,7 +367,7 @@ def main():
             for rtol in args.rtol_sweep:
                 ctx.synchronize()
                 w0 = time.perf_counter()
-                u = sysm.solve(mode, rtol=rtol, max_iter=200000, inner_rtol=args.inner_rtol, max_outer=400)
+                u = sysm.solve(mode, rtol=rtol, max_iter=200000, inner_rtol=args.inner_rtol)
                 wall_s = time.perf_counter() - w0
                 sr = sysm.stats
                 err = float(np.linalg.norm(u - u_ref) / nref)

@@ -213,18 +213,25 @@ int tfm_system_info(const tfm_system *sys, int64_t *n_free, int64_t *nnz, int64_
 int tfm_system_download(const tfm_system *sys, int64_t *row_ptrs, int64_t *col_ind,
                         double *values, double *fred, int64_t *free_dofs);
 
-/* TFM_SOLVE_PCG64_MG: FP64 CG preconditioned by a geometric multigrid
-   V-cycle; batched SIMP systems on a tfm_mesh_grid_batch mesh only (one
-   problem per design), TFM_E_UNSUPPORTED elsewhere. */
-enum { TFM_SOLVE_PCG64 = 0, TFM_SOLVE_MIXED = 1, TFM_SOLVE_PCG64_MG = 2 };
+/* TFM_SOLVE_PCG64: Jacobi-preconditioned FP64 CG.  TFM_SOLVE_MIXED: FP32 CG
+   with FP64 residual refinement.  TFM_SOLVE_PCG64_MG: FP64 CG preconditioned
+   by a geometric multigrid V-cycle, for systems on a tfm_mesh_box mesh (one
+   system, one rhs) or on a tfm_mesh_grid_batch mesh (SIMP designs, one
+   problem per design); TFM_E_UNSUPPORTED elsewhere.  TFM_SOLVE_DIRECT: the
+   reference's dense partial-pivot LU (sparse/src/lib.rs:244-278) on the GPU,
+   bit-identical to that loop run sequentially, for at most 8000 free DOFs
+   (TFM_E_UNSUPPORTED above).  PCG64 and MIXED fall back to DIRECT on such
+   small systems when CG breaks down (the reduced matrix is not SPD) or
+   stalls, as the reference solves every system directly. */
+enum { TFM_SOLVE_PCG64 = 0, TFM_SOLVE_MIXED = 1, TFM_SOLVE_PCG64_MG = 2, TFM_SOLVE_DIRECT = 3 };
 typedef struct {
   int mode;           /* TFM_SOLVE_*                                          */
   double rtol;        /* stop when ||f - K u|| <= rtol * ||f|| (FP64 residual) */
   int max_iter;       /* total inner iterations                               */
   double inner_rtol;  /* mixed: FP32 residual reduction between FP64 residual
                          replacements (outside (0,1): 0.3), per segment when
-                         batched                                              */
-  int max_outer;      /* unused (replacements are driven by inner_rtol)       */
+                         batched; refinement cycles run until rtol is met or
+                         max_iter inner iterations are spent                  */
 } tfm_solve_opts;
 typedef struct {
   int iterations;     /* inner PCG iterations in total          */
@@ -249,10 +256,14 @@ int tfm_mesh_box_batch(tfm_ctx *ctx, int64_t P, const int64_t *n3, tfm_mesh **ou
 int tfm_mesh_grid_batch(tfm_ctx *ctx, int64_t P, int64_t nx, int64_t ny, double h,
                         tfm_mesh **out);
 /* Solve a block-diagonal system (P problems, DOF ranges dof_off[p]..dof_off[p+1])
- * for nrhs (1 or 8) right-hand sides each, every (problem, rhs) segment with
- * its own PCG scalars and convergence.  rhs: host [nrhs][n_dof] or NULL (the
- * system's own rhs, nrhs = 1); u: host [nrhs][n_dof]; stats: [P*nrhs]
- * (segment p*nrhs + k).  Homogeneous Dirichlet values only. */
+ * for nrhs >= 1 right-hand sides each, every (problem, rhs) segment with its
+ * own PCG scalars and convergence (the kernels interleave 8 right-hand
+ * sides: other counts run as chunks of 8 and then one at a time).  rhs: host
+ * [nrhs][n_dof] or NULL (the system's own rhs, nrhs = 1); u: host
+ * [nrhs][n_dof]; stats: [P*nrhs] (segment p*nrhs + k).  Homogeneous
+ * Dirichlet values only.  With TFM_SOLVE_DIRECT (P = 1 or any block-diagonal
+ * system up to 8000 free DOFs) this is solve_multi (sparse/src/lib.rs:
+ * 244-278): one dense LU, every right-hand side solved against it. */
 int tfm_system_solve_batch(tfm_system *sys, int64_t P, const int64_t *dof_off, int nrhs,
                            const double *rhs, const tfm_solve_opts *opts, double *u,
                            tfm_solve_stats *stats);

@@ -80,7 +80,7 @@ def test_batched_simp_multi_rhs_matches_direct(ctx):
     s = T.System.create(m, op, 2, bcs, None, flags=T.SYS_FP64 | T.SYS_FP32)
     off = np.arange(P + 1) * ndof1
     u = s.solve_batch(off, rhs=f, nrhs=8, mode=T.SOLVE_PCG64, rtol=1e-12)
-    um = s.solve_batch(off, rhs=f, nrhs=8, mode=T.SOLVE_MIXED, rtol=1e-10, max_outer=200)
+    um = s.solve_batch(off, rhs=f, nrhs=8, mode=T.SOLVE_MIXED, rtol=1e-10)
     coords, conn = O.grid(nx, ny, 1.0)
     ke0 = O.simp_ke_unit(0.3, 1.0)
     lb = [(2 * j * nx + c, 0.0) for j in range(ny) for c in (0, 1)]

@@ -1627,17 +1627,53 @@ extern "C" int tfm_mesh_grid_batch(tfm_ctx *ctx, int64_t P, int64_t nx, int64_t 
 
 extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *dof_off, int nrhs, const double *rhs,
                                       const tfm_solve_opts *opts, double *u, tfm_solve_stats *stats) {
-  if (!s || !dof_off || !u || P <= 0) return TFM_E_INVALID;
+  if (!s || !dof_off || !u || P <= 0 || nrhs < 1) return TFM_E_INVALID;
   tfm_ctx *ctx = s->ctx;
-  if (nrhs != 1 && nrhs != 8) return fail(ctx, TFM_E_UNSUPPORTED, "batched solve supports 1 or 8 right-hand sides");
   if (dof_off[0] != 0 || dof_off[P] != s->ndof) return fail(ctx, TFM_E_INVALID, "dof offsets must cover the system");
   for (int64_t p = 0; p < P; p++)
     if (dof_off[p + 1] < dof_off[p]) return fail(ctx, TFM_E_INVALID, "dof offsets must ascend");
   if (nrhs > 1 && !rhs) return fail(ctx, TFM_E_INVALID, "several right-hand sides need rhs");
   int rc = set_device(ctx);
   if (rc) return rc;
-  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 0.3, 60};
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 0.3};
   if (opts) o = *opts;
+  const int64_t n = s->ndof;
+  if (o.mode == TFM_SOLVE_DIRECT) {
+    // solve_multi: one factorisation of the whole (block-diagonal) system,
+    // every right-hand side solved against it
+    double *d_F = nullptr;
+    if (rhs) {
+      if ((rc = dalloc(ctx, &d_F, (size_t)n * nrhs))) return rc;
+      if (hipMemcpy(d_F, rhs, sizeof(double) * n * nrhs, hipMemcpyHostToDevice) != hipSuccess) {
+        dfree(d_F);
+        return fail(ctx, TFM_E_HIP, "rhs upload failed");
+      }
+    }
+    std::vector<tfm_solve_stats> st((size_t)nrhs);
+    rc = system_solve_direct(s, nrhs, d_F, u, st.data());
+    dfree(d_F);
+    if (!rc && stats)
+      for (int64_t p = 0; p < P; p++)
+        for (int k = 0; k < nrhs; k++) stats[p * nrhs + k] = st[k];
+    return rc;
+  }
+  if (nrhs != 1 && nrhs != 8) {
+    // the kernels are built for 1 and 8 interleaved right-hand sides: chunks
+    // of 8, the remainder one at a time
+    std::vector<tfm_solve_stats> st((size_t)P * 8);
+    int worst = TFM_OK;
+    for (int k0 = 0; k0 < nrhs;) {
+      const int c = nrhs - k0 >= 8 ? 8 : 1;
+      rc = tfm_system_solve_batch(s, P, dof_off, c, rhs + (size_t)k0 * n, &o, u + (size_t)k0 * n, st.data());
+      if (rc && rc != TFM_E_NOT_CONVERGED) return rc;
+      if (rc) worst = rc;
+      if (stats)
+        for (int64_t p = 0; p < P; p++)
+          for (int j = 0; j < c; j++) stats[p * nrhs + k0 + j] = st[(size_t)p * c + j];
+      k0 += c;
+    }
+    return worst;
+  }
   // A segment's rows must not couple to another problem's rows (block
   // diagonal): the kernels reduce per problem and would silently mix them.
   BatchSolver *b = nullptr;
@@ -1651,7 +1687,6 @@ extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *d
     bsolver_destroy(b);
     return rc;
   }
-  const int64_t n = s->ndof;
   if (nrhs == 1) {
     // x is 0 at the fixed DOFs: u = x, then the Dirichlet values in bcs order
     hipError_t e = hipMemcpy(u, b->x, sizeof(double) * n, hipMemcpyDeviceToHost);

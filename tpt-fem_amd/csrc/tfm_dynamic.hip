@@ -259,7 +259,7 @@ extern "C" int tfm_newmark(const tfm_matrix *M, const tfm_matrix *C, const tfm_m
   hipStream_t st = ctx->stream;
   const int nn = (int)n;
   const unsigned g = (unsigned)((nn + 255) / 256);
-  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-12, 100000, 0.3, 60};
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-12, 100000, 0.3};
   if (opts) o = *opts;
   o.mode = TFM_SOLVE_PCG64;
   double *F = nullptr, *buf = nullptr, *hist = nullptr;

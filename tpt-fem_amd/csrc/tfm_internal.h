@@ -208,6 +208,16 @@ void bsolver_destroy(BatchSolver *b);
 // the solver (sliced copy, hierarchy) is kept on the system for later solves.
 int system_solve_mg(tfm_system *s, const tfm_solve_opts &o, double *u, tfm_solve_stats *stats);
 
+// Dense partial-pivot LU of the reduced system (tfm_dense.hip), the
+// reference's direct solve, for at most kDirectMax free DOFs (x stays in one
+// workgroup's LDS during back substitution): d_F [nrhs][ndof] right-hand
+// sides (0 at fixed DOFs) -> d_x [nrhs][ndof] (0 at fixed DOFs).
+constexpr int64_t kDirectMax = 8000;
+int direct_solve_dev(tfm_system *s, int nrhs, const double *d_F, double *d_x);
+// The same with the FP64 certificate and the Dirichlet scatter into host u
+// [nrhs][ndof]; d_F NULL: the system's own rhs (nrhs = 1).
+int system_solve_direct(tfm_system *s, int nrhs, const double *d_F, double *u, tfm_solve_stats *stats);
+
 // Geometric multigrid for P batched topopt grids (tfm_mg.hip).
 struct MgData;
 int mg_levels(int nx, int ny, std::vector<int> *dims);

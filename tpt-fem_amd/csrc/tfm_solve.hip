@@ -1273,22 +1273,129 @@ extern "C" int tfm_system_download(const tfm_system *s, int64_t *row_ptrs, int64
   return TFM_OK;
 }
 
+// Rows per CSR-stream chunk of the system's matrix (from its longest row).
+static int system_chunk_rows(tfm_system *s, int *S) {
+  tfm_ctx *ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int n = (int)s->ndof;
+  int mnz = 0, *d_m = nullptr;
+  int rc = dalloc(ctx, &d_m, 1);
+  if (rc) return rc;
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(d_m, 0, sizeof(int), st));
+  if (n > 0) hipLaunchKernelGGL(k_max_row_nnz, dim3(grid1(n, 256)), dim3(256), 0, st, n, s->d_rp, d_m);
+  hipError_t e = hipMemcpy(&mnz, d_m, sizeof(int), hipMemcpyDeviceToHost);
+  dfree(d_m);
+  if (e != hipSuccess) return fail(ctx, TFM_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  if (mnz > kWaveCap) return fail(ctx, TFM_E_UNSUPPORTED, "a matrix row is longer than the SpMV chunk buffer");
+  *S = rows_per_chunk(mnz);
+  return TFM_OK;
+}
+
+// The reference's direct solve (solve_multi, crates/tpt-fem-sparse/src/lib.rs:
+// 244-278) of the reduced system for nrhs right-hand sides (d_F: device
+// [nrhs][ndof], 0 at fixed DOFs; NULL: the system's own rhs), then the
+// solve_with_dirichlet scatter into u [nrhs][ndof].  Each solution's FP64
+// residual is certified: a matrix that is singular to working precision
+// (factors without an exact zero pivot but leaves a residual above 1e-6 of
+// ||f||) is reported as singular.
+int tfm::system_solve_direct(tfm_system *s, int nrhs, const double *d_F, double *u, tfm_solve_stats *stats) {
+  tfm_ctx *ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int n = (int)s->ndof;
+  int S = 1, rc;
+  if (s->n_free == 0)
+    return fail(ctx, TFM_E_NUMERIC,
+                "sparse numeric factorization/solve failed: rhs length 0 does not match matrix dimension 1");
+  if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values (TFM_SYS_FP64)");
+  if ((rc = system_chunk_rows(s, &S))) return rc;
+  const double *F = d_F ? d_F : s->d_f;
+  double *x = nullptr, *r = nullptr, *part = nullptr;
+  Scal *sc = nullptr;
+  auto cleanup = [&]() {
+    dfree(x);
+    dfree(r);
+    dfree(part);
+    dfree(sc);
+  };
+  if ((rc = dalloc(ctx, &x, (size_t)n * nrhs)) || (rc = dalloc(ctx, &r, n)) ||
+      (rc = dalloc(ctx, &part, P_SLOTS * kMaxBlocks)) || (rc = dalloc(ctx, &sc, 1))) {
+    cleanup();
+    return rc;
+  }
+  TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev0, st));
+  if ((rc = direct_solve_dev(s, nrhs, F, x))) {
+    cleanup();
+    return rc;
+  }
+  TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, st));
+  const SpOp<double> K64 = csr_op<double>(n, s->d_rp, s->d_ci, s->d_v64, S);
+  double worst = 0.0;
+  std::vector<double> hx((size_t)n * nrhs);
+  for (int k = 0; k < nrhs && !rc; k++) {
+    residual_dispatch(S, st, K64.nb, n, s->d_rp, s->d_ci, s->d_v64, x + (size_t)k * n, F + (size_t)k * n, r, nullptr,
+                      sc, part);
+    Scal h{};
+    hipError_t e = hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = fail(ctx, TFM_E_HIP, std::string("residual: ") + hipGetErrorString(e));
+    const double rel = h.rr0 > 0 ? std::sqrt(h.rr / h.rr0) : std::sqrt(h.rr);
+    worst = std::max(worst, rel);
+  }
+  if (!rc && hipMemcpy(hx.data(), x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(ctx, TFM_E_HIP, "solution download failed");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  cleanup();
+  if (rc) return rc;
+  if (!(worst <= 1e-6))
+    return fail(ctx, TFM_E_NUMERIC, "sparse numeric factorization/solve failed: singular matrix");
+  for (int k = 0; k < nrhs; k++) {
+    double *uk = u + (size_t)k * n;
+    const double *xk = hx.data() + (size_t)k * n;
+    for (int64_t i = 0; i < n; i++) uk[i] = 0.0;
+    for (int64_t f : s->free_dofs) uk[f] = xk[f];
+    for (size_t b = 0; b < s->bc_dof.size(); b++) uk[s->bc_dof[b]] = s->bc_val[b];
+  }
+  if (stats)
+    for (int k = 0; k < nrhs; k++) stats[k] = tfm_solve_stats{0, 0, worst, ms * 1e-3};
+  return TFM_OK;
+}
+
+static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, tfm_solve_stats *stats,
+                             bool *not_spd);
+
 extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, double *u, tfm_solve_stats *stats) {
   if (!s || !u) return TFM_E_INVALID;
   tfm_ctx *ctx = s->ctx;
   int rc = set_device(ctx);
   if (rc) return rc;
-  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 1e-5, 40};
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 1e-5};
   if (opts) o = *opts;
   if (o.max_iter <= 0) o.max_iter = 20000;
-  if (o.mode != TFM_SOLVE_PCG64 && o.mode != TFM_SOLVE_MIXED && o.mode != TFM_SOLVE_PCG64_MG)
+  if (o.mode != TFM_SOLVE_PCG64 && o.mode != TFM_SOLVE_MIXED && o.mode != TFM_SOLVE_PCG64_MG &&
+      o.mode != TFM_SOLVE_DIRECT)
     return fail(ctx, TFM_E_INVALID, "unknown solve mode");
+  if (o.mode == TFM_SOLVE_DIRECT) return system_solve_direct(s, 1, nullptr, u, stats);
   if (o.mode == TFM_SOLVE_PCG64_MG) {
     if (s->n_free == 0)
       return fail(ctx, TFM_E_NUMERIC,
                   "sparse numeric factorization/solve failed: rhs length 0 does not match matrix dimension 1");
     return system_solve_mg(s, o, u, stats);  // the batched solver with one problem, kept on the system
   }
+  bool not_spd = false;
+  rc = system_solve_iter(s, o, u, stats, &not_spd);
+  // The reference factors every reduced system directly, SPD or not: a system
+  // on which CG breaks down or stalls (not SPD, or too ill-conditioned for
+  // the iteration budget) gets the dense LU when it is small enough.
+  if ((not_spd || rc == TFM_E_NOT_CONVERGED) && s->n_free <= kDirectMax)
+    return system_solve_direct(s, 1, nullptr, u, stats);
+  return rc;
+}
+
+static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, tfm_solve_stats *stats,
+                             bool *not_spd) {
+  tfm_ctx *ctx = s->ctx;
+  int rc = TFM_OK;
   const int n = (int)s->ndof;
   const bool mixed = o.mode == TFM_SOLVE_MIXED;
   // No free DOF: the reference's solve gets the 1 x 1 CSR of an empty
@@ -1356,6 +1463,7 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
   if (hbad) {
     cleanup();
+    *not_spd = true;
     return fail(ctx, TFM_E_NUMERIC,
                 "sparse numeric factorization/solve failed: singular matrix (a free DOF has no positive diagonal)");
   }
@@ -1440,7 +1548,10 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
     stats->rel_residual = relres;
     stats->seconds = ms * 1e-3;
   }
-  if (brk) return fail(ctx, TFM_E_BREAKDOWN, "PCG breakdown: p'Kp <= 0 (matrix not SPD)");
+  if (brk) {
+    *not_spd = true;
+    return fail(ctx, TFM_E_BREAKDOWN, "PCG breakdown: p'Kp <= 0 (matrix not SPD)");
+  }
   // PCG64 stops on its recurrence residual (the true FP64 residual can sit
   // slightly above it at tight tolerances: it is reported, not tested); the
   // refinement loop is driven by the true FP64 residual itself.
@@ -1491,7 +1602,7 @@ extern "C" int tfm_solve_poisson(tfm_ctx *ctx, const tfm_mesh *mesh, double kapp
   op.coef = kappa;
   tfm_system *s = nullptr;
   if ((rc = tfm_system_create(ctx, mesh, &op, 1, n_bc, bc_dof, bc_val, rhs.data(), TFM_SYS_FP64, &s))) return rc;
-  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 40};
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5};
   if (opts) o = *opts;
   rc = tfm_system_solve(s, &o, u, nullptr);
   tfm_system_destroy(s);
@@ -1519,7 +1630,7 @@ extern "C" int tfm_solve_poisson_nr(tfm_ctx *ctx, const tfm_mesh *mesh, double k
   rc = tfm_system_from_matrix(ctx, K, mesh->n_nodes, rhs.data(), n_bc, bc_dof, bc_val, &s);
   tfm_matrix_destroy(K);
   if (rc) return rc;
-  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 40};
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5};
   if (opts) o = *opts;
   rc = tfm_system_solve(s, &o, u, nullptr);
   tfm_system_destroy(s);
@@ -1560,7 +1671,7 @@ extern "C" int tfm_solve_poisson_qp(tfm_ctx *ctx, const tfm_mesh *mesh, double k
     rc = tfm_system_create(ctx, mesh, &op, 1, n_bc, bc_dof, bc_val, rhs.data(), TFM_SYS_FP64, &s);
   }
   if (rc) return rc;
-  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 0};
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5};
   if (opts) o = *opts;
   rc = tfm_system_solve(s, &o, u, nullptr);
   tfm_system_destroy(s);
@@ -1584,7 +1695,7 @@ extern "C" int tfm_solve_elasticity_qp(tfm_ctx *ctx, const tfm_mesh *mesh, int m
   op.nu = nu;
   tfm_system *s = nullptr;
   if ((rc = tfm_system_create(ctx, mesh, &op, dim, n_bc, bc_dof, bc_val, rhs.data(), TFM_SYS_FP64, &s))) return rc;
-  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 0};
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5};
   if (opts) o = *opts;
   rc = tfm_system_solve(s, &o, u, nullptr);
   tfm_system_destroy(s);
@@ -1609,7 +1720,7 @@ extern "C" int tfm_solve_elasticity(tfm_ctx *ctx, const tfm_mesh *mesh, int mode
   op.nu = nu;
   tfm_system *s = nullptr;
   if ((rc = tfm_system_create(ctx, mesh, &op, dim, n_bc, bc_dof, bc_val, rhs.data(), TFM_SYS_FP64, &s))) return rc;
-  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5, 40};
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-13, 100000, 1e-5};
   if (opts) o = *opts;
   rc = tfm_system_solve(s, &o, u, nullptr);
   tfm_system_destroy(s);

@@ -158,7 +158,7 @@ extern "C" int tfm_topopt_simp_batch(tfm_ctx *ctx, int64_t P, const tfm_topopt_p
     if (bc_val[b] != 0.0)
       return fail(ctx, TFM_E_UNSUPPORTED, "topopt: batched solve takes homogeneous Dirichlet values only");
   }
-  tfm_solve_opts o{TFM_SOLVE_PCG64_MG, 1e-12, 100000, 0.3, 60};
+  tfm_solve_opts o{TFM_SOLVE_PCG64_MG, 1e-12, 100000, 0.3};
   if (opts) o = *opts;
   const bool mixed = o.mode == TFM_SOLVE_MIXED;
   int rc = set_device(ctx);

@@ -27,9 +27,11 @@ BAR, PLANE_STRESS, PLANE_STRAIN, CONTINUUM3D = 0, 1, 2, 3
 OP_POISSON, OP_ELASTICITY, OP_SIMP, OP_MASS = 0, 1, 2, 3
 SYS_FP64, SYS_FP32 = 1, 2
 SOLVE_PCG64, SOLVE_MIXED = 0, 1
-# FP64 CG with a geometric-multigrid V-cycle: batched SIMP systems on a
-# Mesh.grid_batch mesh (one problem per design) only
+# FP64 CG with a geometric-multigrid V-cycle: one system on a Mesh.box mesh
+# (one rhs) or SIMP systems on a Mesh.grid_batch mesh (one problem per design)
 SOLVE_PCG64_MG = 2
+# the reference's dense partial-pivot LU on the GPU (at most 8000 free DOFs)
+SOLVE_DIRECT = 3
 NODES = {LINE: 2, TRI: 3, QUAD: 4, TET: 4, HEX: 8}
 DIMS = {LINE: 1, TRI: 2, QUAD: 2, TET: 3, HEX: 3}
 
@@ -81,7 +83,7 @@ class Operator(C.Structure):
 
 
 class SolveOpts(C.Structure):
-    _fields_ = [("mode", INT), ("rtol", D), ("max_iter", INT), ("inner_rtol", D), ("max_outer", INT)]
+    _fields_ = [("mode", INT), ("rtol", D), ("max_iter", INT), ("inner_rtol", D)]
 
 
 class SolveStats(C.Structure):
@@ -527,25 +529,25 @@ class System:
         return (Csr(nrows, ncols, rp, ci[: self.nnz].copy(), v[: self.nnz].copy()),
                 fred[: self.n_free].copy(), free[: self.n_free].copy())
 
-    def solve(self, mode=SOLVE_PCG64, rtol=1e-12, max_iter=100000, inner_rtol=0.3, max_outer=60):
+    def solve(self, mode=SOLVE_PCG64, rtol=1e-12, max_iter=100000, inner_rtol=0.3):
         """Single-system solve.  Mixed mode: inner_rtol is the FP32 residual
         reduction between FP64 residual replacements."""
         u = np.zeros(max(1, self.n_dof))
-        o = SolveOpts(mode=mode, rtol=rtol, max_iter=max_iter, inner_rtol=inner_rtol, max_outer=max_outer)
+        o = SolveOpts(mode=mode, rtol=rtol, max_iter=max_iter, inner_rtol=inner_rtol)
         st = SolveStats()
         self.ctx.check(_lib.tfm_system_solve(self.h, C.byref(o), _ptr(u), C.byref(st)))
         self.stats = st
         return u[: self.n_dof]
 
     def solve_batch(self, dof_off, rhs=None, nrhs=1, mode=SOLVE_PCG64, rtol=1e-10, max_iter=20000,
-                    inner_rtol=0.3, max_outer=60):
+                    inner_rtol=0.3):
         """Block-diagonal batched solve: returns u [nrhs, n_dof] and per-segment stats."""
         off = _i64(dof_off)
         P = len(off) - 1
         r = None if rhs is None else _f64(rhs).reshape(nrhs, self.n_dof)
         u = np.zeros((nrhs, self.n_dof))
         st = (SolveStats * (P * nrhs))()
-        o = SolveOpts(mode=mode, rtol=rtol, max_iter=max_iter, inner_rtol=inner_rtol, max_outer=max_outer)
+        o = SolveOpts(mode=mode, rtol=rtol, max_iter=max_iter, inner_rtol=inner_rtol)
         self.ctx.check(_lib.tfm_system_solve_batch(self.h, P, _ptr(off), nrhs, _ptr(r), C.byref(o), _ptr(u),
                                                    C.cast(st, C.c_void_p)))
         self.batch_stats = [(s.iterations, s.outer, s.rel_residual, s.seconds) for s in st]
@@ -593,7 +595,7 @@ def solve_poisson(mesh, kappa, quad_order, f, bcs, rtol=1e-13, mode=SOLVE_PCG64,
     constant source, no Neumann/Robin terms)."""
     nb, bd, bv = _bc_arrays(bcs)
     u = np.zeros(mesh.n_nodes)
-    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=0)
+    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5)
     if not callable(f) and neumann is None and robin is None:
         mesh.ctx.check(_lib.tfm_solve_poisson(mesh.ctx.h, mesh.h, kappa, quad_order, f, nb, _ptr(bd), _ptr(bv),
                                               C.byref(o), _ptr(u)))
@@ -625,7 +627,7 @@ def solve_elasticity(mesh, model, young, nu, quad_order, body, bcs, rtol=1e-13, 
     multigrid CG (meshes from Mesh.box, constant body force)."""
     nb, bd, bv = _bc_arrays(bcs)
     u = np.zeros(mesh.n_nodes * mesh.dim)
-    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=0)
+    o = SolveOpts(mode=mode, rtol=rtol, max_iter=200000, inner_rtol=1e-5)
     if callable(body):
         xq = mesh.quad_points(quad_order)
         bq = _values_at(body, xq.reshape(-1, mesh.pdim), mesh.dim, vectorized)
@@ -669,7 +671,7 @@ def topopt_simp(ctx, nx, ny, h, e0, nu, vol_frac, penal, filter_radius, max_iter
     comp = np.zeros((Pn, max_iter + 1))
     its = np.zeros(Pn, dtype=np.int32)
     st = (SolveStats * Pn)()
-    o = SolveOpts(mode=mode, rtol=rtol, max_iter=pcg_max_iter, inner_rtol=inner_rtol, max_outer=60)
+    o = SolveOpts(mode=mode, rtol=rtol, max_iter=pcg_max_iter, inner_rtol=inner_rtol)
     ctx.check(_lib.tfm_topopt_simp_batch(ctx.h, Pn, C.byref(prm), _ptr(ff), _ptr(vf), nb, _ptr(bd), _ptr(bv),
                                          C.byref(o), _ptr(rho), _ptr(comp), its.ctypes.data,
                                          C.cast(st, C.c_void_p)))
@@ -724,7 +726,7 @@ def newmark(m, c, k, u0, v0, f, dt, beta, gamma, nsteps, fscale=None, rtol=1e-12
     fs = None if fscale is None else _f64(fscale)
     n = len(u0)
     hist = np.zeros((nsteps + 1, n))
-    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=max_iter, inner_rtol=0.3, max_outer=60)
+    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=max_iter, inner_rtol=0.3)
     st = SolveStats()
     m.ctx.check(_lib.tfm_newmark(m.h, c.h, k.h, n, _ptr(u0), _ptr(v0), _ptr(f), _ptr(fs), dt, beta, gamma, nsteps,
                                  C.byref(o), _ptr(hist), C.byref(st)))
@@ -754,7 +756,7 @@ def solve_poisson_nr(mesh, kappa, quad_order, f, bcs, neumann=None, robin=None, 
     u = np.zeros(mesh.n_nodes)
     g = None if neumann is None else _f64([neumann])
     hh = None if robin is None else _f64([robin])
-    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=200000, inner_rtol=1e-5, max_outer=40)
+    o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=200000, inner_rtol=1e-5)
     mesh.ctx.check(_lib.tfm_solve_poisson_nr(mesh.ctx.h, mesh.h, kappa, quad_order, f, _ptr(g), _ptr(hh), nb,
                                              _ptr(bd), _ptr(bv), C.byref(o), _ptr(u)))
     return u
