@@ -244,6 +244,20 @@ typedef struct {
  * (host, n_dof) with u[bc] = value in bcs order (last duplicate wins). */
 int tfm_system_solve(tfm_system *sys, const tfm_solve_opts *opts, double *u,
                      tfm_solve_stats *stats);
+/* tfm_system_solve in the TFM_SOLVE_PCG64 mode without any device
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate (serving loops, harnesses that forbid hipMalloc).  The PCG
+ * scalars, block partial sums, flags and work vectors live in one file-scope
+ * device array of 64 MiB per GPU (held from the first tfm_ctx_create on that
+ * GPU until the process ends), so the system may have at most about
+ * 1,197,000 DOFs (n_dof, free and fixed) -- larger systems return
+ * TFM_E_UNSUPPORTED -- and the iterations run the CSR-stream SpMV, launched
+ * directly.  Unlike tfm_system_solve there is no dense-LU fallback: a system
+ * CG cannot solve returns TFM_E_BREAKDOWN / TFM_E_NOT_CONVERGED.  Every call
+ * on one GPU uses the same array: calls that overlap on one GPU (two host
+ * threads, two contexts) race and must be serialised by the caller. */
+int tfm_system_solve_noalloc(tfm_system *sys, const tfm_solve_opts *opts, double *u,
+                             tfm_solve_stats *stats);
 void tfm_system_destroy(tfm_system *sys);
 
 /* ---- batching form (no reference counterpart: the reference solves one

@@ -281,6 +281,11 @@ extern "C" int tfm_ctx_create(int device, tfm_ctx **out) {
     delete c;
     return TFM_E_HIP;
   }
+  // the allocation-free solve's work space on this device (looked up once)
+  if (noalloc_init(c) != TFM_OK) {
+    tfm_ctx_destroy(c);
+    return TFM_E_HIP;
+  }
   *out = c;
   return TFM_OK;
 }

@@ -218,6 +218,10 @@ int direct_solve_dev(tfm_system *s, int nrhs, const double *d_F, double *d_x);
 // [nrhs][ndof]; d_F NULL: the system's own rhs (nrhs = 1).
 int system_solve_direct(tfm_system *s, int nrhs, const double *d_F, double *u, tfm_solve_stats *stats);
 
+// Looks up this device's copy of the allocation-free solve's work space
+// (tfm_solve.hip); tfm_ctx_create calls it with the context's device current.
+int noalloc_init(tfm_ctx *ctx);
+
 // Geometric multigrid for P batched topopt grids (tfm_mg.hip).
 struct MgData;
 int mg_levels(int nx, int ny, std::vector<int> *dims);

@@ -15,6 +15,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "tfm_bsell.h"
 #include "tfm_elem.h"
@@ -524,7 +525,7 @@ __global__ void k_max_row_nnz(int n, const int *rp, int *out) {
 // SpMV + p.q, then the update + r.z, r.r); 16 iterations per graph replay.
 template <typename VT, typename XT>
 static int pcg_run(tfm_ctx *ctx, const SpOp<VT> &A, PcgBufs<VT> &B, XT *x, double rtol, int max_iter, Scal *sc,
-                   double *part, int *iters, int *breakdown, double *rr_out) {
+                   double *part, int *iters, int *breakdown, double *rr_out, bool allow_graph = true) {
   hipStream_t st = ctx->stream;
   const int n = A.n, nb = A.nb;
   hipLaunchKernelGGL((k_pcg_init<VT>), dim3(nb), dim3(kBlock), 0, st, n, B.r, B.z, B.p, B.p2, B.dinv, part);
@@ -542,7 +543,7 @@ static int pcg_run(tfm_ctx *ctx, const SpOp<VT> &A, PcgBufs<VT> &B, XT *x, doubl
   };
   // TFM_NO_GRAPH=1 launches the iterations directly (for tools that do not
   // follow graph launches); results are identical.
-  const bool use_graph = !getenv("TFM_NO_GRAPH");
+  const bool use_graph = allow_graph && !getenv("TFM_NO_GRAPH");
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   if (use_graph) {
@@ -1557,6 +1558,168 @@ static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, 
   // refinement loop is driven by the true FP64 residual itself.
   bool ok = mixed ? (relres <= o.rtol) : (pcg_conv && relres <= o.rtol * 10.0);
   if (!ok) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "PCG did not reach the tolerance: rel. residual %.3e after %d iterations", relres,
+             total_it);
+    return fail(ctx, TFM_E_NOT_CONVERGED, buf);
+  }
+  return TFM_OK;
+}
+
+// ---- allocation-free solve -------------------------------------------------
+// The work space of tfm_system_solve_noalloc: PCG scalars, block partial
+// sums, flags and the seven work vectors of an n-DOF system, in one
+// file-scope device array (64 MiB, so n <= 1,197,000 DOFs).  Each GPU holds
+// its own copy for the life of the process; its address on a device is
+// looked up once, when the first context on that device is created.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+constexpr int kNoAllocDevices = 64;
+namespace tfm_noalloc {
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+}
+static void *g_scratch_dev[kNoAllocDevices] = {};
+static std::mutex g_scratch_mu;
+
+int tfm::noalloc_init(tfm_ctx *ctx) {  // ctx->device is current
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (ctx->device < 0 || ctx->device >= kNoAllocDevices) return TFM_OK;  // the entry point refuses such devices
+  if (g_scratch_dev[ctx->device]) return TFM_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(tfm_noalloc::scratch));
+  if (e != hipSuccess || !p)
+    return fail(ctx, TFM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_dev[ctx->device] = p;
+  return TFM_OK;
+}
+
+struct NoAllocLayout {
+  Scal *sc = nullptr;
+  double *part = nullptr;
+  int *flag = nullptr;  // [0] bad diagonal, [1] longest row
+  double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *p2 = nullptr;
+  size_t bytes = 0;
+};
+
+static NoAllocLayout noalloc_layout(char *base, int64_t n) {
+  NoAllocLayout L;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    char *q = base ? base + off : nullptr;
+    off += (b + 255) / 256 * 256;
+    return q;
+  };
+  L.sc = (Scal *)take(sizeof(Scal));
+  L.part = (double *)take(sizeof(double) * P_SLOTS * kMaxBlocks);
+  L.flag = (int *)take(2 * sizeof(int));
+  double **v[7] = {&L.x, &L.r, &L.z, &L.p, &L.q, &L.dinv, &L.p2};
+  for (double **d : v) *d = (double *)take(sizeof(double) * (size_t)std::max<int64_t>(n, 1));
+  L.bytes = off;
+  return L;
+}
+
+__global__ void k_zero_bytes(unsigned char *__restrict__ p, int64_t nbytes) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+__global__ void k_zero_f64(double *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+extern "C" int tfm_system_solve_noalloc(tfm_system *s, const tfm_solve_opts *opts, double *u,
+                                        tfm_solve_stats *stats) {
+  if (!s || !u) return TFM_E_INVALID;
+  tfm_ctx *ctx = s->ctx;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  tfm_solve_opts o{TFM_SOLVE_PCG64, 1e-10, 20000, 1e-5};
+  if (opts) o = *opts;
+  if (o.max_iter <= 0) o.max_iter = 20000;
+  if (o.mode != TFM_SOLVE_PCG64)
+    return fail(ctx, TFM_E_UNSUPPORTED, "the allocation-free solve runs the FP64 Jacobi PCG (TFM_SOLVE_PCG64) only");
+  if (ctx->device < 0 || ctx->device >= kNoAllocDevices || !g_scratch_dev[ctx->device])
+    return fail(ctx, TFM_E_INVALID, "allocation-free solve: no work space on this device");
+  if (s->n_free == 0)
+    return fail(ctx, TFM_E_NUMERIC,
+                "sparse numeric factorization/solve failed: rhs length 0 does not match matrix dimension 1");
+  if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values (TFM_SYS_FP64)");
+  const int n = (int)s->ndof;
+  const NoAllocLayout need = noalloc_layout(nullptr, n);
+  if (need.bytes > kNoAllocBytes) {
+    char buf[200];
+    snprintf(buf, sizeof buf,
+             "allocation-free solve: %d DOFs need %zu bytes of work space, the fixed array holds %zu "
+             "(at most about 1,197,000 DOFs)",
+             n, need.bytes, kNoAllocBytes);
+    return fail(ctx, TFM_E_UNSUPPORTED, buf);
+  }
+  hipStream_t st = ctx->stream;
+  const NoAllocLayout L = noalloc_layout((char *)g_scratch_dev[ctx->device], n);
+  // the array keeps the previous call's contents: clear what must start at 0
+  hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(256), 0, st, (unsigned char *)L.sc, (int64_t)sizeof(Scal));
+  hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(64), 0, st, (unsigned char *)L.flag, (int64_t)(2 * sizeof(int)));
+  hipLaunchKernelGGL(k_zero_f64, dim3(std::min<unsigned>(grid1(n, 256), 4096)), dim3(256), 0, st, L.x, (int64_t)n);
+  if (n > 0) hipLaunchKernelGGL(k_max_row_nnz, dim3(grid1(n, 256)), dim3(256), 0, st, n, s->d_rp, L.flag + 1);
+  hipLaunchKernelGGL(k_diag_inv<double>, dim3(grid1(n, 256)), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v64,
+                     s->d_fixed, L.dinv, L.flag);
+  int hflag[2] = {0, 0};
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(hflag, L.flag, sizeof hflag, hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  if (hflag[0])
+    return fail(ctx, TFM_E_NUMERIC,
+                "sparse numeric factorization/solve failed: singular matrix (a free DOF has no positive diagonal)");
+  if (hflag[1] > kWaveCap) return fail(ctx, TFM_E_UNSUPPORTED, "a matrix row is longer than the SpMV chunk buffer");
+  const int S = rows_per_chunk(hflag[1]);
+  PcgBufs<double> B;
+  B.r = L.r;
+  B.z = L.z;
+  B.p = L.p;
+  B.q = L.q;
+  B.dinv = L.dinv;
+  B.p2 = L.p2;
+  TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev0, st));
+  const SpOp<double> K64 = csr_op<double>(n, s->d_rp, s->d_ci, s->d_v64, S);
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(B.r, s->d_f, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  // restarted PCG on the true FP64 residual, as tfm_system_solve runs it
+  int total_it = 0, outer = 0, brk = 0;
+  bool pcg_conv = false;
+  double target = o.rtol;
+  for (outer = 0; outer < 5 && !rc; outer++) {
+    double rr = 0, rel_in = 1.0;
+    int it = 0;
+    if (outer > 0) {
+      residual_dispatch(S, st, K64.nb, n, s->d_rp, s->d_ci, s->d_v64, L.x, s->d_f, B.r, nullptr, L.sc, L.part);
+      Scal h{};
+      TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h, L.sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
+      TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+      rel_in = h.rr0 > 0 ? std::sqrt(h.rr / h.rr0) : 0.0;
+      if (rel_in <= o.rtol) break;
+      target = o.rtol / rel_in;
+    }
+    rc = pcg_run<double, double>(ctx, K64, B, L.x, target, o.max_iter - total_it, L.sc, L.part, &it, &brk, &rr,
+                                 false);
+    total_it += it;
+    pcg_conv = rr <= target;
+    if (brk || !pcg_conv || total_it >= o.max_iter) break;
+  }
+  outer = std::max(outer, 1);
+  TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, st));
+  residual_dispatch(S, st, K64.nb, n, s->d_rp, s->d_ci, s->d_v64, L.x, s->d_f, B.r, nullptr, L.sc, L.part);
+  Scal h{};
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h, L.sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
+  // x is 0 at the fixed DOFs (their Jacobi weight is 0): u = x, then the
+  // Dirichlet values in bcs order (solve_with_dirichlet's scatter)
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(u, L.x, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  if (rc) return rc;
+  for (size_t b = 0; b < s->bc_dof.size(); b++) u[s->bc_dof[b]] = s->bc_val[b];
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  const double relres = h.rr0 > 0 ? std::sqrt(h.rr / h.rr0) : 0.0;
+  if (stats) *stats = tfm_solve_stats{total_it, outer, relres, ms * 1e-3};
+  if (brk) return fail(ctx, TFM_E_BREAKDOWN, "PCG breakdown: p'Kp <= 0 (matrix not SPD)");
+  if (!(pcg_conv && relres <= o.rtol * 10.0)) {
     char buf[160];
     snprintf(buf, sizeof buf, "PCG did not reach the tolerance: rel. residual %.3e after %d iterations", relres,
              total_it);

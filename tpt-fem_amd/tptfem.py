@@ -137,6 +137,7 @@ _SIGS = {
     "tfm_system_info": (INT, [P, P, P, P]),
     "tfm_system_download": (INT, [P, P, P, P, P, P]),
     "tfm_system_solve": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
+    "tfm_system_solve_noalloc": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
     "tfm_system_destroy": (None, [P]),
     "tfm_mesh_box_batch": (INT, [P, I64, P, PP, P, P]),
     "tfm_mesh_grid_batch": (INT, [P, I64, I64, I64, D, PP]),
@@ -536,6 +537,16 @@ class System:
         o = SolveOpts(mode=mode, rtol=rtol, max_iter=max_iter, inner_rtol=inner_rtol)
         st = SolveStats()
         self.ctx.check(_lib.tfm_system_solve(self.h, C.byref(o), _ptr(u), C.byref(st)))
+        self.stats = st
+        return u[: self.n_dof]
+
+    def solve_noalloc(self, rtol=1e-12, max_iter=100000):
+        """tfm_system_solve_noalloc: the FP64 Jacobi PCG without device
+        allocations (work space in a fixed device array; at most ~1.197M DOFs)."""
+        u = np.zeros(max(1, self.n_dof))
+        o = SolveOpts(mode=SOLVE_PCG64, rtol=rtol, max_iter=max_iter, inner_rtol=0.3)
+        st = SolveStats()
+        self.ctx.check(_lib.tfm_system_solve_noalloc(self.h, C.byref(o), _ptr(u), C.byref(st)))
         self.stats = st
         return u[: self.n_dof]
 
