@@ -209,6 +209,15 @@ int tfm_system_from_matrix(tfm_ctx *ctx, const tfm_matrix *k, int64_t n, const d
  * brings them up to date. */
 int tfm_system_assemble(tfm_system *sys, const tfm_operator *op, int which);
 int tfm_system_info(const tfm_system *sys, int64_t *n_free, int64_t *nnz, int64_t *n_dof);
+/* No counterpart in the reference (diagnostic): which numeric assembly
+ * kernel the system refills with and how its launch shape was chosen.
+ * path: 0 = generic tile kernel, 1 = P1-tet kernel; tile_pairs: element-node
+ * pairs (= threads) per P1-tet tile (192 or 256); source: 1 = timed at
+ * creation, 2 = reused from an earlier system of the same dofs per node,
+ * value arrays and size class (floor(log2(elements))), 3 = forced by
+ * TFM_TET_TILE_PAIRS; ms[2]: the timed pass of each candidate (256, 192)
+ * when source is 1, else 0.  Any pointer may be NULL. */
+int tfm_system_assembly_info(const tfm_system *sys, int *path, int *tile_pairs, int *source, double *ms);
 /* ReducedSystem {kred (as its to_csr()), fred, free}. */
 int tfm_system_download(const tfm_system *sys, int64_t *row_ptrs, int64_t *col_ind,
                         double *values, double *fred, int64_t *free_dofs);

@@ -478,6 +478,46 @@ def test_tet_fast_path_equals_generic_kernel(ctx, monkeypatch):
     csr_equal(kf, kr.to_csr())
 
 
+@pytest.mark.parametrize("dpn", [1, 3])
+def test_tet_tile_sizes_bitexact_and_tuner_record(ctx, monkeypatch, dpn):
+    """A node's pairs never span two P1-tet tiles, so both tile sizes sum
+    every entry in the same element order: FP64 and FP32 matrices are
+    bit-identical across them.  The autotuner's record names the forced
+    size, and an unforced system either timed both candidates (and kept
+    the faster) or reused the choice of an earlier system of its class."""
+    T = tp()
+    n = (7, 6, 5)
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], n)
+    m.jitter(n, 0.2, 5)
+    coords, _ = m.download()
+    cf = 10.0 ** np.random.default_rng(4).uniform(-1, 1, m.n_elem)
+    if dpn == 1:
+        op = T.Operator.poisson(1.0, 2, kappa_e=cf)
+    else:
+        op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=cf)
+    bcs = face_bcs(coords, dpn, [(0, 0.0)])
+    got = {}
+    for pairs in (192, 256):
+        monkeypatch.setenv("TFM_TET_TILE_PAIRS", str(pairs))
+        for flags in (T.SYS_FP64, T.SYS_FP32):
+            s = T.System.create(m, op, dpn, bcs, flags=flags)
+            info = s.assembly_info()
+            assert (info["kernel"], info["tile_pairs"], info["source"]) == ("p1_tet", pairs, "env")
+            got[(pairs, flags)] = s.download()[0]
+    for flags in (T.SYS_FP64, T.SYS_FP32):
+        csr_equal(got[(192, flags)], got[(256, flags)])
+    monkeypatch.delenv("TFM_TET_TILE_PAIRS")
+    first = T.System.create(m, op, dpn, bcs).assembly_info()
+    assert first["kernel"] == "p1_tet" and first["tile_pairs"] in (192, 256)
+    assert first["source"] in ("timed", "cached")
+    if first["source"] == "timed":
+        t256, t192 = first["ms"]["256"], first["ms"]["192"]
+        assert t256 > 0 and t192 > 0
+        assert first["tile_pairs"] == (256 if t256 <= t192 else 192)
+    again = T.System.create(m, op, dpn, bcs).assembly_info()
+    assert (again["source"], again["tile_pairs"]) == ("cached", first["tile_pairs"])
+
+
 @pytest.mark.parametrize("kind", ["poisson", "elasticity"])
 def test_tet_fp32_values_close_to_fp64(ctx, kind):
     """The precision form's FP32 matrix: same pattern, values within FP32

@@ -514,7 +514,7 @@ static int upload_tables(tfm_ctx *ctx, AsmPlan *pl, double **d_tab, int *nq) {
 }
 
 int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
-                const unsigned char *h_fixed, AsmPlan *pl) {
+                const unsigned char *h_fixed, int vals, AsmPlan *pl) {
   int rc;
   if ((rc = validate_operator(ctx, mesh, op, dpn))) return rc;
   if ((rc = ensure_adjacency(const_cast<tfm_mesh *>(mesh)))) return rc;
@@ -595,7 +595,8 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
   const bool tet_op = mesh->cell == TFM_TET && mesh->n_elem > 0 && ((op->kind == TFM_OP_POISSON && dpn == 1) ||
                                                 (op->kind == TFM_OP_ELASTICITY && dpn == 3));
   if (tet_op && mesh->max_adj <= 32 && !getenv("TFM_GENERIC_ASSEMBLY")) {
-    if (tet_plan_create(ctx, mesh, dpn, pl->d_rp, pl->d_ci, &pl->tet) == TFM_OK) {
+    if (tet_plan_tuned(ctx, mesh, dpn, pl->d_rp, pl->d_ci, nnz, P, vals ? vals : TFM_SYS_FP64, &pl->tet,
+                       &pl->tune) == TFM_OK) {
       pl->use_tet = true;
     } else {
       tet_plan_destroy(&pl->tet);
@@ -610,7 +611,7 @@ int plan_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *d_v64, float *d_v32) {
   OpParams P = make_params(pl, pl->d_tab, pl->nq, pl->d_tab + (size_t)pl->nq * kRuleStride);
   int rc;
   if (pl->ntiles == 0) return TFM_OK;
-  if (pl->use_tet) return tet_numeric(ctx, pl->mesh, pl->dpn, pl->tet, P, d_v64, d_v32);
+  if (pl->use_tet) return tet_numeric(ctx, pl->mesh, pl->dpn, pl->tet, pl->d_rp, P, d_v64, d_v32);
   if (d_v64 && (rc = dispatch<M_F64>(ctx, pl, P, nullptr, nullptr, nullptr, d_v64, nullptr, nullptr))) return rc;
   if (d_v32 && (rc = dispatch<M_F32>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, d_v32, nullptr))) return rc;
   TFM_CHECK_HIP(ctx, hipGetLastError());

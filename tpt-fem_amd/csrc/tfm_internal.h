@@ -105,20 +105,34 @@ int exclusive_scan_i32(tfm_ctx *ctx, const int *d_in, int *d_out, int64_t n, int
 
 // P1-tet assembly plan (tfm_asm_tet.hip), built once per pattern: the tile
 // schedule and one item record per row block (n, m) of each tile's nodes
-// (28 bytes: mask u32, lmb u64, dpn x pos int32, pmk u32), stored in the
-// order the numeric pass takes them.
+// (20 bytes: 3 x u32 contribution words, u32 row positions, u32 pmk), stored
+// in the order the numeric pass takes them.
+// How the tet tile size was chosen: source 1 = timed here, 2 = reused from
+// an earlier system of the same shape class, 3 = forced by the environment;
+// ms = measured pass time of each candidate (256, 192 pairs) when timed.
+struct TetTune {
+  int tile_pairs = 0, source = 0;
+  float ms[2] = {0.f, 0.f};
+};
+
 struct TetPlan {
   int *d_tiles = nullptr;  // first node of each tile (ntiles + 1)
   int *d_tpa = nullptr;    // first (node, element) pair of each tile
   int *d_tib = nullptr;    // first item of each tile
+  int *d_tnd = nullptr;    // diagonal blocks of each tile
+  int *d_tw = nullptr;     // longest row (entries) of each tile's nodes: the staging stride
   int ntiles = 0;
+  int tp = 0;              // pairs (and threads) per tile: 192 or 256
   // One item per row block (n, m) of a tile's nodes, in tile order: the
   // tile's diagonal blocks first, then the off-diagonal ones in node and
-  // column order.
-  unsigned *d_mask = nullptr;            // which of n's pairs contribute
-  unsigned long long *d_lmb = nullptr;   // 2-bit local index of m in each pair
-  int *d_pos = nullptr;                  // [DPN][nitems] CSR position of row a's first present entry
-  unsigned *d_pmk = nullptr;             // presence masks of the DPN rows (4 bits each) | first pair of n << 16
+  // column order.  Contribution words [3][nitems]: a diagonal block's first
+  // pair and pair count; an off-diagonal block's LDS element offsets of its
+  // (<= 6) contributions in pair order, 16 bits each; or, for a block with
+  // more contributions, the pair mask and the 2-bit LDS slots of its pairs.
+  unsigned *d_off = nullptr;
+  unsigned *d_pos = nullptr;  // per row a, bits 8a..8a+7: the block's first present entry relative to
+                              // the row's start; bits 24..29: the node's index in its tile
+  unsigned *d_pmk = nullptr;  // presence masks (4 bits per row) | count << 12 | first pair of n << 16
   int64_t nitems = 0;
 };
 
@@ -145,9 +159,11 @@ struct AsmPlan {
   int *d_rp = nullptr;
   int *d_ci = nullptr;
   int64_t nrows = 0, ncols = 0, nnz = 0;
-  // P1-tet fast path (tfm_asm_tet.hip), when its plan could be built
+  // P1-tet fast path (tfm_asm_tet.hip), when its plan could be built, and
+  // how its tile size was chosen
   bool use_tet = false;
   TetPlan tet;
+  TetTune tune;
   // Elements whose coefficient was exactly 0 when the pattern was built (all
   // their entries were skipped, so they own no pattern entries); zero_all:
   // the scalar coefficient was 0.  A re-assembly may not make them non-zero.
@@ -155,12 +171,20 @@ struct AsmPlan {
   bool zero_all = false;
 };
 struct OpParams;
-int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci, TetPlan *tp);
+int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci, int tile_pairs,
+                    TetPlan *tp);
+// Builds the tet plan at the tile size the autotuner picks for this shape
+// class, timing each candidate into scratch value arrays of nnz entries for
+// the arrays `vals` (TFM_SYS_FP64 | TFM_SYS_FP32) names.
+int tet_plan_tuned(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci, int64_t nnz,
+                   const OpParams &P, int vals, TetPlan *tp, TetTune *info);
 void tet_plan_destroy(TetPlan *tp);
-int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const TetPlan &tp, const OpParams &P, double *v64,
-                float *v32);
+int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const TetPlan &tp, const int *d_rp, const OpParams &P,
+                double *v64, float *v32);
+// `vals` names the value arrays later numeric passes write (TFM_SYS_FP64 |
+// TFM_SYS_FP32): what the tile-size autotuner times.
 int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
-                const unsigned char *h_fixed, AsmPlan *plan);
+                const unsigned char *h_fixed, int vals, AsmPlan *plan);
 int plan_set_coefficients(tfm_ctx *ctx, AsmPlan *plan, const tfm_operator *op);
 int plan_numeric(tfm_ctx *ctx, const AsmPlan *plan, double *d_v64, float *d_v32);
 void plan_destroy(AsmPlan *plan);

@@ -862,7 +862,7 @@ extern "C" int tfm_assemble(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operat
   int rc = set_device(ctx);
   if (rc) return rc;
   AsmPlan pl;
-  if ((rc = plan_create(ctx, mesh, op, dpn, nullptr, &pl))) {
+  if ((rc = plan_create(ctx, mesh, op, dpn, nullptr, TFM_SYS_FP64, &pl))) {
     plan_destroy(&pl);
     return rc;
   }
@@ -1182,7 +1182,8 @@ extern "C" int tfm_system_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_o
     return rc;
   }
   s->has_plan = true;
-  if ((rc = plan_create(ctx, mesh, op, dpn, n_bc ? fixed.data() : nullptr, &s->plan))) {
+  if ((rc = plan_create(ctx, mesh, op, dpn, n_bc ? fixed.data() : nullptr, flags & (TFM_SYS_FP64 | TFM_SYS_FP32),
+                       &s->plan))) {
     tfm_system_destroy(s);
     return rc;
   }
@@ -1238,6 +1239,19 @@ extern "C" int tfm_system_info(const tfm_system *s, int64_t *n_free, int64_t *nn
   if (n_free) *n_free = s->n_free;
   if (nnz) *nnz = s->nnz;
   if (n_dof) *n_dof = s->ndof;
+  return TFM_OK;
+}
+
+extern "C" int tfm_system_assembly_info(const tfm_system *s, int *path, int *tile_pairs, int *source, double *ms) {
+  if (!s) return TFM_E_INVALID;
+  const bool tet = s->has_plan && s->plan.use_tet;
+  if (path) *path = tet ? 1 : 0;
+  if (tile_pairs) *tile_pairs = tet ? s->plan.tet.tp : 0;
+  if (source) *source = tet ? s->plan.tune.source : 0;
+  if (ms) {
+    ms[0] = tet ? s->plan.tune.ms[0] : 0.0;
+    ms[1] = tet ? s->plan.tune.ms[1] : 0.0;
+  }
   return TFM_OK;
 }
 

@@ -135,6 +135,7 @@ _SIGS = {
     "tfm_system_from_matrix": (INT, [P, P, I64, P, I64, P, P, PP]),
     "tfm_system_assemble": (INT, [P, C.POINTER(Operator), INT]),
     "tfm_system_info": (INT, [P, P, P, P]),
+    "tfm_system_assembly_info": (INT, [P, P, P, P, P]),
     "tfm_system_download": (INT, [P, P, P, P, P, P]),
     "tfm_system_solve": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
     "tfm_system_solve_noalloc": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
@@ -494,6 +495,14 @@ class System:
         nf, nz, nd = I64(), I64(), I64()
         _lib.tfm_system_info(handle, C.byref(nf), C.byref(nz), C.byref(nd))
         self.n_free, self.nnz, self.n_dof = nf.value, nz.value, nd.value
+
+    def assembly_info(self):
+        """Numeric assembly kernel and how its tile size was chosen."""
+        path, tp, src = C.c_int(), C.c_int(), C.c_int()
+        ms = (C.c_double * 2)()
+        self.ctx.check(_lib.tfm_system_assembly_info(self.h, C.byref(path), C.byref(tp), C.byref(src), ms))
+        return {"kernel": ("generic", "p1_tet")[path.value], "tile_pairs": tp.value,
+                "source": ("fixed", "timed", "cached", "env")[src.value], "ms": {"256": ms[0], "192": ms[1]}}
 
     @staticmethod
     def create(mesh, op, dofs_per_node, bcs, rhs=None, flags=SYS_FP64):
