@@ -28,7 +28,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tpt-fem_amd"))
 
 PEAK_HBM_GBS = 8000.0
-ASM_KERNEL = "tfm::k_asm_tet<3, double>"  # the FP64 P1-tet elasticity assembly kernel
+
+
+def asm_kernel(tile_pairs, vt="double"):
+    """Name rocprofv3 gives the P1-tet elasticity assembly kernel at a tile size."""
+    return f"tfm::k_asm_tet<3, {vt}, {tile_pairs}>"
 
 
 def asm_bytes(n_nodes, n_elem, nnz, n_free, sv, per_elem_coef=True):
@@ -321,8 +325,10 @@ def main():
     op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=ye)
     sysm = T.System.create(mesh, op, 3, bcs, rhs, flags=T.SYS_FP64 | T.SYS_FP32)
     ctx.synchronize()
+    tune = sysm.assembly_info()
+    kernel = asm_kernel(tune["tile_pairs"])
     log(f"setup {time.perf_counter() - t_setup:.2f}s: {mesh.n_elem} tets, {mesh.n_nodes} nodes, "
-        f"n_free {sysm.n_free}, nnz {sysm.nnz}")
+        f"n_free {sysm.n_free}, nnz {sysm.nnz}; assembly kernel {kernel} ({tune['source']})")
 
     # ---- timed assembly --------------------------------------------------
     for _ in range(args.warmup):
@@ -432,11 +438,11 @@ def main():
                                "nu 0.3, x=0 clamped, traction on x=1",
                    "n_elem": mesh.n_elem, "n_nodes": mesh.n_nodes, "n_free": sysm.n_free, "nnz_reduced": sysm.nnz},
         "assembly": {"fp64_ms": ms64, "fp32_ms": ms32, "fp32_elements_per_s": mesh.n_elem / (ms32 * 1e-3),
-                     "fp64_GBps": gbs64, "fp32_GBps": gbs32},
+                     "fp64_GBps": gbs64, "fp32_GBps": gbs32, "autotune": tune},
         "solve": solve,
         "roofline": {"bound": "hbm", "achieved": gbs64, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": gbs64 / PEAK_HBM_GBS, "traffic": measured_traffic(ASM_KERNEL, n),
-                     "kernel": ASM_KERNEL, "algorithmic_bytes_per_launch": b64},
+                     "frac": gbs64 / PEAK_HBM_GBS, "traffic": measured_traffic(kernel, n),
+                     "kernel": kernel, "algorithmic_bytes_per_launch": b64},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
