@@ -218,12 +218,25 @@ int tfm_system_info(const tfm_system *sys, int64_t *n_free, int64_t *nnz, int64_
  * TFM_TET_TILE_PAIRS; ms[2]: the timed pass of each candidate (256, 192)
  * when source is 1, else 0.  Any pointer may be NULL. */
 int tfm_system_assembly_info(const tfm_system *sys, int *path, int *tile_pairs, int *source, double *ms);
+/* No counterpart in the reference (test hook): y = K x with the precision
+ * form's matrix-free FP32 operator (x, y: host, n_dof; x rounded to FP32;
+ * rows of fixed DOFs give 0 and x must be 0 there).  TFM_E_UNSUPPORTED when
+ * the system has no such operator (see TFM_SOLVE_MIXED). */
+int tfm_debug_mf_apply(tfm_system *sys, const double *x, double *y);
 /* ReducedSystem {kred (as its to_csr()), fred, free}. */
 int tfm_system_download(const tfm_system *sys, int64_t *row_ptrs, int64_t *col_ind,
                         double *values, double *fred, int64_t *free_dofs);
 
 /* TFM_SOLVE_PCG64: Jacobi-preconditioned FP64 CG.  TFM_SOLVE_MIXED: FP32 CG
-   with FP64 residual refinement.  TFM_SOLVE_PCG64_MG: FP64 CG preconditioned
+   with FP64 residual refinement (r = f - K u on the FP64 values); its FP32
+   operator is matrix-free on systems built from a tfm_mesh_box mesh of P1
+   tets (Poisson, 3-D elasticity): element contributions recomputed in FP32
+   from the coordinates and coefficients on every product, determinants in
+   FP64 -- no FP32 values needed, but the FP64 values must be current with
+   the last coefficients (TFM_E_INVALID otherwise); elsewhere it is the
+   assembled FP32 values.  TFM_SOLVE_MIXED_CSR: the same refinement on the
+   assembled FP32 values always (single systems; the batched solve takes
+   TFM_SOLVE_MIXED).  TFM_SOLVE_PCG64_MG: FP64 CG preconditioned
    by a geometric multigrid V-cycle, for systems on a tfm_mesh_box mesh (one
    system, one rhs) or on a tfm_mesh_grid_batch mesh (SIMP designs, one
    problem per design); TFM_E_UNSUPPORTED elsewhere.  TFM_SOLVE_DIRECT: the
@@ -231,8 +244,9 @@ int tfm_system_download(const tfm_system *sys, int64_t *row_ptrs, int64_t *col_i
    bit-identical to that loop run sequentially, for at most 8000 free DOFs
    (TFM_E_UNSUPPORTED above).  PCG64 and MIXED fall back to DIRECT on such
    small systems when CG breaks down (the reduced matrix is not SPD) or
-   stalls, as the reference solves every system directly. */
-enum { TFM_SOLVE_PCG64 = 0, TFM_SOLVE_MIXED = 1, TFM_SOLVE_PCG64_MG = 2, TFM_SOLVE_DIRECT = 3 };
+   stalls, as the reference solves every system directly (MIXED_CSR too). */
+enum { TFM_SOLVE_PCG64 = 0, TFM_SOLVE_MIXED = 1, TFM_SOLVE_PCG64_MG = 2, TFM_SOLVE_DIRECT = 3,
+       TFM_SOLVE_MIXED_CSR = 4 };
 typedef struct {
   int mode;           /* TFM_SOLVE_*                                          */
   double rtol;        /* stop when ||f - K u|| <= rtol * ||f|| (FP64 residual) */

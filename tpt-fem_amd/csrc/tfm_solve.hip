@@ -20,51 +20,11 @@
 #include "tfm_bsell.h"
 #include "tfm_elem.h"
 #include "tfm_internal.h"
+#include "tfm_pcg.h"
 
 namespace tfm {
 
-constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 1024;
 constexpr double kFloor2 = 1e-12;  // precision form: FP32 residual floor (squared, relative to the last replacement)
-
-// Partial-sum slots (each kMaxBlocks doubles): p.q, r.z and r.r by iteration
-// parity, and the refinement residual's ||r||^2, ||f||^2.
-enum { P_PQ = 0, P_RZ0 = 1, P_RZ1 = 2, P_RR0 = 3, P_RR1 = 4, P_RES = 5, P_FF = 6, P_SLOTS = 8 };
-
-struct Scal {
-  double rr, rr0, tol2;
-  int done, breakdown, it, max_it;
-};
-
-// Block sum of v, returned to every thread of the block (fixed order).
-__device__ __forceinline__ double block_sum_all(double v) {
-  __shared__ double sh[kBlock / 64];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  __syncthreads();
-  if (l == 0) sh[w] = v;
-  __syncthreads();
-  double t = 0.0;
-#pragma unroll
-  for (int i = 0; i < kBlock / 64; i++) t += sh[i];
-  return t;
-}
-
-// Sum of the nb partials of one slot, identical in every block (each thread
-// takes a fixed strided subset, then the fixed block tree).  The partials
-// were written by an earlier kernel, so no fence is needed.
-__device__ __forceinline__ double sum_part(const double *part, int slot, int nb) {
-  const double *q = part + slot * kMaxBlocks;
-  double t = 0.0;
-  for (int i = threadIdx.x; i < nb; i += kBlock) t += q[i];
-  return block_sum_all(t);
-}
-
-__device__ __forceinline__ void put_part(double *part, int slot, double v) {
-  v = block_sum_all(v);
-  if (threadIdx.x == 0) part[slot * kMaxBlocks + blockIdx.x] = v;
-}
 
 // Wave-level CSR stream: each wavefront independently takes R consecutive
 // rows at a time, loads their whole (contiguous) non-zero range with
@@ -84,40 +44,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-// Start of iteration i (parity e) in every block: converged / iteration
-// limit (sc->done) decided identically by all blocks from the partials of the
-// previous update; beta_i = rz_i / rz_{i-1}.  Returns false (uniformly per
-// block) when the block has nothing to do.
-__device__ __forceinline__ bool dir_prologue(Scal *sc, const double *part, int e, double *beta) {
-  __shared__ int s_done, s_it;
-  if (threadIdx.x == 0) {
-    s_done = sc->done;
-    s_it = sc->it;
-  }
-  __syncthreads();
-  if (s_done) return false;
-  const int nb = gridDim.x;
-  const int it = s_it;
-  if (it > 0) {
-    double rr = sum_part(part, e ? P_RR0 : P_RR1, nb);
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->rr = rr;  // latest recurrence residual, for the host
-    if (rr <= sc->tol2 || it >= sc->max_it) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        sc->done = 1;
-        sc->rr = rr;
-      }
-      return false;
-    }
-  }
-  *beta = 0.0;
-  if (it > 0) {
-    const double rzo = sum_part(part, e ? P_RZ1 : P_RZ0, nb);
-    const double rzn = sum_part(part, e ? P_RZ0 : P_RZ1, nb);
-    if (rzo > 0.0) *beta = rzn / rzo;  // 0 after a restart whose old slot is empty
-  }
-  return true;
 }
 
 // Iteration i (parity e): p_i = z_i + beta_i p_{i-1} on the fly, q = K p_i,
@@ -362,6 +288,12 @@ __global__ void k_diag_inv(int n, const int *__restrict__ rp, const int *__restr
   }
 }
 
+// FP32 copy of an FP64 vector (the matrix-free operator's Jacobi weights).
+__global__ void k_round_f32(int n, const double *__restrict__ a, float *__restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
 __global__ void k_add_f32_to_f64(int n, double *u, const float *d) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) u[i] += (double)d[i];
@@ -466,6 +398,8 @@ struct SpOp {
   int bs = 0, nslices = 0;
   const int *soff = nullptr, *bcol = nullptr;
   const VT *bv = nullptr;
+  bool mf = false;  // FP32 only: the matrix-free box-mesh operator
+  MfBox box;
 };
 
 template <typename VT>
@@ -500,6 +434,12 @@ static SpOp<VT> bsell_op(int n, const BSell &B, const VT *bv) {
 template <typename VT>
 static void dir_launch(const SpOp<VT> &A, hipStream_t st, const VT *z, const VT *pold, VT *pnew, VT *q, Scal *sc,
                        double *part, int e) {
+  if constexpr (sizeof(VT) == 4) {
+    if (A.mf) {
+      mf_dir_launch(A.box, st, A.nb, z, pold, pnew, q, sc, part, e);
+      return;
+    }
+  }
   if (A.bs == 0) {
     dir_dispatch<VT>(A.R, st, A.nb, A.n, A.rp, A.ci, A.v, z, pold, pnew, q, sc, part, e);
     return;
@@ -1242,6 +1182,40 @@ extern "C" int tfm_system_info(const tfm_system *s, int64_t *n_free, int64_t *nn
   return TFM_OK;
 }
 
+extern "C" int tfm_debug_mf_apply(tfm_system *s, const double *x, double *y) {
+  if (!s || !x || !y) return TFM_E_INVALID;
+  tfm_ctx *ctx = s->ctx;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  MfBox box;
+  if (!mf_box_setup(s, &box)) return fail(ctx, TFM_E_UNSUPPORTED, "system has no matrix-free operator");
+  const int64_t n = s->ndof;
+  hipStream_t st = ctx->stream;
+  std::vector<float> hx((size_t)n), hq((size_t)n);
+  for (int64_t i = 0; i < n; i++) hx[i] = (float)x[i];
+  Scal *sc = nullptr;
+  double *part = nullptr;
+  float *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *mfc = nullptr;
+  auto cleanup = [&]() { dfree(sc); dfree(part); dfree(z); dfree(p0); dfree(p1); dfree(q); dfree(mfc); };
+  if ((rc = dalloc(ctx, &sc, 1)) || (rc = dalloc(ctx, &part, P_SLOTS * kMaxBlocks)) || (rc = dalloc(ctx, &z, n)) ||
+      (rc = dalloc(ctx, &p0, n)) || (rc = dalloc(ctx, &p1, n)) || (rc = dalloc(ctx, &q, n)) ||
+      (rc = mf_coef32(ctx, s, &box, &mfc))) {
+    cleanup();
+    return rc;
+  }
+  // iteration 0 of a fresh scalar block: beta = 0, so p = z = x and q = K x
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(sc, 0, sizeof(Scal), st));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(p0, 0, sizeof(float) * n, st));
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(z, hx.data(), sizeof(float) * n, hipMemcpyHostToDevice, st));
+  mf_dir_launch(box, st, mf_grid(box), z, p0, p1, q, sc, part, 0);
+  TFM_CHECK_HIP(ctx, hipGetLastError());
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(hq.data(), q, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  cleanup();
+  for (int64_t i = 0; i < n; i++) y[i] = (double)hq[i];
+  return TFM_OK;
+}
+
 extern "C" int tfm_system_assembly_info(const tfm_system *s, int *path, int *tile_pairs, int *source, double *ms) {
   if (!s) return TFM_E_INVALID;
   const bool tet = s->has_plan && s->plan.use_tet;
@@ -1388,7 +1362,7 @@ extern "C" int tfm_system_solve(tfm_system *s, const tfm_solve_opts *opts, doubl
   if (opts) o = *opts;
   if (o.max_iter <= 0) o.max_iter = 20000;
   if (o.mode != TFM_SOLVE_PCG64 && o.mode != TFM_SOLVE_MIXED && o.mode != TFM_SOLVE_PCG64_MG &&
-      o.mode != TFM_SOLVE_DIRECT)
+      o.mode != TFM_SOLVE_DIRECT && o.mode != TFM_SOLVE_MIXED_CSR)
     return fail(ctx, TFM_E_INVALID, "unknown solve mode");
   if (o.mode == TFM_SOLVE_DIRECT) return system_solve_direct(s, 1, nullptr, u, stats);
   if (o.mode == TFM_SOLVE_PCG64_MG) {
@@ -1412,16 +1386,22 @@ static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, 
   tfm_ctx *ctx = s->ctx;
   int rc = TFM_OK;
   const int n = (int)s->ndof;
-  const bool mixed = o.mode == TFM_SOLVE_MIXED;
+  const bool mixed = o.mode == TFM_SOLVE_MIXED || o.mode == TFM_SOLVE_MIXED_CSR;
+  // The precision form's FP32 operator: matrix-free on box meshes of P1 tets
+  // (TFM_SOLVE_MIXED), else the assembled FP32 values.
+  MfBox box;
+  const bool mf = o.mode == TFM_SOLVE_MIXED && mf_box_setup(s, &box);
   // No free DOF: the reference's solve gets the 1 x 1 CSR of an empty
   // triplet list against a length-0 rhs and fails (sparse/src/lib.rs:244-258).
   if (s->n_free == 0)
     return fail(ctx, TFM_E_NUMERIC,
                 "sparse numeric factorization/solve failed: rhs length 0 does not match matrix dimension 1");
-  if (mixed && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values (TFM_SYS_FP32)");
-  if (mixed && !s->fp32_current())
-    return fail(ctx, TFM_E_INVALID, "mixed solve: the FP32 values predate the last FP64 re-assembly");
   if (!s->d_v64) return fail(ctx, TFM_E_INVALID, "solve needs FP64 values (TFM_SYS_FP64)");
+  if (mf && s->ver64 != s->version)
+    return fail(ctx, TFM_E_INVALID, "mixed solve: the FP64 values predate the last coefficients");
+  if (mixed && !mf && !s->d_v32) return fail(ctx, TFM_E_INVALID, "mixed solve needs FP32 values (TFM_SYS_FP32)");
+  if (mixed && !mf && !s->fp32_current())
+    return fail(ctx, TFM_E_INVALID, "mixed solve: the FP32 values predate the last FP64 re-assembly");
   hipStream_t st = ctx->stream;
   // rows per CSR-stream chunk, from the longest row
   int mnz = 0;
@@ -1442,9 +1422,9 @@ static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, 
   int *bad = nullptr;
   PcgBufs<double> B64;
   PcgBufs<float> B32;
-  float *x32 = nullptr;
+  float *x32 = nullptr, *mfc = nullptr;
   auto cleanup = [&]() {
-    dfree(sc); dfree(part); dfree(x); dfree(bad); dfree(x32);
+    dfree(sc); dfree(part); dfree(x); dfree(bad); dfree(x32); dfree(mfc);
     dfree(B64.r); dfree(B64.z); dfree(B64.p); dfree(B64.q); dfree(B64.dinv); dfree(B64.p2);
     dfree(B32.r); dfree(B32.z); dfree(B32.p); dfree(B32.q); dfree(B32.dinv); dfree(B32.p2);
   };
@@ -1470,9 +1450,10 @@ static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, 
   int nb1 = (int)grid1(n, 256);
   hipLaunchKernelGGL(k_diag_inv<double>, dim3(nb1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v64, s->d_fixed,
                      B64.dinv, bad);
-  if (mixed)
+  if (mixed && !mf)
     hipLaunchKernelGGL(k_diag_inv<float>, dim3(nb1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v32, s->d_fixed,
                        B32.dinv, bad);
+  if (mf) hipLaunchKernelGGL(k_round_f32, dim3(nb1), dim3(256), 0, st, n, B64.dinv, B32.dinv);
   int hbad = 0;
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
@@ -1492,10 +1473,19 @@ static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, 
   const SpOp<double> K64 = csr_op<double>(n, s->d_rp, s->d_ci, s->d_v64, S);
   SpOp<double> A64 = K64;
   SpOp<float> A32 = csr_op<float>(n, s->d_rp, s->d_ci, s->d_v32, S);
+  if (mf && (rc = mf_coef32(ctx, s, &box, &mfc))) {
+    cleanup();
+    return rc;
+  }
+  if (mf) {
+    A32.mf = true;
+    A32.box = box;
+    A32.nb = mf_grid(box);
+  }
   if (s->bsell.state > 0) {
     if (!mixed && !(rc = bsell_values<double>(s, s->d_v64, &s->bsell.d_v64)))
       A64 = bsell_op<double>(n, s->bsell, s->bsell.d_v64);
-    if (mixed && !(rc = bsell_values<float>(s, s->d_v32, &s->bsell.d_v32)))
+    if (mixed && !mf && !(rc = bsell_values<float>(s, s->d_v32, &s->bsell.d_v32)))
       A32 = bsell_op<float>(n, s->bsell, s->bsell.d_v32);
     if (rc) {
       cleanup();

@@ -32,6 +32,9 @@ SOLVE_PCG64, SOLVE_MIXED = 0, 1
 SOLVE_PCG64_MG = 2
 # the reference's dense partial-pivot LU on the GPU (at most 8000 free DOFs)
 SOLVE_DIRECT = 3
+# the precision form on the assembled FP32 values (SOLVE_MIXED is matrix-free
+# on box meshes of P1 tets)
+SOLVE_MIXED_CSR = 4
 NODES = {LINE: 2, TRI: 3, QUAD: 4, TET: 4, HEX: 8}
 DIMS = {LINE: 1, TRI: 2, QUAD: 2, TET: 3, HEX: 3}
 
@@ -136,6 +139,7 @@ _SIGS = {
     "tfm_system_assemble": (INT, [P, C.POINTER(Operator), INT]),
     "tfm_system_info": (INT, [P, P, P, P]),
     "tfm_system_assembly_info": (INT, [P, P, P, P, P]),
+    "tfm_debug_mf_apply": (INT, [P, P, P]),
     "tfm_system_download": (INT, [P, P, P, P, P, P]),
     "tfm_system_solve": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
     "tfm_system_solve_noalloc": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
@@ -495,6 +499,13 @@ class System:
         nf, nz, nd = I64(), I64(), I64()
         _lib.tfm_system_info(handle, C.byref(nf), C.byref(nz), C.byref(nd))
         self.n_free, self.nnz, self.n_dof = nf.value, nz.value, nd.value
+
+    def mf_apply(self, x):
+        """K x by the precision form's matrix-free FP32 operator (test hook)."""
+        xx = _f64(x)
+        y = np.zeros(self.n_dof)
+        self.ctx.check(_lib.tfm_debug_mf_apply(self.h, _ptr(xx), _ptr(y)))
+        return y
 
     def assembly_info(self):
         """Numeric assembly kernel and how its tile size was chosen."""
