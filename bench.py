@@ -12,9 +12,14 @@ mesh (the hot path), with mesh, coefficients and the CSR pattern resident in
 HBM.  value = elements assembled per second by the FP64 (plain) pass.  After
 the timed steps the reduced system is solved with FP64 PCG to a relative
 residual of 1e-12 (the FP64 reference solution); then plain FP64 PCG and
-FP32 PCG with FP64 residual replacement are each timed to 1e-8 relative
-error against that solution (the loosest residual tolerance of a sweep that
-reaches it), and the line reports both times and their ratio.
+FP32 PCG with FP64 residual replacement (on the matrix-free FP32 operator,
+and on the assembled FP32 values) are each timed to 1e-8 relative error
+against that solution (the loosest residual tolerance of a sweep that
+reaches it), and the line reports the times and their ratios, with the
+multigrid CG beside them.  Sub-records c2 (64^3 elasticity), c4 (256 uneven
+Poisson boxes, batched) and c5 (64 SIMP designs x 8 loads, batched) carry
+each workload's assembly passes (time, algorithmic bytes, HBM fraction) and
+its solvers' times to 1e-8 error against an FP64 solve.
 """
 import argparse
 import json
@@ -30,9 +35,9 @@ sys.path.insert(0, os.path.join(ROOT, "tpt-fem_amd"))
 PEAK_HBM_GBS = 8000.0
 
 
-def asm_kernel(tile_pairs, vt="double"):
-    """Name rocprofv3 gives the P1-tet elasticity assembly kernel at a tile size."""
-    return f"tfm::k_asm_tet<3, {vt}, {tile_pairs}>"
+def asm_kernel(tile_pairs, vt="double", dpn=3):
+    """Name rocprofv3 gives the P1-tet assembly kernel at a tile size."""
+    return f"tfm::k_asm_tet<{dpn}, {vt}, {tile_pairs}>"
 
 
 def asm_bytes(n_nodes, n_elem, nnz, n_free, sv, per_elem_coef=True):
@@ -132,15 +137,133 @@ def box_face_nodes(n):
     return ((i * (ny + 1) + j) * (nz + 1) + k)[on]
 
 
-def bench_batch(args, T, log):
+def timed_assembly(ctx, s, steps, warmup):
+    """Mean event-timed FP64 and FP32 numeric passes (ms) of a system."""
+    out = {}
+    for which, name in ((T_SYS[0], "fp64"), (T_SYS[1], "fp32")):
+        for _ in range(warmup):
+            s.assemble(None, which)
+        ts = []
+        for _ in range(steps):
+            ctx.timer_begin()
+            s.assemble(None, which)
+            ts.append(ctx.timer_end())
+        out[name] = float(np.mean(ts))
+    s.assemble(None, 0)
+    return out
+
+
+def assembly_record(s, mesh, ms, per_elem_coef, cell_dpn):
+    """Assembly times, algorithmic GB/s and HBM roofline fraction per pass."""
+    info = s.assembly_info()
+    rec = {"fp64_ms": ms["fp64"], "fp32_ms": ms["fp32"], "autotune": info}
+    for name, sv in (("fp64", 8), ("fp32", 4)):
+        b = asm_bytes(mesh.n_nodes, mesh.n_elem, s.nnz, s.n_free, sv, per_elem_coef)
+        gbs = b / (ms[name] * 1e-3) / 1e9
+        vt = "double" if sv == 8 else "float"
+        kern = (asm_kernel(info["tile_pairs"], vt, cell_dpn[1]) if info["kernel"] == "p1_tet"
+                else f"tfm::k_asm_tile<{cell_dpn[0]}, {cell_dpn[1]}, {2 if sv == 8 else 3}>")
+        rec[name + "_roofline"] = {"kernel": kern, "algorithmic_bytes": b, "GBps": gbs, "frac": gbs / PEAK_HBM_GBS}
+    return rec
+
+
+def sweep(run, err_of, rtols, target, log, name):
+    """The loosest tolerance of `rtols` whose solution is within `target`
+    (relative error) of the FP64 reference: (that solve's record, every try)."""
+    tried = []
+    for rtol in rtols:
+        rec = run(rtol)
+        rec["rel_err_vs_fp64"] = err_of(rec.pop("u"))
+        rec["rtol"] = rtol
+        tried.append(rec)
+        log(f"{name}: rtol {rtol:.0e}: {rec['iters']} it, {rec['s']:.3f} s, err {rec['rel_err_vs_fp64']:.2e}")
+        if rec["rel_err_vs_fp64"] <= target:
+            return rec, tried
+    return None, tried
+
+
+def single_run(T, ctx, s, mode, inner_rtol):
+    def run(rtol):
+        ctx.synchronize()
+        w0 = time.perf_counter()
+        u = s.solve(mode, rtol=rtol, max_iter=200000, inner_rtol=inner_rtol)
+        wall = time.perf_counter() - w0
+        st = s.stats
+        return {"s": st.seconds, "iters": st.iterations, "cycles": st.outer,
+                "ms_per_iter": 1e3 * st.seconds / max(1, st.iterations), "wall_s": wall, "u": u}
+    return run
+
+
+def batch_run(T, ctx, s, mode, off, rhs, nrhs, inner_rtol):
+    def run(rtol):
+        ctx.synchronize()
+        w0 = time.perf_counter()
+        u = s.solve_batch(off, rhs=rhs, nrhs=nrhs, mode=mode, rtol=rtol, max_iter=200000, inner_rtol=inner_rtol)
+        wall = time.perf_counter() - w0
+        st = s.batch_stats
+        return {"s": st[0][3], "iters": max(x[0] for x in st), "max_rel_res": max(x[2] for x in st),
+                "wall_s": wall, "u": u}
+    return run
+
+
+def segment_err(u, u_ref, off):
+    """Largest relative error over the problems (and right-hand sides)."""
+    u, u_ref = np.atleast_2d(u), np.atleast_2d(u_ref)
+    worst = 0.0
+    for k in range(u.shape[0]):
+        for p in range(len(off) - 1):
+            a, b = off[p], off[p + 1]
+            nr = np.linalg.norm(u_ref[k, a:b])
+            if nr > 0:
+                worst = max(worst, float(np.linalg.norm(u[k, a:b] - u_ref[k, a:b]) / nr))
+    return worst
+
+
+BATCH_RTOLS = [1e-8, 3e-9, 1e-9, 3e-10, 1e-10, 3e-11, 1e-11]
+
+
+def bench_c2(args, T, log):
+    """box_mesh 64^3 P1-tet elasticity, E = 1, nu = 0.3, x = 0 clamped,
+    traction on x = 1: assembly and the solvers to 1e-8 error vs FP64."""
+    ctx = T.Context(0)
+    n = 64
+    mesh = T.Mesh.box(ctx, [0.0, 0.0, 0.0], [1.0, 1.0, 1.0], (n, n, n))
+    bcs = [(int(i) * 3 + k, 0.0) for i in mesh.nodes_on_plane(0, 0.0, 1e-9) for k in range(3)]
+    rhs = T.plane_traction(mesh, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
+    s = T.System.create(mesh, T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2), 3, bcs, rhs,
+                        flags=T.SYS_FP64 | T.SYS_FP32)
+    rec = {"workload": "P1-tet elasticity box_mesh 64^3 x6, E 1, nu 0.3, x=0 clamped, traction on x=1",
+           "n_elem": mesh.n_elem, "n_free": s.n_free, "nnz": s.nnz,
+           "assembly": assembly_record(s, mesh, timed_assembly(ctx, s, args.steps, args.warmup), False, (3, 3))}
+    if not args.skip_solve:
+        u_ref = s.solve(T.SOLVE_PCG64_MG, rtol=1e-12, max_iter=5000)
+        nref = np.linalg.norm(u_ref)
+        err = lambda u: float(np.linalg.norm(u - u_ref) / nref)
+        solves = {}
+        for mode, name in ((T.SOLVE_PCG64, "plain_fp64"), (T.SOLVE_MIXED, "mixed_fp32_fp64refine"),
+                           (T.SOLVE_PCG64_MG, "multigrid_fp64")):
+            solves[name], _ = sweep(single_run(T, ctx, s, mode, args.inner_rtol), err, args.rtol_sweep,
+                                    args.err_target, log, "c2 " + name)
+        rp, rm = solves["plain_fp64"], solves["mixed_fp32_fp64refine"]
+        solves["mixed_speedup_vs_plain"] = rp["s"] / rm["s"] if rp and rm else None
+        rec["solve"] = solves
+    s.close()
+    mesh.close()
+    ctx.close()
+    return rec
+
+
+def bench_batch(args, T, log, kind, emit=True):
     """The batching forms: 256 uneven Poisson boxes (kappa_e = exp(N(0,1)),
     f = 1, u = 0 on the faces), or 64 SIMP Q4 cantilever designs x 8 point
     loads.  One assembly pass fills the block-diagonal system; one segmented
-    PCG solves every problem (and right-hand side) together."""
+    PCG solves every problem (and right-hand side) together; each solver is
+    timed to 1e-8 relative error (worst problem and load) against an FP64
+    solve of the batch."""
     ctx = T.Context(0)
     rng = np.random.default_rng(2024)
     t0 = time.perf_counter()
-    if args.workload == "c4":
+    if kind == "c4":
         dims = [tuple(int(v) for v in d) for d in rng.integers(8, 49, size=(256, 3))]
         mesh, noff, eoff = T.Mesh.box_batch(ctx, dims)
         kappa = np.exp(rng.standard_normal(mesh.n_elem))
@@ -148,9 +271,11 @@ def bench_batch(args, T, log):
         bcs = np.stack([ids.astype(np.float64), np.zeros(len(ids))], axis=1)
         rhs = T.source_vector(mesh, 1.0)
         op = T.Operator.poisson(1.0, 2, kappa_e=kappa)
-        dpn, nrhs, off, f = 1, 1, noff, None
+        dpn, nrhs, off, f, cell = 1, 1, noff, None, 3
         desc = (f"256 Poisson boxes, n per axis U{{8..48}}, {mesh.n_elem} tets, kappa_e = exp(N(0,1)), "
                 "f = 1, u = 0 on faces")
+        ref_mode = T.SOLVE_PCG64
+        modes = [(T.SOLVE_PCG64, "plain_fp64"), (T.SOLVE_MIXED, "mixed_fp32_fp64refine")]
     else:
         P, nx, ny = 64, 257, 129
         mesh = T.Mesh.grid_batch(ctx, P, nx, ny, 1.0)
@@ -165,47 +290,40 @@ def bench_batch(args, T, log):
             for k in range(8):
                 f[k, 2 * (p * nx * ny + rows[p, k] * nx + nx - 1) + 1] = -1.0
         op = T.Operator.simp(rho, nu=0.3, penal=3.0, e0=1.0, emin=1e-9, h=1.0)
-        dpn, nrhs, off, rhs = 2, 8, np.arange(P + 1) * nd1, None
+        dpn, nrhs, off, rhs, cell = 2, 8, np.arange(P + 1) * nd1, None, 2
         desc = f"64 SIMP Q4 designs 257x129 nodes x 8 point loads, rho U[0.01,1], p 3, {mesh.n_elem} quads"
+        ref_mode = T.SOLVE_PCG64_MG
+        modes = [(T.SOLVE_PCG64, "plain_fp64"), (T.SOLVE_MIXED, "mixed_fp32_fp64refine"),
+                 (T.SOLVE_PCG64_MG, "multigrid_fp64")]
     s = T.System.create(mesh, op, dpn, bcs, rhs, flags=T.SYS_FP64 | T.SYS_FP32)
     ctx.synchronize()
-    log(f"{args.workload} setup {time.perf_counter() - t0:.2f}s: {desc}; n_free {s.n_free}, nnz {s.nnz}")
-    times = {}
-    for which, name in ((T.SYS_FP64, "fp64"), (T.SYS_FP32, "fp32")):
-        for _ in range(args.warmup):
-            s.assemble(None, which)
-        ts = []
-        for _ in range(args.steps):
-            ctx.timer_begin()
-            s.assemble(None, which)
-            ts.append(ctx.timer_end())
-        times[name] = float(np.mean(ts))
-    log(f"assembly fp64 {times['fp64']:.3f} ms, fp32 {times['fp32']:.3f} ms")
-    s.assemble(None, 0)
-    solves = {}
-    modes = [(T.SOLVE_PCG64, "plain_fp64", 1e-12), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", 1e-10)]
-    if args.workload == "c5":
-        modes.append((T.SOLVE_PCG64_MG, "multigrid_fp64", 1e-12))
-    modes = [m for m in modes if m[1].split("_")[0] in args.solvers]
-    for mode, name, rtol in modes:
-        u = s.solve_batch(off, rhs=f, nrhs=nrhs, mode=mode, rtol=rtol, max_iter=200000)
-        st = s.batch_stats
-        rec = {"rtol": rtol, "s": st[0][3], "max_iters": max(x[0] for x in st),
-               "max_rel_res": max(x[2] for x in st)}
-        if args.workload == "c5":
-            rec["compliance_sum"] = float(sum(f[k] @ u[k] for k in range(nrhs)))
-            if name == modes[-1][1]:  # f.u per design x load, once
-                rec["compliance"] = [[float("%.12g" % (f[k, off[p]:off[p + 1]] @ u[k, off[p]:off[p + 1]]))
-                                      for k in range(nrhs)] for p in range(len(off) - 1)]
-        solves[name] = rec
-        log(f"{name}: {rec}")
-    line = {"metric": "batched assembly + solve", "workload": args.workload, "config": {"workload": desc},
-            "n_elem": mesh.n_elem, "n_free": s.n_free, "nnz": s.nnz,
-            "assembly_ms": times, "elements_per_s_fp64": mesh.n_elem / (times["fp64"] * 1e-3), "solve": solves}
-    print(json.dumps(line), flush=True)
+    log(f"{kind} setup {time.perf_counter() - t0:.2f}s: {desc}; n_free {s.n_free}, nnz {s.nnz}")
+    ms = timed_assembly(ctx, s, args.steps, args.warmup)
+    log(f"{kind} assembly fp64 {ms['fp64']:.3f} ms, fp32 {ms['fp32']:.3f} ms")
+    rec = {"workload": desc, "n_elem": mesh.n_elem, "n_free": s.n_free, "nnz": s.nnz,
+           "assembly": assembly_record(s, mesh, ms, True, (cell, dpn)),
+           "elements_per_s_fp64": mesh.n_elem / (ms["fp64"] * 1e-3)}
+    if not args.skip_solve:
+        u_ref = s.solve_batch(off, rhs=f, nrhs=nrhs, mode=ref_mode, rtol=1e-12, max_iter=200000)
+        err = lambda u: segment_err(u, u_ref, off)
+        solves = {"reference": {"mode": "plain_fp64" if ref_mode == T.SOLVE_PCG64 else "multigrid_fp64",
+                                "rtol": 1e-12}}
+        for mode, name in [m for m in modes if m[1].split("_")[0] in args.solvers]:
+            solves[name], _ = sweep(batch_run(T, ctx, s, mode, off, f, nrhs, args.inner_rtol), err, BATCH_RTOLS,
+                                    args.err_target, log, f"{kind} {name}")
+        rp, rm = solves.get("plain_fp64"), solves.get("mixed_fp32_fp64refine")
+        solves["mixed_speedup_vs_plain"] = rp["s"] / rm["s"] if rp and rm else None
+        rec["solve"] = solves
     s.close()
     mesh.close()
     ctx.close()
+    if emit:
+        line = {"metric": "batched assembly + solve", "workload": kind, "config": {"workload": desc}, **rec}
+        print(json.dumps(line), flush=True)
+    return rec
+
+
+T_SYS = (1, 2)  # TFM_SYS_FP64, TFM_SYS_FP32
 
 
 def bench_simp(args, T, log):
@@ -269,51 +387,10 @@ def bench_simp(args, T, log):
     ctx.close()
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=128, help="cells per axis")
-    ap.add_argument("--err-target", type=float, default=1e-8,
-                    help="relative error vs the FP64 solution (rtol 1e-12) that the timed solves must reach")
-    ap.add_argument("--rtol-sweep", type=float, nargs="+", default=[1e-6, 7e-7, 5e-7, 4e-7, 3e-7, 2e-7, 1.5e-7, 1e-7, 7e-8, 5e-8, 3e-8, 2e-8, 1e-8, 5e-9, 2e-9, 1e-9],
-                    help="residual tolerances tried, loosest first, until the error target is met")
-    ap.add_argument("--inner-rtol", type=float, default=0.3,
-                    help="FP32 residual reduction between FP64 residual replacements")
-    ap.add_argument("--cpu-n", type=int, default=48, help="cells per axis of the CPU-baseline sample (about 10 s on one core)")
-    ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--skip-solve", action="store_true")
-    ap.add_argument("--workload", choices=["c3", "c4", "c5", "simp"], default="c3",
-                    help="c3: the headline single problem; c4 / c5: the batching forms; simp: the batched SIMP "
-                         "loop (own JSON lines)")
-    ap.add_argument("--simp-iters", type=int, default=30,
-                    help="OC iterations of the simp workload (the reference example's max_iter)")
-    ap.add_argument("--solvers", nargs="+", choices=["plain", "mixed", "multigrid"],
-                    default=["plain", "mixed", "multigrid"],
-                    help="solvers timed by every workload (c3 times plain and mixed to the error target, and the "
-                         "multigrid CG beside them)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="only run five 1 GiB device-to-device copies (scale for HBM byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(sharding a problem over several GPUs is designed but not built)", file=sys.stderr)
-        sys.exit(2)
-    if args.copy_probe:
-        copy_probe()
-        return
-
-    import tptfem as T
-
+def bench_c3(args, T, log):
+    """The headline workload: timed assembly passes, then the solvers to the
+    error target against an FP64 reference solve."""
     n = args.n
-    log = lambda *a: print(*a, file=sys.stderr, flush=True)
-    if args.workload == "simp":
-        bench_simp(args, T, log)
-        return
-    if args.workload != "c3":
-        bench_batch(args, T, log)
-        return
     ctx = T.Context(0)
     t_setup = time.perf_counter()
     mesh = T.Mesh.box(ctx, [0.0, 0.0, 0.0], [1.0, 1.0, 1.0], (n, n, n))
@@ -323,12 +400,15 @@ def main():
     bcs = [(int(i) * 3 + k, 0.0) for i in clamp for k in range(3)]
     rhs = T.plane_traction(mesh, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
     op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=ye)
+    ctx.synchronize()
+    t_create = time.perf_counter()
     sysm = T.System.create(mesh, op, 3, bcs, rhs, flags=T.SYS_FP64 | T.SYS_FP32)
     ctx.synchronize()
+    create_s = time.perf_counter() - t_create
     tune = sysm.assembly_info()
     kernel = asm_kernel(tune["tile_pairs"])
-    log(f"setup {time.perf_counter() - t_setup:.2f}s: {mesh.n_elem} tets, {mesh.n_nodes} nodes, "
-        f"n_free {sysm.n_free}, nnz {sysm.nnz}; assembly kernel {kernel} ({tune['source']})")
+    log(f"setup {time.perf_counter() - t_setup:.2f}s (system create {create_s:.3f}s): {mesh.n_elem} tets, "
+        f"{mesh.n_nodes} nodes, n_free {sysm.n_free}, nnz {sysm.nnz}; assembly kernel {kernel} ({tune['source']})")
 
     # ---- timed assembly --------------------------------------------------
     for _ in range(args.warmup):
@@ -364,37 +444,20 @@ def main():
         solve["ref_fp64_rtol1e-12"] = {"s": st.seconds, "iters": st.iterations, "rel_res": st.rel_residual}
         log(f"fp64 ref: {st.iterations} it, {st.seconds:.3f}s, res {st.rel_residual:.2e}")
         nref = np.linalg.norm(u_ref)
-
-        def to_target(mode, name):
-            """Loosest residual tolerance of the sweep whose solution is within
-            --err-target of the FP64 reference; returns that solve's record
-            and every attempt."""
-            tried = []
-            for rtol in args.rtol_sweep:
-                ctx.synchronize()
-                w0 = time.perf_counter()
-                u = sysm.solve(mode, rtol=rtol, max_iter=200000, inner_rtol=args.inner_rtol)
-                wall_s = time.perf_counter() - w0
-                sr = sysm.stats
-                err = float(np.linalg.norm(u - u_ref) / nref)
-                rec = {"rtol": rtol, "s": sr.seconds, "iters": sr.iterations, "cycles": sr.outer,
-                       "rel_err_vs_fp64": err, "ms_per_iter": 1e3 * sr.seconds / max(1, sr.iterations),
-                       "wall_s": wall_s}
-                tried.append(rec)
-                log(f"{name}: rtol {rtol:.0e}: {sr.iterations} it / {sr.outer} cycles, {sr.seconds:.3f}s, "
-                    f"err {err:.2e}")
-                if err <= args.err_target:
-                    return rec, tried
-            return None, tried
-
-        rp, tp = to_target(T.SOLVE_PCG64, "plain") if "plain" in args.solvers else (None, [])
-        rm, tm = to_target(T.SOLVE_MIXED, "mixed") if "mixed" in args.solvers else (None, [])
-        solve.update({
-            "err_target": args.err_target,
-            "plain_fp64": rp, "mixed_fp32_fp64refine": rm,
-            "mixed_speedup_vs_plain": (rp["s"] / rm["s"]) if (rp and rm and rm["s"] > 0) else None,
-            "sweep": {"plain_fp64": tp, "mixed_fp32_fp64refine": tm},
-        })
+        err = lambda u: float(np.linalg.norm(u - u_ref) / nref)
+        run = lambda mode: single_run(T, ctx, sysm, mode, args.inner_rtol)
+        legs = [(T.SOLVE_PCG64, "plain_fp64", "plain"), (T.SOLVE_MIXED, "mixed_fp32_fp64refine", "mixed"),
+                (T.SOLVE_MIXED_CSR, "mixed_csr_fp32_fp64refine", "mixed")]
+        sw = {}
+        for mode, name, key in legs:
+            if key in args.solvers:
+                solve[name], sw[name] = sweep(run(mode), err, args.rtol_sweep, args.err_target, log, name)
+        rp, rm, rc = solve.get("plain_fp64"), solve.get("mixed_fp32_fp64refine"), solve.get("mixed_csr_fp32_fp64refine")
+        solve["err_target"] = args.err_target
+        solve["mixed_operator"] = "matrix-free FP32 (box mesh of P1 tets)"
+        solve["mixed_speedup_vs_plain"] = (rp["s"] / rm["s"]) if (rp and rm and rm["s"] > 0) else None
+        solve["mixed_csr_speedup_vs_plain"] = (rp["s"] / rc["s"]) if (rp and rc and rc["s"] > 0) else None
+        solve["sweep"] = sw
         if "multigrid" in args.solvers:
             # FP64 CG with the box-mesh multigrid V-cycle.  Its first call
             # builds the solver kept on the system (sliced copy, level
@@ -405,11 +468,11 @@ def main():
             w0 = time.perf_counter()
             sysm.solve(T.SOLVE_PCG64_MG, rtol=1e-2, max_iter=1000)
             first_wall = time.perf_counter() - w0
-            rg, tg = to_target(T.SOLVE_PCG64_MG, "multigrid")
+            rg, tg = sweep(run(T.SOLVE_PCG64_MG), err, args.rtol_sweep, args.err_target, log, "multigrid")
             if rg:
                 rg["first_call_wall_s"] = first_wall
             solve["multigrid_fp64"] = rg
-            solve["sweep"]["multigrid_fp64"] = tg
+            sw["multigrid_fp64"] = tg
             if rp and rg:
                 solve["multigrid_speedup_vs_plain_wall"] = rp["wall_s"] / rg["wall_s"]
 
@@ -418,7 +481,8 @@ def main():
         eps, ne, dt = cpu_baseline(args.cpu_n)
         cpu = {"value": eps, "unit": "elements/s", "cores": 1, "kind": "port", "host": host_cpu(),
                "sample": f"oracle assemble+to_csr+reduce_system, {args.cpu_n}^3 jittered elasticity "
-                         f"({ne} tets, {dt:.1f}s)"}
+                         f"({ne} tets, {dt:.1f}s); a port of the reference's loop in C at -O3 without its "
+                         "per-element heap allocations, so faster than the reference itself"}
         log(f"cpu baseline {eps:.0f} elements/s")
 
     line = {
@@ -438,17 +502,80 @@ def main():
                                "nu 0.3, x=0 clamped, traction on x=1",
                    "n_elem": mesh.n_elem, "n_nodes": mesh.n_nodes, "n_free": sysm.n_free, "nnz_reduced": sysm.nnz},
         "assembly": {"fp64_ms": ms64, "fp32_ms": ms32, "fp32_elements_per_s": mesh.n_elem / (ms32 * 1e-3),
-                     "fp64_GBps": gbs64, "fp32_GBps": gbs32, "autotune": tune},
+                     "fp64_GBps": gbs64, "fp32_GBps": gbs32, "autotune": tune,
+                     "fp32_roofline": {"kernel": asm_kernel(tune["tile_pairs"], "float"),
+                                       "algorithmic_bytes": b32, "frac": gbs32 / PEAK_HBM_GBS},
+                     "system_create_s": create_s,
+                     "end_to_end_elements_per_s": mesh.n_elem / create_s},
         "solve": solve,
         "roofline": {"bound": "hbm", "achieved": gbs64, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": gbs64 / PEAK_HBM_GBS, "traffic": measured_traffic(kernel, n),
                      "kernel": kernel, "algorithmic_bytes_per_launch": b64},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
     sysm.close()
     mesh.close()
     ctx.close()
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128, help="cells per axis")
+    ap.add_argument("--err-target", type=float, default=1e-8,
+                    help="relative error vs the FP64 solution (rtol 1e-12) that the timed solves must reach")
+    ap.add_argument("--rtol-sweep", type=float, nargs="+", default=[1e-6, 7e-7, 5e-7, 4e-7, 3e-7, 2e-7, 1.5e-7, 1e-7, 7e-8, 5e-8, 3e-8, 2e-8, 1e-8, 5e-9, 2e-9, 1e-9],
+                    help="residual tolerances tried, loosest first, until the error target is met")
+    ap.add_argument("--inner-rtol", type=float, default=0.3,
+                    help="FP32 residual reduction between FP64 residual replacements")
+    ap.add_argument("--cpu-n", type=int, default=48, help="cells per axis of the CPU-baseline sample (about 10 s on one core)")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-solve", action="store_true")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="c3 only: leave out the c2 / c4 / c5 sub-records of the line")
+    ap.add_argument("--workload", choices=["c3", "c2", "c4", "c5", "simp"], default="c3",
+                    help="c3: the headline single problem (with c2 / c4 / c5 sub-records); c2 / c4 / c5 alone; "
+                         "simp: the batched SIMP loop (own JSON lines)")
+    ap.add_argument("--simp-iters", type=int, default=30,
+                    help="OC iterations of the simp workload (the reference example's max_iter)")
+    ap.add_argument("--solvers", nargs="+", choices=["plain", "mixed", "multigrid"],
+                    default=["plain", "mixed", "multigrid"],
+                    help="solvers timed by every workload (c3 times plain and mixed to the error target, and the "
+                         "multigrid CG beside them)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run five 1 GiB device-to-device copies (scale for HBM byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding a problem over several GPUs is designed but not built)", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
+
+    import tptfem as T
+
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.workload == "simp":
+        bench_simp(args, T, log)
+        return
+    if args.workload == "c2":
+        print(json.dumps({"metric": "assembly + solve", "workload": "c2", **bench_c2(args, T, log)}), flush=True)
+        return
+    if args.workload in ("c4", "c5"):
+        bench_batch(args, T, log, args.workload)
+        return
+    line = bench_c3(args, T, log)
+    if not args.no_sub:
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = min(args.steps, 10), min(args.warmup, 2)
+        line["c2"] = bench_c2(sub, T, log)
+        line["c4"] = bench_batch(sub, T, log, "c4", emit=False)
+        line["c5"] = bench_batch(sub, T, log, "c5", emit=False)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":
