@@ -19,7 +19,8 @@ reaches it), and the line reports the times and their ratios, with the
 multigrid CG beside them.  Sub-records c2 (64^3 elasticity), c4 (256 uneven
 Poisson boxes, batched) and c5 (64 SIMP designs x 8 loads, batched) carry
 each workload's assembly passes (time, algorithmic bytes, HBM fraction) and
-its solvers' times to 1e-8 error against an FP64 solve.
+its solvers' times to 1e-8 error against an FP64 solve; hex8 (64^3 jittered
+Hex8 elasticity) the same, with the FP32 pass on the matrix cores.
 """
 import argparse
 import json
@@ -323,6 +324,65 @@ def bench_batch(args, T, log, kind, emit=True):
     return rec
 
 
+def hex_box(n, seed=1):
+    """n^3 Hex8 box on the unit cube, interior nodes jittered by +-0.2h."""
+    i, j, k = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    coords = np.stack([i, j, k], -1).reshape(-1, 3).astype(np.float64) / n
+    rng = np.random.default_rng(seed)
+    inner = ((i > 0) & (i < n) & (j > 0) & (j < n) & (k > 0) & (k < n)).ravel()
+    coords[inner] += rng.uniform(-0.2 / n, 0.2 / n, (int(inner.sum()), 3))
+    idx = lambda a, b, c: (a * (n + 1) + b) * (n + 1) + c
+    I, J, K = [v.ravel() for v in np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")]
+    conn = np.stack([idx(I, J, K), idx(I + 1, J, K), idx(I + 1, J + 1, K), idx(I, J + 1, K),
+                     idx(I, J, K + 1), idx(I + 1, J, K + 1), idx(I + 1, J + 1, K + 1), idx(I, J + 1, K + 1)], -1)
+    return coords, conn, rng
+
+
+def bench_hex(args, T, log, n=64):
+    """Hex8 3-D elasticity on a jittered n^3 box (E per element log-uniform
+    [0.1, 10], nu 0.3, 2x2x2 Gauss points, x = 0 clamped, traction on x = 1):
+    FP64 and FP32 assembly passes, the FP32 one with the element matrices on
+    the matrix cores and, for comparison, with the per-pair FP32 kernel
+    (TFM_NO_MFMA); then plain and precision-form Jacobi PCG to 1e-8 error."""
+    ctx = T.Context(0)
+    coords, conn, rng = hex_box(n)
+    mesh = T.Mesh.from_arrays(ctx, T.HEX, coords, conn)
+    ye = 10.0 ** rng.uniform(-1, 1, len(conn))
+    bcs = [(int(q) * 3 + c, 0.0) for q in np.flatnonzero(coords[:, 0] == 0.0) for c in range(3)]
+    rhs = T.plane_traction(mesh, 3, 0, 1.0, 1e-9, [0.0, 0.0, -1e-3])
+    s = T.System.create(mesh, T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=ye), 3, bcs, rhs,
+                        flags=T.SYS_FP64 | T.SYS_FP32)
+    ms = timed_assembly(ctx, s, args.steps, args.warmup)
+    os.environ["TFM_NO_MFMA"] = "1"
+    try:
+        ms_pair = timed_assembly(ctx, s, args.steps, args.warmup)["fp32"]
+    finally:
+        del os.environ["TFM_NO_MFMA"]
+    s.assemble(None, 0)
+    log(f"hex8 assembly fp64 {ms['fp64']:.3f} ms, fp32 (MFMA) {ms['fp32']:.3f} ms, fp32 per pair {ms_pair:.3f} ms")
+    kpad = 8
+    rec = {"workload": f"Hex8 elasticity {n}^3 jittered box, E log-uniform [0.1,10], nu 0.3, 2x2x2 Gauss",
+           "n_elem": mesh.n_elem, "n_free": s.n_free, "nnz": s.nnz,
+           "assembly": assembly_record(s, mesh, ms, True, (4, 3)),
+           "fp32_per_pair_kernel_ms": ms_pair, "fp32_mfma_speedup": ms_pair / ms["fp32"],
+           "mfma_flops_per_pass": mesh.n_elem * 2 * 32 * 32 * kpad}
+    if not args.skip_solve:
+        u_ref = s.solve(T.SOLVE_PCG64, rtol=1e-12, max_iter=200000)
+        nref = np.linalg.norm(u_ref)
+        err = lambda u: float(np.linalg.norm(u - u_ref) / nref)
+        solves = {}
+        for mode, name in ((T.SOLVE_PCG64, "plain_fp64"), (T.SOLVE_MIXED, "mixed_fp32_fp64refine")):
+            solves[name], _ = sweep(single_run(T, ctx, s, mode, args.inner_rtol), err, args.rtol_sweep,
+                                    args.err_target, log, "hex8 " + name)
+        rp, rm = solves["plain_fp64"], solves["mixed_fp32_fp64refine"]
+        solves["mixed_speedup_vs_plain"] = rp["s"] / rm["s"] if rp and rm else None
+        rec["solve"] = solves
+    s.close()
+    mesh.close()
+    ctx.close()
+    return rec
+
+
 T_SYS = (1, 2)  # TFM_SYS_FP64, TFM_SYS_FP32
 
 
@@ -536,8 +596,8 @@ def main():
     ap.add_argument("--skip-solve", action="store_true")
     ap.add_argument("--no-sub", action="store_true",
                     help="c3 only: leave out the c2 / c4 / c5 sub-records of the line")
-    ap.add_argument("--workload", choices=["c3", "c2", "c4", "c5", "simp"], default="c3",
-                    help="c3: the headline single problem (with c2 / c4 / c5 sub-records); c2 / c4 / c5 alone; "
+    ap.add_argument("--workload", choices=["c3", "c2", "c4", "c5", "hex8", "simp"], default="c3",
+                    help="c3: the headline single problem (with c2 / c4 / c5 / hex8 sub-records); c2 / c4 / c5 / hex8 alone; "
                          "simp: the batched SIMP loop (own JSON lines)")
     ap.add_argument("--simp-iters", type=int, default=30,
                     help="OC iterations of the simp workload (the reference example's max_iter)")
@@ -565,6 +625,9 @@ def main():
     if args.workload == "c2":
         print(json.dumps({"metric": "assembly + solve", "workload": "c2", **bench_c2(args, T, log)}), flush=True)
         return
+    if args.workload == "hex8":
+        print(json.dumps({"metric": "assembly + solve", "workload": "hex8", **bench_hex(args, T, log)}), flush=True)
+        return
     if args.workload in ("c4", "c5"):
         bench_batch(args, T, log, args.workload)
         return
@@ -575,6 +638,7 @@ def main():
         line["c2"] = bench_c2(sub, T, log)
         line["c4"] = bench_batch(sub, T, log, "c4", emit=False)
         line["c5"] = bench_batch(sub, T, log, "c5", emit=False)
+        line["hex8"] = bench_hex(sub, T, log)
     print(json.dumps(line), flush=True)
 
 

@@ -34,7 +34,8 @@
 
 namespace tfm {
 
-enum { M_COUNT = 0, M_FILL = 1, M_F64 = 2, M_F32 = 3 };
+// M_F32T: FP32 values summed from a table of element matrices (P.ke32)
+enum { M_COUNT = 0, M_FILL = 1, M_F64 = 2, M_F32 = 3, M_F32T = 4 };
 constexpr int kTileNodes = 16;  // nodes per tile (hash tables in LDS)
 constexpr int kHash = 64;       // hash slots per node (> distinct neighbours)
 
@@ -61,10 +62,12 @@ __global__ __launch_bounds__((TileCfg<CELL, DPN>::TP)) void k_asm_tile(
   using C = TileCfg<CELL, DPN>;
   constexpr int NN = C::NN, W = C::W, ENT = C::ENT, TP = C::TP;
   constexpr int DIM = Cell<CELL>::DIM;
-  using VT = typename std::conditional<MODE == M_F32, float, double>::type;
+  using VT = typename std::conditional<MODE == M_F32 || MODE == M_F32T, float, double>::type;
 
-  __shared__ VT s_tab[TP][ENT];
+  // row blocks of the tile's pairs (the table mode reads them from P.ke32)
+  __shared__ VT s_tab[(MODE == M_F32T) ? 1 : TP][ENT];
   __shared__ int s_en[TP][NN];
+  __shared__ int s_pe[(MODE == M_F32T) ? TP : 1];  // table mode: element << 3 | local node
   __shared__ int s_nptr[kTileNodes + 1];
   __shared__ int s_key[kTileNodes][kHash];
   __shared__ unsigned long long s_mask[kTileNodes][kHash];
@@ -107,16 +110,16 @@ __global__ __launch_bounds__((TileCfg<CELL, DPN>::TP)) void k_asm_tile(
     }
     double coef = P.coef_e ? P.coef_e[e] : P.coef;
     double rho = P.rho ? P.rho[e] : 0.0;
-    if constexpr (MODE == M_F32) {
+    if constexpr (MODE == M_F32T) {
+      s_pe[tid] = v;  // phase 2 reads the element's matrix rows from the table
+    } else if constexpr (MODE == M_F32) {
       float tmp[ENT];
       rowblock_f32<CELL, DPN>(X, ln, coef, rho, P, tmp);
 #pragma unroll
       for (int i = 0; i < ENT; i++) s_tab[tid][i] = tmp[i];
     } else {
-      double tmp[ENT];
-      rowblock_f64<CELL, DPN>(X, ln, coef, rho, P, tmp);
-#pragma unroll
-      for (int i = 0; i < ENT; i++) s_tab[tid][i] = tmp[i];
+      // accumulated in place in LDS (no register copy of the row block)
+      rowblock_f64<CELL, DPN>(X, ln, coef, rho, P, s_tab[tid]);
     }
 #pragma unroll
     for (int lm = 0; lm < NN; lm++) {
@@ -137,7 +140,7 @@ __global__ __launch_bounds__((TileCfg<CELL, DPN>::TP)) void k_asm_tile(
   }
   __syncthreads();
 
-  if constexpr (MODE == M_F64 || MODE == M_F32) {
+  if constexpr (MODE == M_F64 || MODE == M_F32 || MODE == M_F32T) {
     // ---- phase 2: one lane per CSR slot, contributions in element order ---
     const int sb = s_rp[0], se = s_rp[nt * DPN];
     for (int s = sb + tid; s < se; s += TP) {
@@ -167,7 +170,13 @@ __global__ __launch_bounds__((TileCfg<CELL, DPN>::TP)) void k_asm_tile(
 #pragma unroll
         for (int q = 1; q < NN; q++)
           if (s_en[i][q] == m) lm = q;
-        VT val = s_tab[i][a * W + lm * DPN + b];
+        VT val;
+        if constexpr (MODE == M_F32T) {
+          const int pv = s_pe[i];
+          val = P.ke32[(int64_t)(pv >> 3) * (W * W) + ((pv & 7) * DPN + a) * W + lm * DPN + b];
+        } else {
+          val = s_tab[i][a * W + lm * DPN + b];
+        }
         if (val != (VT)0) acc += val;
       }
       if constexpr (MODE == M_F64) v64[s] = acc;
@@ -455,6 +464,7 @@ static OpParams make_params(const AsmPlan *pl, const double *d_rule, int nq, con
   P.rho = pl->d_rho;
   P.rule = d_rule;
   P.ke0 = d_ke0;
+  P.ke32 = nullptr;
   return P;
 }
 
@@ -613,13 +623,26 @@ int plan_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *d_v64, float *d_v32) {
   if (pl->ntiles == 0) return TFM_OK;
   if (pl->use_tet) return tet_numeric(ctx, pl->mesh, pl->dpn, pl->tet, pl->d_rp, P, d_v64, d_v32);
   if (d_v64 && (rc = dispatch<M_F64>(ctx, pl, P, nullptr, nullptr, nullptr, d_v64, nullptr, nullptr))) return rc;
-  if (d_v32 && (rc = dispatch<M_F32>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, d_v32, nullptr))) return rc;
+  if (d_v32) {
+    // FP32 Hex8 elasticity: element matrices on the matrix cores, once per
+    // element, then the tile pass sums them
+    if (hex_mfma_eligible(pl)) {
+      if (!pl->d_ke32 && (rc = dalloc(ctx, &pl->d_ke32, pl->mesh->n_elem * 576))) return rc;
+      if ((rc = hex_ke_mfma(ctx, pl, pl->d_ke32))) return rc;
+      OpParams P32 = P;
+      P32.ke32 = pl->d_ke32;
+      if ((rc = dispatch<M_F32T>(ctx, pl, P32, nullptr, nullptr, nullptr, nullptr, d_v32, nullptr))) return rc;
+    } else if ((rc = dispatch<M_F32>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, d_v32, nullptr))) {
+      return rc;
+    }
+  }
   TFM_CHECK_HIP(ctx, hipGetLastError());
   return TFM_OK;
 }
 
 void plan_destroy(AsmPlan *pl) {
   tet_plan_destroy(&pl->tet);
+  dfree(pl->d_ke32);
   dfree(pl->d_tab);
   dfree(pl->d_coef_e);
   dfree(pl->d_rho);

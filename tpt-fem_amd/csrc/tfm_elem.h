@@ -38,6 +38,7 @@ struct OpParams {
   const double *rho;     // device, per element (SIMP), or nullptr
   const double *rule;    // device [nq][kRuleStride]
   const double *ke0;     // device [64] (SIMP unit stiffness)
+  const float *ke32;     // device [n_elem][W][W] FP32 element matrices, or nullptr
 };
 
 // MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs,

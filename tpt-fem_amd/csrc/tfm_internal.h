@@ -169,6 +169,9 @@ struct AsmPlan {
   // the scalar coefficient was 0.  A re-assembly may not make them non-zero.
   std::vector<int64_t> zero_elems;
   bool zero_all = false;
+  // FP32 Hex8 elasticity element matrices ([n_elem][24][24], owned), made by
+  // the first FP32 numeric pass that forms them on the matrix cores
+  mutable float *d_ke32 = nullptr;
 };
 struct OpParams;
 int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci, int tile_pairs,
@@ -187,6 +190,10 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
                 const unsigned char *h_fixed, int vals, AsmPlan *plan);
 int plan_set_coefficients(tfm_ctx *ctx, AsmPlan *plan, const tfm_operator *op);
 int plan_numeric(tfm_ctx *ctx, const AsmPlan *plan, double *d_v64, float *d_v32);
+// FP32 element matrices of 3-D isotropic elasticity on Hex8 cells by MFMA
+// (tfm_hex_mfma.hip); TFM_NO_MFMA in the environment switches them off.
+bool hex_mfma_eligible(const AsmPlan *plan);
+int hex_ke_mfma(tfm_ctx *ctx, const AsmPlan *plan, float *ke);
 void plan_destroy(AsmPlan *plan);
 int validate_operator(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn);
 
