@@ -775,3 +775,75 @@ def test_boundary_points_on_a_jittered_mesh_and_higher_order():
         if order == 3:
             rhs = O.apply_neumann_qp(coords, O.TET, conn, 1, x[:, 0] ** 2 * nrm[:, 0], np.zeros(len(coords)), order)
             np.testing.assert_allclose(rhs.sum(), 1.0, rtol=1e-13)
+
+
+def _fma(a, b, c):
+    """a * b + c rounded once (what a fused multiply-add or a matrix core's
+    accumulate step does), exactly via rationals."""
+    from fractions import Fraction
+
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _hex_elasticity_ke(phys, young, nu, order, fused):
+    """elasticity_element_matrix for a Hex8 in the reference's operation
+    order (crates/tpt-fem-elasticity/src/lib.rs:262-309: D, then per point
+    physical gradients, B, and s = sum_{s,t} B_i[s][a] D[s][t] B_j[t][b]
+    from 0.0, k += w det s), the inner accumulation either separately
+    rounded (the reference) or fused."""
+    c = young / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    m, dd = (1.0 - nu) * c, (1.0 - 2.0 * nu) / 2.0 * c
+    d = [[0.0] * 6 for _ in range(6)]
+    for i in range(3):
+        for j in range(3):
+            d[i][j] = m if i == j else c * nu
+    d[3][3] = d[4][4] = d[5][5] = dd
+    qp, qw = O.cell_quad(O.HEX, order)
+    k = [[0.0] * 24 for _ in range(24)]
+    for q in range(len(qw)):
+        lg = O.grad(O.HEX, qp[q])
+        _, det, inv = O.map_jacobian(phys, lg)
+        det = abs(det)
+        b = []
+        for a in range(8):
+            g = []
+            for i in range(3):
+                s = 0.0
+                for r in range(3):
+                    s += float(lg[a][r]) * float(inv[r][i])
+                g.append(s)
+            bm = [[0.0] * 3 for _ in range(6)]
+            bm[0][0], bm[1][1], bm[2][2] = g[0], g[1], g[2]
+            bm[3][0], bm[3][1], bm[4][1], bm[4][2], bm[5][0], bm[5][2] = g[1], g[0], g[2], g[1], g[2], g[0]
+            b.append(bm)
+        for i in range(8):
+            for j in range(8):
+                for a in range(3):
+                    for bb in range(3):
+                        s = 0.0
+                        for s_ in range(6):
+                            for t in range(6):
+                                x = b[i][s_][a] * d[s_][t]
+                                s = _fma(x, b[j][t][bb], s) if fused else s + x * b[j][t][bb]
+                        k[i * 3 + a][j * 3 + bb] += float(qw[q]) * det * s
+    return np.array(k)
+
+
+def test_fused_accumulation_cannot_reproduce_the_reference_bits():
+    """Why the bit-exact FP64 path stays off the matrix cores (and off FMA):
+    the matrix cores' multiply-accumulate rounds a * b + c once, the
+    reference rounds the product and the sum separately.  The same
+    Hex8 element matrix in the same order, with only that accumulation
+    fused, differs from the reference in most entries (by ~1e-16 of the
+    largest), while the separately rounded restatement reproduces the
+    oracle's matrix bit for bit."""
+    rng = np.random.default_rng(11)
+    phys = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    phys = phys * [1.0, 0.8, 1.3] + rng.uniform(-0.15, 0.15, phys.shape)
+    ref = O.elasticity_element_matrix(O.HEX, phys, O.CONT3D, 2.5, 0.3, 2)
+    sep = _hex_elasticity_ke(phys, 2.5, 0.3, 2, fused=False)
+    assert np.array_equal(sep.view(np.uint64), ref.view(np.uint64))
+    fus = _hex_elasticity_ke(phys, 2.5, 0.3, 2, fused=True)
+    differ = np.count_nonzero(fus != ref)
+    assert differ > 24 * 24 // 4
+    assert np.abs(fus - ref).max() <= 1e-14 * np.abs(ref).max()
