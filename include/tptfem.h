@@ -211,7 +211,8 @@ int tfm_system_assemble(tfm_system *sys, const tfm_operator *op, int which);
 int tfm_system_info(const tfm_system *sys, int64_t *n_free, int64_t *nnz, int64_t *n_dof);
 /* No counterpart in the reference (diagnostic): which numeric assembly
  * kernel the system refills with and how its launch shape was chosen.
- * path: 0 = generic tile kernel, 1 = P1-tet kernel; tile_pairs: element-node
+ * path: 0 = generic tile kernel, 1 = P1-tet kernel, 2 = sorted triplets (node
+ * valence beyond the tile kernels); tile_pairs: element-node
  * pairs (= threads) per P1-tet tile (192 or 256); source: 1 = timed at
  * creation, 2 = reused from an earlier system of the same dofs per node,
  * value arrays and size class (floor(log2(elements))), 3 = forced by

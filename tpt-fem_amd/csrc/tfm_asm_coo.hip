@@ -1,0 +1,283 @@
+// tfm_asm_coo.hip -- assembly by sorted triplets, for meshes whose node
+// valence exceeds the tile kernels' limits (a node in more than 64 elements,
+// or with more distinct neighbours than a tile's per-node hash holds: the
+// Delaunay meshes crates/tpt-fem-mesh-gen/src/lib.rs:163-409 builds and the
+// Gmsh meshes crates/tpt-fem-mesh/src/lib.rs:260-388 imports).
+//
+// This is the reference's own route, restated for the GPU: assemble pushes
+// every non-zero element entry as a triplet in element order, local row,
+// local column (crates/tpt-fem-assembly/src/lib.rs:428-480), and
+// Coo::to_csr sums duplicates in insertion order
+// (crates/tpt-fem-sparse/src/lib.rs:39-130).  Here one thread per
+// (element, local node) pair evaluates its row block and writes its kept
+// triplets at its scanned offset (key = row << 32 | column, payload = the
+// entry's position in element-major order), a stable radix sort by key
+// brings every CSR entry's contributions together in that insertion order,
+// and the numeric pass sums each entry's contributions in that order from a
+// dense table of element values.  Results equal the tile kernels' (and the
+// reference's) bit for bit; it costs a sort at plan creation and a table of
+// all element entries per numeric pass.
+#include <algorithm>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "tfm_elem.h"
+#include "tfm_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace tfm {
+
+enum { C_COUNT = 0, C_FILL = 1, C_F64 = 2, C_F32 = 3 };
+
+// One (element, local node) pair per thread, t = e * NN + ln (element-major).
+template <int CELL, int DPN, int WHAT>
+__global__ __launch_bounds__(128) void k_coo_pairs(int64_t npairs, const double *__restrict__ coords, int pdim,
+                                                    const int *__restrict__ conn, OpParams P,
+                                                    const unsigned char *__restrict__ fixed, int *__restrict__ cnt,
+                                                    const int *__restrict__ off, unsigned long long *__restrict__ keys,
+                                                    int64_t *__restrict__ idx, double *__restrict__ v64,
+                                                    float *__restrict__ v32) {
+  constexpr int NN = Cell<CELL>::NN, DIM = Cell<CELL>::DIM, W = NN * DPN, ND = NN * DPN;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  const int64_t e = t / NN;
+  const int ln = (int)(t - e * NN);
+  int en[NN];
+  double X[NN][3];
+#pragma unroll
+  for (int a = 0; a < NN; a++) {
+    en[a] = conn[e * NN + a];
+#pragma unroll
+    for (int p = 0; p < 3; p++) X[a][p] = p < DIM ? coords[(int64_t)en[a] * pdim + p] : 0.0;
+  }
+  const double coef = P.coef_e ? P.coef_e[e] : P.coef;
+  const double rho = P.rho ? P.rho[e] : 0.0;
+  const int64_t base = e * (ND * ND) + (int64_t)ln * DPN * ND;  // first entry of the pair's rows
+  if constexpr (WHAT == C_F32) {
+    float tmp[DPN * W];
+    rowblock_f32<CELL, DPN>(X, ln, coef, rho, P, tmp);
+#pragma unroll
+    for (int i = 0; i < DPN * W; i++) v32[base + i] = tmp[i];
+  } else {
+    double tmp[DPN * W];
+    rowblock_f64<CELL, DPN>(X, ln, coef, rho, P, tmp);
+    if constexpr (WHAT == C_F64) {
+#pragma unroll
+      for (int i = 0; i < DPN * W; i++) v64[base + i] = tmp[i];
+    } else {
+      int c = 0, o = WHAT == C_FILL ? off[t] : 0;
+#pragma unroll
+      for (int a = 0; a < DPN; a++) {
+        const int row = en[ln] * DPN + a;
+        if (fixed && fixed[row]) continue;
+#pragma unroll
+        for (int lm = 0; lm < NN; lm++)
+#pragma unroll
+          for (int b = 0; b < DPN; b++) {
+            const int col = en[lm] * DPN + b;
+            // the reference pushes non-zero entries (every SIMP entry)
+            if ((P.kind != TFM_OP_SIMP && tmp[a * W + lm * DPN + b] == 0.0) || (fixed && fixed[col])) continue;
+            if constexpr (WHAT == C_FILL) {
+              keys[o + c] = ((unsigned long long)row << 32) | (unsigned)col;
+              idx[o + c] = base + a * ND + lm * DPN + b;
+            }
+            c++;
+          }
+      }
+      if constexpr (WHAT == C_COUNT) cnt[t] = c;
+    }
+  }
+}
+
+// First triplet of each CSR entry: flag[k] = key[k] != key[k - 1].
+__global__ void k_coo_heads(int64_t n, const unsigned long long *__restrict__ keys, int *__restrict__ flag) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) flag[k] = (k == 0 || keys[k] != keys[k - 1]) ? 1 : 0;
+}
+
+// Per CSR entry q (its head k): column, first triplet, and the row count.
+__global__ void k_coo_entries(int64_t n, const unsigned long long *__restrict__ keys, const int *__restrict__ flag,
+                              const int *__restrict__ eid, int *__restrict__ ci, int *__restrict__ seg,
+                              int *__restrict__ rcnt, int *__restrict__ dims) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n || !flag[k]) return;
+  const int q = eid[k];
+  const int row = (int)(keys[k] >> 32), col = (int)(keys[k] & 0xffffffffull);
+  ci[q] = col;
+  seg[q] = (int)k;
+  atomicAdd(&rcnt[row], 1);
+  atomicMax(&dims[0], row + 1);
+  atomicMax(&dims[1], col + 1);
+}
+
+// One CSR entry per thread: its contributions in insertion order from +0.0.
+template <typename VT>
+__global__ void k_coo_sum(int64_t nnz, const int *__restrict__ seg, const int64_t *__restrict__ perm,
+                          const VT *__restrict__ dense, VT *__restrict__ out) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  VT acc = 0;
+  for (int k = seg[q]; k < seg[q + 1]; k++) {
+    const VT v = dense[perm[k]];
+    if (v != (VT)0) acc += v;
+  }
+  out[q] = acc;
+}
+
+template <int WHAT>
+static int coo_launch(tfm_ctx *ctx, const AsmPlan *pl, const OpParams &P, int *cnt, const int *off,
+                      unsigned long long *keys, int64_t *idx, double *v64, float *v32) {
+  const tfm_mesh *m = pl->mesh;
+  const int64_t npairs = m->n_elem * m->nn;
+  if (npairs == 0) return TFM_OK;
+  const dim3 g((unsigned)((npairs + 127) / 128)), b(128);
+#define TFM_COO(C, D)                                                                                           \
+  if (m->cell == C && pl->dpn == D) {                                                                           \
+    hipLaunchKernelGGL((k_coo_pairs<C, D, WHAT>), g, b, 0, ctx->stream, npairs, m->d_coords, m->pdim, m->d_conn, \
+                       P, pl->d_fixed, cnt, off, keys, idx, v64, v32);                                          \
+    TFM_CHECK_HIP(ctx, hipGetLastError());                                                                      \
+    return TFM_OK;                                                                                              \
+  }
+  TFM_COO(TFM_TET, 1)
+  TFM_COO(TFM_TET, 3)
+  TFM_COO(TFM_QUAD, 1)
+  TFM_COO(TFM_QUAD, 2)
+  TFM_COO(TFM_HEX, 1)
+  TFM_COO(TFM_HEX, 3)
+  TFM_COO(TFM_TRI, 1)
+  TFM_COO(TFM_TRI, 2)
+  TFM_COO(TFM_LINE, 1)
+#undef TFM_COO
+  return fail(ctx, TFM_E_UNSUPPORTED, "cell / dofs-per-node combination not built");
+}
+
+int coo_plan_create(tfm_ctx *ctx, AsmPlan *pl, const OpParams &P) {
+  const tfm_mesh *m = pl->mesh;
+  hipStream_t st = ctx->stream;
+  const int64_t npairs = m->n_elem * m->nn;
+  const int nd = m->nn * pl->dpn;
+  if (m->n_elem * nd * nd >= ((int64_t)1 << 40)) return fail(ctx, TFM_E_UNSUPPORTED, "mesh too large for triplets");
+  int *cnt = nullptr, *off = nullptr, *flag = nullptr, *eid = nullptr, *rcnt = nullptr, *dims = nullptr;
+  unsigned long long *keys = nullptr, *keys2 = nullptr;
+  int64_t *idx = nullptr;
+  void *tmp = nullptr;
+  auto cleanup = [&]() {
+    dfree(cnt); dfree(off); dfree(flag); dfree(eid); dfree(rcnt); dfree(dims); dfree(keys); dfree(keys2); dfree(idx);
+    dfree(tmp);
+  };
+  int rc;
+  if ((rc = dalloc(ctx, &cnt, std::max<int64_t>(npairs, 1))) || (rc = dalloc(ctx, &off, npairs + 1))) {
+    cleanup();
+    return rc;
+  }
+  if ((rc = coo_launch<C_COUNT>(ctx, pl, P, cnt, nullptr, nullptr, nullptr, nullptr, nullptr))) {
+    cleanup();
+    return rc;
+  }
+  int64_t ntrip = 0;
+  if ((rc = exclusive_scan_i32(ctx, cnt, off, npairs, &ntrip))) {  // fails beyond 2^31 triplets
+    cleanup();
+    return rc;
+  }
+  if ((rc = dalloc(ctx, &keys, std::max<int64_t>(ntrip, 1))) || (rc = dalloc(ctx, &keys2, std::max<int64_t>(ntrip, 1))) ||
+      (rc = dalloc(ctx, &idx, std::max<int64_t>(ntrip, 1))) ||
+      (rc = dalloc(ctx, &pl->d_coo_perm, std::max<int64_t>(ntrip, 1)))) {
+    cleanup();
+    return rc;
+  }
+  if ((rc = coo_launch<C_FILL>(ctx, pl, P, nullptr, off, keys, idx, nullptr, nullptr))) {
+    cleanup();
+    return rc;
+  }
+  // stable LSD radix sort of (key, element-major position): equal keys keep
+  // their insertion order
+  if (ntrip > 0) {
+    size_t bytes = 0;
+    TFM_CHECK_HIP(ctx, rocprim::radix_sort_pairs(nullptr, bytes, keys, keys2, idx, pl->d_coo_perm, (size_t)ntrip, 0,
+                                                 64, st));
+    if ((rc = dalloc(ctx, (char **)&tmp, bytes))) {
+      cleanup();
+      return rc;
+    }
+    TFM_CHECK_HIP(ctx, rocprim::radix_sort_pairs(tmp, bytes, keys, keys2, idx, pl->d_coo_perm, (size_t)ntrip, 0, 64,
+                                                 st));
+  }
+  if ((rc = dalloc(ctx, &flag, std::max<int64_t>(ntrip, 1))) || (rc = dalloc(ctx, &eid, ntrip + 1))) {
+    cleanup();
+    return rc;
+  }
+  const unsigned gk = (unsigned)((ntrip + 255) / 256);
+  if (ntrip > 0) hipLaunchKernelGGL(k_coo_heads, dim3(gk), dim3(256), 0, st, ntrip, keys2, flag);
+  int64_t nnz = 0;
+  if ((rc = exclusive_scan_i32(ctx, flag, eid, ntrip, &nnz))) {
+    cleanup();
+    return rc;
+  }
+  if ((rc = dalloc(ctx, &pl->d_ci, std::max<int64_t>(nnz, 1))) || (rc = dalloc(ctx, &pl->d_coo_seg, nnz + 1)) ||
+      (rc = dalloc(ctx, &rcnt, pl->ndof + 1)) || (rc = dalloc(ctx, &dims, 2)) ||
+      (rc = dalloc(ctx, &pl->d_rp, pl->ndof + 1))) {
+    cleanup();
+    return rc;
+  }
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(rcnt, 0, sizeof(int) * (pl->ndof + 1), st));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(dims, 0, sizeof(int) * 2, st));
+  if (ntrip > 0)
+    hipLaunchKernelGGL(k_coo_entries, dim3(gk), dim3(256), 0, st, ntrip, keys2, flag, eid, pl->d_ci, pl->d_coo_seg,
+                       rcnt, dims);
+  const int segend = (int)ntrip;
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(pl->d_coo_seg + nnz, &segend, sizeof(int), hipMemcpyHostToDevice, st));
+  int64_t nnz2 = 0;
+  if ((rc = exclusive_scan_i32(ctx, rcnt, pl->d_rp, pl->ndof, &nnz2))) {
+    cleanup();
+    return rc;
+  }
+  int hd[2];
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(hd, dims, sizeof hd, hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  cleanup();
+  pl->use_coo = true;
+  pl->coo_ntrip = ntrip;
+  pl->nnz = nnz;
+  // Coo::to_csr: nrows = max row + 1, ncols = max col + 1, and an empty
+  // triplet list still yields a 1 x 1 shape.
+  pl->nrows = nnz ? hd[0] : 1;
+  pl->ncols = nnz ? hd[1] : 1;
+  return TFM_OK;
+}
+
+int coo_numeric(tfm_ctx *ctx, const AsmPlan *pl, const OpParams &P, double *v64, float *v32) {
+  const tfm_mesh *m = pl->mesh;
+  const int nd = m->nn * pl->dpn;
+  const int64_t dense = m->n_elem * nd * nd;
+  hipStream_t st = ctx->stream;
+  int rc;
+  const unsigned g = (unsigned)((pl->nnz + 255) / 256);
+  if (v64) {
+    if (!pl->d_coo_v64 && (rc = dalloc(ctx, &pl->d_coo_v64, std::max<int64_t>(dense, 1)))) return rc;
+    if ((rc = coo_launch<C_F64>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, pl->d_coo_v64, nullptr))) return rc;
+    if (pl->nnz)
+      hipLaunchKernelGGL(k_coo_sum<double>, dim3(g), dim3(256), 0, st, pl->nnz, pl->d_coo_seg, pl->d_coo_perm,
+                         pl->d_coo_v64, v64);
+  }
+  if (v32) {
+    if (!pl->d_coo_v32 && (rc = dalloc(ctx, &pl->d_coo_v32, std::max<int64_t>(dense, 1)))) return rc;
+    if ((rc = coo_launch<C_F32>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, nullptr, pl->d_coo_v32))) return rc;
+    if (pl->nnz)
+      hipLaunchKernelGGL(k_coo_sum<float>, dim3(g), dim3(256), 0, st, pl->nnz, pl->d_coo_seg, pl->d_coo_perm,
+                         pl->d_coo_v32, v32);
+  }
+  TFM_CHECK_HIP(ctx, hipGetLastError());
+  return TFM_OK;
+}
+
+void coo_plan_destroy(AsmPlan *pl) {
+  dfree(pl->d_coo_perm);
+  dfree(pl->d_coo_seg);
+  dfree(pl->d_coo_v64);
+  dfree(pl->d_coo_v32);
+  pl->use_coo = false;
+}
+
+}  // namespace tfm

@@ -528,8 +528,6 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
   int rc;
   if ((rc = validate_operator(ctx, mesh, op, dpn))) return rc;
   if ((rc = ensure_adjacency(const_cast<tfm_mesh *>(mesh)))) return rc;
-  if (mesh->max_adj > 64)
-    return fail(ctx, TFM_E_UNSUPPORTED, "a node belongs to more than 64 elements");
   pl->mesh = mesh;
   pl->dpn = dpn;
   pl->ndof = mesh->n_nodes * dpn;
@@ -547,8 +545,15 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
     if ((rc = dalloc(ctx, &pl->d_fixed, pl->ndof))) return rc;
     TFM_CHECK_HIP(ctx, hipMemcpyAsync(pl->d_fixed, h_fixed, pl->ndof, hipMemcpyHostToDevice, ctx->stream));
   }
-  // Tile schedule: consecutive nodes, <= TP pairs and <= kTileNodes nodes.
+  // A node in more elements than a tile holds (or than its pair mask has
+  // bits): the sorted-triplet path assembles the whole mesh.
   int TP = tile_pairs_for(mesh->cell, dpn);
+  if (mesh->max_adj > 64 || mesh->max_adj > TP || getenv("TFM_COO_ASSEMBLY")) {
+    if ((rc = upload_tables(ctx, pl, &pl->d_tab, &pl->nq))) return rc;
+    OpParams P = make_params(pl, pl->d_tab, pl->nq, pl->d_tab + (size_t)pl->nq * kRuleStride);
+    return coo_plan_create(ctx, pl, P);
+  }
+  // Tile schedule: consecutive nodes, <= TP pairs and <= kTileNodes nodes.
   pl->tile_pairs = TP;
   std::vector<int> tiles;
   tiles.push_back(0);
@@ -563,7 +568,6 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
     }
   }
   if (mesh->n_nodes > 0) tiles.push_back((int)mesh->n_nodes);
-  if (mesh->max_adj > TP) return fail(ctx, TFM_E_UNSUPPORTED, "node adjacency exceeds the tile size");
   pl->ntiles = (int)tiles.size() - 1;
   if ((rc = dalloc(ctx, &pl->d_tiles, tiles.size()))) return rc;
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(pl->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice,
@@ -595,7 +599,13 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
   dfree(cnt);
   dfree(dims);
   dfree(err);
-  if (herr) return fail(ctx, TFM_E_UNSUPPORTED, "a node has more neighbours than the tile hash holds");
+  if (herr) {
+    // more distinct neighbours than a node's hash holds: sorted triplets
+    dfree(pl->d_rp);
+    dfree(pl->d_ci);
+    pl->d_rp = pl->d_ci = nullptr;
+    return coo_plan_create(ctx, pl, P);
+  }
   // Coo::to_csr: nrows = max row + 1, ncols = max col + 1, and an empty
   // triplet list still yields a 1 x 1 shape.
   pl->nrows = nnz ? hd[0] : 1;
@@ -620,6 +630,7 @@ int plan_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *d_v64, float *d_v32) {
   if (!pl->d_tab) return fail(ctx, TFM_E_INVALID, "assembly plan has no tables");
   OpParams P = make_params(pl, pl->d_tab, pl->nq, pl->d_tab + (size_t)pl->nq * kRuleStride);
   int rc;
+  if (pl->use_coo) return coo_numeric(ctx, pl, P, d_v64, d_v32);
   if (pl->ntiles == 0) return TFM_OK;
   if (pl->use_tet) return tet_numeric(ctx, pl->mesh, pl->dpn, pl->tet, pl->d_rp, P, d_v64, d_v32);
   if (d_v64 && (rc = dispatch<M_F64>(ctx, pl, P, nullptr, nullptr, nullptr, d_v64, nullptr, nullptr))) return rc;
@@ -642,6 +653,7 @@ int plan_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *d_v64, float *d_v32) {
 
 void plan_destroy(AsmPlan *pl) {
   tet_plan_destroy(&pl->tet);
+  coo_plan_destroy(pl);
   dfree(pl->d_ke32);
   dfree(pl->d_tab);
   dfree(pl->d_coef_e);

@@ -172,6 +172,16 @@ struct AsmPlan {
   // FP32 Hex8 elasticity element matrices ([n_elem][24][24], owned), made by
   // the first FP32 numeric pass that forms them on the matrix cores
   mutable float *d_ke32 = nullptr;
+  // sorted-triplet assembly (tfm_asm_coo.hip), for node valences beyond the
+  // tile kernels: per CSR entry its first triplet (d_coo_seg, nnz + 1), the
+  // triplets' element-major positions in key order (d_coo_perm), and the
+  // dense element-value tables of the numeric passes (made on first use)
+  bool use_coo = false;
+  int64_t coo_ntrip = 0;
+  int64_t *d_coo_perm = nullptr;
+  int *d_coo_seg = nullptr;
+  mutable double *d_coo_v64 = nullptr;
+  mutable float *d_coo_v32 = nullptr;
 };
 struct OpParams;
 int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci, int tile_pairs,
@@ -190,6 +200,9 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
                 const unsigned char *h_fixed, int vals, AsmPlan *plan);
 int plan_set_coefficients(tfm_ctx *ctx, AsmPlan *plan, const tfm_operator *op);
 int plan_numeric(tfm_ctx *ctx, const AsmPlan *plan, double *d_v64, float *d_v32);
+int coo_plan_create(tfm_ctx *ctx, AsmPlan *plan, const OpParams &P);
+int coo_numeric(tfm_ctx *ctx, const AsmPlan *plan, const OpParams &P, double *v64, float *v32);
+void coo_plan_destroy(AsmPlan *plan);
 // FP32 element matrices of 3-D isotropic elasticity on Hex8 cells by MFMA
 // (tfm_hex_mfma.hip); TFM_NO_MFMA in the environment switches them off.
 bool hex_mfma_eligible(const AsmPlan *plan);
