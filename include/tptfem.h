@@ -26,7 +26,7 @@ extern "C" {
 #define TFM_E_NOT_CONVERGED (-3) /* PCG hit max_iter (no reference counterpart) */
 #define TFM_E_BREAKDOWN (-4)     /* PCG p'Ap <= 0: operator not SPD           */
 #define TFM_E_HIP (-5)           /* HIP runtime error                          */
-#define TFM_E_UNSUPPORTED (-6)   /* cell/operator combination not built        */
+#define TFM_E_UNSUPPORTED (-6)   /* input outside what is built: keep the CPU path */
 #define TFM_E_MODEL (-7)         /* ElasticityError::ModelDimMismatch (elasticity/src/lib.rs:28-35) */
 #define TFM_E_CFL (-8)           /* DynamicError::CflViolation (dynamic/src/lib.rs:46, 58-61) */
 
