@@ -162,8 +162,14 @@ def assembly_record(s, mesh, ms, per_elem_coef, cell_dpn):
         b = asm_bytes(mesh.n_nodes, mesh.n_elem, s.nnz, s.n_free, sv, per_elem_coef)
         gbs = b / (ms[name] * 1e-3) / 1e9
         vt = "double" if sv == 8 else "float"
-        kern = (asm_kernel(info["tile_pairs"], vt, cell_dpn[1]) if info["kernel"] == "p1_tet"
-                else f"tfm::k_asm_tile<{cell_dpn[0]}, {cell_dpn[1]}, {2 if sv == 8 else 3}>")
+        if info["kernel"] == "p1_tet":
+            kern = asm_kernel(info["tile_pairs"], vt, cell_dpn[1])
+        elif info["kernel"] == "simp_grid":
+            kern = f"tfm::k_simp_rows<{vt}, false>"
+        elif info["kernel"] == "triplets":
+            kern = f"tfm::k_coo_sum<{vt}>"
+        else:
+            kern = f"tfm::k_asm_tile<{cell_dpn[0]}, {cell_dpn[1]}, {2 if sv == 8 else 3}>"
         rec[name + "_roofline"] = {"kernel": kern, "algorithmic_bytes": b, "GBps": gbs, "frac": gbs / PEAK_HBM_GBS}
     return rec
 

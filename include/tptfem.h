@@ -212,7 +212,8 @@ int tfm_system_info(const tfm_system *sys, int64_t *n_free, int64_t *nnz, int64_
 /* No counterpart in the reference (diagnostic): which numeric assembly
  * kernel the system refills with and how its launch shape was chosen.
  * path: 0 = generic tile kernel, 1 = P1-tet kernel, 2 = sorted triplets (node
- * valence beyond the tile kernels); tile_pairs: element-node
+ * valence beyond the tile kernels), 3 = SIMP-grid rows (tfm_mesh_grid_batch
+ * meshes); tile_pairs: element-node
  * pairs (= threads) per P1-tet tile (192 or 256); source: 1 = timed at
  * creation, 2 = reused from an earlier system of the same dofs per node,
  * value arrays and size class (floor(log2(elements))), 3 = forced by

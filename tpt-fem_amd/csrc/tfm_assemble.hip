@@ -523,6 +523,149 @@ static int upload_tables(tfm_ctx *ctx, AsmPlan *pl, double **d_tab, int *nq) {
   return TFM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// SIMP grids (tfm_mesh_grid_batch: P copies of an nx x ny node grid, Q4
+// element (ei, ej) of design p = p (nx-1)(ny-1) + ej (nx-1) + ei with nodes
+// (ei, ej), (ei+1, ej), (ei+1, ej+1), (ei, ej+1)): every element matrix is
+// E(rho_e) Ke_unit (crates/tpt-fem-topopt/src/lib.rs:224, 233-235), so a row
+// of the stiffness is formed directly from the four elements around its
+// node, in ascending element order, each column's contributions summed from
+// +0.0 as the tile kernel sums them -- the same values, bit for bit, at a
+// fraction of the work.
+// ---------------------------------------------------------------------------
+__global__ void k_simp_erho(int64_t ne, const double *__restrict__ rho, double emin, double e0, double penal,
+                            double *__restrict__ er) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e < ne) er[e] = emin + pow(rho[e], penal) * (e0 - emin);
+}
+
+// One thread per DOF row (node n, component a), 256 rows per block: each
+// row's entries (at most 9 neighbours x 2) go to LDS at their CSR offsets,
+// then the block's contiguous CSR range is written with coalesced stores.
+// With CHECK, only compares each row's entry count with the pattern's.
+constexpr int kSimpRows = 256;
+template <typename VT, bool CHECK>
+__global__ __launch_bounds__(kSimpRows) void k_simp_rows(int64_t nrows, int nx, int ny, const double *__restrict__ er,
+                                                          const double *__restrict__ ke0,
+                                                          const unsigned char *__restrict__ fixed,
+                                                          const int *__restrict__ rp, VT *__restrict__ vals,
+                                                          int *__restrict__ bad) {
+  __shared__ VT s_v[CHECK ? 1 : kSimpRows * 18];
+  const int64_t r0 = (int64_t)blockIdx.x * kSimpRows;
+  const int64_t r = r0 + threadIdx.x;
+  const int64_t r1 = r0 + kSimpRows < nrows ? r0 + kSimpRows : nrows;
+  const int base = rp[r0];
+  if (r < nrows && !(fixed && fixed[r])) {
+    const int a = (int)(r & 1);
+    const int64_t n = r >> 1;
+    const int64_t per = (int64_t)nx * ny;
+    const int64_t p = n / per;
+    const int rem = (int)(n - p * per), j = rem / nx, i = rem - j * nx;
+    const int64_t ebase = p * (int64_t)(nx - 1) * (ny - 1);
+    // the node's elements in ascending id: (i-1, j-1), (i, j-1), (i-1, j), (i, j)
+    double E[4];
+    bool has[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int ei = i - 1 + (k & 1), ej = j - 1 + (k >> 1);
+      has[k] = ei >= 0 && ei < nx - 1 && ej >= 0 && ej < ny - 1;
+      E[k] = has[k] ? er[ebase + (int64_t)ej * (nx - 1) + ei] : 0.0;
+    }
+    int q = rp[r];
+    for (int dj = -1; dj <= 1; dj++)
+      for (int di = -1; di <= 1; di++) {
+        const int mi = i + di, mj = j + dj;
+        if (mi < 0 || mi >= nx || mj < 0 || mj >= ny) continue;
+        bool share = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int ei = i - 1 + (k & 1), ej = j - 1 + (k >> 1);
+          share |= has[k] && mi >= ei && mi <= ei + 1 && mj >= ej && mj <= ej + 1;
+        }
+        if (!share) continue;
+        const int64_t m = p * per + (int64_t)mj * nx + mi;
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          if (fixed && fixed[m * 2 + b]) continue;
+          if (!CHECK) {
+            VT acc = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              const int ei = i - 1 + (k & 1), ej = j - 1 + (k >> 1);
+              if (!has[k] || mi < ei || mi > ei + 1 || mj < ej || mj > ej + 1) continue;
+              // local index of (x, y) in element (ei, ej): (0,0) 0, (1,0) 1, (1,1) 2, (0,1) 3
+              const int ln = (j - ej) ? ((i - ei) ? 2 : 3) : (i - ei);
+              const int lm = (mj - ej) ? ((mi - ei) ? 2 : 3) : (mi - ei);
+              VT v;
+              if constexpr (sizeof(VT) == 8) {
+                v = E[k] * ke0[(ln * 2 + a) * 8 + lm * 2 + b];
+              } else {
+                v = (VT)(E[k] * ke0[(ln * 2 + a) * 8 + lm * 2 + b]);  // the FP64 product rounded
+              }
+              if (v != (VT)0) acc += v;
+            }
+            s_v[q - base] = acc;
+          }
+          q++;
+        }
+      }
+    if (CHECK && q != rp[r + 1]) atomicExch(bad, 1);
+  } else if (CHECK && r < nrows && rp[r + 1] != rp[r]) {
+    atomicExch(bad, 1);
+  }
+  if constexpr (!CHECK) {
+    __syncthreads();
+    const int len = rp[r1] - base;
+    for (int k = threadIdx.x; k < len; k += kSimpRows) vals[base + k] = s_v[k];
+  }
+}
+
+bool simp_grid_eligible(const AsmPlan *pl) {
+  const tfm_mesh *m = pl->mesh;
+  return pl->op.kind == TFM_OP_SIMP && pl->dpn == 2 && m->cell == TFM_QUAD && m->grid_P > 0 && m->grid_nx >= 2 &&
+         m->grid_ny >= 2 && m->n_nodes == m->grid_P * m->grid_nx * m->grid_ny &&
+         m->n_elem == m->grid_P * (m->grid_nx - 1) * (m->grid_ny - 1) && pl->d_rho && !getenv("TFM_GENERIC_ASSEMBLY");
+}
+
+static int simp_grid_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *v64, float *v32) {
+  const tfm_mesh *m = pl->mesh;
+  hipStream_t st = ctx->stream;
+  int rc;
+  if (!pl->d_erho && (rc = dalloc(ctx, &pl->d_erho, m->n_elem))) return rc;
+  hipLaunchKernelGGL(k_simp_erho, dim3((unsigned)((m->n_elem + 255) / 256)), dim3(256), 0, st, m->n_elem, pl->d_rho,
+                     pl->op.emin, pl->op.e0, pl->op.penal, pl->d_erho);
+  const int64_t nrows = pl->ndof;
+  const dim3 g((unsigned)((nrows + kSimpRows - 1) / kSimpRows)), b(kSimpRows);
+  const double *ke0 = pl->d_tab + (size_t)pl->nq * kRuleStride;
+  if (v64)
+    hipLaunchKernelGGL((k_simp_rows<double, false>), g, b, 0, st, nrows, (int)m->grid_nx, (int)m->grid_ny,
+                       pl->d_erho, ke0, pl->d_fixed, pl->d_rp, v64, nullptr);
+  if (v32)
+    hipLaunchKernelGGL((k_simp_rows<float, false>), g, b, 0, st, nrows, (int)m->grid_nx, (int)m->grid_ny,
+                       pl->d_erho, ke0, pl->d_fixed, pl->d_rp, v32, nullptr);
+  TFM_CHECK_HIP(ctx, hipGetLastError());
+  return TFM_OK;
+}
+
+// Once per plan: the direct row kernel applies when it reproduces the
+// pattern's row lengths exactly.
+static int simp_grid_check(tfm_ctx *ctx, AsmPlan *pl) {
+  const tfm_mesh *m = pl->mesh;
+  int *bad = nullptr, hb = 0, rc;
+  if ((rc = dalloc(ctx, &bad, 1))) return rc;
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
+  const int64_t nrows = pl->ndof;
+  hipLaunchKernelGGL((k_simp_rows<double, true>), dim3((unsigned)((nrows + kSimpRows - 1) / kSimpRows)),
+                     dim3(kSimpRows), 0, ctx->stream,
+                     nrows, (int)m->grid_nx, (int)m->grid_ny, pl->d_rho, pl->d_rho, pl->d_fixed, pl->d_rp,
+                     (double *)nullptr, bad);
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dfree(bad);
+  pl->use_simp_grid = hb == 0;
+  return TFM_OK;
+}
+
 int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
                 const unsigned char *h_fixed, int vals, AsmPlan *pl) {
   int rc;
@@ -610,6 +753,7 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
   // triplet list still yields a 1 x 1 shape.
   pl->nrows = nnz ? hd[0] : 1;
   pl->ncols = nnz ? hd[1] : 1;
+  if (simp_grid_eligible(pl) && (rc = simp_grid_check(ctx, pl))) return rc;
   // P1-tet Poisson / 3-D elasticity: the balanced fast path, when every node
   // fits its tile limits (otherwise the generic kernel keeps the job).
   const bool tet_op = mesh->cell == TFM_TET && mesh->n_elem > 0 && ((op->kind == TFM_OP_POISSON && dpn == 1) ||
@@ -631,6 +775,7 @@ int plan_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *d_v64, float *d_v32) {
   OpParams P = make_params(pl, pl->d_tab, pl->nq, pl->d_tab + (size_t)pl->nq * kRuleStride);
   int rc;
   if (pl->use_coo) return coo_numeric(ctx, pl, P, d_v64, d_v32);
+  if (pl->use_simp_grid) return simp_grid_numeric(ctx, pl, d_v64, d_v32);
   if (pl->ntiles == 0) return TFM_OK;
   if (pl->use_tet) return tet_numeric(ctx, pl->mesh, pl->dpn, pl->tet, pl->d_rp, P, d_v64, d_v32);
   if (d_v64 && (rc = dispatch<M_F64>(ctx, pl, P, nullptr, nullptr, nullptr, d_v64, nullptr, nullptr))) return rc;
@@ -653,6 +798,7 @@ int plan_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *d_v64, float *d_v32) {
 
 void plan_destroy(AsmPlan *pl) {
   tet_plan_destroy(&pl->tet);
+  dfree(pl->d_erho);
   coo_plan_destroy(pl);
   dfree(pl->d_ke32);
   dfree(pl->d_tab);

@@ -176,6 +176,9 @@ struct AsmPlan {
   // tile kernels: per CSR entry its first triplet (d_coo_seg, nnz + 1), the
   // triplets' element-major positions in key order (d_coo_perm), and the
   // dense element-value tables of the numeric passes (made on first use)
+  // SIMP grids: rows formed directly (tfm_assemble.hip), E(rho_e) per element
+  bool use_simp_grid = false;
+  mutable double *d_erho = nullptr;
   bool use_coo = false;
   int64_t coo_ntrip = 0;
   int64_t *d_coo_perm = nullptr;

@@ -1219,7 +1219,7 @@ extern "C" int tfm_debug_mf_apply(tfm_system *s, const double *x, double *y) {
 extern "C" int tfm_system_assembly_info(const tfm_system *s, int *path, int *tile_pairs, int *source, double *ms) {
   if (!s) return TFM_E_INVALID;
   const bool tet = s->has_plan && s->plan.use_tet;
-  if (path) *path = tet ? 1 : (s->has_plan && s->plan.use_coo ? 2 : 0);
+  if (path) *path = tet ? 1 : !s->has_plan ? 0 : s->plan.use_coo ? 2 : s->plan.use_simp_grid ? 3 : 0;
   if (tile_pairs) *tile_pairs = tet ? s->plan.tet.tp : 0;
   if (source) *source = tet ? s->plan.tune.source : 0;
   if (ms) {

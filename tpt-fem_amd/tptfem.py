@@ -512,7 +512,7 @@ class System:
         path, tp, src = C.c_int(), C.c_int(), C.c_int()
         ms = (C.c_double * 2)()
         self.ctx.check(_lib.tfm_system_assembly_info(self.h, C.byref(path), C.byref(tp), C.byref(src), ms))
-        return {"kernel": ("generic", "p1_tet", "triplets")[path.value], "tile_pairs": tp.value,
+        return {"kernel": ("generic", "p1_tet", "triplets", "simp_grid")[path.value], "tile_pairs": tp.value,
                 "source": ("fixed", "timed", "cached", "env")[src.value], "ms": {"256": ms[0], "192": ms[1]}}
 
     @staticmethod
