@@ -32,9 +32,11 @@ def test_scan_total_guard(ctx):
 
 def test_stale_fp32_values_are_not_used(ctx):
     """Create with FP64 and FP32 values, re-assemble the FP64 values alone with
-    new moduli: the precision form refuses the stale FP32 copy, and the
-    multigrid CG (whose fine passes would read it) solves the new system --
-    against the oracle's dense LU of the new coefficients."""
+    new moduli: the precision form on the assembled FP32 values refuses the
+    stale copy, the matrix-free precision form (which reads the current
+    coefficients) and the multigrid CG (whose fine passes would read the
+    copy) solve the new system -- against the oracle's dense LU of the new
+    coefficients."""
     import tptfem as T
 
     n = (6, 5, 6)
@@ -43,15 +45,17 @@ def test_stale_fp32_values_are_not_used(ctx):
     coef2 = 4.0 * 10.0 ** np.random.default_rng(3).uniform(-1, 1, len(conn))
     s.assemble(T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=coef2), which=T.SYS_FP64)
     with pytest.raises(T.TfmError) as ei:
-        s.solve(T.SOLVE_MIXED, rtol=1e-10)
+        s.solve(T.SOLVE_MIXED_CSR, rtol=1e-10)
     assert ei.value.code == T.E_INVALID
     u = s.solve(T.SOLVE_PCG64_MG, rtol=1e-12, max_iter=2000)
     ur = O.solve_with_dirichlet(O.assemble(coords, conn, O.make_op(O.TET, 3, 1, coef_e=coef2, nu=0.3), 3), rhs, bcs)
     assert np.linalg.norm(u - ur) <= 1e-9 * np.linalg.norm(ur)
     assert np.linalg.norm(u - u0) > 0.1 * np.linalg.norm(ur)  # the solution did change
+    umf = s.solve(T.SOLVE_MIXED, rtol=1e-10, max_iter=20000)
+    assert np.linalg.norm(umf - ur) <= 1e-8 * np.linalg.norm(ur)
     # an FP32 pass with the resident coefficients makes both forms available again
     s.assemble(None, which=T.SYS_FP32)
-    um = s.solve(T.SOLVE_MIXED, rtol=1e-10, max_iter=20000)
+    um = s.solve(T.SOLVE_MIXED_CSR, rtol=1e-10, max_iter=20000)
     assert np.linalg.norm(um - ur) <= 1e-8 * np.linalg.norm(ur)
     s.close()
     m.close()
