@@ -546,6 +546,9 @@ def bench_c3(args, T, log):
     if not args.skip_cpu:
         eps, ne, dt = cpu_baseline(args.cpu_n)
         cpu = {"value": eps, "unit": "elements/s", "cores": 1, "kind": "port", "host": host_cpu(),
+               "scope": "symbolic + numeric (assemble, to_csr, reduce_system) of a fresh system; `value` times "
+                        "the numeric pass on a resident pattern, assembly.end_to_end_elements_per_s the GPU's "
+                        "figure for the CPU's scope",
                "sample": f"oracle assemble+to_csr+reduce_system, {args.cpu_n}^3 jittered elasticity "
                          f"({ne} tets, {dt:.1f}s); a port of the reference's loop in C at -O3 without its "
                          "per-element heap allocations, so faster than the reference itself"}
