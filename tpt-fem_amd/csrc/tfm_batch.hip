@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <functional>
+#include <initializer_list>
 #include <type_traits>
 
 #include "tfm_bsell.h"
@@ -490,6 +492,14 @@ __global__ __launch_bounds__(kBT) void kbx_resid_bs(BView V, const double *__res
                                                     double *__restrict__ r, float *__restrict__ r32,
                                                     const int *__restrict__ keep, const int *__restrict__ sel) {
   const int b = blockIdx.x, pr = V.blk_prob[b];
+  if (sel) {
+    // a problem with no selected segment keeps its partials (and so the
+    // residual its segments were last replaced with)
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < NR; k++) any |= sel[pr * NR + k] != 0;
+    if (!any) return;
+  }
   constexpr int BB = BS * BS;
   const int lane = threadIdx.x & 63;
   const int s = V.blk_s0[b] + (threadIdx.x >> 6);
@@ -932,312 +942,183 @@ static void bb_resid(const tfm_system *s, const BBatch &B, const BView &V, hipSt
   }
 }
 
-// Batched PCG from r (x accumulates).  Segments flagged in `flags` are left
-// alone; the others stop at ||r|| <= rtol ||f|| (recurrence residual; the
-// segments' ||f||^2 must be in place) or max_iter.  Two kernels per
-// iteration, 16 iterations per graph replay.
-// A captured chunk of iterations, kept between runs of one solver: its
-// kernels' arguments change only with the stop rule.
-struct GraphCache {
+// ---------------------------------------------------------------------------
+// One batched solve as a sequence of captured graphs.
+//
+// A solve runs as four stages, each captured into a hipGraph on its first use
+// and replayed afterwards (TFM_NO_GRAPH=1 launches the same kernels
+// directly, for tracers):
+//   A  the caller's pre-stage (the SIMP loop's numeric assembly pass) and the
+//      refresh of the solver's copy: sliced values, Jacobi diagonals,
+//      multigrid levels;
+//   B  the initial FP64 residual and the CG state;
+//   K  a chunk of iterations -- plain/multigrid: 16 (8) PCG iterations;
+//      precision form: 16 FP32 iterations, then the refinement step, decided
+//      per segment on the device (fold the FP32 correction, FP64 residual,
+//      replace the FP32 residual, restart or stop);
+//   C  the FP64 residual certificate of every segment and the caller's
+//      post-stage (the SIMP loop's sensitivities, filter and OC update).
+// The host launches A, B, then K until every segment is done -- it reads a
+// small pinned snapshot per chunk, one chunk behind, so the check overlaps
+// the next chunk -- then C, and reads the segments' state once at the end.
+// ---------------------------------------------------------------------------
+struct Stage {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int max_it = 0;
-  double rtol2 = 0.0;
+  uint64_t key = 0;
   void clear() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
+    key = 0;
   }
 };
 
-template <typename VT, typename XT, int NR>
-static int bpcg(tfm_ctx *ctx, const BBatch &B, const VT *bv, VT *r, VT *z, VT *p0, VT *p1, VT *q, const VT *dinv,
-                XT *x, double rtol, int max_iter, const int *flags, std::vector<Seg> &hs, const MgData *mg = nullptr,
-                GraphCache *gc = nullptr, const Mg3Data *mg3 = nullptr) {
-  hipStream_t st = ctx->stream;
-  const BView V = B.view(max_iter, rtol * rtol);
-  const int nseg = B.P * B.NR;
-  dim3 g(B.nblk), t(kBT);
-  hipLaunchKernelGGL(kb_reset, dim3((nseg + 255) / 256), dim3(256), 0, st, B.seg, nseg, flags, B.it, 1);
-  // multigrid (FP64 only): z = M r by a V-cycle instead of Dinv r, skipping
-  // the segments already done; the update kernels write z = w Dinv r (the
-  // V-cycle's first sweep) and the closing fine sweep replaces their r.z
-  // partials; beta has its own kernel and K p is taken matrix-free
-  static_assert(sizeof(Seg) % sizeof(int) == 0, "Seg stride in ints");
-  const int *sdone = reinterpret_cast<const int *>(reinterpret_cast<const char *>(B.seg) + offsetof(Seg, done));
-  const int sstride = (int)(sizeof(Seg) / sizeof(int));
-  const VT *jac = dinv;
-  MgOp mop{};
-  const double *mmask = nullptr;
-  if constexpr (std::is_same<VT, double>::value) {
-    if (mg) {
-      jac = mg_sweep_weights(mg);
-      mop = mg_fine_op(mg);
-      mmask = mg_fine_dinv(mg);
-    }
-    if (mg3) jac = mg3_sweep_weights(mg3);
-  }
-  // z = M r and the partial r.z into `slot`: the V-cycle up to the finest
-  // level's last sweep, which runs fused with the dot product
-  auto precond = [&](int slot) {
-    if constexpr (std::is_same<VT, double>::value) {
-      const double *yv = mg_apply(mg, r, z, st, sdone, sstride, false);
-      if (yv)
-        hipLaunchKernelGGL((kbx_smooth_rz<NR>), g, t, 0, st, V, mop, mmask, (const double *)r, yv,
-                           (double *)z, slot);
-      else
-        hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, slot);
-    }
-  };
-  // the same on a box system: level 0 on the sliced copy around mg3_cycle
-  auto precond3 = [&](int slot) {
-    if constexpr (std::is_same<VT, double>::value && NR == 1) {
-      if (mg3_depth(mg3) == 0) {
-        mg3_direct(mg3, r, z, st, sdone);
-        hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, slot);
-        return;
-      }
-      double *r0 = mg3_fine_r(mg3);
-      const double *d0 = mg3_fine_dinv(mg3);
-      const float *b32 = B.v32;
-      if (b32) {
-        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_resid<3, float>), g, t, 0, st, V, b32, r, z, r0);
-        else hipLaunchKernelGGL((kbx_mg3_resid<1, float>), g, t, 0, st, V, b32, r, z, r0);
-      } else {
-        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_resid<3, double>), g, t, 0, st, V, bv, r, z, r0);
-        else hipLaunchKernelGGL((kbx_mg3_resid<1, double>), g, t, 0, st, V, bv, r, z, r0);
-      }
-      const double *yv = mg3_cycle(mg3, z, st, sdone);
-      if (b32) {
-        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_smooth<3, float>), g, t, 0, st, V, b32, r, yv, d0, z, slot);
-        else hipLaunchKernelGGL((kbx_mg3_smooth<1, float>), g, t, 0, st, V, b32, r, yv, d0, z, slot);
-      } else {
-        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_smooth<3, double>), g, t, 0, st, V, bv, r, yv, d0, z, slot);
-        else hipLaunchKernelGGL((kbx_mg3_smooth<1, double>), g, t, 0, st, V, bv, r, yv, d0, z, slot);
-      }
-    }
-  };
-  hipLaunchKernelGGL((kbx_init<VT, NR>), g, t, 0, st, V, r, z, p0, p1, jac);
-  if (mg) precond(Q_RZ1);
-  if (mg3) precond3(Q_RZ1);
-  TFM_CHECK_HIP(ctx, hipGetLastError());
-  // iterations per replay (even: buffer parity is static); multigrid runs
-  // need few iterations, so shorter replays waste fewer past convergence
-  const int CH = (mg || mg3) ? 8 : 16;
-  VT *pb[2] = {p0, p1};
-  auto chunk = [&]() {
-    for (int k = 0; k < CH; k++) {
-      const int e = k & 1;
-      if constexpr (std::is_same<VT, double>::value) {
-        if (mg) {
-          hipLaunchKernelGGL((kbx_beta<NR>), dim3(B.P), t, 0, st, V, e);
-          hipLaunchKernelGGL((kbx_dir_p<NR>), g, t, 0, st, V, (const double *)z, (const double *)pb[1 - e], pb[e]);
-          hipLaunchKernelGGL((kbx_q_mg<NR>), g, t, 0, st, V, mop, mmask, (const double *)pb[e], q);
-        } else
-          bb_dir<VT, NR>(B, V, st, bv, z, pb[1 - e], pb[e], q, e);
-      } else {
-        bb_dir<VT, NR>(B, V, st, bv, z, pb[1 - e], pb[e], q, e);
-      }
-      hipLaunchKernelGGL((kbx_update<VT, XT, NR>), g, t, 0, st, V, x, r, z, pb[e], q, jac, e);
-      if constexpr (std::is_same<VT, double>::value) {
-        if (mg) precond(e ? Q_RZ1 : Q_RZ0);
-        if (mg3) precond3(e ? Q_RZ1 : Q_RZ0);
-      }
-    }
-  };
-  const bool use_graph = !getenv("TFM_NO_GRAPH");
-  GraphCache local;
-  GraphCache *G = gc ? gc : &local;
-  if (use_graph && (!G->exec || G->max_it != V.max_it || G->rtol2 != V.rtol2)) {
-    G->clear();
-    TFM_CHECK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    chunk();
-    TFM_CHECK_HIP(ctx, hipStreamEndCapture(st, &G->graph));
-    TFM_CHECK_HIP(ctx, hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
-    G->max_it = V.max_it;
-    G->rtol2 = V.rtol2;
-  }
-  hipGraphExec_t exec = G->exec;
-  hs.resize(nseg);
-  int rc = TFM_OK;
-  // Chunk k + 1 is queued before the host reads the segments' state after
-  // chunk k (pinned copy + event), so the host's check and the next graph
-  // launch overlap the GPU's work; a chunk past convergence only runs
-  // kernels that exit at once.
-  auto launch = [&]() -> bool {
-    if (!use_graph) {
-      chunk();
-      return true;
-    }
-    return hipGraphLaunch(exec, st) == hipSuccess;
-  };
-  auto snapshot = [&]() -> bool {
-    return hipMemcpyAsync(B.hseg, B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st) == hipSuccess &&
-           hipEventRecord(B.ev_seg, st) == hipSuccess;
-  };
-  if (!launch() || !snapshot()) rc = fail(ctx, TFM_E_HIP, "batched PCG launch failed");
-  for (int it = CH; !rc; it += CH) {
-    if (!launch()) {
-      rc = fail(ctx, TFM_E_HIP, "hipGraphLaunch failed");
-      break;
-    }
-    TFM_CHECK_HIP(ctx, hipEventSynchronize(B.ev_seg));
-    bool all = true;
-    for (int q2 = 0; q2 < nseg; q2++) all &= B.hseg[q2].done != 0;
-    if (all || it > max_iter + 2 * CH) break;  // the kernels stop every segment at max_iter
-    if (!snapshot()) rc = fail(ctx, TFM_E_HIP, "hipMemcpyAsync failed");
-  }
-  TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
-  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
-  local.clear();
-  return rc;
+static uint64_t key_of(std::initializer_list<uint64_t> parts) {
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t v : parts) h = (h ^ v) * 1099511628211ull + 0x9e3779b97f4a7c15ull;
+  return h | 1ull;  // never 0, the empty stage's key
 }
 
-// Precision form, per segment: FP32 PCG on the correction with reliable
-// updates.  Whenever a segment's recurrence residual has dropped by delta
-// since its last replacement (or stalls), its FP32 correction is folded into
-// the FP64 solution and its residual replaced by the true FP64 f - K u,
-// keeping its search direction; a segment ends when that residual meets rtol.
-template <int NR>
-static int bmixed(tfm_system *s, const BBatch &B, const BView &V, const double *f, double *x, double *r64,
-                  float *r32, float *z32, float *p32, float *p32b, float *q32, const float *d32, float *x32,
-                  int *odone, const tfm_solve_opts &o, std::vector<int> &od, std::vector<int> &iters,
-                  std::vector<int> &cycles, std::vector<int> &brk) {
-  tfm_ctx *ctx = s->ctx;
+static uint64_t bits_of(double d) {
+  uint64_t u;
+  std::memcpy(&u, &d, sizeof(u));
+  return u;
+}
+
+// Enqueue a stage: replay its graph, capturing it first when its key (the
+// stop rule and the callers' tags, which fix every kernel argument) changed.
+template <typename F>
+static int stage_run(tfm_ctx *ctx, Stage &S, uint64_t key, bool use_graph, F &&enqueue) {
   hipStream_t st = ctx->stream;
-  const int P = B.P, nseg = P * NR;
-  const size_t nv = (size_t)s->ndof * NR;
-  const double delta = (o.inner_rtol > 0.0 && o.inner_rtol < 1.0) ? o.inner_rtol : 0.3;
-  dim3 g(B.nblk), t(kBT);
-  int *sel = nullptr, *rstf = nullptr;
-  int rc;
-  if ((rc = dalloc(ctx, &sel, nseg)) || (rc = dalloc(ctx, &rstf, nseg))) {
-    dfree(sel);
-    return rc;
-  }
-  std::vector<Seg> hs(nseg);
-  std::vector<double> rr_ref(nseg), best(nseg);
-  std::vector<int> since(nseg, 0), hsel(nseg, 0), hrst(nseg, 0), nbrk(nseg, 0);
-  auto residual = [&](const int *sl) -> int {
-    bb_resid<NR>(s, B, V, st, x, f, r64, r32, odone, sl);
-    hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
-    TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
-    return TFM_OK;
-  };
-  // initial residual of every segment
-  TFM_CHECK_HIP(ctx, hipMemcpyAsync(odone, od.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-  if ((rc = residual(nullptr))) {
-    dfree(sel);
-    dfree(rstf);
-    return rc;
-  }
-  for (int q = 0; q < nseg; q++) {
-    const double rel = hs[q].ff > 0 ? std::sqrt(hs[q].rt / hs[q].ff) : 0.0;
-    if (rel <= o.rtol) od[q] = 1;
-    rr_ref[q] = best[q] = hs[q].rt;
-    cycles[q] = 1;
-  }
-  TFM_CHECK_HIP(ctx, hipMemcpyAsync(odone, od.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-  TFM_CHECK_HIP(ctx, hipMemsetAsync(x32, 0, sizeof(float) * nv, st));
-  // Replacements keeping the direction happen at chunk boundaries (the
-  // buffer and partial-slot parity is then consistent).  Inside a chunk a
-  // segment freezes only at the FP32 accuracy floor -- its residual 1e-6
-  // below the last replacement -- or on a breakdown, so a small problem never
-  // iterates on rounding noise; a frozen segment restarts its direction.
-  const BView V32 = B.view(o.max_iter, kFloor2);
-  const int nsb = (nseg + 255) / 256;
-  hipLaunchKernelGGL(kb_reset, dim3(nsb), dim3(256), 0, st, B.seg, nseg, odone, B.it, 0);
-  hipLaunchKernelGGL((kbx_init<float, NR>), g, t, 0, st, V32, r32, z32, p32, p32b, d32);
-  const int CH = 16;
-  float *pb[2] = {p32, p32b};
-  auto chunk = [&]() {
-    for (int k = 0; k < CH; k++) {
-      const int e = k & 1;
-      bb_dir<float, NR>(B, V32, st, B.v32, z32, pb[1 - e], pb[e], q32, e);
-      hipLaunchKernelGGL((kbx_update<float, float, NR>), g, t, 0, st, V32, x32, r32, z32, pb[e], q32, d32, e);
-    }
-  };
-  const bool use_graph = !getenv("TFM_NO_GRAPH");
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  if (use_graph) {
+  if (!use_graph) return enqueue();
+  if (!S.exec || S.key != key) {
+    S.clear();
     TFM_CHECK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    chunk();
-    TFM_CHECK_HIP(ctx, hipStreamEndCapture(st, &graph));
-    TFM_CHECK_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  }
-  constexpr int kStall = 64;  // chunks without a 10 % residual improvement
-  for (int it = 0;; it += CH) {
-    bool all = true;
-    for (int q = 0; q < nseg; q++) all &= od[q] != 0;
-    if (all) break;
-    if (!use_graph) {
-      chunk();
-    } else if (hipGraphLaunch(exec, st) != hipSuccess) {
-      rc = fail(ctx, TFM_E_HIP, "hipGraphLaunch failed");
-      break;
+    const int rc = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc || e != hipSuccess || !g) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return rc ? rc : fail(ctx, TFM_E_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
     }
-    const int done_it = it + CH;
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
-    TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
-    bool any = false;
-    for (int q = 0; q < nseg; q++) {
-      hsel[q] = 0;
-      if (od[q]) continue;
-      if (hs[q].rr < 0.81 * best[q]) {
-        best[q] = hs[q].rr;
-        since[q] = 0;
+    S.graph = g;
+    if (hipGraphInstantiate(&S.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+      S.clear();
+      return fail(ctx, TFM_E_HIP, "hipGraphInstantiate failed");
+    }
+    S.key = key;
+  }
+  if (hipGraphLaunch(S.exec, st) != hipSuccess) return fail(ctx, TFM_E_HIP, "hipGraphLaunch failed");
+  return TFM_OK;
+}
+
+// Clears inside captured stages are fill kernels (no memset nodes).
+template <typename T>
+__global__ void kb_fill(int64_t n, T *__restrict__ p, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+template <typename T>
+static void fill(hipStream_t st, T *p, int64_t n, T v) {
+  if (n <= 0) return;
+  const int64_t g = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(kb_fill<T>, dim3((unsigned)g), dim3(256), 0, st, n, p, v);
+}
+
+// Precision form, refinement state of one segment (device).
+struct Mix {
+  double rr_ref, best;  // FP32 residual^2 at the last replacement; best since
+  int since;            // chunks without a 10 % improvement of best
+  int od;               // refinement over (converged, max_iter or failed)
+  int cycles, iters, nbrk, brk;
+  int sel, rst;         // this chunk: replace the residual; restart the direction
+};
+constexpr int kMixCh = 16;  // FP32 iterations per chunk
+constexpr int kStall = 64;  // chunks without a 10 % residual improvement
+
+// After the first FP64 residual: segments already within rtol are over.
+__global__ void kb_mix_init(const Seg *__restrict__ seg, int nseg, Mix *__restrict__ mx, int *__restrict__ odone,
+                            int *__restrict__ mit, double rtol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *mit = 0;
+  if (i >= nseg) return;
+  const Seg g = seg[i];
+  const double rel = g.ff > 0 ? sqrt(g.rt / g.ff) : 0.0;
+  Mix m{};
+  m.od = rel <= rtol;
+  m.rr_ref = m.best = g.rt;
+  m.cycles = 1;
+  mx[i] = m;
+  odone[i] = m.od;
+}
+
+// After a chunk: which running segments replace their residual -- those
+// whose FP32 residual fell by delta since the last replacement or below the
+// target, that froze (FP32 floor or breakdown), stalled, or spent max_iter.
+// One block; advances the iteration count by a chunk.
+__global__ void kb_mix_select(const Seg *__restrict__ seg, int nseg, Mix *__restrict__ mx, int *__restrict__ sel,
+                              int *__restrict__ mit, double delta2, double rtol2, int max_iter) {
+  const int done_it = *mit + kMixCh;
+  for (int q = threadIdx.x; q < nseg; q += blockDim.x) {
+    Mix m = mx[q];
+    int s = 0;
+    if (!m.od) {
+      const Seg g = seg[q];
+      if (g.rr < 0.81 * m.best) {
+        m.best = g.rr;
+        m.since = 0;
       } else {
-        since[q]++;
+        m.since++;
       }
-      if (hs[q].rr <= delta * delta * rr_ref[q] || hs[q].rr <= o.rtol * o.rtol * hs[q].ff || hs[q].done ||
-          since[q] >= kStall || done_it >= o.max_iter) {
-        hsel[q] = 1;
-        any = true;
-      }
+      if (g.rr <= delta2 * m.rr_ref || g.rr <= rtol2 * g.ff || g.done || m.since >= kStall || done_it >= max_iter)
+        s = 1;
     }
-    if (!any) continue;
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(sel, hsel.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL((kbx_fold<NR>), g, t, 0, st, V, x, x32, sel);
-    if ((rc = residual(sel))) break;
-    for (int q = 0; q < nseg; q++) {
-      hrst[q] = 0;
-      if (!hsel[q]) continue;
-      const double rel = hs[q].ff > 0 ? std::sqrt(hs[q].rt / hs[q].ff) : 0.0;
-      cycles[q] += 1;
-      rr_ref[q] = best[q] = hs[q].rt;
-      since[q] = 0;
-      if (rel <= o.rtol || done_it >= o.max_iter) {
-        od[q] = 1;
-        iters[q] = done_it;
-        hsel[q] = 0;
-      } else if (hs[q].breakdown && ++nbrk[q] > 4) {
-        // an FP32 breakdown (p.q <= 0) that keeps coming back fails the segment
-        brk[q] = 1;
-        od[q] = 1;
-        iters[q] = done_it;
-        hsel[q] = 0;
-      } else if (hs[q].done) {
-        hrst[q] = 1;  // frozen (floor or breakdown): restart the direction
-      }
-    }
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(odone, od.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(sel, hsel.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-    TFM_CHECK_HIP(ctx, hipMemcpyAsync(rstf, hrst.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(kb_mark_done, dim3(nsb), dim3(256), 0, st, B.seg, nseg, odone);
-    hipLaunchKernelGGL((kbx_replace<NR>), g, t, 0, st, V32, r32, z32, d32, sel);
-    hipLaunchKernelGGL((kbx_resume<NR>), g, t, 0, st, V32, p32, p32b, sel, rstf);
+    m.sel = s;
+    sel[q] = s;
+    mx[q] = m;
   }
-  // segments still running (none in normal operation) count their iterations
-  for (int q = 0; q < nseg; q++)
-    if (!iters[q]) iters[q] = hs[q].iters;
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  dfree(sel);
-  dfree(rstf);
-  return rc;
+  __syncthreads();
+  if (threadIdx.x == 0) *mit = done_it;
+}
+
+// After the FP64 residual of the selected segments: stop those within rtol
+// (or out of iterations, or failing on repeated FP32 breakdowns), restart
+// the direction of frozen ones; the others resume with the new residual.
+__global__ void kb_mix_after(const Seg *__restrict__ seg, int nseg, Mix *__restrict__ mx, int *__restrict__ sel,
+                             int *__restrict__ rst, int *__restrict__ odone, const int *__restrict__ mit, double rtol,
+                             int max_iter) {
+  const int done_it = *mit;
+  for (int q = threadIdx.x; q < nseg; q += blockDim.x) {
+    Mix m = mx[q];
+    int r = 0;
+    if (m.sel) {
+      const Seg g = seg[q];
+      const double rel = g.ff > 0 ? sqrt(g.rt / g.ff) : 0.0;
+      m.cycles += 1;
+      m.rr_ref = m.best = g.rt;
+      m.since = 0;
+      if (rel <= rtol || done_it >= max_iter) {
+        m.od = 1;
+        m.iters = done_it;
+        m.sel = 0;
+      } else if (g.breakdown && ++m.nbrk > 4) {
+        // an FP32 breakdown (p.q <= 0) that keeps coming back fails the segment
+        m.brk = 1;
+        m.od = 1;
+        m.iters = done_it;
+        m.sel = 0;
+      } else if (g.done) {
+        r = 1;  // frozen (floor or breakdown): restart the direction
+      }
+    }
+    rst[q] = r;
+    sel[q] = m.sel;
+    odone[q] = m.od;
+    mx[q] = m;
+  }
 }
 
 // Host side of a batched solve: the sliced matrix, work vectors and the
@@ -1252,20 +1133,30 @@ struct BatchSolver {
   float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *d32 = nullptr,
         *x32 = nullptr;
   int *bad = nullptr, *odone = nullptr;
+  Mix *mix = nullptr;            // precision form: per-segment refinement state
+  int *mit = nullptr;            // precision form: iterations run
+  int *sel = nullptr, *rst = nullptr;
+  Mix *hmix = nullptr;           // pinned snapshot of mix (od), one chunk behind
+  hipEvent_t ev_mix = nullptr;
   MgData *mg = nullptr;    // multigrid preconditioner (TFM_SOLVE_PCG64_MG), SIMP grids
   Mg3Data *mg3 = nullptr;  // the same for one system on a box mesh
-  GraphCache graph;        // the plain CG's captured chunk
+  Stage sa, sb, sk, sc;    // the solve's captured stages
 };
 
 void bsolver_destroy(BatchSolver *b) {
   if (!b) return;
+  b->sa.clear();
+  b->sb.clear();
+  b->sk.clear();
+  b->sc.clear();
   bb_free(b->B);
   dfree(b->f); dfree(b->x); dfree(b->r64); dfree(b->d64); dfree(b->z64); dfree(b->p64); dfree(b->p64b);
   dfree(b->q64); dfree(b->r32); dfree(b->z32); dfree(b->p32); dfree(b->p32b); dfree(b->q32); dfree(b->d32);
-  dfree(b->x32); dfree(b->bad); dfree(b->odone);
+  dfree(b->x32); dfree(b->bad); dfree(b->odone); dfree(b->mix); dfree(b->mit); dfree(b->sel); dfree(b->rst);
+  if (b->hmix) (void)hipHostFree(b->hmix);
+  if (b->ev_mix) (void)hipEventDestroy(b->ev_mix);
   mg_destroy(b->mg);
   mg3_destroy(b->mg3);
-  b->graph.clear();
   delete b;
 }
 
@@ -1303,18 +1194,25 @@ int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, int mod
   b->mixed = mixed;
   const int64_t n = s->ndof;
   const size_t nv = (size_t)n * NR;
+  const int nseg = P * NR;
   int rc = bb_setup(s, P, dof_off, NR, b->B);
   if (!rc && !((rc = dalloc(ctx, &b->f, nv)) || (rc = dalloc(ctx, &b->x, nv)) || (rc = dalloc(ctx, &b->r64, nv)) ||
                (rc = dalloc(ctx, &b->d64, n)) || (rc = dalloc(ctx, &b->bad, 1)) ||
-               (rc = dalloc(ctx, &b->odone, P * NR)))) {
+               (rc = dalloc(ctx, &b->odone, nseg)))) {
     if (!mixed)
       (void)((rc = dalloc(ctx, &b->z64, nv)) || (rc = dalloc(ctx, &b->p64, nv)) ||
              (rc = dalloc(ctx, &b->p64b, nv)) || (rc = dalloc(ctx, &b->q64, nv)));
     else
       (void)((rc = dalloc(ctx, &b->r32, nv)) || (rc = dalloc(ctx, &b->z32, nv)) ||
              (rc = dalloc(ctx, &b->p32, nv)) || (rc = dalloc(ctx, &b->p32b, nv)) ||
-             (rc = dalloc(ctx, &b->q32, nv)) || (rc = dalloc(ctx, &b->d32, n)) || (rc = dalloc(ctx, &b->x32, nv)));
+             (rc = dalloc(ctx, &b->q32, nv)) || (rc = dalloc(ctx, &b->d32, n)) || (rc = dalloc(ctx, &b->x32, nv)) ||
+             (rc = dalloc(ctx, &b->mix, nseg)) || (rc = dalloc(ctx, &b->mit, 1)) || (rc = dalloc(ctx, &b->sel, nseg)) ||
+             (rc = dalloc(ctx, &b->rst, nseg)));
   }
+  if (!rc && mixed &&
+      (hipHostMalloc((void **)&b->hmix, sizeof(Mix) * nseg, hipHostMallocDefault) != hipSuccess ||
+       hipEventCreateWithFlags(&b->ev_mix, hipEventDisableTiming) != hipSuccess))
+    rc = fail(ctx, TFM_E_HIP, "pinned refinement-state buffer allocation failed");
   if (!rc && mg2d)
     rc = mg_create(ctx, P, (int)s->plan.mesh->grid_nx, (int)s->plan.mesh->grid_ny, NR, s->plan.ke0, &b->mg);
   if (!rc && mg3d) {
@@ -1359,40 +1257,75 @@ int bsolver_set_rhs(BatchSolver *b, const double *rhs) {
   return TFM_OK;
 }
 
-// Sliced copy and Jacobi of the system's current values (after a
-// re-assembly); a free DOF without a positive diagonal is a numeric error.
-int bsolver_refresh(BatchSolver *b) {
+// Refresh of the solver's copy after a (re-)assembly, in two parts: the
+// host-side checks and first-use allocations (run directly), then the
+// kernels (captured into stage A): sliced values, Jacobi diagonals (a free
+// DOF without a positive diagonal raises the `bad` flag, checked when the
+// solve ends) and the multigrid levels.
+static int refresh_prepare(BatchSolver *b) {
+  tfm_system *s = b->s;
+  tfm_ctx *ctx = s->ctx;
+  const int64_t nv = b->B.nblkst * b->B.bs * b->B.bs;
+  if (b->mixed && !s->fp32_current())
+    return fail(ctx, TFM_E_INVALID, "mixed solve: the FP32 values predate the last FP64 re-assembly");
+  int rc = b->B.v64 ? TFM_OK : dalloc(ctx, &b->B.v64, nv);
+  // The box multigrid's two fine passes may read the FP32 copy, but only
+  // while it holds the same matrix as the FP64 values (same version);
+  // otherwise they run on the FP64 copy.
+  const bool want32 = b->mixed || (b->mg3 && s->fp32_current());
+  if (!rc && want32 && !b->B.v32) rc = dalloc(ctx, &b->B.v32, nv);
+  if (!rc && !want32 && b->B.v32) dfree(b->B.v32);
+  return rc;
+}
+
+static int refresh_enqueue(BatchSolver *b) {
   tfm_system *s = b->s;
   tfm_ctx *ctx = s->ctx;
   hipStream_t st = ctx->stream;
   const int n = (int)s->ndof;
-  // FP64 sliced values serve the plain CG and every mode's FP64 residual
-  if (b->mixed && !s->fp32_current())
-    return fail(ctx, TFM_E_INVALID, "mixed solve: the FP32 values predate the last FP64 re-assembly");
-  int rc = bb_values<double>(s, b->B, s->d_v64, &b->B.v64);
-  // The box multigrid's two fine passes may read the FP32 copy, but only
-  // while it holds the same matrix as the FP64 values (same version);
-  // otherwise they run on the FP64 copy.
-  if (!rc && (b->mixed || (b->mg3 && s->fp32_current()))) rc = bb_values<float>(s, b->B, s->d_v32, &b->B.v32);
-  else if (!rc && b->mg3) dfree(b->B.v32);
-  if (rc) return rc;
-  TFM_CHECK_HIP(ctx, hipMemsetAsync(b->bad, 0, sizeof(int), st));
+  const int64_t nv = b->B.nblkst * b->B.bs * b->B.bs;
+  if (nv > 0) {
+    hipLaunchKernelGGL((k_bs_values<double, double>), dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, nv,
+                       b->B.map, (const double *)s->d_v64, b->B.v64);
+    if (b->B.v32)
+      hipLaunchKernelGGL((k_bs_values<float, float>), dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, nv,
+                         b->B.map, (const float *)s->d_v32, b->B.v32);
+  }
+  fill(st, b->bad, 1, 0);
   const unsigned g1 = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(kb_dinv<double>, dim3(g1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v64, s->d_fixed, b->d64,
-                     b->bad);
-  if (b->mixed)
-    hipLaunchKernelGGL(kb_dinv<float>, dim3(g1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v32, s->d_fixed, b->d32,
-                       b->bad);
+  if (n > 0) {
+    hipLaunchKernelGGL(kb_dinv<double>, dim3(g1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v64, s->d_fixed,
+                       b->d64, b->bad);
+    if (b->mixed)
+      hipLaunchKernelGGL(kb_dinv<float>, dim3(g1), dim3(256), 0, st, n, s->d_rp, s->d_ci, s->d_v32, s->d_fixed,
+                         b->d32, b->bad);
+  }
+  int rc;
   if (b->mg && (rc = mg_update(b->mg, s->plan.d_rho, s->plan.op.penal, s->plan.op.e0, s->plan.op.emin, s->d_fixed)))
     return rc;
   if (b->mg3 && (rc = mg3_update(b->mg3, s->d_rp, s->d_ci, s->d_v64, b->d64))) return rc;
+  TFM_CHECK_HIP(ctx, hipGetLastError());
+  return TFM_OK;
+}
+
+static int bad_check(BatchSolver *b) {
+  tfm_ctx *ctx = b->s->ctx;
   int hbad = 0;
-  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hbad, b->bad, sizeof(int), hipMemcpyDeviceToHost, st));
-  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hbad, b->bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (hbad)
     return fail(ctx, TFM_E_NUMERIC,
                 "sparse numeric factorization/solve failed: singular matrix (a free DOF has no positive diagonal)");
   return TFM_OK;
+}
+
+// The refresh on its own (run directly), for callers that solve many times
+// against one matrix (time stepping).
+int bsolver_refresh(BatchSolver *b) {
+  int rc = refresh_prepare(b);
+  if (!rc) rc = refresh_enqueue(b);
+  if (!rc) rc = bad_check(b);
+  return rc;
 }
 
 const double *bsolver_x(const BatchSolver *b) { return b->x; }
@@ -1416,56 +1349,300 @@ int bsolver_set_rhs_dev(BatchSolver *b, const double *d_rhs) {
   return TFM_OK;
 }
 
+// ---- the plain and multigrid CG (FP64) ------------------------------------
+// Segments stop at ||r|| <= rtol ||f|| (recurrence residual against the
+// segment's ||f||^2) or max_iter.  Two kernels per iteration (Jacobi);
+// multigrid: z = M r by a V-cycle instead of Dinv r, skipping the segments
+// already done; the update kernels write z = w Dinv r (the V-cycle's first
+// sweep) and the closing fine sweep replaces their r.z partials; beta has
+// its own kernel and K p is taken matrix-free (SIMP grids).
 template <int NR>
-static int bsolver_run_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_stats *stats, double *fu) {
+struct PlainCg {
+  BatchSolver *b;
+  hipStream_t st;
+  BView V;
+  dim3 g, t;
+  const double *jac;
+  MgOp mop{};
+  const double *mmask = nullptr;
+  const int *sdone;
+  int sstride;
+  PlainCg(BatchSolver *b_, const tfm_solve_opts &o)
+      : b(b_), st(b_->s->ctx->stream), V(b_->B.view(o.max_iter, o.rtol * o.rtol)), g(b_->B.nblk), t(kBT),
+        jac(b_->d64) {
+    static_assert(sizeof(Seg) % sizeof(int) == 0, "Seg stride in ints");
+    sdone = reinterpret_cast<const int *>(reinterpret_cast<const char *>(b->B.seg) + offsetof(Seg, done));
+    sstride = (int)(sizeof(Seg) / sizeof(int));
+    if (b->mg) {
+      jac = mg_sweep_weights(b->mg);
+      mop = mg_fine_op(b->mg);
+      mmask = mg_fine_dinv(b->mg);
+    }
+    if (b->mg3) jac = mg3_sweep_weights(b->mg3);
+  }
+  int chunk_len() const { return (b->mg || b->mg3) ? 8 : 16; }
+  // z = M r and the partial r.z into `slot`: the V-cycle up to the finest
+  // level's last sweep, which runs fused with the dot product
+  void precond(int slot) {
+    double *r = b->r64, *z = b->z64;
+    if (b->mg) {
+      const double *yv = mg_apply(b->mg, r, z, st, sdone, sstride, false);
+      if (yv)
+        hipLaunchKernelGGL((kbx_smooth_rz<NR>), g, t, 0, st, V, mop, mmask, (const double *)r, yv, z, slot);
+      else
+        hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, slot);
+      return;
+    }
+    if constexpr (NR == 1) {
+      // the same on a box system: level 0 on the sliced copy around mg3_cycle
+      Mg3Data *mg3 = b->mg3;
+      const BBatch &B = b->B;
+      if (mg3_depth(mg3) == 0) {
+        mg3_direct(mg3, r, z, st, sdone);
+        hipLaunchKernelGGL((kbx_rz<NR>), g, t, 0, st, V, (const double *)r, (const double *)z, slot);
+        return;
+      }
+      double *r0 = mg3_fine_r(mg3);
+      const double *d0 = mg3_fine_dinv(mg3);
+      const float *b32 = B.v32;
+      const double *bv = B.v64;
+      if (b32) {
+        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_resid<3, float>), g, t, 0, st, V, b32, r, z, r0);
+        else hipLaunchKernelGGL((kbx_mg3_resid<1, float>), g, t, 0, st, V, b32, r, z, r0);
+      } else {
+        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_resid<3, double>), g, t, 0, st, V, bv, r, z, r0);
+        else hipLaunchKernelGGL((kbx_mg3_resid<1, double>), g, t, 0, st, V, bv, r, z, r0);
+      }
+      const double *yv = mg3_cycle(mg3, z, st, sdone);
+      if (b32) {
+        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_smooth<3, float>), g, t, 0, st, V, b32, r, yv, d0, z, slot);
+        else hipLaunchKernelGGL((kbx_mg3_smooth<1, float>), g, t, 0, st, V, b32, r, yv, d0, z, slot);
+      } else {
+        if (B.bs == 3) hipLaunchKernelGGL((kbx_mg3_smooth<3, double>), g, t, 0, st, V, bv, r, yv, d0, z, slot);
+        else hipLaunchKernelGGL((kbx_mg3_smooth<1, double>), g, t, 0, st, V, bv, r, yv, d0, z, slot);
+      }
+    }
+  }
+  // stage B: r = f - K x (x = 0 or the warm start) with ||f||^2 as the stop
+  // reference, then the CG's first state
+  int init(bool warm) {
+    BatchSolver &S = *b;
+    const BBatch &B = S.B;
+    const int nseg = S.P * NR;
+    const BView V0 = B.view(V.max_it, 0.0);
+    if (!warm) fill(st, S.x, S.s->ndof * NR, 0.0);
+    fill(st, S.odone, nseg, 0);
+    bb_resid<NR>(S.s, B, V0, st, S.x, S.f, S.r64, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(S.P), t, 0, st, V0);
+    hipLaunchKernelGGL(kb_reset, dim3((nseg + 255) / 256), dim3(256), 0, st, B.seg, nseg, (const int *)nullptr, B.it,
+                       1);
+    hipLaunchKernelGGL((kbx_init<double, NR>), g, t, 0, st, V, S.r64, S.z64, S.p64, S.p64b, jac);
+    if (S.mg || S.mg3) precond(Q_RZ1);
+    TFM_CHECK_HIP(S.s->ctx, hipGetLastError());
+    return TFM_OK;
+  }
+  // stage K: chunk_len() iterations (even: the buffer parity is static)
+  int chunk() {
+    BatchSolver &S = *b;
+    const BBatch &B = S.B;
+    double *pb[2] = {S.p64, S.p64b};
+    for (int k = 0; k < chunk_len(); k++) {
+      const int e = k & 1;
+      if (S.mg) {
+        hipLaunchKernelGGL((kbx_beta<NR>), dim3(S.P), t, 0, st, V, e);
+        hipLaunchKernelGGL((kbx_dir_p<NR>), g, t, 0, st, V, (const double *)S.z64, (const double *)pb[1 - e], pb[e]);
+        hipLaunchKernelGGL((kbx_q_mg<NR>), g, t, 0, st, V, mop, mmask, (const double *)pb[e], S.q64);
+      } else {
+        bb_dir<double, NR>(B, V, st, B.v64, S.z64, pb[1 - e], pb[e], S.q64, e);
+      }
+      hipLaunchKernelGGL((kbx_update<double, double, NR>), g, t, 0, st, V, S.x, S.r64, S.z64, pb[e], S.q64, jac, e);
+      if (S.mg || S.mg3) precond(e ? Q_RZ1 : Q_RZ0);
+    }
+    TFM_CHECK_HIP(S.s->ctx, hipGetLastError());
+    return TFM_OK;
+  }
+};
+
+// ---- the precision form ----------------------------------------------------
+// FP32 PCG on the correction with reliable updates, per segment: whenever a
+// segment's recurrence residual has dropped by delta since its last
+// replacement (or stalls), its FP32 correction is folded into the FP64
+// solution and its residual replaced by the true FP64 f - K u, keeping its
+// search direction; a segment ends when that residual meets rtol.
+// Replacements happen at chunk boundaries (the buffer and partial-slot
+// parity is then consistent).  Inside a chunk a segment freezes only at the
+// FP32 accuracy floor -- its residual 1e-6 below the last replacement -- or
+// on a breakdown, so a small problem never iterates on rounding noise; a
+// frozen segment restarts its direction.
+template <int NR>
+struct MixedCg {
+  BatchSolver *b;
+  hipStream_t st;
+  BView V, V32;
+  dim3 g, t;
+  double rtol, delta;
+  int max_iter;
+  MixedCg(BatchSolver *b_, const tfm_solve_opts &o)
+      : b(b_), st(b_->s->ctx->stream), V(b_->B.view(o.max_iter, 0.0)), V32(b_->B.view(o.max_iter, kFloor2)),
+        g(b_->B.nblk), t(kBT), rtol(o.rtol),
+        delta((o.inner_rtol > 0.0 && o.inner_rtol < 1.0) ? o.inner_rtol : 0.3), max_iter(o.max_iter) {}
+  int chunk_len() const { return kMixCh; }
+  int init(bool warm) {
+    BatchSolver &S = *b;
+    const BBatch &B = S.B;
+    const int nseg = S.P * NR;
+    const int nsb = (nseg + 255) / 256;
+    if (!warm) fill(st, S.x, S.s->ndof * NR, 0.0);
+    fill(st, S.odone, nseg, 0);
+    // initial FP64 residual of every segment, rounded into the FP32 residual
+    bb_resid<NR>(S.s, B, V, st, S.x, S.f, S.r64, S.r32, S.odone, nullptr);
+    hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(S.P), t, 0, st, V);
+    hipLaunchKernelGGL(kb_mix_init, dim3(nsb), dim3(256), 0, st, (const Seg *)B.seg, nseg, S.mix, S.odone, S.mit,
+                       rtol);
+    fill(st, S.x32, S.s->ndof * NR, 0.0f);
+    hipLaunchKernelGGL(kb_reset, dim3(nsb), dim3(256), 0, st, B.seg, nseg, (const int *)S.odone, B.it, 0);
+    hipLaunchKernelGGL((kbx_init<float, NR>), g, t, 0, st, V32, S.r32, S.z32, S.p32, S.p32b, (const float *)S.d32);
+    TFM_CHECK_HIP(S.s->ctx, hipGetLastError());
+    return TFM_OK;
+  }
+  int chunk() {
+    BatchSolver &S = *b;
+    const BBatch &B = S.B;
+    const int nseg = S.P * NR;
+    const int nsb = (nseg + 255) / 256;
+    float *pb[2] = {S.p32, S.p32b};
+    for (int k = 0; k < kMixCh; k++) {
+      const int e = k & 1;
+      bb_dir<float, NR>(B, V32, st, B.v32, S.z32, pb[1 - e], pb[e], S.q32, e);
+      hipLaunchKernelGGL((kbx_update<float, float, NR>), g, t, 0, st, V32, S.x32, S.r32, S.z32, pb[e], S.q32,
+                         (const float *)S.d32, e);
+    }
+    // the refinement step; a no-op for segments not selected
+    hipLaunchKernelGGL(kb_mix_select, dim3(1), dim3(256), 0, st, (const Seg *)B.seg, nseg, S.mix, S.sel, S.mit,
+                       delta * delta, rtol * rtol, max_iter);
+    hipLaunchKernelGGL((kbx_fold<NR>), g, t, 0, st, V, S.x, S.x32, (const int *)S.sel);
+    bb_resid<NR>(S.s, B, V, st, S.x, S.f, S.r64, S.r32, S.odone, S.sel);
+    hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(S.P), t, 0, st, V);
+    hipLaunchKernelGGL(kb_mix_after, dim3(1), dim3(256), 0, st, (const Seg *)B.seg, nseg, S.mix, S.sel, S.rst,
+                       S.odone, (const int *)S.mit, rtol, max_iter);
+    hipLaunchKernelGGL(kb_mark_done, dim3(nsb), dim3(256), 0, st, B.seg, nseg, (const int *)S.odone);
+    hipLaunchKernelGGL((kbx_replace<NR>), g, t, 0, st, V32, (const float *)S.r32, S.z32, (const float *)S.d32,
+                       (const int *)S.sel);
+    hipLaunchKernelGGL((kbx_resume<NR>), g, t, 0, st, V32, S.p32, S.p32b, (const int *)S.sel, (const int *)S.rst);
+    TFM_CHECK_HIP(S.s->ctx, hipGetLastError());
+    return TFM_OK;
+  }
+};
+
+template <int NR>
+static int bsolver_solve_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, bool refresh, const SolveHooks *hk,
+                            tfm_solve_stats *stats, double *fu) {
   tfm_system *s = b->s;
   tfm_ctx *ctx = s->ctx;
   hipStream_t st = ctx->stream;
-  const BBatch &B = b->B;
+  BBatch &B = b->B;
   const int P = b->P, nseg = P * NR;
-  const size_t nv = (size_t)s->ndof * NR;
+  const bool use_graph = !getenv("TFM_NO_GRAPH");
   int rc = TFM_OK;
-  if (!warm) TFM_CHECK_HIP(ctx, hipMemsetAsync(b->x, 0, sizeof(double) * nv, st));
-  TFM_CHECK_HIP(ctx, hipMemsetAsync(b->odone, 0, sizeof(int) * nseg, st));
-  std::vector<Seg> hs(nseg);
-  std::vector<int> iters(nseg, 0), cycles(nseg, 0), brk(nseg, 0), od(nseg, 0);
-  const BView V = B.view(o.max_iter, 0.0);
-  dim3 g(B.nblk), t(kBT);
+  if (refresh && (rc = refresh_prepare(b))) return rc;
+  const uint64_t tag = hk ? hk->tag : 0;
+  const uint64_t key = key_of({(uint64_t)o.mode, (uint64_t)o.max_iter, bits_of(o.rtol), bits_of(o.inner_rtol),
+                               (uint64_t)warm, (uint64_t)refresh, tag, (uint64_t)(B.v32 != nullptr)});
+  // A: the caller's pre-stage and the refresh
+  if ((refresh || (hk && hk->pre)) && (rc = stage_run(ctx, b->sa, key, use_graph, [&]() -> int {
+        int r = TFM_OK;
+        if (hk && hk->pre) r = hk->pre();
+        if (!r && refresh) r = refresh_enqueue(b);
+        return r;
+      })))
+    return rc;
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev0, st));
-  if (!b->mixed) {
-    // r = f - K x (x = 0 or the warm start), and ||f||^2 as the stop reference
-    bb_resid<NR>(s, B, V, st, b->x, b->f, b->r64, nullptr, nullptr, nullptr);
-    hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
-    rc = bpcg<double, double, NR>(ctx, B, B.v64, b->r64, b->z64, b->p64, b->p64b, b->q64, b->d64, b->x, o.rtol,
-                                  o.max_iter, nullptr, hs, b->mg, &b->graph, b->mg3);
-    for (int q = 0; q < nseg; q++) {
-      iters[q] = hs[q].iters;
-      brk[q] = hs[q].breakdown;
-      cycles[q] = 1;
-      od[q] = hs[q].done && !hs[q].breakdown && hs[q].iters < o.max_iter;  // converged on its recurrence residual
+  std::vector<Seg> hs(nseg);
+  std::vector<Mix> hm;
+  int chunks = 0;
+  auto drive = [&](auto &cg, void *snap, size_t snap_bytes, hipEvent_t ev, const void *src, auto all_done) -> int {
+    int r = stage_run(ctx, b->sb, key, use_graph, [&]() { return cg.init(warm); });
+    if (r) return r;
+    const int CH = cg.chunk_len();
+    const int cap = o.max_iter / CH + 4;
+    auto launch = [&]() { return stage_run(ctx, b->sk, key, use_graph, [&]() { return cg.chunk(); }); };
+    auto snapshot = [&]() -> bool {
+      return hipMemcpyAsync(snap, src, snap_bytes, hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipEventRecord(ev, st) == hipSuccess;
+    };
+    // chunk k + 1 is queued before the host reads the state after chunk k;
+    // a chunk past convergence only runs kernels that exit at once
+    if ((r = launch())) return r;
+    chunks = 1;
+    if (!snapshot()) return fail(ctx, TFM_E_HIP, "batched solve: snapshot copy failed");
+    for (;;) {
+      if ((r = launch())) return r;
+      chunks++;
+      if (hipEventSynchronize(ev) != hipSuccess) return fail(ctx, TFM_E_HIP, "batched solve: event wait failed");
+      if (all_done() || chunks > cap) break;
+      if (!snapshot()) return fail(ctx, TFM_E_HIP, "batched solve: snapshot copy failed");
     }
+    return TFM_OK;
+  };
+  if (!b->mixed) {
+    PlainCg<NR> cg(b, o);
+    rc = drive(cg, B.hseg, sizeof(Seg) * nseg, B.ev_seg, B.seg, [&]() {
+      for (int q = 0; q < nseg; q++)
+        if (!B.hseg[q].done) return false;
+      return true;
+    });
   } else {
-    rc = bmixed<NR>(s, B, V, b->f, b->x, b->r64, b->r32, b->z32, b->p32, b->p32b, b->q32, b->d32, b->x32, b->odone,
-                    o, od, iters, cycles, brk);
+    MixedCg<NR> cg(b, o);
+    rc = drive(cg, b->hmix, sizeof(Mix) * nseg, b->ev_mix, b->mix, [&]() {
+      for (int q = 0; q < nseg; q++)
+        if (!b->hmix[q].od) return false;
+      return true;
+    });
   }
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, st));
-  // final FP64 residual certificate of every segment
-  bb_resid<NR>(s, B, V, st, b->x, b->f, b->r64, nullptr, nullptr, nullptr);
-  hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), t, 0, st, V);
+  // C: the FP64 residual certificate of every segment, the caller's post-stage
+  if (!rc)
+    rc = stage_run(ctx, b->sc, key, use_graph, [&]() -> int {
+      const BView V0 = B.view(o.max_iter, 0.0);
+      bb_resid<NR>(s, B, V0, st, b->x, b->f, b->r64, nullptr, nullptr, nullptr);
+      hipLaunchKernelGGL((kbx_resid_finish<NR>), dim3(P), dim3(kBT), 0, st, V0);
+      TFM_CHECK_HIP(ctx, hipGetLastError());
+      return (hk && hk->post) ? hk->post() : TFM_OK;
+    });
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    return rc;
+  }
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(hs.data(), B.seg, sizeof(Seg) * nseg, hipMemcpyDeviceToHost, st));
+  if (b->mixed) {
+    hm.resize(nseg);
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(hm.data(), b->mix, sizeof(Mix) * nseg, hipMemcpyDeviceToHost, st));
+  }
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  if (refresh && (rc = bad_check(b))) return rc;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-  if (rc) return rc;
   bool worst_ok = true;
   for (int q = 0; q < nseg; q++) {
     const double rel = hs[q].ff > 0 ? std::sqrt(hs[q].rt / hs[q].ff) : 0.0;
+    int iters, cycles, brk, od;
+    if (!b->mixed) {
+      iters = hs[q].iters;
+      brk = hs[q].breakdown;
+      cycles = 1;
+      od = hs[q].done && !hs[q].breakdown && hs[q].iters < o.max_iter;  // converged on its recurrence residual
+    } else {
+      iters = hm[q].iters ? hm[q].iters : hs[q].iters;  // segments still running count their own
+      brk = hm[q].brk;
+      cycles = hm[q].cycles;
+      od = hm[q].od;
+    }
     // plain: PCG converged on its recurrence residual (the true residual is
     // reported); mixed: the FP64 refinement residual itself met rtol
-    worst_ok &= (b->mixed ? rel <= o.rtol : od[q] != 0) && !brk[q];
+    worst_ok &= (b->mixed ? rel <= o.rtol : od != 0) && !brk;
     if (stats) {
-      stats[q].iterations = iters[q];
-      stats[q].outer = cycles[q];
+      stats[q].iterations = iters;
+      stats[q].outer = cycles;
       stats[q].rel_residual = rel;
       stats[q].seconds = ms * 1e-3;
     }
@@ -1477,8 +1654,15 @@ static int bsolver_run_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, tf
 
 // One batched solve from x = 0 (or, warm, from the last solution): per
 // segment stats and f.u (the compliance of a load case) when asked for.
+// refresh: copy the system's current values into the solver first.
+int bsolver_solve(BatchSolver *b, const tfm_solve_opts &o, bool warm, bool refresh, const SolveHooks *hooks,
+                  tfm_solve_stats *stats, double *fu) {
+  return b->NR == 1 ? bsolver_solve_nr<1>(b, o, warm, refresh, hooks, stats, fu)
+                    : bsolver_solve_nr<8>(b, o, warm, refresh, hooks, stats, fu);
+}
+
 int bsolver_run(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_stats *stats, double *fu) {
-  return b->NR == 1 ? bsolver_run_nr<1>(b, o, warm, stats, fu) : bsolver_run_nr<8>(b, o, warm, stats, fu);
+  return bsolver_solve(b, o, warm, false, nullptr, stats, fu);
 }
 
 int system_solve_mg(tfm_system *s, const tfm_solve_opts &o, double *u, tfm_solve_stats *stats) {
@@ -1489,8 +1673,8 @@ int system_solve_mg(tfm_system *s, const tfm_solve_opts &o, double *u, tfm_solve
     if ((rc = bsolver_create(s, 1, off, 1, TFM_SOLVE_PCG64_MG, &s->mg_solver))) return rc;
   }
   BatchSolver *b = s->mg_solver;
-  if ((rc = bsolver_set_rhs(b, nullptr)) || (rc = bsolver_refresh(b))) return rc;
-  rc = bsolver_run(b, o, false, stats, nullptr);
+  if ((rc = bsolver_set_rhs(b, nullptr))) return rc;
+  rc = bsolver_solve(b, o, false, true, nullptr, stats, nullptr);
   if (rc && rc != TFM_E_NOT_CONVERGED) return rc;
   // x is 0 at the fixed DOFs: u = x, then the Dirichlet values in bcs order
   const hipError_t e = hipMemcpy(u, b->x, sizeof(double) * s->ndof, hipMemcpyDeviceToHost);
@@ -1678,11 +1862,11 @@ extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *d
   // diagonal): the kernels reduce per problem and would silently mix them.
   BatchSolver *b = nullptr;
   if ((rc = bsolver_create(s, (int)P, dof_off, nrhs, o.mode, &b))) return rc;
-  if ((rc = bsolver_set_rhs(b, rhs)) || (rc = bsolver_refresh(b))) {
+  if ((rc = bsolver_set_rhs(b, rhs))) {
     bsolver_destroy(b);
     return rc;
   }
-  rc = bsolver_run(b, o, false, stats, nullptr);
+  rc = bsolver_solve(b, o, false, true, nullptr, stats, nullptr);
   if (rc && rc != TFM_E_NOT_CONVERGED) {
     bsolver_destroy(b);
     return rc;

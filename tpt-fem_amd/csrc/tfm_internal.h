@@ -15,6 +15,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -247,6 +248,20 @@ int bsolver_create(tfm_system *s, int P, const int64_t *dof_off, int NR, int mod
 int bsolver_set_rhs(BatchSolver *b, const double *rhs);
 int bsolver_refresh(BatchSolver *b);
 int bsolver_run(BatchSolver *b, const tfm_solve_opts &o, bool warm, tfm_solve_stats *stats, double *fu);
+// One solve as a sequence of captured graphs (stage A: hooks->pre, e.g. the
+// numeric assembly pass, and, with refresh, the solver's copy of the
+// system's current values; B: initial residual and CG state; K: chunks of
+// iterations until every segment is done; C: the FP64 residual certificate,
+// then hooks->post).  The hooks only enqueue work on the context's stream,
+// with arguments fixed for a given tag (graphs are re-captured when the tag
+// or the stop rule changes).  bsolver_run is bsolver_solve without refresh
+// or hooks.
+struct SolveHooks {
+  std::function<int()> pre, post;
+  uint64_t tag = 0;
+};
+int bsolver_solve(BatchSolver *b, const tfm_solve_opts &o, bool warm, bool refresh, const SolveHooks *hooks,
+                  tfm_solve_stats *stats, double *fu);
 const double *bsolver_x(const BatchSolver *b);  // [n_dof][NR], fixed DOFs 0
 double *bsolver_x_mut(BatchSolver *b);          // the same, to seed a warm start
 int bsolver_set_rhs_dev(BatchSolver *b, const double *d_rhs);  // NR = 1, device [n_dof]

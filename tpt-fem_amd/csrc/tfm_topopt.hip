@@ -224,11 +224,31 @@ extern "C" int tfm_topopt_simp_batch(tfm_ctx *ctx, int64_t P, const tfm_topopt_p
   std::vector<tfm_solve_stats> ss(P);
   if (stats)
     for (int64_t p = 0; p < P; p++) stats[p] = tfm_solve_stats{0, 0, 0.0, 0.0};
+  // the OC step: sensitivities of the last solution, filter, density update
+  auto oc_step = [&]() -> int {
+    hipLaunchKernelGGL(k_topo_sens, dim3(ge), dim3(256), 0, st, n, (const int *)mesh->d_conn, bsolver_x(bs),
+                       (const double *)d_ke0, (const double *)sys->plan.d_rho, prm->penal, prm->e0 - op.emin, d_raw);
+    hipLaunchKernelGGL(k_topo_filter, dim3(ge), dim3(256), 0, st, n, (int)ne, (int)nex, (int)ney, prm->h,
+                       prm->filter_radius, W, (const double *)d_raw, d_sens);
+    hipLaunchKernelGGL(k_topo_oc, dim3((unsigned)P), dim3(kOcThreads), 0, st, (int)ne, (const double *)d_vf,
+                       prm->move_limit, sys->plan.d_rho, (const double *)d_sens);
+    if (hipGetLastError() != hipSuccess) return fail(ctx, TFM_E_HIP, "topopt: element kernel launch failed");
+    return TFM_OK;
+  };
+  // Iterations k >= 2 run as one sequence of captured graphs: the numeric
+  // assembly with the new densities and the solver's refresh, the warm
+  // started solve, then the certificate and the OC step.
+  SolveHooks loop;
+  loop.pre = [&]() { return plan_numeric(ctx, &sys->plan, sys->d_v64, mixed ? sys->d_v32 : nullptr); };
+  loop.post = oc_step;
+  loop.tag = 0x70706f63ull;  // this loop's hooks
   for (int64_t k = 0; k < nc && !rc; k++) {
-    // iteration 1 re-solves the uniform start (:356-367): same system, same u
-    if (k >= 2 && !(rc = plan_numeric(ctx, &sys->plan, sys->d_v64, mixed ? sys->d_v32 : nullptr)))
+    if (k == 0) {
+      rc = bsolver_solve(bs, o, false, true, nullptr, ss.data(), fu.data());
+    } else if (k >= 2) {
       sys->note_values(true, true, mixed && sys->d_v32);
-    if (!rc && k != 1 && !(rc = bsolver_refresh(bs))) rc = bsolver_run(bs, o, k >= 2, ss.data(), fu.data());
+      rc = bsolver_solve(bs, o, true, true, &loop, ss.data(), fu.data());
+    }
     if (rc) break;
     for (int64_t p = 0; p < P; p++) {
       compliance[p * nc + k] = k == 1 ? compliance[p * nc] : fu[p];
@@ -239,14 +259,9 @@ extern "C" int tfm_topopt_simp_batch(tfm_ctx *ctx, int64_t P, const tfm_topopt_p
         stats[p].seconds += ss[p].seconds;
       }
     }
-    if (k == 0) continue;
-    hipLaunchKernelGGL(k_topo_sens, dim3(ge), dim3(256), 0, st, n, (const int *)mesh->d_conn, bsolver_x(bs),
-                       (const double *)d_ke0, (const double *)sys->plan.d_rho, prm->penal, prm->e0 - op.emin, d_raw);
-    hipLaunchKernelGGL(k_topo_filter, dim3(ge), dim3(256), 0, st, n, (int)ne, (int)nex, (int)ney, prm->h,
-                       prm->filter_radius, W, (const double *)d_raw, d_sens);
-    hipLaunchKernelGGL(k_topo_oc, dim3((unsigned)P), dim3(kOcThreads), 0, st, (int)ne, (const double *)d_vf,
-                       prm->move_limit, sys->plan.d_rho, (const double *)d_sens);
-    if (hipGetLastError() != hipSuccess) rc = fail(ctx, TFM_E_HIP, "topopt: element kernel launch failed");
+    // iteration 1 re-solves the uniform start (:356-367): same system, same
+    // u, then its OC step
+    if (k == 1) rc = oc_step();
   }
   if (!rc) {
     if (hipMemcpyAsync(densities, sys->plan.d_rho, sizeof(double) * P * ne, hipMemcpyDeviceToHost, st) != hipSuccess ||
