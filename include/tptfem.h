@@ -105,6 +105,9 @@ int tfm_mesh_download(const tfm_mesh *mesh, double *coords, int64_t *conn);
 /* Mesh::nodes_on_plane (mesh/src/lib.rs:425-431); out holds n_nodes entries. */
 int tfm_mesh_nodes_on_plane(const tfm_mesh *mesh, int axis, double coord, double tol,
                             int64_t *out, int64_t *count);
+/* Releases the caller's handle.  A system built on the mesh holds it too: the
+ * device memory is freed when the handle and every such system are gone, so
+ * a system may outlive the handle (as a Rust owner would drop it). */
 void tfm_mesh_destroy(tfm_mesh *mesh);
 
 /* ---- assembly (crates/tpt-fem-assembly/src/lib.rs:428-480) -------------- */

@@ -1,7 +1,9 @@
 """Guards of the library's contracts on the GPU: the int32 range guard of
 every pattern build's prefix sum, FP32 values that predate an FP64-only
-re-assembly, and re-assembly that would need pattern entries the first
-assembly skipped."""
+re-assembly, re-assembly that would need pattern entries the first
+assembly skipped, and a system that outlives its mesh's handle."""
+import gc
+
 import numpy as np
 import pytest
 
@@ -95,3 +97,42 @@ def test_reassembly_cannot_revive_zero_coefficient_elements(ctx):
         s.assemble(T.Operator.poisson(1.0, 2))
     s.close()
     m.close()
+
+
+@pytest.mark.parametrize("dpn", [1, 3])
+def test_system_outlives_its_mesh_handle(ctx, dpn):
+    """A system holds its mesh: after the caller destroys the mesh handle (and
+    new allocations take any freed memory), re-assembly with new
+    coefficients, the batched solve and the single solve still run on the
+    mesh the system was built on -- bit-identical values and the oracle's
+    dense-LU solution."""
+    import tptfem as T
+
+    n = (4, 3, 5)
+    m, s, (coords, conn, oop, rhs, bcs) = box_system(ctx, n, dpn, seed=5, flags=T.SYS_FP64 | T.SYS_FP32)
+    m.close()
+    del m
+    gc.collect()
+    # same-sized allocations that would reuse the mesh's memory had it been freed
+    others = [T.Mesh.box(ctx, [2, 2, 2], [3, 3, 3], n) for _ in range(3)]
+    coef2 = 2.0 + np.random.default_rng(8).uniform(0, 1, len(conn))
+    op2 = (T.Operator.poisson(1.0, 2, kappa_e=coef2) if dpn == 1
+           else T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=coef2))
+    oop2 = O.make_op(O.TET, 3, 0 if dpn == 1 else 1, coef_e=coef2, nu=0.3)
+    s.assemble(op2, T.SYS_FP64 | T.SYS_FP32)
+    coo = O.assemble(coords, conn, oop2, dpn)
+    kg, fg, _ = s.download()
+    kr, fr, _ = O.reduce_system(coo, rhs, bcs)
+    kr = kr.to_csr()
+    np.testing.assert_array_equal(kg.row_ptrs, kr.row_ptrs)
+    np.testing.assert_array_equal(kg.col_ind, kr.col_ind)
+    assert np.array_equal(kg.values, kr.values)
+    ur = O.solve_with_dirichlet(coo, rhs, bcs)
+    for mode in (T.SOLVE_PCG64, T.SOLVE_MIXED):
+        ub = s.solve_batch([0, s.n_dof], mode=mode, rtol=1e-11)[0]
+        assert np.linalg.norm(ub - ur) <= 1e-8 * np.linalg.norm(ur), mode
+    u = s.solve(T.SOLVE_PCG64, rtol=1e-12)
+    assert np.linalg.norm(u - ur) <= 1e-10 * np.linalg.norm(ur)
+    s.close()
+    for o in others:
+        o.close()

@@ -29,9 +29,10 @@
 // off-diagonal work sit in different wavefronts, so neither waits on the
 // other's loop.  No hashing, searching or atomics.
 //
-// Entry arithmetic is the reference's (tfm_elem.h: bdb3, and quad4 for the
-// sum of the four equal quadrature contributions); this translation unit is
-// built without FMA contraction.
+// FP64 entry arithmetic is the reference's (tfm_elem.h: bdb3, and quad4 for
+// the sum of the four equal quadrature contributions); this translation unit
+// is built without FMA contraction, which only the FP32 pass's own functions
+// switch back on.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,28 +57,12 @@ constexpr int kTetNodes = 32;     // nodes per tile
 constexpr int kTetMaxC = 6;       // contributions an off-diagonal item record lists
 constexpr int kTetStage = 512;    // staged entries of one DOF row of a tile (nodes x longest row)
 
-// Precision form: the constitutive entries per unit modulus, formed once on
-// the host in FP64 (m = (1 - nu) c, c nu, (1 - 2 nu) / 2 c with c = 1 /
-// ((1 + nu)(1 - 2 nu))) and rounded to FP32.
-struct TetF32 {
-  float m, cnu, dd;
+// Precision form: the Lame constants per unit modulus, formed once on the
+// host in FP64 (lambda = nu c, mu = (1 - 2 nu) / 2 c with c = 1 / ((1 + nu)
+// (1 - 2 nu))) and rounded to FP32.
+struct TetLame {
+  float lam, mu;
 };
-
-// FP32 entry of the precision form: vol * B_i^T D B_j (structural zeros dropped).
-__device__ __forceinline__ float bdb3f(const float *gi, const float *gj, int a, int b, float m, float cnu,
-                                       float dd) {
-  switch (a * 3 + b) {
-  case 0: return gi[0] * m * gj[0] + dd * (gi[1] * gj[1] + gi[2] * gj[2]);
-  case 1: return gi[0] * cnu * gj[1] + gi[1] * dd * gj[0];
-  case 2: return gi[0] * cnu * gj[2] + gi[2] * dd * gj[0];
-  case 3: return gi[1] * cnu * gj[0] + gi[0] * dd * gj[1];
-  case 4: return gi[1] * m * gj[1] + dd * (gi[0] * gj[0] + gi[2] * gj[2]);
-  case 5: return gi[1] * cnu * gj[2] + gi[2] * dd * gj[1];
-  case 6: return gi[2] * cnu * gj[0] + gi[0] * dd * gj[2];
-  case 7: return gi[2] * cnu * gj[1] + gi[1] * dd * gj[2];
-  default: return gi[2] * m * gj[2] + dd * (gi[1] * gj[1] + gi[0] * gj[0]);
-  }
-}
 
 // LDS slot of element-local node lm in the strip of a pair whose own node
 // is ln: the own node's block in slot 0, the other three in order.
@@ -97,127 +82,161 @@ __device__ __forceinline__ void tet_slots(const T (&g)[4][3], int ln, T (&gs)[4]
 
 // Element quantities of one (node, element) pair, kept in registers while
 // its row block is emitted one DOF row at a time; gradients in slot order.
-template <int DPN, typename VT>
-struct TetPair;
-
+//
+// Exact pass (FP64, the reference's arithmetic, bit-identical):
 template <int DPN>
-struct TetPair<DPN, double> {
+struct TetPairX {
   double g[4][3];  // physical gradients, slot order (g[0]: the pair's own node)
   double k0, k1;   // Poisson: w * coef, |det|; elasticity: w * |det|
   double m, cnu, dd;  // elasticity: the D entries (constitutive, per element)
 };
 
-template <int DPN>
-struct TetPair<DPN, float> {
-  float g[4][3];
-  float f0, m, cnu, dd;  // vol (* coef for Poisson) and the D entries
+// FP32 pass of the precision form: the isotropic element matrix in closed
+// form, vol (lambda g_i[a] g_j[b] + mu
+// g_i[b] g_j[a] + mu delta_ab g_i . g_j), which is B_i^T D B_j of the
+// reference's D for 3-D continuum, with the pair's own-node terms formed
+// once:  A = vol lambda g_0, Bm = vol mu g_0, d[s] = vol mu g_0 . g_s
+// (Poisson: d[s] = vol coef g_0 . g_s).
+template <int DPN, typename T>
+struct TetPairC {
+  T g[4][3];
+  T A[3], Bm[3], d[4];
 };
 
-// FP64: the reference's map and constitutive arithmetic (bit-identical).
-// FP32 (precision form): J and its determinant in FP64, cofactors, inverse
-// and gradients in FP32; D from host-side factors (P.coef is unused here).
 template <int DPN, typename VT>
+struct TetPairSel {
+  using type = TetPairX<DPN>;
+};
+template <int DPN>
+struct TetPairSel<DPN, float> {
+  using type = TetPairC<DPN, float>;
+};
+
+// Exact: the reference's map and constitutive arithmetic.
+template <int DPN>
 __device__ __forceinline__ void tet_setup(const double (&X)[4][3], int ln, double coef, const OpParams &P,
-                                          const float (&dfac)[3], TetPair<DPN, VT> &pr) {
-  if constexpr (sizeof(VT) == 8) {
-    double g[4][3];
-    const double det = fabs(map_tet(X, g));
-    tet_slots<double>(g, ln, pr.g);
-    const double w = 1.0 / 24.0;
-    if constexpr (DPN == 1) {
-      pr.k0 = w * coef;
-      pr.k1 = det;
-    } else {
-      const DConst D = make_d(TFM_CONTINUUM3D, coef, P.nu);
-      pr.m = D.m;
-      pr.cnu = D.cnu;
-      pr.dd = D.dd;
-      pr.k0 = w * det;
-    }
+                                          const TetLame &, TetPairX<DPN> &pr) {
+  double g[4][3];
+  const double det = fabs(map_tet(X, g));
+  tet_slots<double>(g, ln, pr.g);
+  const double w = 1.0 / 24.0;
+  if constexpr (DPN == 1) {
+    pr.k0 = w * coef;
+    pr.k1 = det;
   } else {
-    double J[9];
+    const DConst D = make_d(TFM_CONTINUUM3D, coef, P.nu);
+    pr.m = D.m;
+    pr.cnu = D.cnu;
+    pr.dd = D.dd;
+    pr.k0 = w * det;
+  }
+}
+
+// FP32: J and its determinant in FP64; the Jacobian entries rounded to FP32,
+// then cofactors, one reciprocal of the determinant, inverse and gradients
+// in FP32 (contraction on: the precision form is not bit-exact).
+template <int DPN, typename T>
+__device__ __forceinline__ void tet_setup(const double (&X)[4][3], int ln, double coef, const OpParams &,
+                                          const TetLame &lame, TetPairC<DPN, T> &pr) {
+#pragma clang fp contract(fast)
+  double J[9];
 #pragma unroll
-    for (int q = 0; q < 3; q++)
+  for (int q = 0; q < 3; q++)
 #pragma unroll
-      for (int r = 0; r < 3; r++) J[q * 3 + r] = -X[0][q] + X[r + 1][q];
-    const double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
-                       J[2] * (J[3] * J[7] - J[4] * J[6]);
-    float m[9];
+    for (int r = 0; r < 3; r++) J[q * 3 + r] = X[r + 1][q] - X[0][q];
+  const double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                     J[2] * (J[3] * J[7] - J[4] * J[6]);
+  T m[9];
 #pragma unroll
-    for (int i = 0; i < 9; i++) m[i] = (float)J[i];
-    const float iv = (float)(1.0 / det);
-    float inv[9];
-    inv[0] = (m[4] * m[8] - m[5] * m[7]) * iv;
-    inv[1] = (m[2] * m[7] - m[1] * m[8]) * iv;
-    inv[2] = (m[1] * m[5] - m[2] * m[4]) * iv;
-    inv[3] = (m[5] * m[6] - m[3] * m[8]) * iv;
-    inv[4] = (m[0] * m[8] - m[2] * m[6]) * iv;
-    inv[5] = (m[2] * m[3] - m[0] * m[5]) * iv;
-    inv[6] = (m[3] * m[7] - m[4] * m[6]) * iv;
-    inv[7] = (m[1] * m[6] - m[0] * m[7]) * iv;
-    inv[8] = (m[0] * m[4] - m[1] * m[3]) * iv;
-    float g[4][3];
+  for (int i = 0; i < 9; i++) m[i] = (T)J[i];
+  const T iv = (T)(1.0 / det);
+  T inv[9];
+  inv[0] = (m[4] * m[8] - m[5] * m[7]) * iv;
+  inv[1] = (m[2] * m[7] - m[1] * m[8]) * iv;
+  inv[2] = (m[1] * m[5] - m[2] * m[4]) * iv;
+  inv[3] = (m[5] * m[6] - m[3] * m[8]) * iv;
+  inv[4] = (m[0] * m[8] - m[2] * m[6]) * iv;
+  inv[5] = (m[2] * m[3] - m[0] * m[5]) * iv;
+  inv[6] = (m[3] * m[7] - m[4] * m[6]) * iv;
+  inv[7] = (m[1] * m[6] - m[0] * m[7]) * iv;
+  inv[8] = (m[0] * m[4] - m[1] * m[3]) * iv;
+  T g[4][3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      g[0][i] = -inv[i] - inv[3 + i] - inv[6 + i];
-      g[1][i] = inv[i];
-      g[2][i] = inv[3 + i];
-      g[3][i] = inv[6 + i];
+  for (int i = 0; i < 3; i++) {
+    g[0][i] = -inv[i] - inv[3 + i] - inv[6 + i];
+    g[1][i] = inv[i];
+    g[2][i] = inv[3 + i];
+    g[3][i] = inv[6 + i];
+  }
+  tet_slots<T>(g, ln, pr.g);
+  const T vol = (T)(fabs(det) * (1.0 / 6.0));  // the four equal-weight points
+  T dot[4];
+#pragma unroll
+  for (int sl = 0; sl < 4; sl++)
+    dot[sl] = pr.g[0][0] * pr.g[sl][0] + pr.g[0][1] * pr.g[sl][1] + pr.g[0][2] * pr.g[sl][2];
+  if constexpr (DPN == 1) {
+    const T f = vol * (T)coef;
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) pr.d[sl] = f * dot[sl];
+  } else {
+    const T ve = vol * (T)coef;
+    const T lam = ve * (T)lame.lam;
+    const T mu = ve * (T)lame.mu;
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      pr.A[p] = lam * pr.g[0][p];
+      pr.Bm[p] = mu * pr.g[0][p];
     }
-    tet_slots<float>(g, ln, pr.g);
-    const float vol = (float)(fabs(det) * (1.0 / 6.0));  // the four equal-weight points
-    if constexpr (DPN == 1) {
-      pr.f0 = vol * (float)coef;
-    } else {
-      const float e = (float)coef;
-      pr.f0 = vol;
-      pr.m = e * dfac[0];
-      pr.cnu = e * dfac[1];
-      pr.dd = e * dfac[2];
-    }
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) pr.d[sl] = mu * dot[sl];
   }
 }
 
 // Row a of the pair's row block: K_e[ln*DPN + a][lm*DPN + b] for the local
 // node lm in slot sl into ob[(sl * DPN + b) * S] (ob: the pair's cell of the
 // strip planes, S their stride).
-template <int DPN, typename VT, int S>
-__device__ __forceinline__ void tet_row(const TetPair<DPN, VT> &pr, int a, VT *ob) {
-  if constexpr (sizeof(VT) == 8) {
-    if constexpr (DPN == 1) {
+template <int DPN, int S>
+__device__ __forceinline__ void tet_row(const TetPairX<DPN> &pr, int a, double *ob) {
+  if constexpr (DPN == 1) {
 #pragma unroll
-      for (int sl = 0; sl < 4; sl++) {
-        const double dot = pr.g[0][0] * pr.g[sl][0] + pr.g[0][1] * pr.g[sl][1] + pr.g[0][2] * pr.g[sl][2];
-        const double c = pr.k0 * dot * pr.k1;
-        ob[sl * S] = quad4(c);
-      }
-    } else {
-      DConst D{};
-      D.m = pr.m;
-      D.cnu = pr.cnu;
-      D.dd = pr.dd;
-#pragma unroll
-      for (int sl = 0; sl < 4; sl++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-          const double c = pr.k0 * bdb3(pr.g[0], pr.g[sl], a, b, D);
-          ob[(sl * 3 + b) * S] = quad4(c);
-        }
+    for (int sl = 0; sl < 4; sl++) {
+      const double dot = pr.g[0][0] * pr.g[sl][0] + pr.g[0][1] * pr.g[sl][1] + pr.g[0][2] * pr.g[sl][2];
+      const double c = pr.k0 * dot * pr.k1;
+      ob[sl * S] = quad4(c);
     }
   } else {
-    if constexpr (DPN == 1) {
+    DConst D{};
+    D.m = pr.m;
+    D.cnu = pr.cnu;
+    D.dd = pr.dd;
 #pragma unroll
-      for (int sl = 0; sl < 4; sl++)
-        ob[sl * S] = pr.f0 * (pr.g[0][0] * pr.g[sl][0] + pr.g[0][1] * pr.g[sl][1] + pr.g[0][2] * pr.g[sl][2]);
-    } else {
+    for (int sl = 0; sl < 4; sl++)
 #pragma unroll
-      for (int sl = 0; sl < 4; sl++)
-#pragma unroll
-        for (int b = 0; b < 3; b++)
-          ob[(sl * 3 + b) * S] = pr.f0 * bdb3f(pr.g[0], pr.g[sl], a, b, pr.m, pr.cnu, pr.dd);
-    }
+      for (int b = 0; b < 3; b++) {
+        const double c = pr.k0 * bdb3(pr.g[0], pr.g[sl], a, b, D);
+        ob[(sl * 3 + b) * S] = quad4(c);
+      }
   }
 }
+
+template <int DPN, int S, typename T>
+__device__ __forceinline__ void tet_row(const TetPairC<DPN, T> &pr, int a, T *ob) {
+#pragma clang fp contract(fast)
+  if constexpr (DPN == 1) {
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) ob[sl * S] = pr.d[sl];
+  } else {
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        T v = pr.A[a] * pr.g[sl][b] + pr.Bm[b] * pr.g[sl][a];
+        if (a == b) v += pr.d[sl];
+        ob[(sl * 3 + b) * S] = v;
+      }
+  }
+}
+
 
 // Plan record of one item (row block), for all DPN rows.
 struct TetBlk {
@@ -303,9 +322,8 @@ __global__ __launch_bounds__(TP) void k_asm_tet(const double *__restrict__ coord
                                                      const int *__restrict__ rp, int ntiles, OpParams P,
                                                      const unsigned *__restrict__ woff,
                                                      const unsigned *__restrict__ ipos, const unsigned *__restrict__ ipmk,
-                                                     int64_t nitems, TetF32 f32, VT *__restrict__ vals) {
+                                                     int64_t nitems, TetLame lame, VT *__restrict__ vals) {
   constexpr int S = TP + 1;
-  const float dfac[3] = {f32.m, f32.cnu, f32.dd};
   // one DOF row of every pair: s_c[(slot * DPN + b) * S + pair]
   __shared__ VT s_c[4 * DPN * S];
   // one DOF row of the tile's nodes, node i's row at s_st[i * W ...]
@@ -356,8 +374,8 @@ __global__ __launch_bounds__(TP) void k_asm_tet(const double *__restrict__ coord
   const bool live0 = tid < nw && (tid < no || tid >= no_pad);
   if (live0) B0 = tet_blk(unit_item(tid, &b0), nitems, woff, ipos, ipmk);
 
-  TetPair<DPN, VT> pr;
-  tet_setup<DPN, VT>(X, ln, coef, P, dfac, pr);
+  typename TetPairSel<DPN, VT>::type pr;
+  tet_setup<DPN>(X, ln, coef, P, lame, pr);
   VT *const ob = s_c + tid;
   // copy-out: staging entry t = i * W + j (node i, entry j); i = t / W as
   // (t * ceil(2^20 / W)) >> 20, exact for t < 2^9 and W < 2^8 (the error
@@ -368,7 +386,7 @@ __global__ __launch_bounds__(TP) void k_asm_tet(const double *__restrict__ coord
 
 #pragma unroll
   for (int a = 0; a < DPN; a++) {
-    if (pair) tet_row<DPN, VT, S>(pr, a, ob);
+    if (pair) tet_row<DPN, S>(pr, a, ob);
     __syncthreads();  // strips written (and the previous row's copy-out done with s_st)
     for (int w = tid; w < nw; w += TP) {
       if (w >= no && w < no_pad) continue;  // padding of the off-diagonal wavefronts
@@ -699,17 +717,16 @@ int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const TetPlan &tp, 
   if (tp.ntiles == 0) return TFM_OK;
   hipStream_t st = ctx->stream;
   dim3 g(xcd_grid(tp.ntiles)), t(tp.tp);
-  TetF32 f32;
+  TetLame lame;
   {
     const double c = 1.0 / ((1.0 + P.nu) * (1.0 - 2.0 * P.nu));
-    f32.m = (float)((1.0 - P.nu) * c);
-    f32.cnu = (float)(c * P.nu);
-    f32.dd = (float)((1.0 - 2.0 * P.nu) / 2.0 * c);
+    lame.lam = (float)(c * P.nu);
+    lame.mu = (float)((1.0 - 2.0 * P.nu) / 2.0 * c);
   }
-#define TFM_TET(D, VT, TPV, V)                                                                                    \
-  hipLaunchKernelGGL((k_asm_tet<D, VT, TPV>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e, tp.d_tiles, \
-                     tp.d_tpa, tp.d_tib, tp.d_tnd, tp.d_tw, d_rp, tp.ntiles, P, tp.d_off, tp.d_pos, tp.d_pmk,        \
-                     tp.nitems, f32, V)
+#define TFM_TET(D, VT, TPV, V)                                                                                      \
+  hipLaunchKernelGGL((k_asm_tet<D, VT, TPV>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e,              \
+                     tp.d_tiles, tp.d_tpa, tp.d_tib, tp.d_tnd, tp.d_tw, d_rp, tp.ntiles, P, tp.d_off, tp.d_pos,      \
+                     tp.d_pmk, tp.nitems, lame, V)
 #define TFM_TET_TP(D, VT, V)             \
   do {                                   \
     if (tp.tp == 256)                    \

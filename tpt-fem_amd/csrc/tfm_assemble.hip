@@ -672,6 +672,7 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
   if ((rc = validate_operator(ctx, mesh, op, dpn))) return rc;
   if ((rc = ensure_adjacency(const_cast<tfm_mesh *>(mesh)))) return rc;
   pl->mesh = mesh;
+  mesh_retain(mesh);  // released by plan_destroy: a system may outlive the caller's handle
   pl->dpn = dpn;
   pl->ndof = mesh->n_nodes * dpn;
   if (pl->ndof >= INT32_MAX) return fail(ctx, TFM_E_UNSUPPORTED, "too many DOFs for int32 indexing");
@@ -808,6 +809,8 @@ void plan_destroy(AsmPlan *pl) {
   dfree(pl->d_tiles);
   dfree(pl->d_rp);
   dfree(pl->d_ci);
+  if (pl->mesh) mesh_release(pl->mesh);
+  pl->mesh = nullptr;
 }
 
 }  // namespace tfm

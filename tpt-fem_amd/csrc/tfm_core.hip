@@ -531,11 +531,21 @@ extern "C" int tfm_mesh_nodes_on_plane(const tfm_mesh *m, int axis, double coord
   return TFM_OK;
 }
 
-extern "C" void tfm_mesh_destroy(tfm_mesh *m) {
-  if (!m) return;
+namespace tfm {
+void mesh_retain(const tfm_mesh *m) { m->refs.fetch_add(1, std::memory_order_relaxed); }
+void mesh_release(const tfm_mesh *cm) {
+  if (cm->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  tfm_mesh *m = const_cast<tfm_mesh *>(cm);
   dfree(m->d_coords);
   dfree(m->d_conn);
   dfree(m->d_n2e_ptr);
   dfree(m->d_n2e);
   delete m;
+}
+}  // namespace tfm
+
+// The caller's handle; the device memory goes with the last holder (systems
+// built on the mesh hold it too).
+extern "C" void tfm_mesh_destroy(tfm_mesh *m) {
+  if (m) tfm::mesh_release(m);
 }

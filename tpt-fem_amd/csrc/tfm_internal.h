@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <functional>
@@ -55,6 +56,9 @@ struct tfm_mesh {
   int64_t grid_P = 0, grid_nx = 0, grid_ny = 0;
   // tfm_mesh_box: cells per axis (0 otherwise; jitter keeps the node grid)
   int64_t box_n[3] = {0, 0, 0};
+  // holders: the caller's handle and every assembly plan built on the mesh
+  // (systems keep theirs); the last release frees it
+  mutable std::atomic<int> refs{1};
 };
 
 struct tfm_matrix {
@@ -70,6 +74,8 @@ namespace tfm {
 int fail(tfm_ctx *ctx, int code, const std::string &msg);
 int set_device(tfm_ctx *ctx);
 int ensure_adjacency(tfm_mesh *m);
+void mesh_retain(const tfm_mesh *m);
+void mesh_release(const tfm_mesh *m);
 int scratch(tfm_ctx *ctx, size_t bytes, void **out);
 
 template <typename T>
