@@ -309,6 +309,21 @@ int tfm_mesh_grid_batch(tfm_ctx *ctx, int64_t P, int64_t nx, int64_t ny, double 
 int tfm_system_solve_batch(tfm_system *sys, int64_t P, const int64_t *dof_off, int nrhs,
                            const double *rhs, const tfm_solve_opts *opts, double *u,
                            tfm_solve_stats *stats);
+/* Batching form, re-assembly and solve together (no reference counterpart:
+ * the reference assembles then solves one problem per call): op's
+ * coefficients (NULL: the current ones; kind, model and rule unchanged) are
+ * assembled into every value array the system holds and the batched solve of
+ * tfm_system_solve_batch follows, as one sequence of captured hipGraph
+ * launches -- numeric assembly and the solver's refresh, initial state,
+ * chunks of iterations until every segment is done, residual certificate.
+ * The solver and its graphs stay with the system, so a later call with the
+ * same P, DOF ranges, nrhs and mode replays them.  Results equal
+ * tfm_system_assemble followed by tfm_system_solve_batch.  TFM_SOLVE_DIRECT
+ * and rhs counts other than 1 and 8 run those two calls. */
+int tfm_system_assemble_solve_batch(tfm_system *sys, const tfm_operator *op, int64_t P,
+                                    const int64_t *dof_off, int nrhs, const double *rhs,
+                                    const tfm_solve_opts *opts, double *u,
+                                    tfm_solve_stats *stats);
 
 /* ---- SIMP topology optimization (crates/tpt-fem-topopt/src/lib.rs) ------ */
 /* TopOptParams (topopt/src/lib.rs:119-137), the Grid given by Grid::new's

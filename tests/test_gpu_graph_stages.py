@@ -110,3 +110,34 @@ def test_simp_loop_replays_equal_direct_launches(ctx, mode):
         rr, cr, _ = O.topopt_simp(nx, ny, 1.0, 1.0, 0.3, vfs[p], 3.0, 1.5, iters, 0.2, F[p], bcs)
         assert np.abs(g0.densities[p] - rr).max() <= 1e-6, p
         np.testing.assert_allclose(g0.compliance[p], cr, rtol=1e-7, atol=0)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_assemble_solve_batch_equals_assemble_then_solve(ctx, mode):
+    """tfm_system_assemble_solve_batch (the numeric pass in the solve's first
+    graph stage) gives bitwise what tfm_system_assemble + solve_batch give,
+    call after call with new coefficients (replayed graphs), and the values
+    it leaves equal a fresh assembly."""
+    import tptfem as T
+
+    flags = T.SYS_FP64 | T.SYS_FP32
+    sm = T.SOLVE_PCG64 if mode == "plain" else T.SOLVE_MIXED
+    m, noff, eoff, kappa, s = poisson_batch(T, ctx, flags)
+    _, _, _, _, r = poisson_batch(T, ctx, flags)
+    rng = np.random.default_rng(21)
+    for rep in range(3):
+        k = kappa * np.exp(rng.uniform(-1, 1, m.n_elem))
+        op = T.Operator.poisson(1.0, 2, kappa_e=k)
+        u = s.assemble_solve_batch(op, noff, mode=sm, rtol=1e-11)
+        st = [x[:3] for x in s.batch_stats]
+        r.assemble(op, flags)
+        ur = r.solve_batch(noff, mode=sm, rtol=1e-11)
+        assert np.array_equal(u, ur), rep
+        assert st == [x[:3] for x in r.batch_stats], rep
+        Ks, Kr = s.download()[0], r.download()[0]
+        assert np.array_equal(Ks.values, Kr.values), rep
+    # the current coefficients again (op None) and the eager path agree
+    u1 = s.assemble_solve_batch(None, noff, mode=sm, rtol=1e-11)
+    with no_graph():
+        u2 = s.assemble_solve_batch(None, noff, mode=sm, rtol=1e-11)
+    assert np.array_equal(u1, u2) and np.array_equal(u1, u)

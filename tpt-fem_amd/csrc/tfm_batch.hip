@@ -1809,8 +1809,26 @@ extern "C" int tfm_mesh_grid_batch(tfm_ctx *ctx, int64_t P, int64_t nx, int64_t 
   return TFM_OK;
 }
 
-extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *dof_off, int nrhs, const double *rhs,
-                                      const tfm_solve_opts *opts, double *u, tfm_solve_stats *stats) {
+// The system's batched solver for (P, DOF ranges, rhs count, mode), built on
+// first use and kept with its captured graphs.
+static int batch_solver_for(tfm_system *s, int64_t P, const int64_t *dof_off, int nrhs, int mode, BatchSolver **out) {
+  std::vector<uint64_t> parts{(uint64_t)P, (uint64_t)nrhs, (uint64_t)mode};
+  uint64_t key = key_of({(uint64_t)P, (uint64_t)nrhs, (uint64_t)mode});
+  for (int64_t p = 0; p <= P; p++) key = key_of({key, (uint64_t)dof_off[p]});
+  if (!s->batch_solver || s->batch_key != key) {
+    bsolver_destroy(s->batch_solver);
+    s->batch_solver = nullptr;
+    s->batch_key = 0;
+    const int rc = bsolver_create(s, (int)P, dof_off, nrhs, mode, &s->batch_solver);
+    if (rc) return rc;
+    s->batch_key = key;
+  }
+  *out = s->batch_solver;
+  return TFM_OK;
+}
+
+static int solve_batch(tfm_system *s, int64_t P, const int64_t *dof_off, int nrhs, const double *rhs,
+                       const tfm_solve_opts *opts, double *u, tfm_solve_stats *stats, const SolveHooks *hooks) {
   if (!s || !dof_off || !u || P <= 0 || nrhs < 1) return TFM_E_INVALID;
   tfm_ctx *ctx = s->ctx;
   if (dof_off[0] != 0 || dof_off[P] != s->ndof) return fail(ctx, TFM_E_INVALID, "dof offsets must cover the system");
@@ -1848,7 +1866,7 @@ extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *d
     int worst = TFM_OK;
     for (int k0 = 0; k0 < nrhs;) {
       const int c = nrhs - k0 >= 8 ? 8 : 1;
-      rc = tfm_system_solve_batch(s, P, dof_off, c, rhs + (size_t)k0 * n, &o, u + (size_t)k0 * n, st.data());
+      rc = solve_batch(s, P, dof_off, c, rhs + (size_t)k0 * n, &o, u + (size_t)k0 * n, st.data(), nullptr);
       if (rc && rc != TFM_E_NOT_CONVERGED) return rc;
       if (rc) worst = rc;
       if (stats)
@@ -1861,27 +1879,18 @@ extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *d
   // A segment's rows must not couple to another problem's rows (block
   // diagonal): the kernels reduce per problem and would silently mix them.
   BatchSolver *b = nullptr;
-  if ((rc = bsolver_create(s, (int)P, dof_off, nrhs, o.mode, &b))) return rc;
-  if ((rc = bsolver_set_rhs(b, rhs))) {
-    bsolver_destroy(b);
-    return rc;
-  }
-  rc = bsolver_solve(b, o, false, true, nullptr, stats, nullptr);
-  if (rc && rc != TFM_E_NOT_CONVERGED) {
-    bsolver_destroy(b);
-    return rc;
-  }
+  if ((rc = batch_solver_for(s, P, dof_off, nrhs, o.mode, &b)) || (rc = bsolver_set_rhs(b, rhs))) return rc;
+  rc = bsolver_solve(b, o, false, true, hooks, stats, nullptr);
+  if (rc && rc != TFM_E_NOT_CONVERGED) return rc;
   if (nrhs == 1) {
     // x is 0 at the fixed DOFs: u = x, then the Dirichlet values in bcs order
     hipError_t e = hipMemcpy(u, b->x, sizeof(double) * n, hipMemcpyDeviceToHost);
-    bsolver_destroy(b);
     if (e != hipSuccess) return fail(ctx, TFM_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
     for (size_t q = 0; q < s->bc_dof.size(); q++) u[s->bc_dof[q]] = s->bc_val[q];
     return rc;
   }
   std::vector<double> hx((size_t)n * nrhs);
   hipError_t e = hipMemcpy(hx.data(), b->x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost);
-  bsolver_destroy(b);
   if (e != hipSuccess) return fail(ctx, TFM_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
   // u[k][*]: free entries from x, then Dirichlet values in bcs order
   for (int k = 0; k < nrhs; k++) {
@@ -1890,5 +1899,45 @@ extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *d
     for (int64_t fdof : s->free_dofs) uk[fdof] = hx[(size_t)fdof * nrhs + k];
     for (size_t q = 0; q < s->bc_dof.size(); q++) uk[s->bc_dof[q]] = s->bc_val[q];
   }
+  return rc;
+}
+
+extern "C" int tfm_system_solve_batch(tfm_system *s, int64_t P, const int64_t *dof_off, int nrhs, const double *rhs,
+                                      const tfm_solve_opts *opts, double *u, tfm_solve_stats *stats) {
+  return solve_batch(s, P, dof_off, nrhs, rhs, opts, u, stats, nullptr);
+}
+
+extern "C" int tfm_system_assemble_solve_batch(tfm_system *s, const tfm_operator *op, int64_t P,
+                                               const int64_t *dof_off, int nrhs, const double *rhs,
+                                               const tfm_solve_opts *opts, double *u, tfm_solve_stats *stats) {
+  if (!s) return TFM_E_INVALID;
+  tfm_ctx *ctx = s->ctx;
+  if (!s->has_plan) return fail(ctx, TFM_E_INVALID, "system was not built from a mesh");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  const int mode = opts ? opts->mode : TFM_SOLVE_PCG64;
+  if (mode == TFM_SOLVE_DIRECT || (nrhs != 1 && nrhs != 8)) {
+    // no graph sequence for these: the re-assembly, then the solve
+    if ((rc = tfm_system_assemble(s, op, 0))) return rc;
+    return solve_batch(s, P, dof_off, nrhs, rhs, opts, u, stats, nullptr);
+  }
+  if (op) {
+    if (op->kind != s->plan.op.kind || op->model != s->plan.op.model || op->quad_order != s->plan.op.quad_order)
+      return fail(ctx, TFM_E_INVALID, "re-assembly must keep the operator kind, model and rule");
+    if ((rc = plan_set_coefficients(ctx, &s->plan, op))) return rc;
+    if (op->kind == TFM_OP_SIMP)
+      TFM_CHECK_HIP(ctx, hipMemcpyAsync(s->plan.d_tab + (size_t)s->plan.nq * kRuleStride, s->plan.ke0,
+                                        sizeof(double) * 64, hipMemcpyHostToDevice, ctx->stream));
+  }
+  // every value array the system holds is re-assembled in the solve's first
+  // stage (the versions are noted first: the solver checks them)
+  double *w64 = s->d_v64;
+  float *w32 = s->d_v32;
+  s->note_values(op != nullptr, w64 != nullptr, w32 != nullptr);
+  SolveHooks hk;
+  hk.pre = [s, ctx, w64, w32]() { return plan_numeric(ctx, &s->plan, w64, w32); };
+  hk.tag = key_of({0x61736d62ull, (uint64_t)(w64 != nullptr), (uint64_t)(w32 != nullptr)});
+  rc = solve_batch(s, P, dof_off, nrhs, rhs, opts, u, stats, &hk);
+  if (rc && rc != TFM_E_NOT_CONVERGED) (void)plan_numeric(ctx, &s->plan, w64, w32);  // values match the coefficients
   return rc;
 }

@@ -364,6 +364,11 @@ struct tfm_system {
   bool owns_pattern = false;
   tfm::BSell bsell;           // solver copy, built on the first solve
   tfm::BatchSolver *mg_solver = nullptr;  // multigrid solver of tfm_system_solve, built on its first use
+  // batched solver of tfm_system_solve_batch / tfm_system_assemble_solve_batch,
+  // kept (with its captured graphs) while P, the DOF ranges, the rhs count
+  // and the mode stay the same
+  tfm::BatchSolver *batch_solver = nullptr;
+  uint64_t batch_key = 0;
   // Coefficient versions: `version` counts the coefficient sets the system
   // has been assembled with; each numeric pass stamps the value arrays it
   // wrote with the current one.  FP32 values describe the same matrix as

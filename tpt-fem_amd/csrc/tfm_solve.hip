@@ -1748,6 +1748,7 @@ extern "C" void tfm_system_destroy(tfm_system *s) {
   dfree(s->d_v32);
   bsell_destroy(&s->bsell);
   bsolver_destroy(s->mg_solver);
+  bsolver_destroy(s->batch_solver);
   dfree(s->d_f);
   delete s;
 }

@@ -147,6 +147,7 @@ _SIGS = {
     "tfm_mesh_box_batch": (INT, [P, I64, P, PP, P, P]),
     "tfm_mesh_grid_batch": (INT, [P, I64, I64, I64, D, PP]),
     "tfm_system_solve_batch": (INT, [P, I64, P, INT, P, C.POINTER(SolveOpts), P, P]),
+    "tfm_system_assemble_solve_batch": (INT, [P, P, I64, P, INT, P, C.POINTER(SolveOpts), P, P]),
     "tfm_topopt_simp": (INT, [P, C.POINTER(TopOptParams), P, I64, P, P, C.POINTER(SolveOpts), P, P, P, P]),
     "tfm_topopt_simp_batch": (INT, [P, I64, C.POINTER(TopOptParams), P, P, I64, P, P, C.POINTER(SolveOpts), P, P,
                                     P, P]),
@@ -581,6 +582,23 @@ class System:
         o = SolveOpts(mode=mode, rtol=rtol, max_iter=max_iter, inner_rtol=inner_rtol)
         self.ctx.check(_lib.tfm_system_solve_batch(self.h, P, _ptr(off), nrhs, _ptr(r), C.byref(o), _ptr(u),
                                                    C.cast(st, C.c_void_p)))
+        self.batch_stats = [(s.iterations, s.outer, s.rel_residual, s.seconds) for s in st]
+        return u
+
+    def assemble_solve_batch(self, op, dof_off, rhs=None, nrhs=1, mode=SOLVE_PCG64, rtol=1e-10, max_iter=20000,
+                             inner_rtol=0.3):
+        """Re-assembly with op's coefficients (None: the current ones) and the
+        batched solve as one sequence of captured graph launches; returns u
+        [nrhs, n_dof] like solve_batch."""
+        off = _i64(dof_off)
+        P = len(off) - 1
+        r = None if rhs is None else _f64(rhs).reshape(nrhs, self.n_dof)
+        u = np.zeros((nrhs, self.n_dof))
+        st = (SolveStats * (P * nrhs))()
+        o = SolveOpts(mode=mode, rtol=rtol, max_iter=max_iter, inner_rtol=inner_rtol)
+        self.ctx.check(_lib.tfm_system_assemble_solve_batch(
+            self.h, C.byref(op) if op is not None else None, P, _ptr(off), nrhs, _ptr(r), C.byref(o), _ptr(u),
+            C.cast(st, C.c_void_p)))
         self.batch_stats = [(s.iterations, s.outer, s.rel_residual, s.seconds) for s in st]
         return u
 
