@@ -167,7 +167,7 @@ def assembly_record(s, mesh, ms, per_elem_coef, cell_dpn):
         elif info["kernel"] == "simp_grid":
             kern = f"tfm::k_simp_rows<{vt}, false>"
         elif info["kernel"] == "triplets":
-            kern = f"tfm::k_coo_sum<{vt}>"
+            kern = f"tfm::k_coo_sum<{vt}, int>"
         else:
             kern = f"tfm::k_asm_tile<{cell_dpn[0]}, {cell_dpn[1]}, {2 if sv == 8 else 3}>"
         rec[name + "_roofline"] = {"kernel": kern, "algorithmic_bytes": b, "GBps": gbs, "frac": gbs / PEAK_HBM_GBS}

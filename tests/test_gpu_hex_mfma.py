@@ -9,6 +9,12 @@ within 2e-6 of each row's largest entry) for Gauss-Legendre orders 1, 2 and
 3 (1, 8 and 27 points: the padded inner dimension 2, 8 and 28), against the
 per-pair FP32 kernel it replaces (TFM_NO_MFMA), and through a precision-form
 solve against the oracle's dense LU.
+
+Hex8 elasticity is assembled from per-element tables by the sorted-triplet
+sum: FP64 element matrices formed once per element by a wavefront in the
+reference's operation order, FP32 ones on the matrix cores.  Its FP64
+values are the oracle's bit for bit and those of the tile kernels
+(TFM_HEX_TILE), before and after re-assembly.
 """
 import numpy as np
 import pytest
@@ -83,4 +89,53 @@ def test_hex_precision_form_solve_vs_dense_lu(ctx):
     u = s.solve(T.SOLVE_MIXED, rtol=1e-12, max_iter=20000)
     u_ref = O.solve_with_dirichlet(O.assemble(coords, conn, ref, 3), rhs, bcs)
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) <= 1e-10
+    m.close()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_hex_table_route_fp64_bitexact(ctx, order, monkeypatch):
+    T, m, s, coords, conn, ref, bcs, rhs = setup(ctx, (4, 3, 5), order, 11 + order, 3)  # FP64 | FP32
+    assert s.assembly_info()["kernel"] == "triplets"
+    K, f, _ = s.download()
+    kr, fr, _ = O.reduce_system(O.assemble(coords, conn, ref, 3), rhs, bcs)
+    kr = kr.to_csr()
+    np.testing.assert_array_equal(K.row_ptrs, kr.row_ptrs)
+    np.testing.assert_array_equal(K.col_ind, kr.col_ind)
+    assert np.array_equal(K.values, kr.values)
+    # the tile kernels give the same FP64 values, and the same FP32 sums of the
+    # same FP32 element tables
+    monkeypatch.setenv("TFM_HEX_TILE", "1")
+    ye = 10.0 ** np.random.default_rng(11 + order).uniform(-1, 1, len(conn))
+    op = T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, order, young_e=ye)
+    t64 = T.System.create(m, op, 3, bcs, rhs, flags=T.SYS_FP64)
+    assert t64.assembly_info()["kernel"] != "triplets"
+    assert np.array_equal(t64.download()[0].values, K.values)
+    t32 = T.System.create(m, op, 3, bcs, rhs, flags=T.SYS_FP32)
+    s32 = None
+    monkeypatch.delenv("TFM_HEX_TILE")
+    s32 = T.System.create(m, op, 3, bcs, rhs, flags=T.SYS_FP32)
+    assert np.array_equal(t32.download()[0].values, s32.download()[0].values)
+    # re-assembly with new moduli
+    ye2 = ye * np.exp(np.random.default_rng(5).uniform(-1, 1, len(conn)))
+    s.assemble(T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, order, young_e=ye2), T.SYS_FP64)
+    kr2, _, _ = O.reduce_system(O.assemble(coords, conn, O.make_op(O.HEX, 3, 1, order=order, coef_e=ye2, nu=0.3), 3),
+                                rhs, bcs)
+    assert np.array_equal(s.download()[0].values, kr2.to_csr().values)
+    for x in (s, t64, t32, s32):
+        x.close()
+    m.close()
+
+
+def test_hex_table_route_one_shot_assemble(ctx):
+    """tfm_assemble (assemble(...).to_csr(), no reduction) on the same route."""
+    import tptfem as T
+
+    coords, conn = hex_mesh((3, 3, 2), 0.25, 4)
+    ye = 10.0 ** np.random.default_rng(4).uniform(-1, 1, len(conn))
+    m = T.Mesh.from_arrays(ctx, T.HEX, coords, conn)
+    K = T.assemble(m, 3, T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=ye)).to_csr()
+    R = O.assemble(coords, conn, O.make_op(O.HEX, 3, 1, order=2, coef_e=ye, nu=0.3), 3).to_csr()
+    np.testing.assert_array_equal(K.row_ptrs, R.row_ptrs)
+    np.testing.assert_array_equal(K.col_ind, R.col_ind)
+    assert np.array_equal(K.values, R.values)
     m.close()

@@ -98,23 +98,81 @@ __global__ void k_coo_heads(int64_t n, const unsigned long long *__restrict__ ke
 }
 
 // Per CSR entry q (its head k): column, first triplet, and the row count.
-__global__ void k_coo_entries(int64_t n, const unsigned long long *__restrict__ keys, const int *__restrict__ flag,
-                              const int *__restrict__ eid, int *__restrict__ ci, int *__restrict__ seg,
-                              int *__restrict__ rcnt, int *__restrict__ dims) {
+// Keys are sorted, so a wavefront's heads come in runs of equal rows: one
+// atomic per run.  The largest row and column go through a block maximum
+// into one of kSpread slot pairs (atomics on a single word from every block
+// would serialise), which k_coo_dims folds.
+constexpr int kSpread = 256;
+__global__ __launch_bounds__(256) void k_coo_entries(int64_t n, const unsigned long long *__restrict__ keys,
+                                                     const int *__restrict__ flag, const int *__restrict__ eid,
+                                                     int *__restrict__ ci, int *__restrict__ seg,
+                                                     int *__restrict__ rcnt, int *__restrict__ spread) {
+  __shared__ int s_m[2][4];
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= n || !flag[k]) return;
-  const int q = eid[k];
-  const int row = (int)(keys[k] >> 32), col = (int)(keys[k] & 0xffffffffull);
-  ci[q] = col;
-  seg[q] = (int)k;
-  atomicAdd(&rcnt[row], 1);
-  atomicMax(&dims[0], row + 1);
-  atomicMax(&dims[1], col + 1);
+  const bool head = k < n && flag[k];
+  int row = -1, col = -1;
+  if (head) {
+    const int q = eid[k];
+    row = (int)(keys[k] >> 32);
+    col = (int)(keys[k] & 0xffffffffull);
+    ci[q] = col;
+    seg[q] = (int)k;
+  }
+  const int lane = threadIdx.x & 63;
+  unsigned long long left = __ballot(head);
+  while (left) {
+    const int lead = __ffsll((long long)left) - 1;
+    const int r = __shfl(row, lead, 64);
+    const unsigned long long run = __ballot(head && row == r);
+    if (lane == lead) atomicAdd(&rcnt[r], __popcll(run));
+    left &= ~run;
+  }
+  int mr = row, mc = col;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mr = max(mr, __shfl_xor(mr, o, 64));
+    mc = max(mc, __shfl_xor(mc, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_m[0][w] = mr;
+    s_m[1][w] = mc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    mr = max(max(s_m[0][0], s_m[0][1]), max(s_m[0][2], s_m[0][3]));
+    mc = max(max(s_m[1][0], s_m[1][1]), max(s_m[1][2], s_m[1][3]));
+    if (mr >= 0) {
+      int *sl = spread + 2 * (blockIdx.x % kSpread);
+      atomicMax(&sl[0], mr + 1);
+      atomicMax(&sl[1], mc + 1);
+    }
+  }
+}
+
+// dims = the slot pairs' maxima (one block of kSpread threads).
+__global__ void k_coo_dims(const int *__restrict__ spread, int *__restrict__ dims) {
+  __shared__ int s[2][kSpread];
+  const int t = threadIdx.x;
+  s[0][t] = spread[2 * t];
+  s[1][t] = spread[2 * t + 1];
+  __syncthreads();
+  for (int o = kSpread / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      s[0][t] = max(s[0][t], s[0][t + o]);
+      s[1][t] = max(s[1][t], s[1][t + o]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    dims[0] = s[0][0];
+    dims[1] = s[1][0];
+  }
 }
 
 // One CSR entry per thread: its contributions in insertion order from +0.0.
-template <typename VT>
-__global__ void k_coo_sum(int64_t nnz, const int *__restrict__ seg, const int64_t *__restrict__ perm,
+template <typename VT, typename IT>
+__global__ void k_coo_sum(int64_t nnz, const int *__restrict__ seg, const IT *__restrict__ perm,
                           const VT *__restrict__ dense, VT *__restrict__ out) {
   const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (q >= nnz) return;
@@ -124,6 +182,89 @@ __global__ void k_coo_sum(int64_t nnz, const int *__restrict__ seg, const int64_
     if (v != (VT)0) acc += v;
   }
   out[q] = acc;
+}
+
+// FP64 element matrices of 3-D elasticity on Hex8 cells, the reference's
+// values bit for bit (crates/tpt-fem-elasticity/src/lib.rs:284-307), into the
+// dense table ke[e][24][24] (row (i, a), column (j, b)) the sorted-triplet sum
+// reads.  One wavefront per element: lane q < nq forms quadrature point q's
+// map (map_point, as every FP64 pass does) into LDS; then lane (i, j) adds
+// the nine entries K[(i,a),(j,b)] += (w |det|) * bdb3(g_i, g_j, a, b) over the
+// points in rule order, from 0.0 -- the per-pair row block's operations and
+// order -- with (a, b) the same across the wavefront; the 576 values leave
+// through LDS as one contiguous block.  The element is formed once, where a
+// per-pair pass forms it once per element node.
+constexpr int kKeWaves = 4;
+constexpr int kKeMaxQ = 27;  // Gauss-Legendre orders 1-3
+
+__device__ __forceinline__ void ke_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(64 * kKeWaves) void k_hex_ke64(int64_t ne, const double *__restrict__ coords,
+                                                             const int *__restrict__ conn, OpParams P,
+                                                             double *__restrict__ ke) {
+  __shared__ double s_x[kKeWaves][8][3];
+  __shared__ double s_g[kKeWaves][kKeMaxQ][8][3];
+  __shared__ double s_wd[kKeWaves][kKeMaxQ];
+  __shared__ double s_k[kKeWaves][576];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * kKeWaves;
+  const int nq = P.nq;
+  for (int64_t e = (int64_t)blockIdx.x * kKeWaves + w; e < ne; e += nwaves) {
+    if (l < 24) s_x[w][l / 3][l % 3] = coords[(int64_t)conn[e * 8 + l / 3] * 3 + l % 3];
+    const double coef = P.coef_e ? P.coef_e[e] : P.coef;
+    const DConst D = make_d(P.model, coef, P.nu);
+    ke_wave_sync();
+    if (l < nq) {
+      double X[8][3];
+#pragma unroll
+      for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) X[a][p] = s_x[w][a][p];
+      const double *R = P.rule + l * kRuleStride;
+      double g[8][3];
+      const double det = fabs(map_point<8, 3>(X, R + 12, g));
+      s_wd[w][l] = R[0] * det;
+#pragma unroll
+      for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) s_g[w][l][a][p] = g[a][p];
+    }
+    ke_wave_sync();
+    const int i = l >> 3, j = l & 7;
+    double acc[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) acc[c] = 0.0;
+    for (int q = 0; q < nq; q++) {
+      const double wd = s_wd[w][q];
+      double gi[3], gj[3];
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        gi[p] = s_g[w][q][i][p];
+        gj[p] = s_g[w][q][j][p];
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) acc[a * 3 + b] += wd * bdb3(gi, gj, a, b, D);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) s_k[w][(i * 3 + a) * 24 + j * 3 + b] = acc[a * 3 + b];
+    ke_wave_sync();
+#pragma unroll
+    for (int t = 0; t < 9; t++) ke[e * 576 + t * 64 + l] = s_k[w][t * 64 + l];
+    ke_wave_sync();  // s_x, s_g and s_k are rewritten for the next element
+  }
+}
+
+bool hex_table_route(const tfm_mesh *m, int dpn, const tfm_operator *op) {
+  return m->cell == TFM_HEX && dpn == 3 && op->kind == TFM_OP_ELASTICITY && op->model == TFM_CONTINUUM3D &&
+         op->quad_order >= 1 && op->quad_order <= 3 && !getenv("TFM_HEX_TILE");
 }
 
 template <int WHAT>
@@ -151,6 +292,11 @@ static int coo_launch(tfm_ctx *ctx, const AsmPlan *pl, const OpParams &P, int *c
   TFM_COO(TFM_LINE, 1)
 #undef TFM_COO
   return fail(ctx, TFM_E_UNSUPPORTED, "cell / dofs-per-node combination not built");
+}
+
+__global__ void k_narrow(int64_t n, const int64_t *__restrict__ a, int *__restrict__ b) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    b[k] = (int)a[k];
 }
 
 int coo_plan_create(tfm_ctx *ctx, AsmPlan *pl, const OpParams &P) {
@@ -216,16 +362,18 @@ int coo_plan_create(tfm_ctx *ctx, AsmPlan *pl, const OpParams &P) {
     return rc;
   }
   if ((rc = dalloc(ctx, &pl->d_ci, std::max<int64_t>(nnz, 1))) || (rc = dalloc(ctx, &pl->d_coo_seg, nnz + 1)) ||
-      (rc = dalloc(ctx, &rcnt, pl->ndof + 1)) || (rc = dalloc(ctx, &dims, 2)) ||
+      (rc = dalloc(ctx, &rcnt, pl->ndof + 1)) || (rc = dalloc(ctx, &dims, 2 + 2 * kSpread)) ||
       (rc = dalloc(ctx, &pl->d_rp, pl->ndof + 1))) {
     cleanup();
     return rc;
   }
   TFM_CHECK_HIP(ctx, hipMemsetAsync(rcnt, 0, sizeof(int) * (pl->ndof + 1), st));
-  TFM_CHECK_HIP(ctx, hipMemsetAsync(dims, 0, sizeof(int) * 2, st));
-  if (ntrip > 0)
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(dims, 0, sizeof(int) * (2 + 2 * kSpread), st));
+  if (ntrip > 0) {
     hipLaunchKernelGGL(k_coo_entries, dim3(gk), dim3(256), 0, st, ntrip, keys2, flag, eid, pl->d_ci, pl->d_coo_seg,
-                       rcnt, dims);
+                       rcnt, dims + 2);
+    hipLaunchKernelGGL(k_coo_dims, dim3(1), dim3(kSpread), 0, st, (const int *)(dims + 2), dims);
+  }
   const int segend = (int)ntrip;
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(pl->d_coo_seg + nnz, &segend, sizeof(int), hipMemcpyHostToDevice, st));
   int64_t nnz2 = 0;
@@ -237,6 +385,17 @@ int coo_plan_create(tfm_ctx *ctx, AsmPlan *pl, const OpParams &P) {
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(hd, dims, sizeof hd, hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
   cleanup();
+  // positions fit 32 bits for tables below 2^31 entries: the sum pass then
+  // reads half the index bytes
+  if (m->n_elem * nd * nd < INT32_MAX) {
+    if ((rc = dalloc(ctx, &pl->d_coo_perm32, std::max<int64_t>(ntrip, 1)))) return rc;
+    if (ntrip > 0)
+      hipLaunchKernelGGL(k_narrow, dim3((unsigned)std::min<int64_t>((ntrip + 255) / 256, 65536)), dim3(256), 0, st,
+                         ntrip, (const int64_t *)pl->d_coo_perm, pl->d_coo_perm32);
+    TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+    dfree(pl->d_coo_perm);
+    pl->d_coo_perm = nullptr;
+  }
   pl->use_coo = true;
   pl->coo_ntrip = ntrip;
   pl->nnz = nnz;
@@ -247,26 +406,45 @@ int coo_plan_create(tfm_ctx *ctx, AsmPlan *pl, const OpParams &P) {
   return TFM_OK;
 }
 
+template <typename VT>
+static void coo_sum(tfm_ctx *ctx, const AsmPlan *pl, const VT *dense, VT *out) {
+  if (!pl->nnz) return;
+  const dim3 g((unsigned)((pl->nnz + 255) / 256)), b(256);
+  if (pl->d_coo_perm32)
+    hipLaunchKernelGGL((k_coo_sum<VT, int>), g, b, 0, ctx->stream, pl->nnz, pl->d_coo_seg,
+                       (const int *)pl->d_coo_perm32, dense, out);
+  else
+    hipLaunchKernelGGL((k_coo_sum<VT, int64_t>), g, b, 0, ctx->stream, pl->nnz, pl->d_coo_seg,
+                       (const int64_t *)pl->d_coo_perm, dense, out);
+}
+
 int coo_numeric(tfm_ctx *ctx, const AsmPlan *pl, const OpParams &P, double *v64, float *v32) {
   const tfm_mesh *m = pl->mesh;
   const int nd = m->nn * pl->dpn;
   const int64_t dense = m->n_elem * nd * nd;
-  hipStream_t st = ctx->stream;
+  const bool hex = hex_table_route(m, pl->dpn, &pl->op) && P.nq <= kKeMaxQ;
   int rc;
-  const unsigned g = (unsigned)((pl->nnz + 255) / 256);
   if (v64) {
     if (!pl->d_coo_v64 && (rc = dalloc(ctx, &pl->d_coo_v64, std::max<int64_t>(dense, 1)))) return rc;
-    if ((rc = coo_launch<C_F64>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, pl->d_coo_v64, nullptr))) return rc;
-    if (pl->nnz)
-      hipLaunchKernelGGL(k_coo_sum<double>, dim3(g), dim3(256), 0, st, pl->nnz, pl->d_coo_seg, pl->d_coo_perm,
-                         pl->d_coo_v64, v64);
+    if (hex) {
+      // each element once, by a wavefront (the table is the per-pair layout)
+      if (m->n_elem > 0)
+        hipLaunchKernelGGL(k_hex_ke64, dim3((unsigned)std::min<int64_t>((m->n_elem + kKeWaves - 1) / kKeWaves, 16384)),
+                           dim3(64 * kKeWaves), 0, ctx->stream, m->n_elem, m->d_coords, m->d_conn, P, pl->d_coo_v64);
+    } else if ((rc = coo_launch<C_F64>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, pl->d_coo_v64, nullptr))) {
+      return rc;
+    }
+    coo_sum<double>(ctx, pl, pl->d_coo_v64, v64);
   }
   if (v32) {
     if (!pl->d_coo_v32 && (rc = dalloc(ctx, &pl->d_coo_v32, std::max<int64_t>(dense, 1)))) return rc;
-    if ((rc = coo_launch<C_F32>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, nullptr, pl->d_coo_v32))) return rc;
-    if (pl->nnz)
-      hipLaunchKernelGGL(k_coo_sum<float>, dim3(g), dim3(256), 0, st, pl->nnz, pl->d_coo_seg, pl->d_coo_perm,
-                         pl->d_coo_v32, v32);
+    if (hex_mfma_eligible(pl)) {
+      // the matrix cores' element table has the same [e][24][24] layout
+      if ((rc = hex_ke_mfma(ctx, pl, pl->d_coo_v32))) return rc;
+    } else if ((rc = coo_launch<C_F32>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, nullptr, pl->d_coo_v32))) {
+      return rc;
+    }
+    coo_sum<float>(ctx, pl, pl->d_coo_v32, v32);
   }
   TFM_CHECK_HIP(ctx, hipGetLastError());
   return TFM_OK;
@@ -274,6 +452,7 @@ int coo_numeric(tfm_ctx *ctx, const AsmPlan *pl, const OpParams &P, double *v64,
 
 void coo_plan_destroy(AsmPlan *pl) {
   dfree(pl->d_coo_perm);
+  dfree(pl->d_coo_perm32);
   dfree(pl->d_coo_seg);
   dfree(pl->d_coo_v64);
   dfree(pl->d_coo_v32);

@@ -692,7 +692,7 @@ int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int 
   // A node in more elements than a tile holds (or than its pair mask has
   // bits): the sorted-triplet path assembles the whole mesh.
   int TP = tile_pairs_for(mesh->cell, dpn);
-  if (mesh->max_adj > 64 || mesh->max_adj > TP || getenv("TFM_COO_ASSEMBLY")) {
+  if (mesh->max_adj > 64 || mesh->max_adj > TP || getenv("TFM_COO_ASSEMBLY") || hex_table_route(mesh, dpn, op)) {
     if ((rc = upload_tables(ctx, pl, &pl->d_tab, &pl->nq))) return rc;
     OpParams P = make_params(pl, pl->d_tab, pl->nq, pl->d_tab + (size_t)pl->nq * kRuleStride);
     return coo_plan_create(ctx, pl, P);

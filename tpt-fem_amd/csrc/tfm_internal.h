@@ -189,6 +189,7 @@ struct AsmPlan {
   bool use_coo = false;
   int64_t coo_ntrip = 0;
   int64_t *d_coo_perm = nullptr;
+  int *d_coo_perm32 = nullptr;  // the same positions in 32 bits, when the table has < 2^31 entries
   int *d_coo_seg = nullptr;
   mutable double *d_coo_v64 = nullptr;
   mutable float *d_coo_v32 = nullptr;
@@ -213,6 +214,10 @@ int plan_numeric(tfm_ctx *ctx, const AsmPlan *plan, double *d_v64, float *d_v32)
 int coo_plan_create(tfm_ctx *ctx, AsmPlan *plan, const OpParams &P);
 int coo_numeric(tfm_ctx *ctx, const AsmPlan *plan, const OpParams &P, double *v64, float *v32);
 void coo_plan_destroy(AsmPlan *plan);
+// 3-D elasticity on Hex8 cells with Gauss-Legendre order 1-3 is assembled by
+// sorted triplets from per-element tables (FP64 exact, FP32 on the matrix
+// cores); TFM_HEX_TILE in the environment keeps the tile kernels.
+bool hex_table_route(const tfm_mesh *m, int dpn, const tfm_operator *op);
 // FP32 element matrices of 3-D isotropic elasticity on Hex8 cells by MFMA
 // (tfm_hex_mfma.hip); TFM_NO_MFMA in the environment switches them off.
 bool hex_mfma_eligible(const AsmPlan *plan);
