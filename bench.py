@@ -138,6 +138,13 @@ def box_face_nodes(n):
     return ((i * (ny + 1) + j) * (nz + 1) + k)[on]
 
 
+def pcg_iter_bytes(nnz, n, sv):
+    """Algorithmic bytes of one PCG iteration on the reduced CSR system:
+    (s_v + 4) nnz + 4 (n + 1) + 2 s_v n for the SpMV, 16 s_v n for the vector
+    updates (s_v: value bytes)."""
+    return (sv + 4) * nnz + 4 * (n + 1) + 2 * sv * n + 16 * sv * n
+
+
 def timed_assembly(ctx, s, steps, warmup):
     """Mean event-timed FP64 and FP32 numeric passes (ms) of a system."""
     out = {}
@@ -519,6 +526,16 @@ def bench_c3(args, T, log):
             if key in args.solvers:
                 solve[name], sw[name] = sweep(run(mode), err, args.rtol_sweep, args.err_target, log, name)
         rp, rm, rc = solve.get("plain_fp64"), solve.get("mixed_fp32_fp64refine"), solve.get("mixed_csr_fp32_fp64refine")
+        # per-iteration HBM rate of the assembled-matrix legs: the algorithmic
+        # bytes of one PCG iteration (CSR SpMV + 16 s n of vector traffic)
+        # over the measured time per iteration
+        for name, sv in (("plain_fp64", 8), ("mixed_csr_fp32_fp64refine", 4)):
+            r = solve.get(name)
+            if r and r["ms_per_iter"] > 0:
+                b = pcg_iter_bytes(sysm.nnz, sysm.n_free, sv)
+                gbs = b / (r["ms_per_iter"] * 1e-3) / 1e9
+                r["iteration_roofline"] = {"algorithmic_bytes": b, "GBps": gbs, "frac": gbs / PEAK_HBM_GBS,
+                                           "kernels": "k_pcg_dir_bsell + k_pcg_update"}
         solve["err_target"] = args.err_target
         solve["mixed_operator"] = "matrix-free FP32 (box mesh of P1 tets)"
         solve["mixed_speedup_vs_plain"] = (rp["s"] / rm["s"]) if (rp and rm and rm["s"] > 0) else None
