@@ -627,77 +627,115 @@ static int tet_tiles(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_r
   return TFM_OK;
 }
 
-int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci, int tile_pairs,
-                    TetPlan *tp) {
+// Per-node row-block lists of the pattern (independent of the tile size):
+// block count, first block, diagonal flag, contributing-pair masks, local
+// index bits, first present entry and presence masks per DOF row.
+struct TetBlocks {
+  int *bptr = nullptr;
+  unsigned char *hasdiag = nullptr, *bpm = nullptr;
+  unsigned *bmask = nullptr;
+  unsigned long long *blmb = nullptr;
+  int *bpos = nullptr;
+  int64_t nblk = 0;
+};
+
+static void tet_blocks_destroy(TetBlocks *B) {
+  dfree(B->bptr);
+  dfree(B->hasdiag);
+  dfree(B->bmask);
+  dfree(B->blmb);
+  dfree(B->bpos);
+  dfree(B->bpm);
+  *B = TetBlocks{};
+}
+
+static int tet_blocks_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci,
+                             TetBlocks *B) {
   const int64_t N = mesh->n_nodes;
+  hipStream_t st = ctx->stream;
+  int rc;
+  int *cnt = nullptr, *err = nullptr;
+  auto fail_out = [&](int code) {
+    dfree(cnt);
+    dfree(err);
+    tet_blocks_destroy(B);
+    return code;
+  };
+  if ((rc = dalloc(ctx, &cnt, N)) || (rc = dalloc(ctx, &B->bptr, N + 1)) || (rc = dalloc(ctx, &err, 1)) ||
+      (rc = dalloc(ctx, &B->hasdiag, N)))
+    return fail_out(rc);
+  const unsigned g = (unsigned)((N + 255) / 256);
+  if (N > 0) {
+    if (dpn == 1)
+      hipLaunchKernelGGL(k_tet_plan_count<1>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, cnt, B->hasdiag);
+    else
+      hipLaunchKernelGGL(k_tet_plan_count<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, cnt, B->hasdiag);
+  }
+  if ((rc = exclusive_scan_i32(ctx, cnt, B->bptr, N, &B->nblk))) return fail_out(rc);
+  const int64_t nblk = B->nblk;
+  if ((rc = dalloc(ctx, &B->bmask, nblk)) || (rc = dalloc(ctx, &B->blmb, nblk)) ||
+      (rc = dalloc(ctx, &B->bpos, nblk * dpn)) || (rc = dalloc(ctx, &B->bpm, nblk * dpn)))
+    return fail_out(rc);
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), st));
+  if (N > 0) {
+    if (dpn == 1)
+      hipLaunchKernelGGL(k_tet_plan_fill<1>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_conn,
+                         mesh->d_n2e_ptr, mesh->d_n2e, B->bptr, B->hasdiag, B->bmask, B->blmb, B->bpos, B->bpm, err);
+    else
+      hipLaunchKernelGGL(k_tet_plan_fill<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_conn,
+                         mesh->d_n2e_ptr, mesh->d_n2e, B->bptr, B->hasdiag, B->bmask, B->blmb, B->bpos, B->bpm, err);
+  }
+  int herr = 0;
+  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
+  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  dfree(cnt);
+  dfree(err);
+  if (herr) {
+    tet_blocks_destroy(B);
+    return fail(ctx, TFM_E_UNSUPPORTED, "pattern not covered by the tet plan");
+  }
+  return TFM_OK;
+}
+
+// The tiles of one tile size and their item records, from the block lists.
+static int tet_plan_from_blocks(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const TetBlocks &B,
+                                int tile_pairs, TetPlan *tp) {
   hipStream_t st = ctx->stream;
   int rc;
   if (tile_pairs != 192 && tile_pairs != 256) return fail(ctx, TFM_E_INVALID, "tet tile size must be 192 or 256 pairs");
   tp->tp = tile_pairs;
   if ((rc = tet_tiles(ctx, mesh, dpn, d_rp, tile_pairs, tp))) return rc;
-  // per-node block lists (temporaries), then the item records
-  int *cnt = nullptr, *err = nullptr, *bptr = nullptr, *bpos = nullptr;
-  unsigned char *hasdiag = nullptr, *bpm = nullptr;
-  unsigned *bmask = nullptr;
-  unsigned long long *blmb = nullptr;
-  auto cleanup = [&]() {
-    dfree(cnt);
-    dfree(err);
-    dfree(bptr);
-    dfree(hasdiag);
-    dfree(bmask);
-    dfree(blmb);
-    dfree(bpos);
-    dfree(bpm);
-  };
-  if ((rc = dalloc(ctx, &cnt, N)) || (rc = dalloc(ctx, &bptr, N + 1)) || (rc = dalloc(ctx, &err, 1)) ||
-      (rc = dalloc(ctx, &hasdiag, N))) {
-    cleanup();
-    return rc;
-  }
-  const unsigned g = (unsigned)((N + 255) / 256);
-  if (N > 0) {
-    if (dpn == 1)
-      hipLaunchKernelGGL(k_tet_plan_count<1>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, cnt, hasdiag);
-    else
-      hipLaunchKernelGGL(k_tet_plan_count<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, cnt, hasdiag);
-  }
-  int64_t nblk = 0;
-  if ((rc = exclusive_scan_i32(ctx, cnt, bptr, N, &nblk))) {
-    cleanup();
-    return rc;
-  }
+  const int64_t nblk = B.nblk;
   tp->nitems = nblk;
-  if ((rc = dalloc(ctx, &bmask, nblk)) || (rc = dalloc(ctx, &blmb, nblk)) || (rc = dalloc(ctx, &bpos, nblk * dpn)) ||
-      (rc = dalloc(ctx, &bpm, nblk * dpn)) || (rc = dalloc(ctx, &tp->d_tib, (int64_t)tp->ntiles + 1)) ||
-      (rc = dalloc(ctx, &tp->d_tnd, (int64_t)tp->ntiles)) || (rc = dalloc(ctx, &tp->d_off, nblk * 3)) ||
-      (rc = dalloc(ctx, &tp->d_pos, nblk)) || (rc = dalloc(ctx, &tp->d_pmk, nblk))) {
-    cleanup();
+  if ((rc = dalloc(ctx, &tp->d_tib, (int64_t)tp->ntiles + 1)) || (rc = dalloc(ctx, &tp->d_tnd, (int64_t)tp->ntiles)) ||
+      (rc = dalloc(ctx, &tp->d_off, nblk * 3)) || (rc = dalloc(ctx, &tp->d_pos, nblk)) ||
+      (rc = dalloc(ctx, &tp->d_pmk, nblk)))
     return rc;
-  }
-  TFM_CHECK_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), st));
-  if (N > 0) {
+  if (mesh->n_nodes > 0 && tp->ntiles > 0) {
     const unsigned gt = (unsigned)((tp->ntiles + 255) / 256);
-    if (dpn == 1) {
-      hipLaunchKernelGGL(k_tet_plan_fill<1>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_conn,
-                         mesh->d_n2e_ptr, mesh->d_n2e, bptr, hasdiag, bmask, blmb, bpos, bpm, err);
+    if (dpn == 1)
       hipLaunchKernelGGL(k_tet_plan_items<1>, dim3(gt), dim3(256), 0, st, tp->ntiles, tp->d_tiles, tp->d_tpa,
-                         mesh->d_n2e_ptr, mesh->d_n2e, d_rp, bptr, hasdiag, bmask, blmb, bpos, bpm, nblk, tp->d_tib,
-                         tp->d_tnd, tp->d_off, tp->d_pos, tp->d_pmk, tile_pairs + 1);
-    } else {
-      hipLaunchKernelGGL(k_tet_plan_fill<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_conn,
-                         mesh->d_n2e_ptr, mesh->d_n2e, bptr, hasdiag, bmask, blmb, bpos, bpm, err);
+                         mesh->d_n2e_ptr, mesh->d_n2e, d_rp, B.bptr, B.hasdiag, B.bmask, B.blmb, B.bpos, B.bpm, nblk,
+                         tp->d_tib, tp->d_tnd, tp->d_off, tp->d_pos, tp->d_pmk, tile_pairs + 1);
+    else
       hipLaunchKernelGGL(k_tet_plan_items<3>, dim3(gt), dim3(256), 0, st, tp->ntiles, tp->d_tiles, tp->d_tpa,
-                         mesh->d_n2e_ptr, mesh->d_n2e, d_rp, bptr, hasdiag, bmask, blmb, bpos, bpm, nblk, tp->d_tib,
-                         tp->d_tnd, tp->d_off, tp->d_pos, tp->d_pmk, tile_pairs + 1);
-    }
+                         mesh->d_n2e_ptr, mesh->d_n2e, d_rp, B.bptr, B.hasdiag, B.bmask, B.blmb, B.bpos, B.bpm, nblk,
+                         tp->d_tib, tp->d_tnd, tp->d_off, tp->d_pos, tp->d_pmk, tile_pairs + 1);
   }
-  int herr = 0;
-  TFM_CHECK_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
-  TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
-  cleanup();
-  if (herr) return fail(ctx, TFM_E_UNSUPPORTED, "pattern not covered by the tet plan");
+  TFM_CHECK_HIP(ctx, hipGetLastError());
   return TFM_OK;
+}
+
+int tet_plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci, int tile_pairs,
+                    TetPlan *tp) {
+  TetBlocks B;
+  int rc = tet_blocks_create(ctx, mesh, dpn, d_rp, d_ci, &B);
+  if (rc) return rc;
+  rc = tet_plan_from_blocks(ctx, mesh, dpn, d_rp, B, tile_pairs, tp);
+  // the block lists are read by the item kernel: freed once it has run
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = fail(ctx, TFM_E_HIP, "tet plan build failed");
+  tet_blocks_destroy(&B);
+  return rc;
 }
 
 void tet_plan_destroy(TetPlan *tp) {
@@ -816,10 +854,17 @@ int tet_plan_tuned(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp,
     dfree(v64);
     return rc;
   }
+  // the candidates share the pattern's per-node block lists
+  TetBlocks B;
+  if ((rc = tet_blocks_create(ctx, mesh, dpn, d_rp, d_ci, &B))) {
+    dfree(v64);
+    dfree(v32);
+    return rc;
+  }
   int best = -1;
   for (int c = 0; c < 2; c++) {
     TetPlan cp;
-    if (tet_plan_create(ctx, mesh, dpn, d_rp, d_ci, cand[c], &cp) != TFM_OK) {
+    if (tet_plan_from_blocks(ctx, mesh, dpn, d_rp, B, cand[c], &cp) != TFM_OK) {
       tet_plan_destroy(&cp);
       ctx->err.clear();
       continue;
@@ -827,6 +872,8 @@ int tet_plan_tuned(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp,
     if ((rc = tet_time_pass(ctx, mesh, dpn, cp, d_rp, P, v64, v32, &info->ms[c]))) {
       tet_plan_destroy(&cp);
       if (best >= 0) tet_plan_destroy(tp);
+      (void)hipStreamSynchronize(ctx->stream);
+      tet_blocks_destroy(&B);
       dfree(v64);
       dfree(v32);
       return rc;
@@ -839,6 +886,8 @@ int tet_plan_tuned(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp,
       tet_plan_destroy(&cp);
     }
   }
+  (void)hipStreamSynchronize(ctx->stream);  // the item kernels have read the block lists
+  tet_blocks_destroy(&B);
   dfree(v64);
   dfree(v32);
   if (best < 0) return fail(ctx, TFM_E_UNSUPPORTED, "no tet tile size fits this mesh");

@@ -64,8 +64,13 @@ __global__ __launch_bounds__((TileCfg<CELL, DPN>::TP)) void k_asm_tile(
   constexpr int DIM = Cell<CELL>::DIM;
   using VT = typename std::conditional<MODE == M_F32 || MODE == M_F32T, float, double>::type;
 
-  // row blocks of the tile's pairs (the table mode reads them from P.ke32)
-  __shared__ VT s_tab[(MODE == M_F32T) ? 1 : TP][ENT];
+  // Row blocks of the tile's pairs (the table mode reads them from P.ke32).
+  // The symbolic passes only need which entries are non-zero: one bit per
+  // entry when the block has at most 64 (the values stay in registers), so
+  // their LDS is a fraction of the numeric pass's and more tiles run per CU.
+  constexpr bool kBits = MODE <= M_FILL && ENT <= 64;
+  __shared__ VT s_tab[(MODE == M_F32T || kBits) ? 1 : TP][ENT];
+  __shared__ unsigned long long s_nzb[kBits ? TP : 1];
   __shared__ int s_en[TP][NN];
   __shared__ int s_pe[(MODE == M_F32T) ? TP : 1];  // table mode: element << 3 | local node
   __shared__ int s_nptr[kTileNodes + 1];
@@ -117,6 +122,14 @@ __global__ __launch_bounds__((TileCfg<CELL, DPN>::TP)) void k_asm_tile(
       rowblock_f32<CELL, DPN>(X, ln, coef, rho, P, tmp);
 #pragma unroll
       for (int i = 0; i < ENT; i++) s_tab[tid][i] = tmp[i];
+    } else if constexpr (kBits) {
+      double tmp[ENT];
+      rowblock_f64<CELL, DPN>(X, ln, coef, rho, P, tmp);
+      unsigned long long nzb = 0ull;
+#pragma unroll
+      for (int i = 0; i < ENT; i++)
+        if (tmp[i] != 0.0) nzb |= 1ull << i;
+      s_nzb[tid] = nzb;
     } else {
       // accumulated in place in LDS (no register copy of the row block)
       rowblock_f64<CELL, DPN>(X, ln, coef, rho, P, s_tab[tid]);
@@ -203,8 +216,12 @@ __global__ __launch_bounds__((TileCfg<CELL, DPN>::TP)) void k_asm_tile(
         for (int a = 0; a < DPN; a++)
 #pragma unroll
           for (int b = 0; b < DPN; b++)
-            if (P.kind == TFM_OP_SIMP || s_tab[i][a * W + lm * DPN + b] != 0.0)
-              nz |= 1u << (a * DPN + b);
+          {
+            bool keep;
+            if constexpr (kBits) keep = (s_nzb[i] >> (a * W + lm * DPN + b)) & 1ull;
+            else keep = s_tab[i][a * W + lm * DPN + b] != 0.0;
+            if (P.kind == TFM_OP_SIMP || keep) nz |= 1u << (a * DPN + b);
+          }
       }
       s_nz[tn][h] = (unsigned short)nz;
       int rank = 0;
