@@ -254,6 +254,13 @@ def bench_c2(args, T, log):
         nref = np.linalg.norm(u_ref)
         err = lambda u: float(np.linalg.norm(u - u_ref) / nref)
         solves = {}
+        # the configuration's own solve: Jacobi PCG to relative residual 1e-12
+        r12 = single_run(T, ctx, s, T.SOLVE_PCG64, args.inner_rtol)(1e-12)
+        r12["rel_err_vs_multigrid_1e-12"] = err(r12.pop("u"))
+        r12["rel_residual"] = s.stats.rel_residual
+        r12["rtol"] = 1e-12
+        solves["jacobi_fp64_rtol1e-12"] = r12
+        log(f"c2 jacobi to 1e-12: {r12['iters']} it, {r12['s']:.3f} s, true residual {r12['rel_residual']:.2e}")
         for mode, name in ((T.SOLVE_PCG64, "plain_fp64"), (T.SOLVE_MIXED, "mixed_fp32_fp64refine"),
                            (T.SOLVE_PCG64_MG, "multigrid_fp64")):
             solves[name], _ = sweep(single_run(T, ctx, s, mode, args.inner_rtol), err, args.rtol_sweep,
