@@ -455,13 +455,14 @@ __global__ void k_tet_plan_count(int N, const int *__restrict__ rp, const int *_
   hasdiag[n] = hd;
 }
 
+// Block columns of node n in block order (the diagonal first, then the
+// others ascending) and, per DOF row, each block's first present entry and
+// presence mask.
 template <int DPN>
-__global__ void k_tet_plan_fill(int N, const int *__restrict__ rp, const int *__restrict__ ci,
-                                const int *__restrict__ conn, const int *__restrict__ n2e_ptr,
-                                const int *__restrict__ n2e, const int *__restrict__ bptr,
-                                const unsigned char *__restrict__ hasdiag, unsigned *__restrict__ bmask,
-                                unsigned long long *__restrict__ blmb, int *__restrict__ ipos,
-                                unsigned char *__restrict__ ipm, int *__restrict__ err) {
+__global__ void k_tet_plan_cols(int N, const int *__restrict__ rp, const int *__restrict__ ci,
+                                const int *__restrict__ n2e_ptr, const int *__restrict__ bptr,
+                                const unsigned char *__restrict__ hasdiag, int *__restrict__ bcol,
+                                int *__restrict__ ipos, unsigned char *__restrict__ ipm, int *__restrict__ err) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   int j[DPN], e[DPN];
@@ -470,32 +471,17 @@ __global__ void k_tet_plan_fill(int N, const int *__restrict__ rp, const int *__
     j[a] = rp[n * DPN + a];
     e[a] = rp[n * DPN + a + 1];
   }
-  const int pa = n2e_ptr[n], np = n2e_ptr[n + 1] - pa;
-  if (np > 32) {
+  if (n2e_ptr[n + 1] - n2e_ptr[n] > 32) {
     atomicExch(err, 1);
     return;
   }
   const int bb = bptr[n], nb = bptr[n + 1] - bb;
-  // the diagonal block is stored first, the others follow in column order
   const int hd = hasdiag[n];
   bool seen = false;
   for (int m, kc = 0; (m = tet_merge_min<DPN>(ci, j, e)) != INT_MAX; kc++) {
     const int k = (m == n) ? 0 : kc + (hd && !seen ? 1 : 0);
     seen |= m == n;
-    unsigned mask = 0u;
-    unsigned long long lmb = 0ull;
-    for (int p = 0; p < np; p++) {
-      const int el = n2e[pa + p] >> 3;
-#pragma unroll
-      for (int lm = 0; lm < 4; lm++)
-        if (conn[(int64_t)el * 4 + lm] == m) {
-          mask |= 1u << p;
-          lmb |= (unsigned long long)lm << (2 * p);
-        }
-    }
-    if (mask == 0u) atomicExch(err, 1);  // an entry no element produces
-    bmask[bb + k] = mask;
-    blmb[bb + k] = lmb;
+    bcol[bb + k] = m;
 #pragma unroll
     for (int a = 0; a < DPN; a++) {
       const int64_t gi = (int64_t)DPN * bb + a * nb + k;
@@ -508,33 +494,88 @@ __global__ void k_tet_plan_fill(int N, const int *__restrict__ rp, const int *__
   }
 }
 
-// Item records in the numeric pass's order, one thread per tile: the tile's
+// For every (node n, pair p) and every local node lm of p's element: the
+// block of n's list holding column node conn[e][lm] (binary search in the
+// ascending part; the node itself is block 0) gets pair bit p and local
+// index lm.  Blocks no element produces keep mask 0 (checked after).
+__global__ void k_tet_plan_pairs(int N, const int *__restrict__ conn, const int *__restrict__ n2e_ptr,
+                                 const int *__restrict__ n2e, const int *__restrict__ bptr,
+                                 const unsigned char *__restrict__ hasdiag, const int *__restrict__ bcol,
+                                 unsigned *__restrict__ bmask, unsigned long long *__restrict__ blmb) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const int pa = n2e_ptr[n], np = n2e_ptr[n + 1] - pa;
+  if (np > 32) return;
+  const int bb = bptr[n], nb = bptr[n + 1] - bb, hd = hasdiag[n];
+  for (int p = 0; p < np; p++) {
+    const int el = n2e[pa + p] >> 3;
+#pragma unroll
+    for (int lm = 0; lm < 4; lm++) {
+      const int x = conn[(int64_t)el * 4 + lm];
+      int k = -1;
+      if (x == n) {
+        if (hd) k = 0;
+      } else {
+        int lo = hd, hi = nb;  // ascending columns in [hd, nb)
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (bcol[bb + mid] < x) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo < nb && bcol[bb + lo] == x) k = lo;
+      }
+      if (k >= 0) {
+        atomicOr(&bmask[bb + k], 1u << p);
+        atomicOr(&blmb[bb + k], (unsigned long long)lm << (2 * p));
+      }
+    }
+  }
+}
+
+// An entry of the pattern that no element produces cannot be assembled here.
+__global__ void k_tet_plan_check(int64_t nblk, const unsigned *__restrict__ bmask, int *__restrict__ err) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b < nblk && bmask[b] == 0u) atomicExch(err, 1);
+}
+
+// Item records in the numeric pass's order, one thread per node: each tile's
 // diagonal blocks (first in each node's block list) in node order, then the
 // off-diagonal blocks in node and column order.  A tile's items occupy the
-// block range of its nodes, [bptr[n0], bptr[n1]).  Each off-diagonal block's
-// contributing pairs j (ascending) become LDS element offsets
-// slot * DPN * S + kb + j (S = TP + 1), slot = the column node's slot in pair j's
-// strip (tet_slot with the pair's own local index, n2e & 7).
+// block range of its nodes, [bptr[n0], bptr[n1]); a node's places in it come
+// from dpre, the exclusive prefix of the diagonal flags over all nodes.  Each
+// off-diagonal block's contributing pairs j (ascending) become LDS element
+// offsets slot * DPN * S + kb + j (S = TP + 1), slot = the column node's slot
+// in pair j's strip (tet_slot with the pair's own local index, n2e & 7).
 template <int DPN>
-__global__ void k_tet_plan_items(int ntiles, const int *__restrict__ tiles, const int *__restrict__ tpa,
+__global__ void k_tet_plan_items(int N, int ntiles, const int *__restrict__ tiles, const int *__restrict__ tpa,
                                  const int *__restrict__ n2e_ptr, const int *__restrict__ n2e,
                                  const int *__restrict__ rp, const int *__restrict__ bptr,
-                                 const unsigned char *__restrict__ hasdiag, const unsigned *__restrict__ bmask,
-                                 const unsigned long long *__restrict__ blmb, const int *__restrict__ bpos,
-                                 const unsigned char *__restrict__ bpm, int64_t nitems, int *__restrict__ tib,
-                                 int *__restrict__ tnd, unsigned *__restrict__ woff, unsigned *__restrict__ ipos,
-                                 unsigned *__restrict__ ipmk, int S) {
-  const int tile = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tile >= ntiles) return;
+                                 const unsigned char *__restrict__ hasdiag, const int *__restrict__ dpre,
+                                 const unsigned *__restrict__ bmask, const unsigned long long *__restrict__ blmb,
+                                 const int *__restrict__ bpos, const unsigned char *__restrict__ bpm, int64_t nitems,
+                                 int *__restrict__ tib, int *__restrict__ tnd, unsigned *__restrict__ woff,
+                                 unsigned *__restrict__ ipos, unsigned *__restrict__ ipmk, int S) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  // the tile holding n: the last tile start <= n
+  int lo = 0, hi = ntiles;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tiles[mid] <= n) lo = mid;
+    else hi = mid;
+  }
+  const int tile = lo;
   const int n0 = tiles[tile], n1 = tiles[tile + 1];
   const int b0 = bptr[n0];
-  tib[tile] = b0;
-  if (tile == ntiles - 1) tib[ntiles] = bptr[n1];
-  int nd = 0;
-  for (int n = n0; n < n1; n++) nd += hasdiag[n];
-  tnd[tile] = nd;
-  int d = 0, o = 0;
-  for (int n = n0; n < n1; n++) {
+  const int nd = dpre[n1] - dpre[n0];
+  if (n == n0) {
+    tib[tile] = b0;
+    tnd[tile] = nd;
+    if (tile == ntiles - 1) tib[ntiles] = bptr[n1];
+  }
+  const int d = dpre[n] - dpre[n0];             // diagonal blocks of the tile's nodes before n
+  const int o = (bptr[n] - b0) - d;             // off-diagonal blocks before n
+  {
     const int bb = bptr[n], nb = bptr[n + 1] - bb, hd = hasdiag[n];
     const int pn = n2e_ptr[n], npn = n2e_ptr[n + 1] - pn;
     const unsigned kb = (unsigned)(pn - tpa[tile]);
@@ -550,10 +591,10 @@ __global__ void k_tet_plan_items(int ntiles, const int *__restrict__ tiles, cons
         unsigned long long sl = 0ull;
         int c = 0;
         for (unsigned mm = mask; mm; mm &= mm - 1) {
-          const int j = __ffs((int)mm) - 1;
-          const int slot = tet_slot((int)((lmb >> (2 * j)) & 3ull), n2e[pn + j] & 7);
-          sl |= (unsigned long long)slot << (2 * j);
-          if (cnt <= (unsigned)kTetMaxC) w[c >> 1] |= ((unsigned)(slot * DPN * S) + kb + (unsigned)j) << (16 * (c & 1));
+          const int jj = __ffs((int)mm) - 1;
+          const int slot = tet_slot((int)((lmb >> (2 * jj)) & 3ull), n2e[pn + jj] & 7);
+          sl |= (unsigned long long)slot << (2 * jj);
+          if (cnt <= (unsigned)kTetMaxC) w[c >> 1] |= ((unsigned)(slot * DPN * S) + kb + (unsigned)jj) << (16 * (c & 1));
           c++;
         }
         if (cnt > (unsigned)kTetMaxC) {
@@ -575,24 +616,23 @@ __global__ void k_tet_plan_items(int ntiles, const int *__restrict__ tiles, cons
       ipos[item] = prw;
       ipmk[item] = pmw;
     }
-    d += hd;
-    o += nb - hd;
   }
+}
+
+// dpre = exclusive prefix of the diagonal flags (input for the scan).
+__global__ void k_tet_flags(int N, const unsigned char *__restrict__ hasdiag, int *__restrict__ f) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) f[n] = hasdiag[n];
 }
 
 // Tile schedule: consecutive nodes with <= TP pairs, <= kTetNodes nodes
 // and nodes x longest row <= kTetStage (the staging buffer); rows longer
 // than 255 entries, or nodes in more than 32 elements, leave the fast path.
-static int tet_tiles(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, int TP, TetPlan *tp) {
+static int tet_tiles(tfm_ctx *ctx, const tfm_mesh *mesh, const std::vector<unsigned short> &rw, int TP,
+                     TetPlan *tp) {
   const int64_t N = mesh->n_nodes;
   const std::vector<int> &p = mesh->h_n2e_ptr;
-  std::vector<int> rp((size_t)(N * dpn + 1));
-  TFM_CHECK_HIP(ctx, hipMemcpy(rp.data(), d_rp, sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
-  auto rowmax = [&](int n) {
-    int w = 0;
-    for (int a = 0; a < dpn; a++) w = std::max(w, rp[(size_t)n * dpn + a + 1] - rp[(size_t)n * dpn + a]);
-    return w;
-  };
+  auto rowmax = [&](int n) { return (int)rw[(size_t)n]; };
   std::vector<int> t, tw;
   t.push_back(0);
   int start = 0, w = 0;
@@ -632,21 +672,34 @@ static int tet_tiles(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_r
 // index bits, first present entry and presence masks per DOF row.
 struct TetBlocks {
   int *bptr = nullptr;
+  int *dpre = nullptr;  // exclusive prefix of hasdiag over the nodes (N + 1)
   unsigned char *hasdiag = nullptr, *bpm = nullptr;
   unsigned *bmask = nullptr;
   unsigned long long *blmb = nullptr;
   int *bpos = nullptr;
   int64_t nblk = 0;
+  std::vector<unsigned short> rowmax;  // host: longest row per node (saturated at 65535), for the tile schedule
 };
 
 static void tet_blocks_destroy(TetBlocks *B) {
   dfree(B->bptr);
+  dfree(B->dpre);
   dfree(B->hasdiag);
   dfree(B->bmask);
   dfree(B->blmb);
   dfree(B->bpos);
   dfree(B->bpm);
+  B->rowmax.clear();
   *B = TetBlocks{};
+}
+
+// Longest of node n's DPN rows, saturated at 65535 (the tile limit is 255).
+__global__ void k_tet_rowmax(int N, int dpn, const int *__restrict__ rp, unsigned short *__restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  int w = 0;
+  for (int a = 0; a < dpn; a++) w = max(w, rp[n * dpn + a + 1] - rp[n * dpn + a]);
+  out[n] = (unsigned short)min(w, 65535);
 }
 
 static int tet_blocks_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const int *d_rp, const int *d_ci,
@@ -654,10 +707,12 @@ static int tet_blocks_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const 
   const int64_t N = mesh->n_nodes;
   hipStream_t st = ctx->stream;
   int rc;
-  int *cnt = nullptr, *err = nullptr;
+  int *cnt = nullptr, *err = nullptr, *bcol = nullptr;
   auto fail_out = [&](int code) {
+    (void)hipStreamSynchronize(st);
     dfree(cnt);
     dfree(err);
+    dfree(bcol);
     tet_blocks_destroy(B);
     return code;
   };
@@ -672,24 +727,47 @@ static int tet_blocks_create(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const 
       hipLaunchKernelGGL(k_tet_plan_count<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, cnt, B->hasdiag);
   }
   if ((rc = exclusive_scan_i32(ctx, cnt, B->bptr, N, &B->nblk))) return fail_out(rc);
+  // the diagonal flags' prefix (cnt is free again)
+  int64_t ndiag = 0;
+  if ((rc = dalloc(ctx, &B->dpre, N + 1))) return fail_out(rc);
+  if (N > 0) hipLaunchKernelGGL(k_tet_flags, dim3(g), dim3(256), 0, st, (int)N, (const unsigned char *)B->hasdiag, cnt);
+  if ((rc = exclusive_scan_i32(ctx, cnt, B->dpre, N, &ndiag))) return fail_out(rc);
+  // per-node longest row for the host's tile schedule (cnt's bytes reused)
+  B->rowmax.resize((size_t)N);
+  if (N > 0) {
+    hipLaunchKernelGGL(k_tet_rowmax, dim3(g), dim3(256), 0, st, (int)N, dpn, d_rp, (unsigned short *)cnt);
+    TFM_CHECK_HIP(ctx, hipMemcpyAsync(B->rowmax.data(), cnt, sizeof(unsigned short) * (size_t)N,
+                                      hipMemcpyDeviceToHost, st));
+    TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+  }
   const int64_t nblk = B->nblk;
   if ((rc = dalloc(ctx, &B->bmask, nblk)) || (rc = dalloc(ctx, &B->blmb, nblk)) ||
       (rc = dalloc(ctx, &B->bpos, nblk * dpn)) || (rc = dalloc(ctx, &B->bpm, nblk * dpn)))
     return fail_out(rc);
   TFM_CHECK_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), st));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(B->bmask, 0, sizeof(unsigned) * std::max<int64_t>(nblk, 1), st));
+  TFM_CHECK_HIP(ctx, hipMemsetAsync(B->blmb, 0, sizeof(unsigned long long) * std::max<int64_t>(nblk, 1), st));
   if (N > 0) {
+    if ((rc = dalloc(ctx, &bcol, std::max<int64_t>(nblk, 1)))) return fail_out(rc);
     if (dpn == 1)
-      hipLaunchKernelGGL(k_tet_plan_fill<1>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_conn,
-                         mesh->d_n2e_ptr, mesh->d_n2e, B->bptr, B->hasdiag, B->bmask, B->blmb, B->bpos, B->bpm, err);
+      hipLaunchKernelGGL(k_tet_plan_cols<1>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_n2e_ptr, B->bptr,
+                         B->hasdiag, bcol, B->bpos, B->bpm, err);
     else
-      hipLaunchKernelGGL(k_tet_plan_fill<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_conn,
-                         mesh->d_n2e_ptr, mesh->d_n2e, B->bptr, B->hasdiag, B->bmask, B->blmb, B->bpos, B->bpm, err);
+      hipLaunchKernelGGL(k_tet_plan_cols<3>, dim3(g), dim3(256), 0, st, (int)N, d_rp, d_ci, mesh->d_n2e_ptr, B->bptr,
+                         B->hasdiag, bcol, B->bpos, B->bpm, err);
+    hipLaunchKernelGGL(k_tet_plan_pairs, dim3(g), dim3(256), 0, st, (int)N, mesh->d_conn, mesh->d_n2e_ptr,
+                       mesh->d_n2e, B->bptr, B->hasdiag, (const int *)bcol, B->bmask, B->blmb);
+    if (nblk > 0)
+      hipLaunchKernelGGL(k_tet_plan_check, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st, nblk,
+                         (const unsigned *)B->bmask, err);
+    TFM_CHECK_HIP(ctx, hipGetLastError());
   }
   int herr = 0;
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
   dfree(cnt);
   dfree(err);
+  dfree(bcol);
   if (herr) {
     tet_blocks_destroy(B);
     return fail(ctx, TFM_E_UNSUPPORTED, "pattern not covered by the tet plan");
@@ -704,7 +782,7 @@ static int tet_plan_from_blocks(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, con
   int rc;
   if (tile_pairs != 192 && tile_pairs != 256) return fail(ctx, TFM_E_INVALID, "tet tile size must be 192 or 256 pairs");
   tp->tp = tile_pairs;
-  if ((rc = tet_tiles(ctx, mesh, dpn, d_rp, tile_pairs, tp))) return rc;
+  if ((rc = tet_tiles(ctx, mesh, B.rowmax, tile_pairs, tp))) return rc;
   const int64_t nblk = B.nblk;
   tp->nitems = nblk;
   if ((rc = dalloc(ctx, &tp->d_tib, (int64_t)tp->ntiles + 1)) || (rc = dalloc(ctx, &tp->d_tnd, (int64_t)tp->ntiles)) ||
@@ -712,15 +790,16 @@ static int tet_plan_from_blocks(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, con
       (rc = dalloc(ctx, &tp->d_pmk, nblk)))
     return rc;
   if (mesh->n_nodes > 0 && tp->ntiles > 0) {
-    const unsigned gt = (unsigned)((tp->ntiles + 255) / 256);
+    const int N = (int)mesh->n_nodes;
+    const unsigned gn = (unsigned)((N + 255) / 256);
     if (dpn == 1)
-      hipLaunchKernelGGL(k_tet_plan_items<1>, dim3(gt), dim3(256), 0, st, tp->ntiles, tp->d_tiles, tp->d_tpa,
-                         mesh->d_n2e_ptr, mesh->d_n2e, d_rp, B.bptr, B.hasdiag, B.bmask, B.blmb, B.bpos, B.bpm, nblk,
-                         tp->d_tib, tp->d_tnd, tp->d_off, tp->d_pos, tp->d_pmk, tile_pairs + 1);
+      hipLaunchKernelGGL(k_tet_plan_items<1>, dim3(gn), dim3(256), 0, st, N, tp->ntiles, tp->d_tiles, tp->d_tpa,
+                         mesh->d_n2e_ptr, mesh->d_n2e, d_rp, B.bptr, B.hasdiag, B.dpre, B.bmask, B.blmb, B.bpos,
+                         B.bpm, nblk, tp->d_tib, tp->d_tnd, tp->d_off, tp->d_pos, tp->d_pmk, tile_pairs + 1);
     else
-      hipLaunchKernelGGL(k_tet_plan_items<3>, dim3(gt), dim3(256), 0, st, tp->ntiles, tp->d_tiles, tp->d_tpa,
-                         mesh->d_n2e_ptr, mesh->d_n2e, d_rp, B.bptr, B.hasdiag, B.bmask, B.blmb, B.bpos, B.bpm, nblk,
-                         tp->d_tib, tp->d_tnd, tp->d_off, tp->d_pos, tp->d_pmk, tile_pairs + 1);
+      hipLaunchKernelGGL(k_tet_plan_items<3>, dim3(gn), dim3(256), 0, st, N, tp->ntiles, tp->d_tiles, tp->d_tpa,
+                         mesh->d_n2e_ptr, mesh->d_n2e, d_rp, B.bptr, B.hasdiag, B.dpre, B.bmask, B.blmb, B.bpos,
+                         B.bpm, nblk, tp->d_tib, tp->d_tnd, tp->d_off, tp->d_pos, tp->d_pmk, tile_pairs + 1);
   }
   TFM_CHECK_HIP(ctx, hipGetLastError());
   return TFM_OK;
