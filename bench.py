@@ -99,6 +99,60 @@ def cpu_baseline(n, seed=7):
     return len(conn) / dt, len(conn), dt
 
 
+def cpu_baseline_c1(reps=20):
+    """C1's reference pipeline on one core: solve_poisson on the 8^3 box
+    (assemble, to_csr, reduce_system, dense LU), mean wall time of `reps`
+    calls, and the solution."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    coords, conn = O.box_mesh([0, 0, 0], [1, 1, 1], (8, 8, 8))
+    bcs = [(int(v), 0.0) for v in sorted({int(v) for a in range(3) for c in (0.0, 1.0)
+                                          for v in O.nodes_on_plane(coords, a, c, 1e-9)})]
+    with one_core():
+        u = O.solve_poisson(coords, O.TET, conn, 1.0, 2, 1.0, bcs)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            O.solve_poisson(coords, O.TET, conn, 1.0, 2, 1.0, bcs)
+        dt = (time.perf_counter() - t0) / reps
+    return dt, u
+
+
+def bench_c1(args, T, log, reps=20):
+    """The CPU-runnable configuration: Poisson -lap u = 1 on the unit cube,
+    box_mesh 8^3 x 6 P1 tets, k = 1, order 2, u = 0 on the six faces, FP64
+    assembly and the direct (dense LU) solve, as one solve_poisson call on
+    the GPU; checked against the discrete solution's closed form (DST-I:
+    u_max = 0.05491766911624052 at node (4, 4, 4))."""
+    ctx = T.Context(0)
+    mesh = T.Mesh.box(ctx, [0.0, 0.0, 0.0], [1.0, 1.0, 1.0], (8, 8, 8))
+    bcs = [(int(v), 0.0) for v in sorted({int(v) for a in range(3) for c in (0.0, 1.0)
+                                          for v in mesh.nodes_on_plane(a, c, 1e-9)})]
+    u = T.solve_poisson(mesh, 1.0, 2, 1.0, bcs, mode=T.SOLVE_DIRECT)  # warm-up (code objects)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        u = T.solve_poisson(mesh, 1.0, 2, 1.0, bcs, mode=T.SOLVE_DIRECT)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    umax = 0.05491766911624052
+    rec = {"workload": "Poisson box_mesh 8^3 x6 P1 tets, k 1, order 2, u = 0 on all faces, dense LU",
+           "n_elem": mesh.n_elem, "n_nodes": mesh.n_nodes,
+           "gpu_solve_poisson_ms": dt * 1e3, "u_max": float(u.max()),
+           "u_max_rel_err_vs_closed_form": abs(float(u.max()) - umax) / umax}
+    log(f"c1 solve_poisson (GPU, dense LU): {dt * 1e3:.2f} ms per call, u_max {u.max():.17f}")
+    if not args.skip_cpu:
+        dtc, uc = cpu_baseline_c1(reps)
+        rec["cpu_baseline"] = {"ms": dtc * 1e3, "cores": 1, "kind": "port",
+                               "sample": "oracle solve_poisson (assemble, to_csr, reduce_system, dense LU), the "
+                                         "whole configuration",
+                               "rel_diff_gpu_vs_cpu": float(np.abs(u - uc).max() / np.abs(uc).max())}
+        log(f"c1 oracle solve_poisson (1 core): {dtc * 1e3:.2f} ms per call")
+    mesh.close()
+    ctx.close()
+    return rec
+
+
 def measured_traffic(kernel, n):
     """HBM bytes per launch of `kernel` on the n^3 workload, from the committed
     rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
@@ -628,9 +682,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve", action="store_true")
     ap.add_argument("--no-sub", action="store_true",
-                    help="c3 only: leave out the c2 / c4 / c5 sub-records of the line")
-    ap.add_argument("--workload", choices=["c3", "c2", "c4", "c5", "hex8", "simp"], default="c3",
-                    help="c3: the headline single problem (with c2 / c4 / c5 / hex8 sub-records); c2 / c4 / c5 / hex8 alone; "
+                    help="c3 only: leave out the c1 / c2 / c4 / c5 / hex8 sub-records of the line")
+    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "hex8", "simp"], default="c3",
+                    help="c3: the headline single problem (with c1 / c2 / c4 / c5 / hex8 sub-records); c1 / c2 / c4 / c5 / hex8 alone; "
                          "simp: the batched SIMP loop (own JSON lines)")
     ap.add_argument("--simp-iters", type=int, default=30,
                     help="OC iterations of the simp workload (the reference example's max_iter)")
@@ -655,6 +709,9 @@ def main():
     if args.workload == "simp":
         bench_simp(args, T, log)
         return
+    if args.workload == "c1":
+        print(json.dumps({"metric": "assembly + direct solve", "workload": "c1", **bench_c1(args, T, log)}), flush=True)
+        return
     if args.workload == "c2":
         print(json.dumps({"metric": "assembly + solve", "workload": "c2", **bench_c2(args, T, log)}), flush=True)
         return
@@ -668,6 +725,7 @@ def main():
     if not args.no_sub:
         sub = argparse.Namespace(**vars(args))
         sub.steps, sub.warmup = min(args.steps, 10), min(args.warmup, 2)
+        line["c1"] = bench_c1(sub, T, log)
         line["c2"] = bench_c2(sub, T, log)
         line["c4"] = bench_batch(sub, T, log, "c4", emit=False)
         line["c5"] = bench_batch(sub, T, log, "c5", emit=False)
