@@ -115,6 +115,15 @@ void tfm_mesh_destroy(tfm_mesh *mesh);
  * element entries skipped, columns ascending, nrows = max row + 1. */
 int tfm_assemble(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op,
                  int dofs_per_node, tfm_matrix **out);
+/* assemble(mesh, dpn, op) itself (assembly/src/lib.rs:458-480): the Coo's
+ * triplets in push order -- element order, then local row, then local
+ * column, exact-zero entries skipped (every SIMP entry kept) -- as host rows,
+ * cols, vals.  Two calls: with rows, cols and vals NULL, *count receives the
+ * number of triplets; then *count must hold the buffers' capacity and
+ * receives the number written. */
+int tfm_assemble_coo(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op,
+                     int dofs_per_node, int64_t *count, int64_t *rows, int64_t *cols,
+                     double *vals);
 /* elasticity_lumped_mass(mesh, model, density, quad_order).to_csr()
  * (elasticity/src/lib.rs:432-457): the diagonal CSR whose entry on each DOF
  * of a node is the sum of the node's consistent-mass rows over dim.  */

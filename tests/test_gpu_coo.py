@@ -10,7 +10,8 @@ per-element coefficients, on a 20,000-point random Delaunay mesh (generated
 here with scipy's Qhull from a seeded point set) and on the reference's own
 Delaunay box mesh (`tests/golden/vtk/mesh_gen_box_delaunay.vtk`); the path
 forced on a box mesh equals the tile kernels; a solve on it matches the
-oracle's dense LU.
+oracle's dense LU.  The triplet export (tfm_assemble_coo) is assemble's Coo
+itself: the oracle's triplet list, element by element in push order.
 """
 import os
 
@@ -110,4 +111,63 @@ def test_forced_triplet_path_equals_tile_kernels(ctx, monkeypatch, kind):
     u = trip.solve(T.SOLVE_PCG64, rtol=1e-12)
     u_ref = O.solve_with_dirichlet(O.assemble(coords, conn, ref, dpn), rhs, bcs)
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) <= 1e-10
+    m.close()
+
+
+def _coo_equal(g, r, exact=True):
+    rows, cols, vals = g
+    np.testing.assert_array_equal(rows, r.rows)
+    np.testing.assert_array_equal(cols, r.cols)
+    if exact:
+        assert np.array_equal(vals, r.vals)
+    else:
+        np.testing.assert_allclose(vals, r.vals, rtol=1e-14, atol=0)
+
+
+def test_triplet_export_is_the_reference_coo(ctx):
+    """tfm_assemble_coo against the oracle's assemble (rows, cols and values in
+    push order): the C1 box (30,720 triplets: 49,152 element entries less
+    18,432 exact zeros), jittered P1-tet elasticity with per-element moduli,
+    Q4 plane stress, Hex8 elasticity, SIMP Q4 (every entry pushed, values to
+    the device pow's 1e-14), and an element-free mesh."""
+    import tptfem as T
+
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], (8, 8, 8))
+    c, e = m.download()
+    g = T.assemble_coo(m, 1, T.Operator.poisson(1.0, 2))
+    assert len(g[0]) == 30720
+    _coo_equal(g, O.assemble(c, e, O.make_op(O.TET, 3, 0), 1))
+    m.close()
+
+    c, e = O.box_mesh([0, 0, 0], [1, 1, 1], (3, 4, 2))
+    c = O.jitter_interior(c, (3, 4, 2), 0.2, 3)
+    ye = 10.0 ** np.random.default_rng(3).uniform(-1, 1, len(e))
+    m = T.Mesh.from_arrays(ctx, T.TET, c, e)
+    _coo_equal(T.assemble_coo(m, 3, T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=ye)),
+               O.assemble(c, e, O.make_op(O.TET, 3, 1, coef_e=ye, nu=0.3), 3))
+    m.close()
+
+    qc, qe = O.grid(7, 5, 0.3)
+    qc = qc + np.random.default_rng(4).uniform(-0.05, 0.05, qc.shape)
+    m = T.Mesh.from_arrays(ctx, T.QUAD, qc, qe)
+    _coo_equal(T.assemble_coo(m, 2, T.Operator.elasticity(T.PLANE_STRESS, 3.0, 0.25, 2)),
+               O.assemble(qc, qe, O.make_op(O.QUAD, 2, 1, coef=3.0, nu=0.25, model=T.PLANE_STRESS), 2))
+    rho = np.random.default_rng(1).uniform(0.01, 1.0, len(qe))
+    _coo_equal(T.assemble_coo(m, 2, T.Operator.simp(rho, 0.3, 3.0, 1.0, 1e-9, 1.0)),
+               O.assemble(qc, qe, O.make_op(O.QUAD, 2, 2, rho=rho, penal=3.0, e0=1.0, emin=1e-9,
+                                            ke0=O.simp_ke_unit(0.3, 1.0)), 2), exact=False)
+    m.close()
+
+    from test_gpu_hex_mfma import hex_mesh
+
+    hc, he = hex_mesh((3, 2, 2), 0.2, 6)
+    ye = 10.0 ** np.random.default_rng(6).uniform(-1, 1, len(he))
+    m = T.Mesh.from_arrays(ctx, T.HEX, hc, he)
+    _coo_equal(T.assemble_coo(m, 3, T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=ye)),
+               O.assemble(hc, he, O.make_op(O.HEX, 3, 1, coef_e=ye, nu=0.3), 3))
+    m.close()
+
+    m = T.Mesh.from_arrays(ctx, T.TET, c, np.zeros((0, 4), np.int64))
+    rows, cols, vals = T.assemble_coo(m, 1, T.Operator.poisson(1.0, 2))
+    assert len(rows) == len(cols) == len(vals) == 0
     m.close()

@@ -459,4 +459,75 @@ void coo_plan_destroy(AsmPlan *pl) {
   pl->use_coo = false;
 }
 
+// Triplet t of the export: its row, column and value.
+__global__ void k_coo_export(int64_t n, const unsigned long long *__restrict__ keys, const int64_t *__restrict__ idx,
+                             const double *__restrict__ dense, int64_t *__restrict__ rows, int64_t *__restrict__ cols,
+                             double *__restrict__ vals) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long k = keys[t];
+    rows[t] = (int64_t)(k >> 32);
+    cols[t] = (int64_t)(k & 0xffffffffull);
+    vals[t] = dense[idx[t]];
+  }
+}
+
 }  // namespace tfm
+
+using namespace tfm;
+
+// assemble(mesh, dpn, op) as the reference's Coo: the count pass, its scan
+// (each (element, local node) pair's first triplet), the fill pass (keys and
+// element-major positions in push order) and the FP64 element table.
+extern "C" int tfm_assemble_coo(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
+                                int64_t *count, int64_t *rows, int64_t *cols, double *vals) {
+  if (!ctx || !mesh || !op || !count) return TFM_E_INVALID;
+  const bool want = rows || cols || vals;
+  if (want && !(rows && cols && vals)) return fail(ctx, TFM_E_INVALID, "rows, cols and vals go together");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  AsmPlan pl;
+  OpParams P{};
+  int *cnt = nullptr, *off = nullptr;
+  unsigned long long *keys = nullptr;
+  int64_t *idx = nullptr, *d_rows = nullptr, *d_cols = nullptr;
+  double *dense = nullptr, *d_vals = nullptr;
+  auto done = [&](int code) {
+    (void)hipStreamSynchronize(st);
+    dfree(cnt); dfree(off); dfree(keys); dfree(idx); dfree(d_rows); dfree(d_cols); dfree(dense); dfree(d_vals);
+    plan_destroy(&pl);
+    return code;
+  };
+  if ((rc = plan_tables(ctx, mesh, op, dpn, &pl, &P))) return done(rc);
+  const int64_t npairs = mesh->n_elem * mesh->nn;
+  const int nd = mesh->nn * dpn;
+  if ((rc = dalloc(ctx, &cnt, std::max<int64_t>(npairs, 1))) || (rc = dalloc(ctx, &off, npairs + 1))) return done(rc);
+  if ((rc = coo_launch<C_COUNT>(ctx, &pl, P, cnt, nullptr, nullptr, nullptr, nullptr, nullptr))) return done(rc);
+  int64_t ntrip = 0;
+  if ((rc = exclusive_scan_i32(ctx, cnt, off, npairs, &ntrip))) return done(rc);
+  if (!want) {
+    *count = ntrip;
+    return done(TFM_OK);
+  }
+  if (*count < ntrip) return done(fail(ctx, TFM_E_INVALID, "buffers hold fewer entries than the triplets"));
+  *count = ntrip;
+  if (ntrip == 0) return done(TFM_OK);
+  if ((rc = dalloc(ctx, &keys, ntrip)) || (rc = dalloc(ctx, &idx, ntrip)) ||
+      (rc = dalloc(ctx, &dense, std::max<int64_t>(mesh->n_elem * nd * nd, 1))) || (rc = dalloc(ctx, &d_rows, ntrip)) ||
+      (rc = dalloc(ctx, &d_cols, ntrip)) || (rc = dalloc(ctx, &d_vals, ntrip)))
+    return done(rc);
+  if ((rc = coo_launch<C_FILL>(ctx, &pl, P, nullptr, off, keys, idx, nullptr, nullptr)) ||
+      (rc = coo_launch<C_F64>(ctx, &pl, P, nullptr, nullptr, nullptr, nullptr, dense, nullptr)))
+    return done(rc);
+  hipLaunchKernelGGL(k_coo_export, dim3((unsigned)std::min<int64_t>((ntrip + 255) / 256, 65536)), dim3(256), 0, st,
+                     ntrip, (const unsigned long long *)keys, (const int64_t *)idx, (const double *)dense, d_rows,
+                     d_cols, d_vals);
+  TFM_CHECK_HIP(ctx, hipGetLastError());
+  if (hipMemcpyAsync(rows, d_rows, sizeof(int64_t) * ntrip, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(cols, d_cols, sizeof(int64_t) * ntrip, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(vals, d_vals, sizeof(double) * ntrip, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return done(fail(ctx, TFM_E_HIP, "triplet download failed"));
+  return done(TFM_OK);
+}
+

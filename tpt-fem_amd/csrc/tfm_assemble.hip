@@ -540,6 +540,21 @@ static int upload_tables(tfm_ctx *ctx, AsmPlan *pl, double **d_tab, int *nq) {
   return TFM_OK;
 }
 
+// A plan's operator tables without a pattern: what the triplet export needs.
+int plan_tables(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn, AsmPlan *pl, OpParams *P) {
+  int rc;
+  if ((rc = validate_operator(ctx, mesh, op, dpn))) return rc;
+  pl->mesh = mesh;
+  mesh_retain(mesh);  // released by plan_destroy
+  pl->dpn = dpn;
+  pl->ndof = mesh->n_nodes * dpn;
+  if (pl->ndof >= INT32_MAX) return fail(ctx, TFM_E_UNSUPPORTED, "too many DOFs for int32 indexing");
+  if ((rc = plan_set_coefficients(ctx, pl, op))) return rc;
+  if ((rc = upload_tables(ctx, pl, &pl->d_tab, &pl->nq))) return rc;
+  *P = make_params(pl, pl->d_tab, pl->nq, pl->d_tab + (size_t)pl->nq * kRuleStride);
+  return TFM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // SIMP grids (tfm_mesh_grid_batch: P copies of an nx x ny node grid, Q4
 // element (ei, ej) of design p = p (nx-1)(ny-1) + ej (nx-1) + ei with nodes

@@ -210,6 +210,7 @@ int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const TetPlan &tp, 
 int plan_create(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn,
                 const unsigned char *h_fixed, int vals, AsmPlan *plan);
 int plan_set_coefficients(tfm_ctx *ctx, AsmPlan *plan, const tfm_operator *op);
+int plan_tables(tfm_ctx *ctx, const tfm_mesh *mesh, const tfm_operator *op, int dpn, AsmPlan *plan, OpParams *P);
 int plan_numeric(tfm_ctx *ctx, const AsmPlan *plan, double *d_v64, float *d_v32);
 int coo_plan_create(tfm_ctx *ctx, AsmPlan *plan, const OpParams &P);
 int coo_numeric(tfm_ctx *ctx, const AsmPlan *plan, const OpParams &P, double *v64, float *v32);

@@ -116,6 +116,7 @@ _SIGS = {
     "tfm_mesh_nodes_on_plane": (INT, [P, INT, D, D, P, P]),
     "tfm_mesh_destroy": (None, [P]),
     "tfm_assemble": (INT, [P, P, C.POINTER(Operator), INT, PP]),
+    "tfm_assemble_coo": (INT, [P, P, C.POINTER(Operator), INT, C.POINTER(I64), P, P, P]),
     "tfm_lumped_mass": (INT, [P, P, D, INT, PP]),
     "tfm_matrix_create": (INT, [P, I64, I64, P, P, P, PP]),
     "tfm_matrix_info": (INT, [P, P, P, P]),
@@ -399,6 +400,19 @@ def assemble(mesh, dofs_per_node, op):
     h = C.c_void_p()
     mesh.ctx.check(_lib.tfm_assemble(mesh.ctx.h, mesh.h, C.byref(op), dofs_per_node, C.byref(h)))
     return Matrix(mesh.ctx, h)
+
+
+def assemble_coo(mesh, dofs_per_node, op):
+    """assemble(mesh, dpn, op) as the reference's Coo (crates/tpt-fem-assembly/src/lib.rs:458-480):
+    (rows, cols, vals) in push order."""
+    n = I64(0)
+    mesh.ctx.check(_lib.tfm_assemble_coo(mesh.ctx.h, mesh.h, C.byref(op), dofs_per_node, C.byref(n), None, None,
+                                         None))
+    rows, cols, vals = np.zeros(max(1, n.value), np.int64), np.zeros(max(1, n.value), np.int64), np.zeros(max(1, n.value))
+    mesh.ctx.check(_lib.tfm_assemble_coo(mesh.ctx.h, mesh.h, C.byref(op), dofs_per_node, C.byref(n), _ptr(rows),
+                                         _ptr(cols), _ptr(vals)))
+    k = n.value
+    return rows[:k], cols[:k], vals[:k]
 
 
 # ---- loads ------------------------------------------------------------------
