@@ -31,6 +31,12 @@ namespace tfm {
 constexpr int kBW = 4;         // waves (= slices) per block
 constexpr double kFloor2 = 1e-12;  // precision form: FP32 residual floor (squared, relative to the last replacement)
 constexpr int kBT = 64 * kBW;  // threads per block
+constexpr int kScalarBlocks = 512;  // blocks of one problem beyond which its CG scalars are formed once
+// TFM_SCALAR_BLOCKS=<n> moves that threshold (both modes give the same values).
+static int scalar_blocks() {
+  const char *e = getenv("TFM_SCALAR_BLOCKS");
+  return e ? atoi(e) : kScalarBlocks;
+}
 
 // Partial-sum slots, each [nblk][NR]: p.q; r.z and r.r by iteration parity;
 // refinement residual ||f - K u||^2, ||f||^2, f.u.
@@ -39,7 +45,8 @@ enum { Q_PQ = 0, Q_RZ0, Q_RZ1, Q_RR0, Q_RR1, Q_RES, Q_FF, Q_FU, Q_N };
 struct Seg {
   double rr, rr0;          // latest recurrence ||r||^2 and the initial one
   double rt, ff, fu;       // refinement residual: ||f - K u||^2, ||f||^2, f.u
-  double beta;             // multigrid mode: this iteration's beta
+  double beta;             // scalar mode: this iteration's beta
+  double alpha;            // scalar mode: this iteration's alpha
   int done, iters, breakdown, pad;
 };
 
@@ -152,12 +159,28 @@ __global__ __launch_bounds__(kBT) void kbx_init(BView V, const VT *__restrict__ 
 // Iteration i (parity e): convergence of each segment, beta, p_i = z_i +
 // beta p_{i-1} on the fly, q = K p_i on the node-blocked slices (one lane per
 // block row, NR right-hand sides at once), partial p.q.
-template <typename VT, int BS, int NR>
+template <typename VT, int BS, int NR, bool SC>
 __global__ __launch_bounds__(kBT) void kbx_dir(BView V, const VT *__restrict__ bv, const VT *__restrict__ z,
                                                const VT *__restrict__ pold, VT *__restrict__ pnew,
                                                VT *__restrict__ q, int e) {
   const int b = blockIdx.x, pr = V.blk_prob[b];
   Seg *S = V.seg + (size_t)pr * NR;
+  VT beta[NR];
+  if constexpr (SC) {
+    // scalar mode: kbx_beta has decided the stop test and beta
+    __shared__ int s_any;
+    __shared__ double s_beta[NR];
+    if (threadIdx.x == 0) {
+      int a = 0;
+      for (int kk = 0; kk < NR; kk++) a |= !S[kk].done;
+      s_any = a;
+    }
+    if (threadIdx.x < NR) s_beta[threadIdx.x] = S[threadIdx.x].done ? 0.0 : S[threadIdx.x].beta;
+    __syncthreads();
+    if (!s_any) return;  // uniform per block
+#pragma unroll
+    for (int k = 0; k < NR; k++) beta[k] = (VT)s_beta[k];
+  } else {
   __shared__ int s_it, s_done[NR];
   if (threadIdx.x == 0) s_it = *V.it;
   if (threadIdx.x < NR) s_done[threadIdx.x] = S[threadIdx.x].done;
@@ -168,7 +191,6 @@ __global__ __launch_bounds__(kBT) void kbx_dir(BView V, const VT *__restrict__ b
   prob_sum<NR>(V, e ? Q_RZ0 : Q_RZ1, pr, rzn);
   prob_sum<NR>(V, e ? Q_RZ1 : Q_RZ0, pr, rzo);
   const bool first = b == V.prob_blk0[pr] && threadIdx.x == 0;
-  VT beta[NR];
   bool any = false;
 #pragma unroll
   for (int k = 0; k < NR; k++) {
@@ -190,6 +212,7 @@ __global__ __launch_bounds__(kBT) void kbx_dir(BView V, const VT *__restrict__ b
     }
   }
   if (!any) return;  // uniform per block
+  }
   constexpr int BB = BS * BS;
   const int lane = threadIdx.x & 63;
   const int s = V.blk_s0[b] + (threadIdx.x >> 6);
@@ -262,6 +285,30 @@ __global__ __launch_bounds__(kBT) void kbx_beta(BView V, int e) {
     if (d) {
       S[k].done = 1;
       S[k].iters = it;
+    }
+  }
+}
+
+// Scalar mode, one block per problem: alpha = r.z / p.q of each live
+// segment (the same fixed-order sums the update's blocks would form), a
+// breakdown (p.q <= 0) ends the segment.
+template <int NR>
+__global__ __launch_bounds__(kBT) void kbx_alpha(BView V, int e) {
+  const int pr = blockIdx.x;
+  Seg *S = V.seg + (size_t)pr * NR;
+  double pq[NR], rz[NR];
+  prob_sum<NR>(V, Q_PQ, pr, pq);
+  prob_sum<NR>(V, e ? Q_RZ0 : Q_RZ1, pr, rz);
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < NR; k++) {
+    S[k].alpha = 0.0;
+    if (S[k].done) continue;
+    if (!(pq[k] > 0.0)) {
+      S[k].breakdown = 1;
+      S[k].done = 1;
+      S[k].iters = *V.it + 1;  // as the update's counter reads once advanced
+    } else {
+      S[k].alpha = rz[k] / pq[k];
     }
   }
 }
@@ -369,8 +416,9 @@ __global__ __launch_bounds__(kBT) void kbx_smooth_rz(BView V, MgOp o, const doub
 
 // alpha = r.z / p.q per segment; x += alpha p, r -= alpha q, z = Dinv r on
 // the active segments; partial r.z, r.r.  Block 0 advances the counter.  With
-// dinv null z is left to a separate preconditioner and kbx_rz.
-template <typename VT, typename XT, int NR>
+// dinv null z is left to a separate preconditioner and kbx_rz.  SC (scalar
+// mode): kbx_alpha has formed alpha and marked breakdowns.
+template <typename VT, typename XT, int NR, bool SC = false>
 __global__ __launch_bounds__(kBT) void kbx_update(BView V, XT *__restrict__ x, VT *__restrict__ r,
                                                   VT *__restrict__ z, const VT *__restrict__ p,
                                                   const VT *__restrict__ q, const VT *__restrict__ dinv, int e) {
@@ -384,11 +432,15 @@ __global__ __launch_bounds__(kBT) void kbx_update(BView V, XT *__restrict__ x, V
 #pragma unroll
   for (int k = 0; k < NR; k++) any |= !s_done[k];
   if (!any) return;  // uniform per block
+  double al[NR];
+  if constexpr (SC) {
+#pragma unroll
+    for (int k = 0; k < NR; k++) al[k] = s_done[k] ? 0.0 : S[k].alpha;
+  } else {
   double pq[NR], rz[NR];
   prob_sum<NR>(V, Q_PQ, pr, pq);
   prob_sum<NR>(V, e ? Q_RZ0 : Q_RZ1, pr, rz);
   const bool first = b == V.prob_blk0[pr] && threadIdx.x == 0;
-  double al[NR];
 #pragma unroll
   for (int k = 0; k < NR; k++) {
     al[k] = 0.0;
@@ -402,6 +454,7 @@ __global__ __launch_bounds__(kBT) void kbx_update(BView V, XT *__restrict__ x, V
     } else {
       al[k] = rz[k] / pq[k];
     }
+  }
   }
   const int k = threadIdx.x % NR;
   double a = al[0];
@@ -786,6 +839,7 @@ __global__ void kb_reset(Seg *seg, int nseg, const int *flags, int *it, int ref_
 // problem, the sliced copy of the system matrix and the device view.
 struct BBatch {
   int P = 0, NR = 1, bs = 1, nblk = 0, nslices = 0;
+  int max_pnb = 0;  // most blocks of one problem
   int64_t nblkst = 0;
   int *blk_prob = nullptr, *blk_s0 = nullptr, *blk_s1 = nullptr, *prob_blk0 = nullptr, *prob_nblk = nullptr;
   int *sb = nullptr, *soff = nullptr, *bcol = nullptr, *map = nullptr;  // map: slot -> CSR position (-1: none)
@@ -854,6 +908,7 @@ static int bb_setup(tfm_system *s, int P, const int64_t *dof_off, int NR, BBatch
   B.bs = bs;
   B.nblk = (int)bp.size();
   B.nslices = (int)sb.size() - 1;
+  B.max_pnb = pnb.empty() ? 0 : *std::max_element(pnb.begin(), pnb.end());
   int rc;
   int *cnt = nullptr, *w = nullptr;
   if ((rc = dalloc(ctx, &B.blk_prob, B.nblk)) || (rc = dalloc(ctx, &B.blk_s0, B.nblk)) ||
@@ -914,14 +969,14 @@ static int bb_values(tfm_system *s, BBatch &B, const VT *src, VT **dst) {
   return TFM_OK;
 }
 
-template <typename VT, int NR>
+template <typename VT, int NR, bool SC = false>
 static void bb_dir(const BBatch &B, const BView &V, hipStream_t st, const VT *bv, const VT *z, const VT *pold,
                    VT *pnew, VT *q, int e) {
   dim3 g(B.nblk), t(kBT);
   switch (B.bs) {
-  case 3: hipLaunchKernelGGL((kbx_dir<VT, 3, NR>), g, t, 0, st, V, bv, z, pold, pnew, q, e); break;
-  case 2: hipLaunchKernelGGL((kbx_dir<VT, 2, NR>), g, t, 0, st, V, bv, z, pold, pnew, q, e); break;
-  default: hipLaunchKernelGGL((kbx_dir<VT, 1, NR>), g, t, 0, st, V, bv, z, pold, pnew, q, e); break;
+  case 3: hipLaunchKernelGGL((kbx_dir<VT, 3, NR, SC>), g, t, 0, st, V, bv, z, pold, pnew, q, e); break;
+  case 2: hipLaunchKernelGGL((kbx_dir<VT, 2, NR, SC>), g, t, 0, st, V, bv, z, pold, pnew, q, e); break;
+  default: hipLaunchKernelGGL((kbx_dir<VT, 1, NR, SC>), g, t, 0, st, V, bv, z, pold, pnew, q, e); break;
   }
 }
 
@@ -1367,9 +1422,13 @@ struct PlainCg {
   const double *mmask = nullptr;
   const int *sdone;
   int sstride;
-  PlainCg(BatchSolver *b_, const tfm_solve_opts &o)
+  // Scalar mode for problems spread over many blocks: the stop test, beta
+  // and alpha formed once per problem (kbx_beta, kbx_alpha) instead of by
+  // every block re-summing all of its problem's partials; same values.
+  bool sc;
+  PlainCg(BatchSolver *b_, const tfm_solve_opts &o, bool scalar)
       : b(b_), st(b_->s->ctx->stream), V(b_->B.view(o.max_iter, o.rtol * o.rtol)), g(b_->B.nblk), t(kBT),
-        jac(b_->d64) {
+        jac(b_->d64), sc(scalar) {
     static_assert(sizeof(Seg) % sizeof(int) == 0, "Seg stride in ints");
     sdone = reinterpret_cast<const int *>(reinterpret_cast<const char *>(b->B.seg) + offsetof(Seg, done));
     sstride = (int)(sizeof(Seg) / sizeof(int));
@@ -1452,10 +1511,19 @@ struct PlainCg {
         hipLaunchKernelGGL((kbx_beta<NR>), dim3(S.P), t, 0, st, V, e);
         hipLaunchKernelGGL((kbx_dir_p<NR>), g, t, 0, st, V, (const double *)S.z64, (const double *)pb[1 - e], pb[e]);
         hipLaunchKernelGGL((kbx_q_mg<NR>), g, t, 0, st, V, mop, mmask, (const double *)pb[e], S.q64);
+      } else if (sc) {
+        hipLaunchKernelGGL((kbx_beta<NR>), dim3(S.P), t, 0, st, V, e);
+        bb_dir<double, NR, true>(B, V, st, B.v64, S.z64, pb[1 - e], pb[e], S.q64, e);
       } else {
         bb_dir<double, NR>(B, V, st, B.v64, S.z64, pb[1 - e], pb[e], S.q64, e);
       }
-      hipLaunchKernelGGL((kbx_update<double, double, NR>), g, t, 0, st, V, S.x, S.r64, S.z64, pb[e], S.q64, jac, e);
+      if (sc) {
+        hipLaunchKernelGGL((kbx_alpha<NR>), dim3(S.P), t, 0, st, V, e);
+        hipLaunchKernelGGL((kbx_update<double, double, NR, true>), g, t, 0, st, V, S.x, S.r64, S.z64, pb[e], S.q64,
+                           jac, e);
+      } else {
+        hipLaunchKernelGGL((kbx_update<double, double, NR>), g, t, 0, st, V, S.x, S.r64, S.z64, pb[e], S.q64, jac, e);
+      }
       if (S.mg || S.mg3) precond(e ? Q_RZ1 : Q_RZ0);
     }
     TFM_CHECK_HIP(S.s->ctx, hipGetLastError());
@@ -1546,8 +1614,10 @@ static int bsolver_solve_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, 
   int rc = TFM_OK;
   if (refresh && (rc = refresh_prepare(b))) return rc;
   const uint64_t tag = hk ? hk->tag : 0;
+  const bool scalar = !b->mixed && B.max_pnb > scalar_blocks();
   const uint64_t key = key_of({(uint64_t)o.mode, (uint64_t)o.max_iter, bits_of(o.rtol), bits_of(o.inner_rtol),
-                               (uint64_t)warm, (uint64_t)refresh, tag, (uint64_t)(B.v32 != nullptr)});
+                               (uint64_t)warm, (uint64_t)refresh, tag, (uint64_t)(B.v32 != nullptr),
+                               (uint64_t)scalar});
   // A: the caller's pre-stage and the refresh
   if ((refresh || (hk && hk->pre)) && (rc = stage_run(ctx, b->sa, key, use_graph, [&]() -> int {
         int r = TFM_OK;
@@ -1585,7 +1655,7 @@ static int bsolver_solve_nr(BatchSolver *b, const tfm_solve_opts &o, bool warm, 
     return TFM_OK;
   };
   if (!b->mixed) {
-    PlainCg<NR> cg(b, o);
+    PlainCg<NR> cg(b, o, scalar);
     rc = drive(cg, B.hseg, sizeof(Seg) * nseg, B.ev_seg, B.seg, [&]() {
       for (int q = 0; q < nseg; q++)
         if (!B.hseg[q].done) return false;
