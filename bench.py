@@ -200,9 +200,10 @@ def pcg_iter_bytes(nnz, n, sv):
 
 
 def timed_assembly(ctx, s, steps, warmup):
-    """Mean event-timed FP64 and FP32 numeric passes (ms) of a system."""
+    """Mean event-timed numeric passes (ms) of a system: FP64, FP32 and both
+    arrays in one call."""
     out = {}
-    for which, name in ((T_SYS[0], "fp64"), (T_SYS[1], "fp32")):
+    for which, name in ((T_SYS[0], "fp64"), (T_SYS[1], "fp32"), (T_SYS[0] | T_SYS[1], "fp64_fp32")):
         for _ in range(warmup):
             s.assemble(None, which)
         ts = []
@@ -218,7 +219,7 @@ def timed_assembly(ctx, s, steps, warmup):
 def assembly_record(s, mesh, ms, per_elem_coef, cell_dpn):
     """Assembly times, algorithmic GB/s and HBM roofline fraction per pass."""
     info = s.assembly_info()
-    rec = {"fp64_ms": ms["fp64"], "fp32_ms": ms["fp32"], "autotune": info}
+    rec = {"fp64_ms": ms["fp64"], "fp32_ms": ms["fp32"], "fp64_fp32_ms": ms["fp64_fp32"], "autotune": info}
     for name, sv in (("fp64", 8), ("fp32", 4)):
         b = asm_bytes(mesh.n_nodes, mesh.n_elem, s.nnz, s.n_free, sv, per_elem_coef)
         gbs = b / (ms[name] * 1e-3) / 1e9
@@ -561,13 +562,24 @@ def bench_c3(args, T, log):
         ctx.timer_begin()
         sysm.assemble(None, T.SYS_FP32)
         t32.append(ctx.timer_end())
+    # both arrays of the precision form in one call: one pass writing the
+    # FP64 values and their FP32 roundings
+    tb = []
+    for _ in range(args.steps):
+        ctx.timer_begin()
+        sysm.assemble(None, T.SYS_FP64 | T.SYS_FP32)
+        tb.append(ctx.timer_end())
     ms64 = float(np.mean(t64))
     ms32 = float(np.mean(t32))
+    msb = float(np.mean(tb))
     b64 = asm_bytes(mesh.n_nodes, mesh.n_elem, sysm.nnz, sysm.n_free, 8)
     b32 = asm_bytes(mesh.n_nodes, mesh.n_elem, sysm.nnz, sysm.n_free, 4)
     gbs64 = b64 / (ms64 * 1e-3) / 1e9
     gbs32 = b32 / (ms32 * 1e-3) / 1e9
-    log(f"assembly fp64 {ms64:.3f} ms ({gbs64:.0f} GB/s), fp32 {ms32:.3f} ms ({gbs32:.0f} GB/s)")
+    bb = b64 + 4 * sysm.nnz
+    gbsb = bb / (msb * 1e-3) / 1e9
+    log(f"assembly fp64 {ms64:.3f} ms ({gbs64:.0f} GB/s), fp32 {ms32:.3f} ms ({gbs32:.0f} GB/s), "
+        f"both {msb:.3f} ms ({gbsb:.0f} GB/s)")
     sysm.assemble(None, 0)
 
     # ---- solves ------------------------------------------------------------
@@ -652,6 +664,9 @@ def bench_c3(args, T, log):
                      "fp64_GBps": gbs64, "fp32_GBps": gbs32, "autotune": tune,
                      "fp32_roofline": {"kernel": asm_kernel(tune["tile_pairs"], "float"),
                                        "algorithmic_bytes": b32, "frac": gbs32 / PEAK_HBM_GBS},
+                     "fp64_fp32_ms": msb,
+                     "fp64_fp32_roofline": {"kernel": f"tfm::k_asm_tet_dual<3, {tune['tile_pairs']}>",
+                                            "algorithmic_bytes": bb, "GBps": gbsb, "frac": gbsb / PEAK_HBM_GBS},
                      "system_create_s": create_s,
                      "end_to_end_elements_per_s": mesh.n_elem / create_s},
         "solve": solve,

@@ -240,6 +240,10 @@ int tfm_debug_mf_apply(tfm_system *sys, const double *x, double *y);
 /* ReducedSystem {kred (as its to_csr()), fred, free}. */
 int tfm_system_download(const tfm_system *sys, int64_t *row_ptrs, int64_t *col_ind,
                         double *values, double *fred, int64_t *free_dofs);
+/* The FP32 values of a system that holds them, in tfm_system_download's
+ * order (nnz entries, widened to double); TFM_E_INVALID without them.  No
+ * reference counterpart (diagnostic of the precision form's matrix). */
+int tfm_debug_system_values32(const tfm_system *sys, double *values);
 
 /* TFM_SOLVE_PCG64: Jacobi-preconditioned FP64 CG.  TFM_SOLVE_MIXED: FP32 CG
    with FP64 residual refinement (r = f - K u on the FP64 values); its FP32

@@ -572,3 +572,49 @@ def test_tet_fp32_values_close_to_fp64(ctx, kind):
     k32, _, _ = s.download()
     kr, _, _ = O.reduce_system(O.assemble(coords, conn, ref, dpn), np.zeros(len(coords) * dpn), bcs)
     csr_equal(k32, kr.to_csr(), exact=False, rtol=1e-5)
+
+
+@pytest.mark.parametrize("kind", ["poisson", "elasticity"])
+def test_tet_both_arrays_fp32_is_rounded_fp64(ctx, kind):
+    """A system holding both value arrays fills them in one pass: the FP32
+    values are the exact FP64 values rounded to nearest, at creation and
+    after a re-assembly with new coefficients; a re-assembly of the FP32
+    array alone takes the FP32 pass (within 1e-5 of each row's maximum)."""
+    T = tp()
+    n = (6, 4, 5)
+    m = T.Mesh.box(ctx, [0, 0, 0], [1, 1, 1], n)
+    m.jitter(n, 0.2, 12)
+    coords, conn = m.download()
+    rng = np.random.default_rng(9)
+    dpn = 1 if kind == "poisson" else 3
+
+    def make(cf):
+        if kind == "poisson":
+            return T.Operator.poisson(1.0, 2, kappa_e=cf), O.make_op(O.TET, 3, 0, coef_e=cf)
+        return (T.Operator.elasticity(T.CONTINUUM3D, 1.0, 0.3, 2, young_e=cf),
+                O.make_op(O.TET, 3, 1, coef_e=cf, nu=0.3))
+
+    def oracle_csr(ref):
+        kr, _, _ = O.reduce_system(O.assemble(coords, conn, ref, dpn), np.zeros(len(coords) * dpn), bcs)
+        return kr.to_csr()
+
+    bcs = face_bcs(coords, dpn, [(0, 0.0)])
+    op, ref = make(10.0 ** rng.uniform(-1, 1, m.n_elem))
+    s = T.System.create(m, op, dpn, bcs, flags=T.SYS_FP64 | T.SYS_FP32)
+    assert s.assembly_info()["kernel"] == "p1_tet"
+    for step in range(2):
+        k64, _, _ = s.download()
+        csr_equal(k64, oracle_csr(ref))
+        np.testing.assert_array_equal(s.values32(), k64.values.astype(np.float32).astype(np.float64))
+        op, ref = make(10.0 ** rng.uniform(-1, 1, m.n_elem))
+        s.assemble(op, T.SYS_FP64 | T.SYS_FP32)
+    op, ref = make(10.0 ** rng.uniform(-1, 1, m.n_elem))
+    s.assemble(op, T.SYS_FP32)
+    kc = oracle_csr(ref)
+    v32 = s.values32()
+    for i in range(len(kc.row_ptrs) - 1):
+        r0, r1 = kc.row_ptrs[i], kc.row_ptrs[i + 1]
+        if r1 > r0:
+            assert np.abs(v32[r0:r1] - kc.values[r0:r1]).max() <= 1e-5 * np.abs(kc.values[r0:r1]).max(), i
+    s.close()
+    m.close()

@@ -316,15 +316,18 @@ __device__ __forceinline__ void tet_diag(const VT *s_c, const TetBlk &B, int a, 
   if ((pm >> b) & 1u) s_st[tet_stage(B, a, W) + __popc(pm & ((1u << b) - 1u))] = acc;
 }
 
-template <int DPN, typename VT, int TP>
-__global__ __launch_bounds__(TP) void k_asm_tet(const double *__restrict__ coords, const int *__restrict__ conn,
-                                                     const int *__restrict__ n2e, const int *__restrict__ tiles,
-                                                     const int *__restrict__ tpa, const int *__restrict__ tib,
-                                                     const int *__restrict__ tnd, const int *__restrict__ tw,
-                                                     const int *__restrict__ rp, int ntiles, OpParams P,
-                                                     const unsigned *__restrict__ woff,
-                                                     const unsigned *__restrict__ ipos, const unsigned *__restrict__ ipmk,
-                                                     int64_t nitems, TetLame lame, VT *__restrict__ vals) {
+// One tile of the numeric pass.  X32 (FP64 pass only): the copy-out also
+// writes every value rounded to FP32 into v32, the FP32 array of a system
+// that holds both.
+template <int DPN, typename VT, int TP, bool X32>
+__device__ __forceinline__ void tet_pass(const double *__restrict__ coords, const int *__restrict__ conn,
+                                         const int *__restrict__ n2e, const int *__restrict__ tiles,
+                                         const int *__restrict__ tpa, const int *__restrict__ tib,
+                                         const int *__restrict__ tnd, const int *__restrict__ tw,
+                                         const int *__restrict__ rp, int ntiles, const OpParams &P,
+                                         const unsigned *__restrict__ woff, const unsigned *__restrict__ ipos,
+                                         const unsigned *__restrict__ ipmk, int64_t nitems, const TetLame &lame,
+                                         VT *__restrict__ vals, float *__restrict__ v32) {
   constexpr int S = TP + 1;
   // one DOF row of every pair: s_c[(slot * DPN + b) * S + pair]
   __shared__ VT s_c[4 * DPN * S];
@@ -416,10 +419,43 @@ __global__ __launch_bounds__(TP) void k_asm_tet(const double *__restrict__ coord
       if (t < nst) {
         const int i = (int)(((unsigned)t * wm) >> 20), j = t - i * W;
         const int g = s_rp[DPN * i + a], len = s_rp[DPN * i + a + 1] - g;
-        if (j < len) vals[g + j] = s_st[t];
+        if (j < len) {
+          vals[g + j] = s_st[t];
+          if constexpr (X32) v32[g + j] = (float)s_st[t];
+        }
       }
     }
   }
+}
+
+template <int DPN, typename VT, int TP>
+__global__ __launch_bounds__(TP) void k_asm_tet(const double *__restrict__ coords, const int *__restrict__ conn,
+                                                const int *__restrict__ n2e, const int *__restrict__ tiles,
+                                                const int *__restrict__ tpa, const int *__restrict__ tib,
+                                                const int *__restrict__ tnd, const int *__restrict__ tw,
+                                                const int *__restrict__ rp, int ntiles, OpParams P,
+                                                const unsigned *__restrict__ woff, const unsigned *__restrict__ ipos,
+                                                const unsigned *__restrict__ ipmk, int64_t nitems, TetLame lame,
+                                                VT *__restrict__ vals) {
+  tet_pass<DPN, VT, TP, false>(coords, conn, n2e, tiles, tpa, tib, tnd, tw, rp, ntiles, P, woff, ipos, ipmk, nitems,
+                               lame, vals, nullptr);
+}
+
+// Both value arrays in one pass: the exact FP64 values and the same values
+// rounded to FP32 (for a system holding both, in place of a separate FP32
+// pass).
+template <int DPN, int TP>
+__global__ __launch_bounds__(TP) void k_asm_tet_dual(const double *__restrict__ coords, const int *__restrict__ conn,
+                                                     const int *__restrict__ n2e, const int *__restrict__ tiles,
+                                                     const int *__restrict__ tpa, const int *__restrict__ tib,
+                                                     const int *__restrict__ tnd, const int *__restrict__ tw,
+                                                     const int *__restrict__ rp, int ntiles, OpParams P,
+                                                     const unsigned *__restrict__ woff,
+                                                     const unsigned *__restrict__ ipos,
+                                                     const unsigned *__restrict__ ipmk, int64_t nitems, TetLame lame,
+                                                     double *__restrict__ vals, float *__restrict__ v32) {
+  tet_pass<DPN, double, TP, true>(coords, conn, n2e, tiles, tpa, tib, tnd, tw, rp, ntiles, P, woff, ipos, ipmk,
+                                  nitems, lame, vals, v32);
 }
 
 // ---- plan -------------------------------------------------------------------
@@ -853,13 +889,27 @@ int tet_numeric(tfm_ctx *ctx, const tfm_mesh *mesh, int dpn, const TetPlan &tp, 
     else                                 \
       TFM_TET(D, VT, 192, V);            \
   } while (0)
-  if (dpn == 1) {
+#define TFM_TET_DUAL(D, TPV)                                                                                      \
+  hipLaunchKernelGGL((k_asm_tet_dual<D, TPV>), g, t, 0, st, mesh->d_coords, mesh->d_conn, mesh->d_n2e, tp.d_tiles, \
+                     tp.d_tpa, tp.d_tib, tp.d_tnd, tp.d_tw, d_rp, tp.ntiles, P, tp.d_off, tp.d_pos, tp.d_pmk,      \
+                     tp.nitems, lame, v64, v32)
+  if (v64 && v32) {
+    // both arrays: the FP64 pass writes the FP32 copy too
+    if (dpn == 1) {
+      if (tp.tp == 256) TFM_TET_DUAL(1, 256);
+      else TFM_TET_DUAL(1, 192);
+    } else {
+      if (tp.tp == 256) TFM_TET_DUAL(3, 256);
+      else TFM_TET_DUAL(3, 192);
+    }
+  } else if (dpn == 1) {
     if (v64) TFM_TET_TP(1, double, v64);
     if (v32) TFM_TET_TP(1, float, v32);
   } else {
     if (v64) TFM_TET_TP(3, double, v64);
     if (v32) TFM_TET_TP(3, float, v32);
   }
+#undef TFM_TET_DUAL
 #undef TFM_TET_TP
 #undef TFM_TET
   TFM_CHECK_HIP(ctx, hipGetLastError());

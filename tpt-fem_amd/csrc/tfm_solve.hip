@@ -1229,19 +1229,21 @@ extern "C" int tfm_system_assembly_info(const tfm_system *s, int *path, int *til
   return TFM_OK;
 }
 
-extern "C" int tfm_system_download(const tfm_system *s, int64_t *row_ptrs, int64_t *col_ind, double *values,
-                                   double *fred, int64_t *free_dofs) {
-  if (!s) return TFM_E_INVALID;
+// The reduced system in reference order; values from the FP32 array when
+// fp32 is set (or the system holds no FP64 values).
+static int system_download(const tfm_system *s, int64_t *row_ptrs, int64_t *col_ind, double *values, double *fred,
+                           int64_t *free_dofs, bool fp32) {
   tfm_ctx *ctx = s->ctx;
   int rc = set_device(ctx);
   if (rc) return rc;
   int64_t n = s->ndof;
+  const bool use32 = fp32 || !s->d_v64;
   std::vector<int> rp(n + 1), ci(s->nnz);
-  std::vector<double> v(s->d_v64 ? s->nnz : 0), f(n);
-  std::vector<float> v32(!s->d_v64 && s->d_v32 ? s->nnz : 0);
+  std::vector<double> v(!use32 ? s->nnz : 0), f(n);
+  std::vector<float> v32(use32 && s->d_v32 ? s->nnz : 0);
   TFM_CHECK_HIP(ctx, hipMemcpy(rp.data(), s->d_rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
   if (s->nnz) TFM_CHECK_HIP(ctx, hipMemcpy(ci.data(), s->d_ci, sizeof(int) * s->nnz, hipMemcpyDeviceToHost));
-  if (s->nnz && s->d_v64) TFM_CHECK_HIP(ctx, hipMemcpy(v.data(), s->d_v64, sizeof(double) * s->nnz, hipMemcpyDeviceToHost));
+  if (s->nnz && v.size()) TFM_CHECK_HIP(ctx, hipMemcpy(v.data(), s->d_v64, sizeof(double) * s->nnz, hipMemcpyDeviceToHost));
   if (s->nnz && v32.size()) TFM_CHECK_HIP(ctx, hipMemcpy(v32.data(), s->d_v32, sizeof(float) * s->nnz, hipMemcpyDeviceToHost));
   TFM_CHECK_HIP(ctx, hipMemcpy(f.data(), s->d_f, sizeof(double) * n, hipMemcpyDeviceToHost));
   std::vector<int64_t> red((size_t)n, -1);
@@ -1252,7 +1254,7 @@ extern "C" int tfm_system_download(const tfm_system *s, int64_t *row_ptrs, int64
     int64_t g = s->free_dofs[q];
     for (int j = rp[g]; j < rp[g + 1]; j++) {
       if (col_ind) col_ind[o] = red[ci[j]];
-      if (values) values[o] = s->d_v64 ? v[j] : (double)v32[j];
+      if (values) values[o] = use32 ? (double)v32[j] : v[j];
       o++;
     }
     if (row_ptrs) row_ptrs[q + 1] = o;
@@ -1260,6 +1262,18 @@ extern "C" int tfm_system_download(const tfm_system *s, int64_t *row_ptrs, int64
     if (free_dofs) free_dofs[q] = g;
   }
   return TFM_OK;
+}
+
+extern "C" int tfm_system_download(const tfm_system *s, int64_t *row_ptrs, int64_t *col_ind, double *values,
+                                   double *fred, int64_t *free_dofs) {
+  if (!s) return TFM_E_INVALID;
+  return system_download(s, row_ptrs, col_ind, values, fred, free_dofs, false);
+}
+
+extern "C" int tfm_debug_system_values32(const tfm_system *s, double *values) {
+  if (!s) return TFM_E_INVALID;
+  if (!s->d_v32) return fail(s->ctx, TFM_E_INVALID, "system holds no FP32 values");
+  return system_download(s, nullptr, nullptr, values, nullptr, nullptr, true);
 }
 
 // Rows per CSR-stream chunk of the system's matrix (from its longest row).

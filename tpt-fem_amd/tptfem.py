@@ -141,6 +141,7 @@ _SIGS = {
     "tfm_system_info": (INT, [P, P, P, P]),
     "tfm_system_assembly_info": (INT, [P, P, P, P, P]),
     "tfm_debug_mf_apply": (INT, [P, P, P]),
+    "tfm_debug_system_values32": (INT, [P, P]),
     "tfm_system_download": (INT, [P, P, P, P, P, P]),
     "tfm_system_solve": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
     "tfm_system_solve_noalloc": (INT, [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]),
@@ -564,6 +565,12 @@ class System:
         nrows = self.n_free
         return (Csr(nrows, ncols, rp, ci[: self.nnz].copy(), v[: self.nnz].copy()),
                 fred[: self.n_free].copy(), free[: self.n_free].copy())
+
+    def values32(self):
+        """The FP32 values (widened), in download()'s order."""
+        v = np.zeros(max(1, self.nnz))
+        self.ctx.check(_lib.tfm_debug_system_values32(self.h, _ptr(v)))
+        return v[: self.nnz].copy()
 
     def solve(self, mode=SOLVE_PCG64, rtol=1e-12, max_iter=100000, inner_rtol=0.3):
         """Single-system solve.  Mixed mode: inner_rtol is the FP32 residual
