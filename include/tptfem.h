@@ -210,7 +210,9 @@ int tfm_system_from_matrix(tfm_ctx *ctx, const tfm_matrix *k, int64_t n, const d
 /* Numeric re-assembly on the existing pattern (the hot path).  op may carry
  * new coefficients (coef_e, rho), or be NULL to reuse the device-resident
  * ones.  which: TFM_SYS_FP64 and/or TFM_SYS_FP32 value arrays to refill (0 =
- * every array the system holds).
+ * every array the system holds).  Refilling both on the P1-tet kernel is one
+ * pass whose FP32 values are the FP64 values rounded to nearest; the FP32
+ * array alone takes the FP32 pass (FP64 geometry, FP32 element arithmetic).
  * The pattern was built from the creation coefficients: an element whose
  * coefficient was exactly 0 then has no entries in it, so giving it a
  * non-zero coefficient returns TFM_E_INVALID (create the system again); a
