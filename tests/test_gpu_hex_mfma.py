@@ -102,6 +102,8 @@ def test_hex_table_route_fp64_bitexact(ctx, order, monkeypatch):
     np.testing.assert_array_equal(K.row_ptrs, kr.row_ptrs)
     np.testing.assert_array_equal(K.col_ind, kr.col_ind)
     assert np.array_equal(K.values, kr.values)
+    # both arrays in one sum: the FP32 values are the FP64 values rounded
+    np.testing.assert_array_equal(s.values32(), K.values.astype(np.float32).astype(np.float64))
     # the tile kernels give the same FP64 values, and the same FP32 sums of the
     # same FP32 element tables
     monkeypatch.setenv("TFM_HEX_TILE", "1")

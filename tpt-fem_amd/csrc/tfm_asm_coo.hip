@@ -184,6 +184,22 @@ __global__ void k_coo_sum(int64_t nnz, const int *__restrict__ seg, const IT *__
   out[q] = acc;
 }
 
+// Both value arrays: the FP64 sum and its rounding to FP32.
+template <typename IT>
+__global__ void k_coo_sum_dual(int64_t nnz, const int *__restrict__ seg, const IT *__restrict__ perm,
+                               const double *__restrict__ dense, double *__restrict__ out,
+                               float *__restrict__ out32) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  double acc = 0;
+  for (int k = seg[q]; k < seg[q + 1]; k++) {
+    const double v = dense[perm[k]];
+    if (v != 0.0) acc += v;
+  }
+  out[q] = acc;
+  out32[q] = (float)acc;
+}
+
 // FP64 element matrices of 3-D elasticity on Hex8 cells, the reference's
 // values bit for bit (crates/tpt-fem-elasticity/src/lib.rs:284-307), into the
 // dense table ke[e][24][24] (row (i, a), column (j, b)) the sorted-triplet sum
@@ -418,6 +434,17 @@ static void coo_sum(tfm_ctx *ctx, const AsmPlan *pl, const VT *dense, VT *out) {
                        (const int64_t *)pl->d_coo_perm, dense, out);
 }
 
+static void coo_sum_dual(tfm_ctx *ctx, const AsmPlan *pl, const double *dense, double *out, float *out32) {
+  if (!pl->nnz) return;
+  const dim3 g((unsigned)((pl->nnz + 255) / 256)), b(256);
+  if (pl->d_coo_perm32)
+    hipLaunchKernelGGL((k_coo_sum_dual<int>), g, b, 0, ctx->stream, pl->nnz, pl->d_coo_seg,
+                       (const int *)pl->d_coo_perm32, dense, out, out32);
+  else
+    hipLaunchKernelGGL((k_coo_sum_dual<int64_t>), g, b, 0, ctx->stream, pl->nnz, pl->d_coo_seg,
+                       (const int64_t *)pl->d_coo_perm, dense, out, out32);
+}
+
 int coo_numeric(tfm_ctx *ctx, const AsmPlan *pl, const OpParams &P, double *v64, float *v32) {
   const tfm_mesh *m = pl->mesh;
   const int nd = m->nn * pl->dpn;
@@ -433,6 +460,12 @@ int coo_numeric(tfm_ctx *ctx, const AsmPlan *pl, const OpParams &P, double *v64,
                            dim3(64 * kKeWaves), 0, ctx->stream, m->n_elem, m->d_coords, m->d_conn, P, pl->d_coo_v64);
     } else if ((rc = coo_launch<C_F64>(ctx, pl, P, nullptr, nullptr, nullptr, nullptr, pl->d_coo_v64, nullptr))) {
       return rc;
+    }
+    if (v32) {
+      // both arrays: the FP32 values are the FP64 sums rounded to nearest
+      coo_sum_dual(ctx, pl, pl->d_coo_v64, v64, v32);
+      TFM_CHECK_HIP(ctx, hipGetLastError());
+      return TFM_OK;
     }
     coo_sum<double>(ctx, pl, pl->d_coo_v64, v64);
   }
