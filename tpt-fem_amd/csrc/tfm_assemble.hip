@@ -232,6 +232,7 @@ __global__ __launch_bounds__((TileCfg<CELL, DPN>::TP)) void k_asm_tile(
       s_ord[tn][rank] = (unsigned char)h;
     }
     __syncthreads();
+    int dmax = 0;  // row count (count pass) / column count (fill pass) seen by this thread
     for (int r = tid; r < nt * DPN; r += TP) {
       int tn = r / DPN, a = r % DPN;
       int row = (n0 + tn) * DPN + a;
@@ -256,11 +257,12 @@ __global__ __launch_bounds__((TileCfg<CELL, DPN>::TP)) void k_asm_tile(
       }
       if (MODE == M_COUNT) {
         row_cnt[row] = c;
-        if (c > 0) atomicMax(&dims[0], row + 1);
+        if (c > 0) dmax = max(dmax, row + 1);
       } else if (maxcol >= 0) {
-        atomicMax(&dims[1], maxcol + 1);
+        dmax = max(dmax, maxcol + 1);
       }
     }
+    block_max(dmax, MODE == M_COUNT ? &dims[0] : &dims[1]);
   }
 }
 

@@ -191,20 +191,23 @@ __global__ void k_adj_fill(const int *conn, int64_t n_elem, int nn, const int *p
 
 // Sort each node's list by element id (insertion sort; lists are short).
 // Atomics make the fill order arbitrary; the sort makes it deterministic.
-__global__ void k_adj_sort(const int *ptr, int *n2e, int64_t n_nodes, int *maxadj) {
+__global__ __launch_bounds__(256) void k_adj_sort(const int *ptr, int *n2e, int64_t n_nodes, int *maxadj) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= n_nodes) return;
-  int b = ptr[n], e = ptr[n + 1];
-  for (int i = b + 1; i < e; i++) {
-    int v = n2e[i];
-    int j = i - 1;
-    while (j >= b && n2e[j] > v) {
-      n2e[j + 1] = n2e[j];
-      j--;
+  int len = 0;
+  if (n < n_nodes) {
+    int b = ptr[n], e = ptr[n + 1];
+    for (int i = b + 1; i < e; i++) {
+      int v = n2e[i];
+      int j = i - 1;
+      while (j >= b && n2e[j] > v) {
+        n2e[j + 1] = n2e[j];
+        j--;
+      }
+      n2e[j + 1] = v;
     }
-    n2e[j + 1] = v;
+    len = e - b;
   }
-  atomicMax(maxadj, e - b);
+  block_max(len, maxadj);
 }
 
 __global__ void k_nodes_on_plane(const double *coords, int64_t n, int pdim, int axis, double coord,

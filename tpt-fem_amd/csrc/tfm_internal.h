@@ -71,6 +71,23 @@ struct tfm_matrix {
 
 namespace tfm {
 
+// Maximum of v over the block (a whole number of wavefronts, at most 1024
+// threads: wave shuffles, then LDS) into *out with one atomic per block:
+// same-word atomics from every wave of a large grid serialise at L2.  Every
+// thread of the block must call it.
+__device__ __forceinline__ void block_max(int v, int *out) {
+  __shared__ int s_max[16];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = s_max[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = max(m, s_max[w]);
+    atomicMax(out, m);
+  }
+}
+
 int fail(tfm_ctx *ctx, int code, const std::string &msg);
 int set_device(tfm_ctx *ctx);
 int ensure_adjacency(tfm_mesh *m);
@@ -387,4 +404,5 @@ struct tfm_system {
     if (w32) ver32 = version;
   }
   bool fp32_current() const { return d_v32 != nullptr && ver32 == ver64; }
+  int csr_chunk = 0;  // rows per CSR-stream chunk (from the longest row), once known
 };

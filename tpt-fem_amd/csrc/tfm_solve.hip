@@ -455,9 +455,10 @@ static void dir_launch(const SpOp<VT> &A, hipStream_t st, const VT *z, const VT 
 #undef TFM_BS
 }
 
-__global__ void k_max_row_nnz(int n, const int *rp, int *out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicMax(out, rp[i + 1] - rp[i]);
+__global__ __launch_bounds__(256) void k_max_row_nnz(int n, const int *rp, int *out) {
+  int m = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = max(m, rp[i + 1] - rp[i]);
+  block_max(m, out);
 }
 
 // Runs PCG on K x = r0 (x accumulates; r holds r0 on entry) until
@@ -1278,6 +1279,10 @@ extern "C" int tfm_debug_system_values32(const tfm_system *s, double *values) {
 
 // Rows per CSR-stream chunk of the system's matrix (from its longest row).
 static int system_chunk_rows(tfm_system *s, int *S) {
+  if (s->csr_chunk > 0) {  // the pattern never changes
+    *S = s->csr_chunk;
+    return TFM_OK;
+  }
   tfm_ctx *ctx = s->ctx;
   hipStream_t st = ctx->stream;
   const int n = (int)s->ndof;
@@ -1285,12 +1290,15 @@ static int system_chunk_rows(tfm_system *s, int *S) {
   int rc = dalloc(ctx, &d_m, 1);
   if (rc) return rc;
   TFM_CHECK_HIP(ctx, hipMemsetAsync(d_m, 0, sizeof(int), st));
-  if (n > 0) hipLaunchKernelGGL(k_max_row_nnz, dim3(grid1(n, 256)), dim3(256), 0, st, n, s->d_rp, d_m);
+  if (n > 0)
+    hipLaunchKernelGGL(k_max_row_nnz, dim3(std::min<unsigned>(grid1(n, 256), 2048)), dim3(256), 0, st, n, s->d_rp,
+                       d_m);
   hipError_t e = hipMemcpy(&mnz, d_m, sizeof(int), hipMemcpyDeviceToHost);
   dfree(d_m);
   if (e != hipSuccess) return fail(ctx, TFM_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
   if (mnz > kWaveCap) return fail(ctx, TFM_E_UNSUPPORTED, "a matrix row is longer than the SpMV chunk buffer");
   *S = rows_per_chunk(mnz);
+  s->csr_chunk = *S;
   return TFM_OK;
 }
 
@@ -1418,18 +1426,11 @@ static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, 
     return fail(ctx, TFM_E_INVALID, "mixed solve: the FP32 values predate the last FP64 re-assembly");
   hipStream_t st = ctx->stream;
   // rows per CSR-stream chunk, from the longest row
-  int mnz = 0;
+  int S = 1;
   {
-    int *d_m = nullptr;
-    int rc0 = dalloc(ctx, &d_m, 1);
+    const int rc0 = system_chunk_rows(s, &S);
     if (rc0) return rc0;
-    TFM_CHECK_HIP(ctx, hipMemsetAsync(d_m, 0, sizeof(int), st));
-    if (n > 0) hipLaunchKernelGGL(k_max_row_nnz, dim3(grid1(n, 256)), dim3(256), 0, st, n, s->d_rp, d_m);
-    TFM_CHECK_HIP(ctx, hipMemcpy(&mnz, d_m, sizeof(int), hipMemcpyDeviceToHost));
-    dfree(d_m);
   }
-  if (mnz > kWaveCap) return fail(ctx, TFM_E_UNSUPPORTED, "a matrix row is longer than the SpMV chunk buffer");
-  const int S = rows_per_chunk(mnz);
   // device work space
   Scal *sc = nullptr;
   double *part = nullptr, *x = nullptr;
