@@ -253,6 +253,58 @@ __global__ __launch_bounds__(kBlock) void k_residual64(int n, const int *__restr
   put_part(part, P_FF, ff);
 }
 
+// r = f - K u on the node-blocked sliced copy (the iteration's traversal);
+// rows of fixed DOFs (empty CSR rows) give 0; optional FP32 copy; partial
+// ||r||^2, ||f||^2.
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_resid_bsell(int n, int nslices, const int *__restrict__ soff,
+                                                         const int *__restrict__ bcol, const double *__restrict__ bv,
+                                                         const int *__restrict__ rp, const double *__restrict__ u,
+                                                         const double *__restrict__ f, double *__restrict__ r,
+                                                         float *__restrict__ r32, double *part) {
+  constexpr int BB = BS * BS;
+  const int lane = threadIdx.x & 63;
+  const int nwaves = gridDim.x * (kBlock / 64);
+  const int gw = xcd_item(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  const int per = (nslices + nwaves - 1) / nwaves;
+  const int s1 = min(nslices, (gw + 1) * per);
+  double rr = 0.0, ff = 0.0;
+  for (int s = gw * per; s < s1; s++) {
+    const int o = soff[s], w = (soff[s + 1] - o) >> 6;
+    const int *cp = bcol + o + lane;
+    const double *vp = bv + (int64_t)o * BB + lane;
+    double acc[BS];
+#pragma unroll
+    for (int i = 0; i < BS; i++) acc[i] = 0.0;
+#pragma unroll 2
+    for (int k = 0; k < w; k++) {
+      const int c = cp[k * 64] * BS;
+      double xv[BS];
+#pragma unroll
+      for (int j = 0; j < BS; j++) xv[j] = u[c + j];
+      const double *vk = vp + (int64_t)k * (64 * BB);
+#pragma unroll
+      for (int i = 0; i < BS; i++)
+#pragma unroll
+        for (int j = 0; j < BS; j++) acc[i] += vk[(i * BS + j) * 64] * xv[j];
+    }
+    const int r0 = (s * 64 + lane) * BS;
+    if (r0 < n) {
+#pragma unroll
+      for (int i = 0; i < BS; i++) {
+        const int row = r0 + i;
+        const double ri = (rp[row] == rp[row + 1]) ? 0.0 : f[row] - acc[i];
+        r[row] = ri;
+        if (r32) r32[row] = (float)ri;
+        rr += ri * ri;
+        ff += f[row] * f[row];
+      }
+    }
+  }
+  put_part(part, P_RES, rr);
+  put_part(part, P_FF, ff);
+}
+
 __global__ void k_residual_finish(Scal *sc, const double *part, int nb) {
   double rr = sum_part(part, P_RES, nb), ff = sum_part(part, P_FF, nb);
   if (threadIdx.x == 0) {
@@ -548,6 +600,26 @@ __global__ void k_fold(int n, double *__restrict__ x, float *__restrict__ d) {
   }
 }
 
+// Residual r = f - K u through the operator: the sliced copy when it holds
+// the FP64 values, else the CSR stream; then sc->rr, sc->rr0.
+static void residual_op(const SpOp<double> &K, hipStream_t st, const double *u, const double *f, double *r,
+                        float *r32, Scal *sc, double *part) {
+  if (K.bs == 0) {
+    residual_dispatch(K.R, st, K.nb, K.n, K.rp, K.ci, K.v, u, f, r, r32, sc, part);
+    return;
+  }
+#define TFM_BS(BSZ)                                                                                             \
+  hipLaunchKernelGGL((k_resid_bsell<BSZ>), dim3(K.nb), dim3(kBlock), 0, st, K.n, K.nslices, K.soff, K.bcol, K.bv, \
+                     K.rp, u, f, r, r32, part)
+  switch (K.bs) {
+  case 3: TFM_BS(3); break;
+  case 2: TFM_BS(2); break;
+  default: TFM_BS(1); break;
+  }
+#undef TFM_BS
+  hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(kBlock), 0, st, sc, part, K.nb);
+}
+
 // Precision form: FP32 PCG on the correction with reliable updates.  The
 // FP32 iteration runs on; whenever its recurrence residual has dropped by
 // delta since the last replacement, the FP32 correction is folded into the
@@ -560,15 +632,12 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
                      Scal *sc, double *part, int *iters, int *repl, int *breakdown, double *relres) {
   hipStream_t st = ctx->stream;
   const int n = A.n, nb = A.nb;
-  const int S = K64.R, nbr = K64.nb;
-  const int *rp = K64.rp, *ci = K64.ci;
-  const double *v64 = K64.v;
   const unsigned g1 = (unsigned)((n + 255) / 256);
   Scal h{};
   *iters = 0;
   *repl = 0;
   *breakdown = 0;
-  residual_dispatch(S, st, nbr, n, rp, ci, v64, x, f, r64, B.r, sc, part);
+  residual_op(K64, st, x, f, r64, B.r, sc, part);
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
   *relres = h.rr0 > 0 ? std::sqrt(h.rr / h.rr0) : 0.0;
@@ -628,7 +697,7 @@ static int mixed_run(tfm_ctx *ctx, const SpOp<double> &K64, const SpOp<float> &A
     }
     if (h.rr <= delta * delta * rr_ref || h.rr <= target2 || h.done || since >= kStall) {
       hipLaunchKernelGGL(k_fold, dim3(g1), dim3(256), 0, st, n, x, x32);
-      residual_dispatch(S, st, nbr, n, rp, ci, v64, x, f, r64, B.r, sc, part);
+      residual_op(K64, st, x, f, r64, B.r, sc, part);
       Scal h2{};
       TFM_CHECK_HIP(ctx, hipMemcpyAsync(&h2, sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
       TFM_CHECK_HIP(ctx, hipStreamSynchronize(st));
@@ -1497,9 +1566,15 @@ static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, 
     A32.box = box;
     A32.nb = mf_grid(box);
   }
+  SpOp<double> R64 = K64;  // the FP64 residuals' operator
   if (s->bsell.state > 0) {
-    if (!mixed && !(rc = bsell_values<double>(s, s->d_v64, &s->bsell.d_v64)))
-      A64 = bsell_op<double>(n, s->bsell, s->bsell.d_v64);
+    // FP64 values in the sliced copy: the plain iteration's operator, and
+    // the precision form's residual replacements
+    if (!(rc = bsell_values<double>(s, s->d_v64, &s->bsell.d_v64))) {
+      R64 = bsell_op<double>(n, s->bsell, s->bsell.d_v64);
+      R64.rp = s->d_rp;
+      if (!mixed) A64 = R64;
+    }
     if (mixed && !mf && !(rc = bsell_values<float>(s, s->d_v32, &s->bsell.d_v32)))
       A32 = bsell_op<float>(n, s->bsell, s->bsell.d_v32);
     if (rc) {
@@ -1542,7 +1617,7 @@ static int system_solve_iter(tfm_system *s, const tfm_solve_opts &o, double *u, 
     // FP32 PCG with FP64 reliable-update refinement; inner_rtol is the
     // residual reduction between FP64 replacements (0 picks 0.3).
     const double delta = (o.inner_rtol > 0.0 && o.inner_rtol < 1.0) ? o.inner_rtol : 0.3;
-    rc = mixed_run(ctx, K64, A32, s->d_f, B32, B64.r, x32, x, o.rtol, delta,
+    rc = mixed_run(ctx, R64, A32, s->d_f, B32, B64.r, x32, x, o.rtol, delta,
                    o.max_iter, sc, part, &total_it, &outer, &brk, &relres);
   }
   TFM_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, st));
