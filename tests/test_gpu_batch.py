@@ -78,6 +78,9 @@ def test_batched_simp_multi_rhs_matches_direct(ctx):
             f[k, 2 * node + 1] = -1.0
     op = T.Operator.simp(rho, nu=0.3, penal=3.0, e0=1.0, emin=1e-9, h=1.0)
     s = T.System.create(m, op, 2, bcs, None, flags=T.SYS_FP64 | T.SYS_FP32)
+    assert s.assembly_info()["kernel"] == "simp_grid"
+    # both arrays in one pass: the FP32 values are the FP64 values rounded
+    np.testing.assert_array_equal(s.values32(), s.download()[0].values.astype(np.float32).astype(np.float64))
     off = np.arange(P + 1) * ndof1
     u = s.solve_batch(off, rhs=f, nrhs=8, mode=T.SOLVE_PCG64, rtol=1e-12)
     um = s.solve_batch(off, rhs=f, nrhs=8, mode=T.SOLVE_MIXED, rtol=1e-10)

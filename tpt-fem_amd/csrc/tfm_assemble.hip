@@ -577,13 +577,15 @@ __global__ void k_simp_erho(int64_t ne, const double *__restrict__ rho, double e
 // row's entries (at most 9 neighbours x 2) go to LDS at their CSR offsets,
 // then the block's contiguous CSR range is written with coalesced stores.
 // With CHECK, only compares each row's entry count with the pattern's.
+// X32 (FP64 only): also stores every value rounded to FP32 into v32, the
+// FP32 array of a system that holds both.
 constexpr int kSimpRows = 256;
-template <typename VT, bool CHECK>
+template <typename VT, bool CHECK, bool X32 = false>
 __global__ __launch_bounds__(kSimpRows) void k_simp_rows(int64_t nrows, int nx, int ny, const double *__restrict__ er,
                                                           const double *__restrict__ ke0,
                                                           const unsigned char *__restrict__ fixed,
                                                           const int *__restrict__ rp, VT *__restrict__ vals,
-                                                          int *__restrict__ bad) {
+                                                          int *__restrict__ bad, float *__restrict__ v32) {
   __shared__ VT s_v[CHECK ? 1 : kSimpRows * 18];
   const int64_t r0 = (int64_t)blockIdx.x * kSimpRows;
   const int64_t r = r0 + threadIdx.x;
@@ -650,7 +652,10 @@ __global__ __launch_bounds__(kSimpRows) void k_simp_rows(int64_t nrows, int nx, 
   if constexpr (!CHECK) {
     __syncthreads();
     const int len = rp[r1] - base;
-    for (int k = threadIdx.x; k < len; k += kSimpRows) vals[base + k] = s_v[k];
+    for (int k = threadIdx.x; k < len; k += kSimpRows) {
+      vals[base + k] = s_v[k];
+      if constexpr (X32) v32[base + k] = (float)s_v[k];
+    }
   }
 }
 
@@ -671,12 +676,17 @@ static int simp_grid_numeric(tfm_ctx *ctx, const AsmPlan *pl, double *v64, float
   const int64_t nrows = pl->ndof;
   const dim3 g((unsigned)((nrows + kSimpRows - 1) / kSimpRows)), b(kSimpRows);
   const double *ke0 = pl->d_tab + (size_t)pl->nq * kRuleStride;
-  if (v64)
+  if (v64 && v32) {
+    // both arrays in one pass: the FP32 values are the FP64 sums rounded
+    hipLaunchKernelGGL((k_simp_rows<double, false, true>), g, b, 0, st, nrows, (int)m->grid_nx, (int)m->grid_ny,
+                       pl->d_erho, ke0, pl->d_fixed, pl->d_rp, v64, nullptr, v32);
+  } else if (v64) {
     hipLaunchKernelGGL((k_simp_rows<double, false>), g, b, 0, st, nrows, (int)m->grid_nx, (int)m->grid_ny,
-                       pl->d_erho, ke0, pl->d_fixed, pl->d_rp, v64, nullptr);
-  if (v32)
+                       pl->d_erho, ke0, pl->d_fixed, pl->d_rp, v64, nullptr, nullptr);
+  } else if (v32) {
     hipLaunchKernelGGL((k_simp_rows<float, false>), g, b, 0, st, nrows, (int)m->grid_nx, (int)m->grid_ny,
-                       pl->d_erho, ke0, pl->d_fixed, pl->d_rp, v32, nullptr);
+                       pl->d_erho, ke0, pl->d_fixed, pl->d_rp, v32, nullptr, nullptr);
+  }
   TFM_CHECK_HIP(ctx, hipGetLastError());
   return TFM_OK;
 }
@@ -692,7 +702,7 @@ static int simp_grid_check(tfm_ctx *ctx, AsmPlan *pl) {
   hipLaunchKernelGGL((k_simp_rows<double, true>), dim3((unsigned)((nrows + kSimpRows - 1) / kSimpRows)),
                      dim3(kSimpRows), 0, ctx->stream,
                      nrows, (int)m->grid_nx, (int)m->grid_ny, pl->d_rho, pl->d_rho, pl->d_fixed, pl->d_rp,
-                     (double *)nullptr, bad);
+                     (double *)nullptr, bad, (float *)nullptr);
   TFM_CHECK_HIP(ctx, hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   TFM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   dfree(bad);
