@@ -97,6 +97,8 @@ __device__ __forceinline__ void prob_sum(const BView &V, int slot, int p, double
   const double *q = V.part + ((size_t)slot * V.nblk + b0) * NR;
 #pragma unroll
   for (int k = 0; k < NR; k++) out[k] = 0.0;
+  // unrolled so the loads issue ahead of the adds, whose order is unchanged
+#pragma unroll 8
   for (int i = threadIdx.x; i < nb; i += kBT)
 #pragma unroll
     for (int k = 0; k < NR; k++) out[k] += q[(size_t)i * NR + k];
